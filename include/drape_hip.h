@@ -1,0 +1,238 @@
+/* drape_hip.h — C-ABI of the MI355X implicit-Euler Newton–PCG cloth step.
+ *
+ * This is the drop-in boundary for the reference's `drape::sim::Solver` surface
+ * (libs/sim/include/drape/sim/solver.hpp:85-98) and its backend switch
+ * (libs/sim/include/drape/sim/solver_factory.hpp:12-20). Every entry point is plain C:
+ * opaque handles, plain pointers and sizes, an int status code, and a thread-local last-error
+ * string. A C++ caller uses the header-only `drape::sim::HipSolver` wrapper
+ * (include/drape/sim/hip_solver.hpp), which maps status codes back to the reference's
+ * exception types.
+ *
+ * Status codes mirror the exceptions the reference throws:
+ *   DRAPE_OK                      0
+ *   DRAPE_ERR_INVALID_ARGUMENT    std::invalid_argument  (sim_model.cpp:81-86, :201, :257-258; cpu_solver.cpp:341)
+ *   DRAPE_ERR_OUT_OF_RANGE        std::out_of_range      (sim_model.cpp:15, :184, :190)
+ *   DRAPE_ERR_RUNTIME             std::runtime_error     (device failures, metal_device.cpp:58-88, :136)
+ *   DRAPE_ERR_LOGIC               std::logic_error       (API misuse, metal_device.cpp:108, :115, :126)
+ */
+#ifndef DRAPE_HIP_H
+#define DRAPE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DRAPE_OK 0
+#define DRAPE_ERR_INVALID_ARGUMENT 1
+#define DRAPE_ERR_OUT_OF_RANGE 2
+#define DRAPE_ERR_RUNTIME 3
+#define DRAPE_ERR_LOGIC 4
+
+#define DRAPE_PHASE_ASSEMBLING 0 /* SimPhase::Assembling (solver.hpp:17) */
+#define DRAPE_PHASE_SETTLING 1   /* SimPhase::Settling */
+
+/* FabricPhysical (libs/core/include/drape/core/fabric.hpp:10-18), SI units. */
+typedef struct drape_fabric {
+  double stretch_warp, stretch_weft, stretch_bias; /* N/m */
+  double strain_limit_warp, strain_limit_weft;     /* fraction */
+  double bend_warp, bend_weft;                     /* N m */
+  double weight;                                   /* kg/m^2 */
+  double thickness;                                /* m */
+  double friction, damping;                        /* 0..1 */
+} drape_fabric;
+
+/* SimSettings (solver.hpp:19-30). drape_settings_default() fills the reference defaults. */
+typedef struct drape_settings {
+  double gravity[3];
+  double collision_offset;
+  double assembly_closing_speed;
+  double assembly_extra_damping;
+  double damping_scale;
+  double strain_limit_stiffness_scale;
+  double seam_stiffness;
+  double friction_slip_speed;
+  int32_t max_cg_iterations;
+  int32_t pad0;
+  double cg_tolerance;
+} drape_settings;
+
+/* SimMesh (libs/core/include/drape/core/sim_mesh.hpp:11-27), one garment, flat arrays. */
+typedef struct drape_mesh_desc {
+  uint32_t n_vertices;
+  const double* rest;            /* 2 per vertex, pattern space (m) */
+  const uint32_t* vertex_piece;  /* 1 per vertex */
+  uint32_t n_triangles;
+  const uint32_t* triangles;     /* 3 per triangle, CCW in pattern space */
+  const uint32_t* triangle_piece;
+  const double* vertex_area;     /* 1/3 of incident rest areas (m^2) */
+  const uint8_t* boundary;       /* may be NULL (unused by the step) */
+  uint32_t n_pieces;
+  const double* piece_grain;     /* 2 per piece, unit grainline */
+  uint32_t n_seam_pairs;
+  const uint32_t* seam_pairs;    /* 4 per pair: a, b, seam, index (SeamPair, sim_mesh.hpp:8-13) */
+} drape_mesh_desc;
+
+/* SimGarment (solver.hpp:32-37). */
+typedef struct drape_garment_desc {
+  const char* id;
+  drape_mesh_desc mesh;
+  const double* initial_positions; /* 3 per mesh vertex */
+  uint32_t n_piece_fabrics;
+  const drape_fabric* piece_fabric; /* one per piece */
+} drape_garment_desc;
+
+/* CollisionField (libs/core/include/drape/core/collision_field.hpp:11-20): signed distance on a regular grid,
+ * samples at origin + (i,j,k) * cell_size, x fastest. The library copies the values at build time. */
+typedef struct drape_field_desc {
+  double origin[3];
+  double cell_size;
+  int32_t dims[3];
+  int32_t pad0;
+  double band;
+  const float* values; /* dims[0]*dims[1]*dims[2] */
+} drape_field_desc;
+
+/* SimScene (solver.hpp:49-55). body may be NULL. Pin/force particle indices are global. */
+typedef struct drape_scene_desc {
+  uint32_t n_garments;
+  const drape_garment_desc* garments;
+  const drape_field_desc* body;
+  uint32_t n_pins;
+  const uint32_t* pin_particles;
+  const double* pin_positions; /* 3 per pin */
+  uint32_t n_forces;
+  const uint32_t* force_particles;
+  const double* force_vectors; /* 3 per force, newtons */
+  drape_settings settings;
+} drape_scene_desc;
+
+/* SimStats (solver.hpp:63-69). */
+typedef struct drape_stats {
+  int32_t has_non_finite;
+  int32_t pad0;
+  double kinetic_energy;
+  double rms_speed;
+  double max_seam_gap;
+  double max_body_penetration;
+} drape_stats;
+
+/* Work counters of the step (not exposed by the reference's Solver; phase-b spec :196 notes their absence).
+ * Cumulative since build or the last drape_solver_reset_counters(). Summed over segments. */
+typedef struct drape_counters {
+  uint64_t substeps;
+  uint64_t newton_iterations;
+  uint64_t pcg_iterations;
+  uint64_t line_search_trials;
+  uint64_t warm_start_fallbacks; /* learned guesses rejected because they raised the energy */
+  uint64_t warm_start_accepted;
+} drape_counters;
+
+typedef struct drape_solver drape_solver; /* opaque */
+
+/* Last error message of the calling thread ("" if none). */
+const char* drape_last_error(void);
+
+/* Reference defaults of SimSettings (solver.hpp:19-30). */
+int drape_settings_default(drape_settings* out);
+
+/* Backend probe (solver_factory.hpp:17-18, isAvailable): 1 when a gfx950 device is usable, else 0 (reason in
+ * drape_last_error). */
+int drape_hip_available(void);
+
+/* makeSolver(SolverBackend::Hip) (solver_factory.hpp:20): runtime_error naming "hip" when no device. */
+int drape_solver_create(drape_solver** out);
+void drape_solver_destroy(drape_solver* s);
+
+/* Solver::build (solver.hpp:88). The scene is copied except body values, which are copied to the device. */
+int drape_solver_build(drape_solver* s, const drape_scene_desc* scene);
+
+/* Batched build: n independent scenes stepped together, one segment each (own particles, own body field, own
+ * line search and convergence). Particles are concatenated in scene order. drape_solver_build(s, sc) is
+ * drape_solver_build_batch(s, 1, sc). */
+int drape_solver_build_batch(drape_solver* s, uint32_t n_scenes, const drape_scene_desc* scenes);
+
+/* Solver::updateFabric / setPhase / range / particleCount (solver.hpp:89-90, :96-97). For batched builds the
+ * garment id is looked up in scene order; the first match wins. */
+int drape_solver_update_fabric(drape_solver* s, const char* garment, uint32_t n, const drape_fabric* piece_fabric);
+int drape_solver_set_phase(drape_solver* s, const char* garment, int32_t phase);
+int drape_solver_range(const drape_solver* s, const char* garment, uint32_t* offset, uint32_t* count);
+int drape_solver_particle_count(const drape_solver* s, size_t* out);
+int drape_solver_segment_count(const drape_solver* s, uint32_t* out);
+int drape_solver_segment_range(const drape_solver* s, uint32_t segment, uint32_t* offset, uint32_t* count);
+
+/* Solver::step (solver.hpp:91): substeps = max(1, substeps), h = dt / substeps, iterations = max(1, iterations)
+ * (cpu_solver.cpp:333-338). */
+int drape_solver_step(drape_solver* s, double dt, int32_t substeps, int32_t iterations);
+
+/* Solver::readPositions (solver.hpp:92): out holds 3 floats per particle; n must equal particleCount()
+ * (invalid_argument otherwise, cpu_solver.cpp:340-343). */
+int drape_solver_read_positions(const drape_solver* s, float* out, size_t n);
+int drape_solver_read_velocities(const drape_solver* s, float* out, size_t n);
+
+/* Solver::stats (solver.hpp:93), computed on the device (FP64 reductions). */
+int drape_solver_stats(const drape_solver* s, drape_stats* out);
+
+/* Solver::snapshot / restore (solver.hpp:94-95) as one flat byte blob: query the size, save into a caller
+ * buffer, restore from one. Layout: positions, velocities (float3 each), phases, assemblyTime, seamStartGap,
+ * contactForce (sim_model.cpp:243-274). */
+int drape_solver_snapshot_size(const drape_solver* s, size_t* bytes);
+int drape_solver_snapshot_save(const drape_solver* s, void* out, size_t bytes);
+int drape_solver_snapshot_restore(drape_solver* s, const void* in, size_t bytes);
+
+int drape_solver_counters(const drape_solver* s, drape_counters* out);
+int drape_solver_reset_counters(drape_solver* s);
+
+/* Execution options. graph: capture each substep's launch sequence in a hipGraph and replay it (1, default) or
+ * launch eagerly (0). Results are bit-identical either way. */
+int drape_solver_set_graph(drape_solver* s, int32_t enabled);
+
+/* ---- Structure and kernel access (parity tests and benchmarks; same kernels as the step) ---- */
+
+/* Block-CSR pattern and term slot tables built on the device (sim_model.cpp:41-74). Sizes first (any pointer may
+ * be NULL to skip), then copies. */
+typedef struct drape_structure_sizes {
+  uint32_t n_particles, n_stretch, n_hinges, n_seams, n_blocks;
+} drape_structure_sizes;
+int drape_solver_structure_sizes(const drape_solver* s, drape_structure_sizes* out);
+int drape_solver_read_structure(const drape_solver* s, uint32_t* row_start, uint32_t* col_index, uint32_t* diag_slot,
+                                uint32_t* stretch_slots, uint32_t* hinge_slots, uint32_t* seam_slots);
+/* Terms as built (sim_model.cpp:107-175): stretch vertices (3 each), hinge vertices (4 each), hinge K, seam
+ * a/b (2 each). */
+int drape_solver_read_terms(const drape_solver* s, uint32_t* stretch_v, uint32_t* hinge_v, double* hinge_k,
+                            uint32_t* seam_ab);
+
+/* Assembles at substep start with displacement u (3 floats per particle, relative to the current positions)
+ * and no contact: total energy (FP64 sum), gradient (3 per particle) and the blocks in BSR slot order (9 per
+ * block, row-major). Uses h for the inertia term. Any output may be NULL. */
+int drape_solver_debug_assemble(drape_solver* s, double h, const float* u, double* energy, float* grad,
+                                float* blocks);
+/* y = A x over the last assembled matrix, pinned rows/columns excluded (cpu_solver.cpp:147-159). */
+int drape_solver_debug_spmv(drape_solver* s, const float* x, float* y);
+
+/* Standalone block-CSR SpMV on caller arrays (host memory): copies in, runs the solver's SpMV kernel `reps`
+ * times, copies y out, and reports the mean kernel time in microseconds (HIP events). pinned may be NULL. */
+int drape_bsr_spmv(uint32_t n, uint32_t n_blocks, const uint32_t* row_start, const uint32_t* col_index,
+                   const float* blocks /* 9 per block, row-major */, const uint8_t* pinned, const float* x,
+                   float* y, int32_t reps, double* mean_us);
+
+/* Batch term evaluation on the device (the FP32 term math of the step): n samples of one kind.
+ * kind 0 stretch: x 9 floats, params [dmInv00, dmInv10, dmInv01, dmInv11, area], fabric per sample;
+ * kind 1 hinge:   x 12 floats, params [K, restAngle];
+ * kind 2 seam:    x 6 floats,  params [target, K];
+ * kind 3 friction: x 3 floats (x - x0), params [nx, ny, nz, muLambda, eps].
+ * Outputs: energy (n), grad (n * 3N), hess (n * 9 N^2, block (a,b) row-major at (a*N+b)*9). */
+int drape_eval_terms(int32_t kind, uint32_t n, const float* x, const float* params, const drape_fabric* fabrics,
+                     float limit_scale, float* energy, float* grad, float* hess);
+
+/* Body field queries on the device: phi (n) and unit gradient (3n) at points p (3n), FP32
+ * (CollisionField::sample/gradient, collision_field.cpp:8-41). */
+int drape_field_query(const drape_field_desc* field, uint32_t n, const float* p, float* phi, float* normal);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DRAPE_HIP_H */
