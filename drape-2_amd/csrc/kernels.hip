@@ -1,0 +1,1063 @@
+// kernels.hip — the HIP kernels of the implicit-Euler Newton–PCG cloth step on MI355X (gfx950).
+//
+// Execution model: one lane per particle row. Row kernels run 256-thread workgroups over 256-row tiles; the SELL-64
+// matrix kernels give each wavefront one 64-row slice. Workgroup ids are remapped so that each of the eight XCDs
+// (which receive workgroups round-robin) walks a contiguous range of tiles, keeping neighbouring rows — and the
+// vector entries their block columns gather — in one XCD's L2.
+//
+// Determinism: no floating-point atomics. Every sum runs in a fixed order: a lane accumulates its row's terms in
+// the reference's gather order (gpu_solver_core.cpp:184-224), FP64 partial sums per tile reduce in a fixed
+// shuffle tree, and every workgroup that needs a segment total re-sums the same partials in the same order.
+// Integer atomics appear only in the structure builder's counting pass, whose results are order independent (the
+// lists they fill are sorted afterwards).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "kernels.hpp"
+#include "term_math.hpp"
+
+namespace dk {
+
+namespace {
+
+void hipCheck(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string("hip: ") + what + ": " + hipGetErrorString(e));
+}
+
+inline uint32_t xcdGrid(uint32_t count) { return 8u * ((count + 7u) >> 3); }
+
+}  // namespace
+
+// Workgroup -> tile such that XCD (bid % 8) covers the contiguous range [x * per, (x + 1) * per).
+__device__ inline uint32_t xcdMap(uint32_t bid, uint32_t count) {
+  const uint32_t per = (count + 7u) >> 3;
+  return (bid & 7u) * per + (bid >> 3);
+}
+
+// Sum over a 256-thread workgroup, fixed order; the result is returned to every thread.
+__device__ inline double blockSum256(double v, double* lds) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) lds[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) lds[4] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+  __syncthreads();
+  return lds[4];
+}
+
+__device__ inline float blockMax256(float v, float* lds) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_down(v, off, 64));
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) lds[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) lds[4] = fmaxf(fmaxf(lds[0], lds[1]), fmaxf(lds[2], lds[3]));
+  __syncthreads();
+  return lds[4];
+}
+
+// Sum of part[begin, end) in a fixed order, identical in every workgroup that calls it.
+__device__ inline double rangeSum256(const double* part, uint32_t begin, uint32_t end, double* lds) {
+  double s = 0.0;
+  for (uint32_t i = begin + threadIdx.x; i < end; i += 256u) s += part[i];
+  return blockSum256(s, lds);
+}
+
+__device__ inline float rangeMax256(const float* part, uint32_t begin, uint32_t end, float* lds) {
+  float s = 0.0f;
+  for (uint32_t i = begin + threadIdx.x; i < end; i += 256u) s = fmaxf(s, part[i]);
+  return blockMax256(s, lds);
+}
+
+__device__ inline double waveSum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;  // valid in lane 0
+}
+
+__device__ inline float contactRadius(const SegParams& P, const Fabric32& f) {
+  return P.collisionOffset + 0.5f * f.thickness;
+}
+
+// Term vertex position relative to the term's first vertex at substep start: (xStart_k - xStart_v0) + u_k
+// (kernels.h:131-134). `alpha` > 0 evaluates at the line-search trial u + alpha step.
+__device__ inline F3 termPos(const View& v, uint32_t v0, uint32_t vk, float alpha) {
+  F3 uk = v.u[vk];
+  if (alpha != 0.0f) uk = uk + alpha * v.step[vk];
+  return (v.xStart[vk] - v.xStart[v0]) + uk;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Substep setup: predict + contact setup (kernels.h:34-82; cpu_solver.cpp:234-263).
+
+__global__ void __launch_bounds__(256) kPredictContact(View v) {
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t s = v.tileSeg[tile];
+  const SegParams& P = v.seg[s];
+  SegState* S = v.state + s;
+  if (tile == P.tileBegin && threadIdx.x == 0) {
+    S->newtonDone = 0u;
+    S->pcgConverged = 0u;
+    S->lsDone = 0u;
+    S->cntSubsteps += 1u;
+  }
+  const uint32_t row = tile * kTile + threadIdx.x;
+  const uint32_t flags = v.rowFlags[row];
+  const F3 xs = v.x[row];
+  v.xStart[row] = xs;
+  const float lastForce = v.cforce[row];
+  uint32_t cf = 0u;
+  float lambda = 0.0f;
+  v.cforce[row] = 0.0f;
+  if (flags & (kRowPinned | kRowPad)) {
+    v.yRel[row] = f3(0, 0, 0);
+    v.u[row] = f3(0, 0, 0);
+  } else {
+    const float h = P.h;
+    const F3 vel = v.v[row];
+    const F3 g = v.garments[v.rowGarment[row]].phase == 1u ? f3(P.gx, P.gy, P.gz) : f3(0, 0, 0);
+    v.yRel[row] = h * vel + (h * h) * (g + v.fExt[row] / v.mass[row]);
+    // Warm start u = h (v + dv): dv = 0 is the reference's previous-velocity guess (cpu_solver.cpp:244).
+    v.u[row] = h * (vel + v.warm[row]);
+    if (P.field.present) {
+      const Fabric32 fab = v.fabrics[v.rowFabric[row]];
+      const float radius = contactRadius(P, fab);
+      const float phi = fieldSample(v.fieldValues, P.field, xs);
+      if (phi < radius + P.contactReach) {
+        const F3 n = fieldGradient(v.fieldValues, P.field, xs);
+        if (!isZero(n)) {
+          cf = kContactNear;
+          v.cnormal[row] = make_float4(n.x, n.y, n.z, radius - phi);
+          lambda = lastForce;
+        }
+      }
+    }
+  }
+  v.cflags[row] = cf;
+  v.clambda[row] = lambda;
+}
+
+// Holds near particles on their contact plane, projecting any below it (kernels.h:86-99; cpu_solver.cpp:218-223).
+__global__ void __launch_bounds__(256) kActivate(View v) {
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t s = v.tileSeg[tile];
+  if (v.state[s].newtonDone) return;
+  const SegParams& P = v.seg[s];
+  const uint32_t row = tile * kTile + threadIdx.x;
+  uint32_t cf = v.cflags[row];
+  if (!(cf & kContactNear)) return;
+  const float4 cn = v.cnormal[row];
+  const F3 n = f3(cn.x, cn.y, cn.z);
+  F3 ui = v.u[row];
+  const float gap = cn.w - dot(n, ui);
+  if (gap > -P.contactSnap) cf |= kContactActive;
+  if ((cf & kContactActive) && gap > 0.0f) {
+    ui += gap * n;
+    v.u[row] = ui;
+  }
+  v.cflags[row] = cf;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Energy of a row's own term (inertia + friction, kernels.h:104-129) and the terms it owns (local vertex 0).
+
+template <bool Deriv>
+__device__ inline float ownTerm(const View& v, const SegParams& P, uint32_t row, uint32_t flags, float alpha, F3& g,
+                                M3& H) {
+  float e = 0.0f;
+  F3 ui = v.u[row];
+  if (alpha != 0.0f) ui = ui + alpha * v.step[row];
+  if (!(flags & kRowPinned)) {
+    const float k = v.mass[row] * P.invH2;
+    const F3 d = ui - v.yRel[row];
+    e = 0.5f * k * len2(d);
+    if (Deriv) {
+      g = k * d;
+      H = identityScaled(k);
+    }
+  }
+  const float lambda = v.clambda[row];
+  if (lambda > 0.0f) {
+    const float4 cn = v.cnormal[row];
+    const Fabric32 fab = v.fabrics[v.rowFabric[row]];
+    e += frictionTerm<Deriv>(ui, f3(cn.x, cn.y, cn.z), fab.friction * lambda, P.frictionSlip * P.h, g, H);
+  }
+  return e;
+}
+
+__device__ inline float seamTarget(const View& v, const SegParams& P, const Seam32& sm) {
+  const Garment32 g = v.garments[sm.garment];
+  return g.phase == 0u ? fmaxf(0.0f, sm.startGap - P.closingSpeed * g.assemblyTime) : 0.0f;
+}
+
+// Evaluates incidence `e` of `row`: returns the term energy; with Deriv adds the row's gradient to gt and its
+// blocks (row, b) to H[0..m-1], and reports m and the slot list of row a.
+template <bool Deriv>
+__device__ inline float incidence(const View& v, const SegParams& P, uint32_t e, float alpha, F3& gt, M3 H[4],
+                                  uint32_t& m, const uint32_t*& slots) {
+  const uint32_t kind = incKind(e), t = incTerm(e), a = incLocal(e);
+  if (kind == kKindStretch) {
+    const Stretch32 st = v.stretch[t];
+    F3 x[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) x[b] = termPos(v, st.v[0], st.v[b], alpha);
+    m = 3;
+    slots = v.stretchSlots + 9u * t + 3u * a;
+    return stretchRow<Deriv>(st, x, v.fabrics[st.fabric], P.limitScale, int(a), gt, H);
+  }
+  if (kind == kKindHinge) {
+    const Hinge32 hg = v.hinges[t];
+    F3 x[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) x[b] = termPos(v, hg.v[0], hg.v[b], alpha);
+    m = 4;
+    slots = v.hingeSlots + 16u * t + 4u * a;
+    return hingeRow<Deriv>(hg, x, int(a), gt, H);
+  }
+  const Seam32 sm = v.seams[t];
+  F3 x[2];
+  x[0] = termPos(v, sm.a, sm.a, alpha);
+  x[1] = termPos(v, sm.a, sm.b, alpha);
+  m = 2;
+  slots = v.seamSlots + 4u * t + 2u * a;
+  return seamRow<Deriv>(x, seamTarget(v, P, sm), P.seamStiffness, int(a), gt, H);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Assembly: energy, gradient and the block-CSR Hessian, atomic-free. One wavefront per 64-row slice; each lane
+// owns a row, evaluates every term incident to it (kernels.h:104-224 semantics) and accumulates its blocks in LDS
+// in the slice's SELL layout; the slice is then written out with coalesced stores. Contact release follows
+// (kernels.h:227-231).
+
+__global__ void __launch_bounds__(64) kAssemble(View v) {
+  extern __shared__ float acc[];  // K * 9 * 64
+  const uint32_t slice = xcdMap(blockIdx.x, v.nSlices);
+  if (slice >= v.nSlices) return;
+  const uint32_t tile = slice / kSlicesPerTile;
+  const uint32_t s = v.tileSeg[tile];
+  if (v.state[s].newtonDone) return;
+  const SegParams& P = v.seg[s];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t row = slice * kSlice + lane;
+  const uint32_t K = v.sellK[slice];
+  for (uint32_t i = lane; i < K * 9u * kSlice; i += kSlice) acc[i] = 0.0f;
+  __syncthreads();
+  const uint32_t flags = v.rowFlags[row];
+  double rowE = 0.0;
+  F3 g = f3(0, 0, 0);
+  if (!(flags & kRowPad)) {
+    const uint32_t rs = v.rowStart[row];
+    M3 Hown = zeroM();
+    rowE += double(ownTerm<true>(v, P, row, flags, 0.0f, g, Hown));
+    {
+      const uint32_t kd = v.diagSlot[row] - rs;
+#pragma unroll
+      for (int c = 0; c < 9; ++c) acc[(kd * 9u + c) * kSlice + lane] += Hown.m[c];
+    }
+    const uint32_t i1 = v.incStart[row + 1];
+    for (uint32_t i = v.incStart[row]; i < i1; ++i) {
+      const uint32_t e = v.inc[i];
+      F3 gt = f3(0, 0, 0);
+      M3 H[4] = {zeroM(), zeroM(), zeroM(), zeroM()};
+      uint32_t m;
+      const uint32_t* slots;
+      const float E = incidence<true>(v, P, e, 0.0f, gt, H, m, slots);
+      if (incLocal(e) == 0u) rowE += double(E);
+      g += gt;
+      for (uint32_t b = 0; b < m; ++b) {
+        const uint32_t kk = slots[b] - rs;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) acc[(kk * 9u + c) * kSlice + lane] += H[b].m[c];
+      }
+    }
+  }
+  if (flags & kRowPinned) g = f3(0, 0, 0);
+  v.grad[row] = g;
+  uint32_t cf = v.cflags[row];
+  if (cf & kContactActive) {
+    const float4 cn = v.cnormal[row];
+    if (dot(g, f3(cn.x, cn.y, cn.z)) < 0.0f) v.cflags[row] = cf & ~kContactActive;
+  }
+  rowE = waveSum(rowE);
+  if (lane == 0) v.partE[slice] = rowE;
+  __syncthreads();
+  float* out = v.sellVal + v.sellBase[slice];
+  for (uint32_t i = lane; i < K * 9u * kSlice; i += kSlice) out[i] = acc[i];
+}
+
+// e0 of this Newton iteration = sum of the assembly's energy partials (gpu_solver_core.cpp:344).
+__global__ void __launch_bounds__(256) kNewtonBegin(View v) {
+  __shared__ double lds[8];
+  const uint32_t s = blockIdx.x;
+  SegState* S = v.state + s;
+  if (S->newtonDone) return;
+  const SegParams& P = v.seg[s];
+  const double e = rangeSum256(v.partE, P.tileBegin * kSlicesPerTile, P.tileEnd * kSlicesPerTile, lds);
+  if (threadIdx.x == 0) {
+    S->e0 = e;
+    S->cntNewton += 1u;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Block-Jacobi PCG restricted to the contact planes (cpu_solver.cpp:169-213; kernels.h:237-347), FP32 vectors and
+// FP64 dot products. Per iteration two kernels; the search-direction update p = z + beta p is folded into the
+// next SpMV (each lane recomputes it for the columns it gathers), with p double-buffered by iteration parity.
+
+__device__ inline const float* diagBlock(const View& v, uint32_t row) {
+  const uint32_t slice = row / kSlice, lane = row % kSlice;
+  const uint32_t k = v.diagSlot[row] - v.rowStart[row];
+  return v.sellVal + v.sellBase[slice] + k * 9u * kSlice + lane;  // component c at [c * 64]
+}
+
+__global__ void __launch_bounds__(256) kPcgInit(View v) {
+  __shared__ double lds[8];
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t s = v.tileSeg[tile];
+  SegState* S = v.state + s;
+  if (S->newtonDone) return;
+  const SegParams& P = v.seg[s];
+  if (tile == P.tileBegin && threadIdx.x == 0) {
+    S->pcgConverged = 0u;
+    S->pcgIter = 0u;
+  }
+  const uint32_t row = tile * kTile + threadIdx.x;
+  const uint32_t flags = v.rowFlags[row];
+  float M[9];
+  if (flags & kRowPinned) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) M[c] = 0.0f;
+  } else {
+    const float* d = diagBlock(v, row);
+    float A[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) A[c] = d[c * kSlice];
+    if (!invertSpd3(A, M)) {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) M[c] = (c % 4 == 0) ? 1.0f : 0.0f;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 9; ++c) v.precond[9u * row + c] = M[c];
+  const uint32_t cf = v.cflags[row];
+  const float4 cn = v.cnormal[row];
+  const F3 ri = filterContact(cf, cn, -v.grad[row]);
+  M3 Mm;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) Mm.m[c] = M[c];
+  const F3 zi = filterContact(cf, cn, mul(Mm, ri));
+  v.step[row] = f3(0, 0, 0);
+  v.r[row] = ri;
+  v.z[row] = zi;
+  v.pA[row] = zi;
+  const double rz = double(ri.x) * zi.x + double(ri.y) * zi.y + double(ri.z) * zi.z;
+  const double rr = double(ri.x) * ri.x + double(ri.y) * ri.y + double(ri.z) * ri.z;
+  const double srz = blockSum256(rz, lds);
+  const double srr = blockSum256(rr, lds);
+  if (threadIdx.x == 0) {
+    v.partA[tile] = srz;
+    v.partB[tile] = srr;
+  }
+}
+
+// Iteration k, first half: convergence test of the previous update, beta, p = z + beta p, ap = filter(A p), p.ap.
+__global__ void __launch_bounds__(256) kPcgSpmv(View v, uint32_t k, const F3* __restrict__ pOld, F3* pNew) {
+  __shared__ double lds[8];
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t s = v.tileSeg[tile];
+  SegState* S = v.state + s;
+  const SegParams& P = v.seg[s];
+  const double rzNew = rangeSum256(v.partA, P.tileBegin, P.tileEnd, lds);
+  const double rr = rangeSum256(v.partB, P.tileBegin, P.tileEnd, lds);
+  if (S->newtonDone) return;
+  const bool first = tile == P.tileBegin && threadIdx.x == 0;
+  uint32_t conv;
+  double beta = 0.0;
+  if (k == 0) {
+    const double r0 = sqrt(rr);
+    conv = r0 > 0.0 ? 0u : 1u;
+    if (first) {
+      S->rzPar[0] = rzNew;
+      S->r0 = r0;
+      S->rr = rr;
+      S->beta = 0.0;
+      if (conv) S->pcgConverged = 1u;
+    }
+  } else {
+    conv = S->pcgConverged;
+    if (!conv) {
+      if (sqrt(rr) <= P.cgTolerance * S->r0) {
+        conv = 1u;
+      } else {
+        beta = rzNew / S->rzPar[(k + 1u) & 1u];
+      }
+    }
+    if (first) {
+      S->rzPar[k & 1u] = rzNew;
+      S->rr = rr;
+      S->beta = beta;
+      if (conv) S->pcgConverged = 1u;
+    }
+  }
+  if (k >= P.maxCg) {
+    if (first) S->pcgConverged = 1u;
+    conv = 1u;
+  }
+  if (conv) return;
+  const float b = float(beta);
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t slice = tile * kSlicesPerTile + wave;
+  const uint32_t row = slice * kSlice + lane;
+  const uint32_t K = v.sellK[slice];
+  const uint32_t base = v.sellBase[slice];
+  const float* __restrict__ val = v.sellVal + base + lane;
+  const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
+  const F3* __restrict__ z = v.z;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  for (uint32_t kk = 0; kk < K; ++kk) {
+    const uint32_t c = col[kk * kSlice];
+    const F3 pj = z[c] + b * pOld[c];
+    const float* bv = val + kk * 9u * kSlice;
+    a0 += bv[0 * kSlice] * pj.x + bv[1 * kSlice] * pj.y + bv[2 * kSlice] * pj.z;
+    a1 += bv[3 * kSlice] * pj.x + bv[4 * kSlice] * pj.y + bv[5 * kSlice] * pj.z;
+    a2 += bv[6 * kSlice] * pj.x + bv[7 * kSlice] * pj.y + bv[8 * kSlice] * pj.z;
+  }
+  const uint32_t flags = v.rowFlags[row];
+  F3 acc = (flags & kRowPinned) ? f3(0, 0, 0) : f3(a0, a1, a2);
+  const F3 pi = z[row] + b * pOld[row];
+  pNew[row] = pi;
+  const F3 api = filterContact(v.cflags[row], v.cnormal[row], acc);
+  v.ap[row] = api;
+  const double pap = double(pi.x) * api.x + double(pi.y) * api.y + double(pi.z) * api.z;
+  const double sp = blockSum256(pap, lds);
+  if (threadIdx.x == 0) v.partC[tile] = sp;
+}
+
+// Iteration k, second half: pAp, alpha, step += alpha p, r -= alpha ap, z = filter(M r); partials r.r and r.z.
+__global__ void __launch_bounds__(256) kPcgUpdate(View v, uint32_t k, const F3* __restrict__ p) {
+  __shared__ double lds[8];
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t s = v.tileSeg[tile];
+  SegState* S = v.state + s;
+  const SegParams& P = v.seg[s];
+  const double pAp = rangeSum256(v.partC, P.tileBegin, P.tileEnd, lds);
+  if (S->newtonDone || S->pcgConverged || k >= P.maxCg) return;
+  const bool first = tile == P.tileBegin && threadIdx.x == 0;
+  if (!(pAp > 0.0)) {
+    if (first) S->pcgConverged = 1u;
+    return;
+  }
+  const double alpha = S->rzPar[k & 1u] / pAp;
+  if (first) {
+    S->pAp = pAp;
+    S->alpha = alpha;
+    S->pcgIter += 1u;
+    S->cntPcg += 1u;
+  }
+  const float a = float(alpha);
+  const uint32_t row = tile * kTile + threadIdx.x;
+  const F3 pi = p[row];
+  v.step[row] = v.step[row] + a * pi;
+  const F3 ri = v.r[row] - a * v.ap[row];
+  v.r[row] = ri;
+  M3 M;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) M.m[c] = v.precond[9u * row + c];
+  const F3 zi = filterContact(v.cflags[row], v.cnormal[row], mul(M, ri));
+  v.z[row] = zi;
+  const double rr = double(ri.x) * ri.x + double(ri.y) * ri.y + double(ri.z) * ri.z;
+  const double rz = double(ri.x) * zi.x + double(ri.y) * zi.y + double(ri.z) * zi.z;
+  const double srr = blockSum256(rr, lds);
+  const double srz = blockSum256(rz, lds);
+  if (threadIdx.x == 0) {
+    v.partA[tile] = srz;
+    v.partB[tile] = srr;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// After the solve: each held particle's normal force from the unfiltered residual -grad - A step
+// (cpu_solver.cpp:210-212; kernels.h:391-396), and the Newton slope grad . step partials.
+
+__global__ void __launch_bounds__(256) kContactForceSlope(View v) {
+  __shared__ double lds[8];
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t s = v.tileSeg[tile];
+  if (v.state[s].newtonDone) return;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t slice = tile * kSlicesPerTile + wave;
+  const uint32_t row = slice * kSlice + lane;
+  const uint32_t cf = v.cflags[row];
+  const F3 gi = v.grad[row];
+  const F3 si = v.step[row];
+  float force = 0.0f;
+  if (cf & kContactActive) {
+    const uint32_t K = v.sellK[slice];
+    const uint32_t base = v.sellBase[slice];
+    const float* val = v.sellVal + base + lane;
+    const uint32_t* col = v.sellCol + base / 9u + lane;
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    for (uint32_t kk = 0; kk < K; ++kk) {
+      const uint32_t c = col[kk * kSlice];
+      const F3 pj = v.step[c];
+      const float* bv = val + kk * 9u * kSlice;
+      a0 += bv[0 * kSlice] * pj.x + bv[1 * kSlice] * pj.y + bv[2 * kSlice] * pj.z;
+      a1 += bv[3 * kSlice] * pj.x + bv[4 * kSlice] * pj.y + bv[5 * kSlice] * pj.z;
+      a2 += bv[6 * kSlice] * pj.x + bv[7 * kSlice] * pj.y + bv[8 * kSlice] * pj.z;
+    }
+    const F3 resid = -gi - f3(a0, a1, a2);
+    const float4 cn = v.cnormal[row];
+    force = fmaxf(0.0f, -dot(resid, f3(cn.x, cn.y, cn.z)));
+  }
+  v.cforce[row] = force;
+  const double sl = double(gi.x) * si.x + double(gi.y) * si.y + double(gi.z) * si.z;
+  const double ss = blockSum256(sl, lds);
+  if (threadIdx.x == 0) v.partA[tile] = ss;
+}
+
+// slope = grad . step; no line search when it is not negative (cpu_solver.cpp:267-268).
+__global__ void __launch_bounds__(256) kNewtonSlope(View v) {
+  __shared__ double lds[8];
+  const uint32_t s = blockIdx.x;
+  SegState* S = v.state + s;
+  if (S->newtonDone) return;
+  const SegParams& P = v.seg[s];
+  const double slope = rangeSum256(v.partA, P.tileBegin, P.tileEnd, lds);
+  if (threadIdx.x == 0) {
+    S->slope = slope;
+    if (!(slope < 0.0)) {
+      S->newtonDone = 1u;
+    } else {
+      S->lsAlpha = 1.0;
+      S->lsDone = 0u;
+      S->lsTrial = 0u;
+    }
+  }
+}
+
+// Line-search trial energy at u + alpha step (kernels.h:399-403 + energy-only term evaluation).
+__global__ void __launch_bounds__(256) kEnergyTrial(View v) {
+  __shared__ double lds[8];
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t s = v.tileSeg[tile];
+  const SegState* S = v.state + s;
+  if (S->newtonDone || S->lsDone) return;
+  const SegParams& P = v.seg[s];
+  const float alpha = float(S->lsAlpha);
+  const uint32_t row = tile * kTile + threadIdx.x;
+  const uint32_t flags = v.rowFlags[row];
+  double rowE = 0.0;
+  if (!(flags & kRowPad)) {
+    F3 g;
+    M3 H;
+    rowE += double(ownTerm<false>(v, P, row, flags, alpha, g, H));
+    const uint32_t i1 = v.incStart[row + 1];
+    for (uint32_t i = v.incStart[row]; i < i1; ++i) {
+      const uint32_t e = v.inc[i];
+      if (incLocal(e) != 0u) continue;
+      F3 gt;
+      M3 Hs[4];
+      uint32_t m;
+      const uint32_t* slots;
+      rowE += double(incidence<false>(v, P, e, alpha, gt, Hs, m, slots));
+    }
+  }
+  const double se = blockSum256(rowE, lds);
+  if (threadIdx.x == 0) v.partC[tile] = se;
+}
+
+// Armijo test E(trial) <= e0 + 1e-4 alpha slope; else halve, at most 20 trials (cpu_solver.cpp:269-274).
+__global__ void __launch_bounds__(256) kLineSearchDecide(View v) {
+  __shared__ double lds[8];
+  const uint32_t s = blockIdx.x;
+  SegState* S = v.state + s;
+  if (S->newtonDone || S->lsDone) return;
+  const SegParams& P = v.seg[s];
+  const double e = rangeSum256(v.partC, P.tileBegin, P.tileEnd, lds);
+  if (threadIdx.x == 0) {
+    const double alpha = S->lsAlpha;
+    S->eTrial = e;
+    S->cntLs += 1u;
+    if (e <= S->e0 + 1e-4 * alpha * S->slope) {
+      S->lsDone = 1u;
+    } else {
+      S->lsAlpha = alpha * 0.5;
+      S->lsTrial += 1u;
+      if (S->lsTrial >= 20u) S->lsDone = 1u;
+    }
+  }
+}
+
+// u += alpha step; max |move|; friction for the next Newton iteration uses this solve's normal forces
+// (kernels.h:407-414; cpu_solver.cpp:275-282).
+__global__ void __launch_bounds__(256) kApply(View v) {
+  __shared__ float lds[8];
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t s = v.tileSeg[tile];
+  const SegState* S = v.state + s;
+  if (S->newtonDone) return;
+  const float alpha = float(S->lsAlpha);
+  const uint32_t row = tile * kTile + threadIdx.x;
+  const F3 d = alpha * v.step[row];
+  v.u[row] = v.u[row] + d;
+  const float moved = fmaxf(fabsf(d.x), fmaxf(fabsf(d.y), fabsf(d.z)));
+  v.clambda[row] = (v.cflags[row] & kContactActive) ? v.cforce[row] : 0.0f;
+  const float m = blockMax256(moved, lds);
+  if (threadIdx.x == 0) v.partMax[tile] = m;
+}
+
+// Stops the Newton loop once the largest move is below 1e-9 m (cpu_solver.cpp:283).
+__global__ void __launch_bounds__(256) kNewtonEnd(View v) {
+  __shared__ float lds[8];
+  const uint32_t s = blockIdx.x;
+  SegState* S = v.state + s;
+  if (S->newtonDone) return;
+  const SegParams& P = v.seg[s];
+  const float m = rangeMax256(v.partMax, P.tileBegin, P.tileEnd, lds);
+  if (threadIdx.x == 0) {
+    S->maxMove = m;
+    if (double(m) < 1e-9) S->newtonDone = 1u;
+  }
+}
+
+// x = xStart + u with the post-solve safety push-out, pins, damped velocity (kernels.h:419-448;
+// cpu_solver.cpp:285-310).
+__global__ void __launch_bounds__(256) kFinish(View v) {
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t s = v.tileSeg[tile];
+  const SegParams& P = v.seg[s];
+  const uint32_t row = tile * kTile + threadIdx.x;
+  const uint32_t flags = v.rowFlags[row];
+  if (flags & kRowPad) return;
+  const Fabric32 fab = v.fabrics[v.rowFabric[row]];
+  const F3 xs = v.xStart[row];
+  F3 xn = xs + v.u[row];
+  if (flags & kRowPinned) {
+    xn = v.pinPos[row];
+  } else if (P.field.present) {
+    const float radius = contactRadius(P, fab);
+    if (fieldSample(v.fieldValues, P.field, xn) < radius - P.contactSnap) {
+      if (isZero(fieldGradient(v.fieldValues, P.field, xn))) {
+        xn = landOnBody(v.fieldValues, P.field, radius, xs, xn);
+      } else {
+        collideBody(v.fieldValues, P.field, radius, fab.friction, xs, xn);
+      }
+    }
+  }
+  float c = P.dampingScale * fab.damping;
+  if (v.garments[v.rowGarment[row]].phase == 0u) c += P.assemblyExtraDamping;
+  v.x[row] = xn;
+  v.v[row] = ((xn - xs) / P.h) * fmaxf(0.0f, 1.0f - c * P.h);
+}
+
+__global__ void kFill(uint32_t* p, uint32_t value, uint64_t count) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = value;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Debug / standalone kernels.
+
+// out = A in over free rows and columns, no filter (multiply, cpu_solver.cpp:147-159).
+__global__ void __launch_bounds__(256) kSpmvPlain(View v, const F3* __restrict__ in, F3* out) {
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t slice = tile * kSlicesPerTile + wave;
+  const uint32_t row = slice * kSlice + lane;
+  const uint32_t K = v.sellK[slice];
+  const uint32_t base = v.sellBase[slice];
+  const float* __restrict__ val = v.sellVal + base + lane;
+  const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  for (uint32_t kk = 0; kk < K; ++kk) {
+    const uint32_t c = col[kk * kSlice];
+    const F3 pj = in[c];
+    const float* bv = val + kk * 9u * kSlice;
+    a0 += bv[0 * kSlice] * pj.x + bv[1 * kSlice] * pj.y + bv[2 * kSlice] * pj.z;
+    a1 += bv[3 * kSlice] * pj.x + bv[4 * kSlice] * pj.y + bv[5 * kSlice] * pj.z;
+    a2 += bv[6 * kSlice] * pj.x + bv[7 * kSlice] * pj.y + bv[8 * kSlice] * pj.z;
+  }
+  out[row] = (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : f3(a0, a1, a2);
+}
+
+// SELL values -> canonical slot order (9 floats per slot), for parity checks of the assembled matrix.
+__global__ void __launch_bounds__(256) kSellToSlots(View v, float* out) {
+  const uint32_t row = blockIdx.x * 256u + threadIdx.x;
+  if (row >= v.nRows) return;
+  const uint32_t slice = row / kSlice, lane = row % kSlice;
+  const uint32_t rs = v.rowStart[row], re = v.rowStart[row + 1];
+  const float* base = v.sellVal + v.sellBase[slice] + lane;
+  for (uint32_t sl = rs; sl < re; ++sl)
+    for (int c = 0; c < 9; ++c) out[9u * sl + c] = base[((sl - rs) * 9u + c) * kSlice];
+}
+
+// Batch term evaluation (full gradient and Hessian from the row kernels, a = 0..N-1).
+__global__ void kEvalTerms(int kind, uint32_t n, const float* x, const float* params, const Fabric32* fabrics,
+                           float limitScale, float* energy, float* grad, float* hess) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int N = kind == 0 ? 3 : kind == 1 ? 4 : kind == 2 ? 2 : 1;
+  const int np = kind == 0 ? 5 : kind == 1 ? 2 : kind == 2 ? 2 : 5;
+  F3 xs[4];
+  for (int a = 0; a < N; ++a) xs[a] = f3(x[(i * N + a) * 3], x[(i * N + a) * 3 + 1], x[(i * N + a) * 3 + 2]);
+  const float* pr = params + size_t(i) * np;
+  float E = 0.0f;
+  for (int a = 0; a < N; ++a) {
+    F3 ga = f3(0, 0, 0);
+    M3 H[4] = {zeroM(), zeroM(), zeroM(), zeroM()};
+    float e = 0.0f;
+    if (kind == 0) {
+      Stretch32 t{};
+      for (int q = 0; q < 4; ++q) t.dmInv[q] = pr[q];
+      t.area = pr[4];
+      e = stretchRow<true>(t, xs, fabrics[i], limitScale, a, ga, H);
+    } else if (kind == 1) {
+      Hinge32 t{};
+      t.K = pr[0];
+      t.restAngle = pr[1];
+      e = hingeRow<true>(t, xs, a, ga, H);
+    } else if (kind == 2) {
+      e = seamRow<true>(xs, pr[0], pr[1], a, ga, H);
+    } else {
+      e = frictionTerm<true>(xs[0], f3(pr[0], pr[1], pr[2]), pr[3], pr[4], ga, H[0]);
+    }
+    if (a == 0) E = e;
+    grad[(size_t(i) * N + a) * 3 + 0] = ga.x;
+    grad[(size_t(i) * N + a) * 3 + 1] = ga.y;
+    grad[(size_t(i) * N + a) * 3 + 2] = ga.z;
+    for (int b = 0; b < N; ++b)
+      for (int c = 0; c < 9; ++c) hess[(size_t(i) * N * N + a * N + b) * 9 + c] = H[b].m[c];
+  }
+  energy[i] = E;
+}
+
+__global__ void kFieldQuery(const float* values, Field32 f, uint32_t n, const float* p, float* phi, float* normal) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const F3 q = f3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+  phi[i] = fieldSample(values, f, q);
+  const F3 g = fieldGradient(values, f, q);
+  normal[3 * i] = g.x;
+  normal[3 * i + 1] = g.y;
+  normal[3 * i + 2] = g.z;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Structure builder: incidence lists, block-CSR pattern, diagonal slots, term slot tables and the SELL layout, all
+// on the device, reproducing sim_model.cpp:41-74 exactly.
+
+__device__ inline bool termItem(const BuildView& b, uint32_t i, uint32_t& kind, uint32_t& t, uint32_t& a,
+                                uint32_t& vtx) {
+  const uint32_t nS = 3u * b.nStretch, nH = 4u * b.nHinge, nK = 2u * b.nSeam;
+  if (i < nS) {
+    kind = kKindStretch;
+    t = i / 3u;
+    a = i % 3u;
+    vtx = b.stretch[t].v[a];
+    return true;
+  }
+  i -= nS;
+  if (i < nH) {
+    kind = kKindHinge;
+    t = i / 4u;
+    a = i % 4u;
+    vtx = b.hinges[t].v[a];
+    return true;
+  }
+  i -= nH;
+  if (i < nK) {
+    kind = kKindSeam;
+    t = i / 2u;
+    a = i % 2u;
+    vtx = a == 0 ? b.seams[t].a : b.seams[t].b;
+    return true;
+  }
+  return false;
+}
+
+__global__ void kIncCount(BuildView b, uint32_t items) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= items) return;
+  uint32_t kind, t, a, vtx;
+  if (termItem(b, i, kind, t, a, vtx)) atomicAdd(b.incCount + vtx, 1u);
+}
+
+__global__ void kIncFill(BuildView b, uint32_t items, uint32_t* cursor) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= items) return;
+  uint32_t kind, t, a, vtx;
+  if (!termItem(b, i, kind, t, a, vtx)) return;
+  const uint32_t pos = atomicAdd(cursor + vtx, 1u);
+  b.inc[b.incStart[vtx] + pos] = incEncode(kind, t, a);
+}
+
+// Exclusive scan of in[0, n) into out[0, n] (one 1024-thread workgroup; the builder runs once per build).
+__global__ void __launch_bounds__(1024) kScan(const uint32_t* in, uint32_t* out, uint32_t n) {
+  __shared__ uint32_t buf[1024];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < n; base += 1024u) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t val = i < n ? in[i] : 0u;
+    buf[threadIdx.x] = val;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+      const uint32_t add = threadIdx.x >= off ? buf[threadIdx.x - off] : 0u;
+      __syncthreads();
+      buf[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (i < n) out[i] = carry + buf[threadIdx.x] - val;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += buf[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
+// Sorted, duplicate-free column list of a row: {row} and every vertex of every incident term.
+__device__ inline uint32_t rowColumns(const BuildView& b, uint32_t row, uint32_t* cols, uint32_t* err) {
+  uint32_t n = 0;
+  auto insert = [&](uint32_t c) {
+    uint32_t pos = 0;
+    while (pos < n && cols[pos] < c) ++pos;
+    if (pos < n && cols[pos] == c) return;
+    if (n >= kMaxRow) {
+      *err = 2u;
+      return;
+    }
+    for (uint32_t q = n; q > pos; --q) cols[q] = cols[q - 1];
+    cols[pos] = c;
+    ++n;
+  };
+  insert(row);
+  for (uint32_t i = b.incStart[row]; i < b.incStart[row + 1]; ++i) {
+    const uint32_t e = b.inc[i];
+    const uint32_t kind = incKind(e), t = incTerm(e);
+    if (kind == kKindStretch) {
+      for (int q = 0; q < 3; ++q) insert(b.stretch[t].v[q]);
+    } else if (kind == kKindHinge) {
+      for (int q = 0; q < 4; ++q) insert(b.hinges[t].v[q]);
+    } else {
+      insert(b.seams[t].a);
+      insert(b.seams[t].b);
+    }
+  }
+  return n;
+}
+
+// Sorts each row's incidences by (kind, term, local) and counts its columns.
+__global__ void kRowSortCount(BuildView b) {
+  const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= b.nRows) return;
+  const uint32_t i0 = b.incStart[row], i1 = b.incStart[row + 1];
+  if (i1 - i0 > kMaxIncidence) {
+    *b.error = 1u;
+    b.rowLen[row] = 1;
+    return;
+  }
+  for (uint32_t i = i0 + 1; i < i1; ++i) {
+    const uint32_t key = b.inc[i];
+    uint32_t j = i;
+    while (j > i0 && b.inc[j - 1] > key) {
+      b.inc[j] = b.inc[j - 1];
+      --j;
+    }
+    b.inc[j] = key;
+  }
+  uint32_t cols[kMaxRow];
+  b.rowLen[row] = rowColumns(b, row, cols, b.error);
+}
+
+__global__ void kPatternFill(BuildView b) {
+  const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= b.nRows) return;
+  uint32_t cols[kMaxRow];
+  const uint32_t n = rowColumns(b, row, cols, b.error);
+  const uint32_t rs = b.rowStart[row];
+  for (uint32_t k = 0; k < n; ++k) {
+    b.colIndex[rs + k] = cols[k];
+    if (cols[k] == row) b.diagSlot[row] = rs + k;
+  }
+}
+
+__device__ inline uint32_t lowerBound(const uint32_t* colIndex, uint32_t lo, uint32_t hi, uint32_t c) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (colIndex[mid] < c)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// slot(v_a, v_b) for every term and (a, b) (sim_model.cpp:35-39, :62-72).
+__global__ void kSlots(BuildView b) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t total = b.nStretch + b.nHinge + b.nSeam;
+  if (i >= total) return;
+  uint32_t vs[4];
+  uint32_t m;
+  uint32_t* out;
+  if (i < b.nStretch) {
+    m = 3;
+    for (int q = 0; q < 3; ++q) vs[q] = b.stretch[i].v[q];
+    out = b.stretchSlots + 9u * i;
+  } else if (i < b.nStretch + b.nHinge) {
+    const uint32_t t = i - b.nStretch;
+    m = 4;
+    for (int q = 0; q < 4; ++q) vs[q] = b.hinges[t].v[q];
+    out = b.hingeSlots + 16u * t;
+  } else {
+    const uint32_t t = i - b.nStretch - b.nHinge;
+    m = 2;
+    vs[0] = b.seams[t].a;
+    vs[1] = b.seams[t].b;
+    out = b.seamSlots + 4u * t;
+  }
+  for (uint32_t a = 0; a < m; ++a)
+    for (uint32_t c = 0; c < m; ++c)
+      out[a * m + c] = lowerBound(b.colIndex, b.rowStart[vs[a]], b.rowStart[vs[a] + 1], vs[c]);
+}
+
+__global__ void kSellK(BuildView b) {
+  const uint32_t slice = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slice >= b.nSlices) return;
+  uint32_t K = 0;
+  for (uint32_t l = 0; l < kSlice; ++l) K = max(K, b.rowLen[slice * kSlice + l]);
+  b.sellK[slice] = K;
+  b.sellSize[slice] = K * 9u * kSlice;
+}
+
+__global__ void kSellCol(BuildView b, const uint32_t* sellBase) {
+  const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= b.nRows) return;
+  const uint32_t slice = row / kSlice, lane = row % kSlice;
+  const uint32_t K = b.sellK[slice];
+  const uint32_t base = sellBase[slice] / 9u;
+  const uint32_t rs = b.rowStart[row], len = b.rowStart[row + 1] - rs;
+  for (uint32_t k = 0; k < K; ++k) {
+    uint32_t c = b.nRows;
+    if (k < len) {
+      c = b.colIndex[rs + k];
+      if (b.rowFlags[c] & kRowPinned) c = b.nRows;
+    }
+    b.sellCol[base + k * kSlice + lane] = c;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Launchers.
+
+void launchPredictContact(const View& v, hipStream_t st) {
+  kPredictContact<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);
+}
+void launchActivate(const View& v, hipStream_t st) { kActivate<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
+void launchAssemble(const View& v, uint32_t maxK, bool, hipStream_t st) {
+  kAssemble<<<xcdGrid(v.nSlices), 64, maxK * 9u * kSlice * sizeof(float), st>>>(v);
+}
+void launchNewtonBegin(const View& v, hipStream_t st) { kNewtonBegin<<<v.nSegments, 256, 0, st>>>(v); }
+void launchPcgInit(const View& v, hipStream_t st) { kPcgInit<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
+void launchPcgIteration(const View& v, uint32_t k, hipStream_t st) {
+  F3* pOld = (k & 1u) ? v.pB : v.pA;
+  F3* pNew = (k & 1u) ? v.pA : v.pB;
+  kPcgSpmv<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, k, pOld, pNew);
+  kPcgUpdate<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, k, pNew);
+}
+void launchContactForceSlope(const View& v, hipStream_t st) {
+  kContactForceSlope<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);
+}
+void launchNewtonSlope(const View& v, hipStream_t st) { kNewtonSlope<<<v.nSegments, 256, 0, st>>>(v); }
+void launchEnergyTrial(const View& v, hipStream_t st) { kEnergyTrial<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
+void launchLineSearchDecide(const View& v, hipStream_t st) { kLineSearchDecide<<<v.nSegments, 256, 0, st>>>(v); }
+void launchApply(const View& v, hipStream_t st) { kApply<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
+void launchNewtonEnd(const View& v, hipStream_t st) { kNewtonEnd<<<v.nSegments, 256, 0, st>>>(v); }
+void launchFinish(const View& v, hipStream_t st) { kFinish<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
+void launchFill(uint32_t* p, uint32_t value, uint64_t count, hipStream_t st) {
+  if (count == 0) return;
+  kFill<<<uint32_t((count + 255) / 256), 256, 0, st>>>(p, value, count);
+}
+void launchSpmvPlain(const View& v, const F3* in, F3* out, hipStream_t st) {
+  kSpmvPlain<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, in, out);
+}
+void launchSellToSlots(const View& v, float* out, hipStream_t st) {
+  kSellToSlots<<<(v.nRows + 255) / 256, 256, 0, st>>>(v, out);
+}
+void launchEvalTerms(int kind, uint32_t n, const float* x, const float* params, const Fabric32* fabrics,
+                     float limitScale, float* energy, float* grad, float* hess, hipStream_t st) {
+  if (n == 0) return;
+  kEvalTerms<<<(n + 63) / 64, 64, 0, st>>>(kind, n, x, params, fabrics, limitScale, energy, grad, hess);
+}
+void launchFieldQuery(const float* values, Field32 field, uint32_t n, const float* p, float* phi, float* normal,
+                      hipStream_t st) {
+  if (n == 0) return;
+  kFieldQuery<<<(n + 255) / 256, 256, 0, st>>>(values, field, n, p, phi, normal);
+}
+
+void launchBuildStructure(const BuildView& bIn, uint32_t& nBlocks, uint64_t& sellFloats, uint32_t& maxK,
+                          uint32_t*& colIndexOut, uint32_t*& sellColOut, hipStream_t st) {
+  BuildView b = bIn;
+  const uint32_t items = 3u * b.nStretch + 4u * b.nHinge + 2u * b.nSeam;
+  const uint32_t nRows = b.nRows;
+  hipCheck(hipMemsetAsync(b.incCount, 0, sizeof(uint32_t) * nRows, st), "memset");
+  hipCheck(hipMemsetAsync(b.error, 0, sizeof(uint32_t), st), "memset");
+  if (items) kIncCount<<<(items + 255) / 256, 256, 0, st>>>(b, items);
+  kScan<<<1, 1024, 0, st>>>(b.incCount, b.incStart, nRows);
+  hipCheck(hipMemsetAsync(b.incCount, 0, sizeof(uint32_t) * nRows, st), "memset");
+  if (items) kIncFill<<<(items + 255) / 256, 256, 0, st>>>(b, items, b.incCount);
+  kRowSortCount<<<(nRows + 127) / 128, 128, 0, st>>>(b);
+  kScan<<<1, 1024, 0, st>>>(b.rowLen, b.rowStart, nRows);
+  uint32_t hostTotal = 0, hostErr = 0;
+  hipCheck(hipMemcpyAsync(&hostTotal, b.rowStart + nRows, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "copy");
+  hipCheck(hipMemcpyAsync(&hostErr, b.error, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "copy");
+  hipCheck(hipStreamSynchronize(st), "sync");
+  if (hostErr) {
+    throw std::invalid_argument(hostErr == 1u ? "hip: a particle has more than 96 incident terms"
+                                              : "hip: a particle couples to more than 64 particles");
+  }
+  nBlocks = hostTotal;
+  hipCheck(hipMalloc(&b.colIndex, sizeof(uint32_t) * std::max<uint32_t>(nBlocks, 1)), "malloc colIndex");
+  kPatternFill<<<(nRows + 127) / 128, 128, 0, st>>>(b);
+  const uint32_t terms = b.nStretch + b.nHinge + b.nSeam;
+  if (terms) kSlots<<<(terms + 255) / 256, 256, 0, st>>>(b);
+  kSellK<<<(b.nSlices + 255) / 256, 256, 0, st>>>(b);
+  kScan<<<1, 1024, 0, st>>>(b.sellSize, b.sellSize, b.nSlices);
+  uint32_t hostSell = 0;
+  hipCheck(hipMemcpyAsync(&hostSell, b.sellSize + b.nSlices, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "copy");
+  hipCheck(hipStreamSynchronize(st), "sync");
+  sellFloats = hostSell;
+  hipCheck(hipMalloc(&b.sellCol, sizeof(uint32_t) * std::max<uint64_t>(hostSell / 9u, 1)), "malloc sellCol");
+  kSellCol<<<(nRows + 255) / 256, 256, 0, st>>>(b, b.sellSize);
+  // maxK: copy sellK back (small) to size the assembly kernel's LDS.
+  std::string err;
+  {
+    uint32_t* hk = static_cast<uint32_t*>(malloc(sizeof(uint32_t) * b.nSlices));
+    hipCheck(hipMemcpyAsync(hk, b.sellK, sizeof(uint32_t) * b.nSlices, hipMemcpyDeviceToHost, st), "copy");
+    hipCheck(hipStreamSynchronize(st), "sync");
+    maxK = 0;
+    for (uint32_t i = 0; i < b.nSlices; ++i) maxK = std::max(maxK, hk[i]);
+    free(hk);
+  }
+  colIndexOut = b.colIndex;
+  sellColOut = b.sellCol;
+}
+
+}  // namespace dk

@@ -1,0 +1,124 @@
+// kernels.hpp — device view of the solver state and the host-side launchers of the HIP kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "layout.hpp"
+
+namespace dk {
+
+// Pointers and sizes every kernel reads. Vectors over rows hold nRows + 1 entries: entry nRows is the zero
+// "dummy column" that padding and pinned-column SELL entries point at.
+struct View {
+  uint32_t nRows, nTiles, nSlices, nSegments;
+  uint32_t nStretch, nHinge, nSeam, pad0;
+  // per row
+  const uint32_t* rowFlags;
+  const float* mass;
+  const uint32_t* rowFabric;
+  const uint32_t* rowGarment;
+  const F3* fExt;
+  const F3* pinPos;
+  F3* x;
+  F3* v;
+  F3* xStart;
+  F3* yRel;
+  F3* u;
+  F3* grad;
+  F3* step;
+  F3* r;
+  F3* z;
+  F3* pA;        // PCG search direction, double-buffered by iteration parity
+  F3* pB;
+  F3* ap;
+  F3* warm;      // learned warm-start velocity change per row (zero when unused)
+  float* precond;  // 9 per row, row-major
+  float4* cnormal; // xyz normal, w = offset r - phi(xStart)
+  uint32_t* cflags;
+  float* cforce;
+  float* clambda;
+  // per tile / slice / segment
+  const uint32_t* tileSeg;
+  const SegParams* seg;
+  SegState* state;
+  double* partA;  // per tile
+  double* partB;  // per tile
+  double* partC;  // per tile
+  double* partE;  // per slice
+  float* partMax; // per tile
+  // terms
+  const Stretch32* stretch;
+  const Hinge32* hinges;
+  const Seam32* seams;
+  const Fabric32* fabrics;
+  const Garment32* garments;
+  // structure
+  const uint32_t* incStart;
+  const uint32_t* inc;
+  const uint32_t* rowStart;
+  const uint32_t* colIndex;
+  const uint32_t* diagSlot;
+  const uint32_t* stretchSlots;
+  const uint32_t* hingeSlots;
+  const uint32_t* seamSlots;
+  const uint32_t* sellK;     // per slice
+  const uint32_t* sellBase;  // per slice, in floats (column base = sellBase / 9)
+  const uint32_t* sellCol;
+  float* sellVal;
+  const float* fieldValues;
+};
+
+// Structure builder inputs/outputs (device pointers).
+struct BuildView {
+  uint32_t nRows, nStretch, nHinge, nSeam, nSlices;
+  const Stretch32* stretch;
+  const Hinge32* hinges;
+  const Seam32* seams;
+  const uint32_t* rowFlags;
+  uint32_t* incCount;   // nRows
+  uint32_t* incStart;   // nRows + 1
+  uint32_t* inc;
+  uint32_t* rowLen;     // nRows
+  uint32_t* rowStart;   // nRows + 1
+  uint32_t* colIndex;
+  uint32_t* diagSlot;
+  uint32_t* stretchSlots;
+  uint32_t* hingeSlots;
+  uint32_t* seamSlots;
+  uint32_t* sellK;      // nSlices
+  uint32_t* sellSize;   // nSlices + 1 (scan of K * 576)
+  uint32_t* sellCol;
+  uint32_t* error;      // nonzero: a row exceeded kMaxIncidence or kMaxRow
+};
+
+// ---- launchers (kernels.hip); all asynchronous on `st` ----
+void launchPredictContact(const View& v, hipStream_t st);
+void launchActivate(const View& v, hipStream_t st);
+void launchAssemble(const View& v, uint32_t maxK, bool contact, hipStream_t st);
+void launchNewtonBegin(const View& v, hipStream_t st);
+void launchPcgInit(const View& v, hipStream_t st);
+void launchPcgIteration(const View& v, uint32_t k, hipStream_t st);
+void launchContactForceSlope(const View& v, hipStream_t st);
+void launchNewtonSlope(const View& v, hipStream_t st);
+void launchEnergyTrial(const View& v, hipStream_t st);
+void launchLineSearchDecide(const View& v, hipStream_t st);
+void launchApply(const View& v, hipStream_t st);
+void launchNewtonEnd(const View& v, hipStream_t st);
+void launchFinish(const View& v, hipStream_t st);
+void launchFill(uint32_t* p, uint32_t value, uint64_t count, hipStream_t st);
+
+// Debug / standalone paths.
+void launchSpmvPlain(const View& v, const F3* in, F3* out, hipStream_t st);
+void launchSellToSlots(const View& v, float* out /* 9 per slot */, hipStream_t st);
+void launchEvalTerms(int kind, uint32_t n, const float* x, const float* params, const Fabric32* fabrics,
+                     float limitScale, float* energy, float* grad, float* hess, hipStream_t st);
+void launchFieldQuery(const float* values, Field32 field, uint32_t n, const float* p, float* phi, float* normal,
+                      hipStream_t st);
+
+// Structure builder (all on the device).
+void launchBuildStructure(const BuildView& b, uint32_t& nBlocks, uint64_t& sellFloats, uint32_t& maxK,
+                          uint32_t*& colIndexOut, uint32_t*& sellColOut, hipStream_t st);
+
+}  // namespace dk

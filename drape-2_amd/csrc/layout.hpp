@@ -1,0 +1,129 @@
+// layout.hpp — device data layout of the MI355X cloth step, shared by the host orchestration and the kernels.
+//
+// Rows (particles) are grouped in slices of 64 (one wavefront, one row per lane) and tiles of 256 (four slices,
+// one 256-thread workgroup). Every segment (an independent scene of a batched build) starts on a tile boundary, so
+// a tile never mixes segments; the gap rows are inert (flag kRowPad, treated as pinned). For a single-segment
+// build the internal row of a particle is its global index in the reference's particle order
+// (libs/sim/include/drape/sim/solver.hpp:84); only padding rows follow the last particle.
+//
+// The Hessian is block-CSR over 3x3 blocks with the reference's pattern and slot numbering
+// (libs/sim/src/sim_model.cpp:41-74): rowStart/colIndex/diagSlot. The block values live in a sliced-ELL layout
+// (SELL-64) so that one lane per row streams them coalesced: slice t holds K_t = max row length of its 64 rows;
+// value (slot k of lane l, component c) is at sellBase[t] + (k * 9 + c) * 64 + l, column index at
+// sellBase[t] / 9 + k * 64 + l. Components are row-major (c = 3 * row + col). Padding entries carry the
+// dummy column `nRows` (a zero entry of every gathered vector) and zero values; so do pinned columns, which the
+// product excludes (cpu_solver.cpp:147-159).
+#pragma once
+
+#include <cstdint>
+
+#ifndef __HIPCC__
+#define __host__
+#define __device__
+#endif
+
+namespace dk {
+
+constexpr uint32_t kSlice = 64;   // rows per slice = lanes per wavefront
+constexpr uint32_t kTile = 256;   // rows per tile = threads per workgroup
+constexpr uint32_t kSlicesPerTile = kTile / kSlice;
+constexpr uint32_t kMaxIncidence = 96;  // term incidences per particle the structure builder accepts
+constexpr uint32_t kMaxRow = 64;        // blocks per row the structure builder accepts
+
+// Row flags.
+constexpr uint32_t kRowPinned = 1u;
+constexpr uint32_t kRowPad = 2u;
+
+// Contact flags (GpuContact nearBody / active, gpu_layout.h:74-78).
+constexpr uint32_t kContactNear = 1u;
+constexpr uint32_t kContactActive = 2u;
+
+// Incidence entry: kind (2 bits) | local vertex a (2 bits) | term index (28 bits); ascending order = (kind, term, a),
+// the reference's gather order (gpu_solver_core.cpp:184-224).
+constexpr uint32_t kKindStretch = 0, kKindHinge = 1, kKindSeam = 2;
+__host__ __device__ inline uint32_t incEncode(uint32_t kind, uint32_t term, uint32_t a) {
+  return (kind << 30) | (term << 2) | a;
+}
+__host__ __device__ inline uint32_t incKind(uint32_t e) { return e >> 30; }
+__host__ __device__ inline uint32_t incTerm(uint32_t e) { return (e >> 2) & 0x0FFFFFFFu; }
+__host__ __device__ inline uint32_t incLocal(uint32_t e) { return e & 3u; }
+
+struct F3 {
+  float x, y, z;
+};
+
+// FabricPhysical fields the kernels read, as float (GpuFabric, gpu_layout.h:26-31).
+struct Fabric32 {
+  float stretchWarp, stretchWeft, stretchBias;
+  float strainLimitWarp, strainLimitWeft;
+  float weight, thickness, friction, damping;
+  float pad[3];
+};
+
+struct Stretch32 {
+  uint32_t v[3];
+  uint32_t fabric;
+  float dmInv[4];  // (0,0), (1,0), (0,1), (1,1)
+  float area;
+  float pad[3];
+};
+
+struct Hinge32 {
+  uint32_t v[4];  // opposite A, opposite B, edge start, edge end
+  float K, restAngle;
+  float pad[2];
+};
+
+struct Seam32 {
+  uint32_t a, b, garment;
+  float startGap;
+};
+
+struct Garment32 {
+  uint32_t phase;  // 0 Assembling, 1 Settling
+  float assemblyTime;
+  uint32_t pad[2];
+};
+
+// Body field of one segment: samples at origin + (i,j,k) cell, x fastest (CollisionField, collision_field.hpp:11-20).
+struct Field32 {
+  float ox, oy, oz, cell;
+  uint32_t dx, dy, dz;
+  float band;
+  uint64_t offset;  // first value in the concatenated field buffer
+  uint32_t present;
+  uint32_t pad;
+};
+
+// Per-segment constants (StepParams, gpu_layout.h:57-62) and layout.
+struct SegParams {
+  float gx, gy, gz, h;
+  float invH2, limitScale, seamStiffness, closingSpeed;
+  float frictionSlip, collisionOffset, dampingScale, assemblyExtraDamping;
+  float contactReach, contactSnap, pad0, pad1;
+  double cgTolerance;
+  uint32_t maxCg;
+  uint32_t tileBegin, tileEnd;  // tiles [tileBegin, tileEnd)
+  uint32_t rowBegin, rowEnd;    // internal rows [rowBegin, rowEnd) (rowEnd padded to a tile)
+  uint32_t pad2;
+  Field32 field;
+};
+
+// Per-segment solver state on the device. Written by the first workgroup of the segment (or by the per-segment
+// control kernels); every other workgroup recomputes the same values from the same partial sums in the same order.
+struct SegState {
+  double rzPar[2];                      // PCG r.z, by iteration parity (iteration k reads [(k+1)&1], writes [k&1])
+  double pAp, alpha, beta, r0, rr;      // PCG (CgScalars, gpu_layout.h:54-57, in FP64)
+  double e0, eTrial, slope, lsAlpha;    // Newton energy, trial energy, grad . step, line-search step
+  float maxMove;
+  uint32_t pcgConverged;  // 1 once |r| <= tol |r0| or pAp <= 0
+  uint32_t pcgIter;
+  uint32_t newtonDone;    // 1 once slope >= 0 or max move < 1e-9 (no more Newton iterations this substep)
+  uint32_t lsDone;        // 1 once the line search accepted a step this Newton iteration
+  uint32_t lsTrial;
+  uint32_t active;        // 1 while this segment runs Newton iterations this substep
+  uint32_t warmRejected;  // 1 when the learned warm start was rejected this substep
+  uint64_t cntNewton, cntPcg, cntLs, cntSubsteps, cntWarmRejected, cntWarmAccepted;
+};
+
+}  // namespace dk

@@ -1,0 +1,844 @@
+// solver.cpp — HipSolver: build, substep orchestration and state access on one MI355X.
+//
+// Substep (cpu_solver.cpp:230-314 semantics, every segment independent):
+//   predict + contact setup
+//   Newton iteration (up to `iterations`, stopping a segment once its slope is not negative or its largest move
+//   is below 1e-9 m):
+//     activate contacts; assemble energy / gradient / block-CSR Hessian; e0
+//     block-Jacobi PCG (two kernels per iteration; the host polls the per-segment converged flags in chunks)
+//     contact forces from the unfiltered residual; slope
+//     Armijo line search (one trial kernel + one decision kernel per trial)
+//     apply; largest move
+//   finish (push-out, pins, damped velocity)
+// All decisions are taken on the device from FP64 sums; the host only reads flags to know when to stop
+// enqueueing work.
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+namespace dh {
+
+namespace {
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string("hip: ") + what + ": " + hipGetErrorString(e));
+}
+
+dk::F3 f3(const Vec3f& v) { return {v.x, v.y, v.z}; }
+dk::F3 f3(const Vec3& v) { return {static_cast<float>(v.x), static_cast<float>(v.y), static_cast<float>(v.z)}; }
+
+dk::Fabric32 fabric32(const Fabric& f) {
+  dk::Fabric32 g{};
+  g.stretchWarp = static_cast<float>(f.stretchWarp);
+  g.stretchWeft = static_cast<float>(f.stretchWeft);
+  g.stretchBias = static_cast<float>(f.stretchBias);
+  g.strainLimitWarp = static_cast<float>(f.strainLimitWarp);
+  g.strainLimitWeft = static_cast<float>(f.strainLimitWeft);
+  g.weight = static_cast<float>(f.weight);
+  g.thickness = static_cast<float>(f.thickness);
+  g.friction = static_cast<float>(f.friction);
+  g.damping = static_cast<float>(f.damping);
+  return g;
+}
+
+constexpr double kContactReach = 0.01;   // sim_model.hpp:14
+constexpr double kContactSnap = 0.0005;  // sim_model.hpp:15
+
+}  // namespace
+
+void DeviceBuffer::allocate(size_t bytes) {
+  release();
+  const size_t b = std::max<size_t>(bytes, 16);
+  check(hipMalloc(&ptr_, b), "hipMalloc");
+  check(hipMemset(ptr_, 0, b), "hipMemset");
+  bytes_ = b;
+}
+
+void DeviceBuffer::release() {
+  if (ptr_) (void)hipFree(ptr_);
+  ptr_ = nullptr;
+  bytes_ = 0;
+}
+
+HipSolver::HipSolver() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) throw std::runtime_error("hip: no GPU device available");
+  hipDeviceProp_t prop;
+  check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    throw std::runtime_error(std::string("hip: device is ") + prop.gcnArchName + ", this build targets gfx950");
+  check(hipSetDevice(0), "hipSetDevice");
+  check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+}
+
+HipSolver::~HipSolver() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (hostStates_) (void)hipHostFree(hostStates_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void HipSolver::requireBuilt() const {
+  if (!built_) throw std::logic_error("hip: solver not built");
+}
+
+template <class T>
+static void upload(DeviceBuffer& b, const std::vector<T>& data, size_t count = 0) {
+  const size_t n = std::max(count, data.size());
+  b.allocate(sizeof(T) * std::max<size_t>(n, 1));
+  if (!data.empty()) check(hipMemcpy(b.as<void>(), data.data(), sizeof(T) * data.size(), hipMemcpyHostToDevice), "upload");
+}
+
+void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
+  Model m = Model::build(scenes, nScenes);
+  check(hipStreamSynchronize(stream_), "sync");
+  model_ = std::move(m);
+  built_ = false;
+  lastH_ = -1.0;
+  const Model& M = model_;
+  nRows_ = M.rowCount();
+  nTiles_ = nRows_ / dk::kTile;
+  nSlices_ = nRows_ / dk::kSlice;
+  const size_t n = M.x.size();
+  rowOf_.resize(n);
+  for (uint32_t i = 0; i < n; ++i) rowOf_[i] = M.internalRow(i);
+
+  // Per-row constants (padding rows are pinned and inert).
+  std::vector<uint32_t> flags(nRows_, dk::kRowPinned | dk::kRowPad), fab(nRows_, 0), gar(nRows_, 0);
+  std::vector<float> mass(nRows_, 1.0f);
+  std::vector<dk::F3> fext(nRows_ + 1, dk::F3{0, 0, 0}), pins(nRows_ + 1, dk::F3{0, 0, 0});
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t r = rowOf_[i];
+    flags[r] = M.pinned[i] ? dk::kRowPinned : 0u;
+    fab[r] = M.pFabric[i];
+    gar[r] = M.pGarment[i];
+    mass[r] = static_cast<float>(M.mass[i]);
+    fext[r] = f3(M.fExt[i]);
+  }
+  for (const auto& [i, pos] : M.pins) pins[rowOf_[i]] = f3(pos);
+  upload(rowFlags_, flags);
+  upload(mass_, mass);
+  upload(rowFabric_, fab);
+  upload(rowGarment_, gar);
+  upload(fExt_, fext);
+  upload(pinPos_, pins);
+  const size_t vecBytes = sizeof(dk::F3) * (nRows_ + 1);
+  for (DeviceBuffer* b : {&x_, &v_, &xStart_, &yRel_, &u_, &grad_, &step_, &r_, &z_, &pA_, &pB_, &ap_, &warm_,
+                          &scratchF3_, &scratchF3b_})
+    b->allocate(vecBytes);
+  precond_.allocate(sizeof(float) * 9 * nRows_);
+  cnormal_.allocate(sizeof(float4) * nRows_);
+  cflags_.allocate(sizeof(uint32_t) * nRows_);
+  cforce_.allocate(sizeof(float) * nRows_);
+  clambda_.allocate(sizeof(float) * nRows_);
+
+  // Segments.
+  std::vector<uint32_t> tileSeg(nTiles_);
+  std::vector<float> fieldValues;
+  std::vector<dk::SegParams> params(M.segments.size());
+  for (uint32_t s = 0; s < M.segments.size(); ++s) {
+    const Segment& sg = M.segments[s];
+    for (uint32_t t = sg.rowBegin / dk::kTile; t < (sg.rowBegin + sg.rowCount) / dk::kTile; ++t) tileSeg[t] = s;
+    dk::SegParams& P = params[s];
+    P = dk::SegParams{};
+    P.gx = static_cast<float>(sg.settings.gravity[0]);
+    P.gy = static_cast<float>(sg.settings.gravity[1]);
+    P.gz = static_cast<float>(sg.settings.gravity[2]);
+    P.limitScale = static_cast<float>(sg.settings.limitScale);
+    P.seamStiffness = static_cast<float>(sg.settings.seamStiffness);
+    P.closingSpeed = static_cast<float>(sg.settings.closingSpeed);
+    P.frictionSlip = static_cast<float>(sg.settings.frictionSlip);
+    P.collisionOffset = static_cast<float>(sg.settings.collisionOffset);
+    P.dampingScale = static_cast<float>(sg.settings.dampingScale);
+    P.assemblyExtraDamping = static_cast<float>(sg.settings.extraDamping);
+    P.contactReach = static_cast<float>(kContactReach);
+    P.contactSnap = static_cast<float>(kContactSnap);
+    P.cgTolerance = sg.settings.cgTolerance;
+    P.maxCg = static_cast<uint32_t>(std::max(0, sg.settings.maxCg));
+    P.tileBegin = sg.rowBegin / dk::kTile;
+    P.tileEnd = (sg.rowBegin + sg.rowCount) / dk::kTile;
+    P.rowBegin = sg.rowBegin;
+    P.rowEnd = sg.rowBegin + sg.rowCount;
+    if (sg.body.present()) {
+      P.field.ox = static_cast<float>(sg.body.origin[0]);
+      P.field.oy = static_cast<float>(sg.body.origin[1]);
+      P.field.oz = static_cast<float>(sg.body.origin[2]);
+      P.field.cell = static_cast<float>(sg.body.cell);
+      P.field.dx = static_cast<uint32_t>(sg.body.dims[0]);
+      P.field.dy = static_cast<uint32_t>(sg.body.dims[1]);
+      P.field.dz = static_cast<uint32_t>(sg.body.dims[2]);
+      P.field.offset = fieldValues.size();
+      P.field.present = 1u;
+      fieldValues.insert(fieldValues.end(), sg.body.values.begin(), sg.body.values.end());
+    }
+    P.field.band = static_cast<float>(sg.body.band);
+  }
+  upload(tileSeg_, tileSeg);
+  upload(segParams_, params);
+  segState_.allocate(sizeof(dk::SegState) * std::max<size_t>(M.segments.size(), 1));
+  if (hostStates_) (void)hipHostFree(hostStates_);
+  hostStates_ = nullptr;
+  check(hipHostMalloc(reinterpret_cast<void**>(&hostStates_), sizeof(dk::SegState) * std::max<size_t>(M.segments.size(), 1),
+                      hipHostMallocDefault),
+        "hipHostMalloc");
+  std::memset(hostStates_, 0, sizeof(dk::SegState) * std::max<size_t>(M.segments.size(), 1));
+  upload(fieldValues_, fieldValues);
+  partA_.allocate(sizeof(double) * nTiles_);
+  partB_.allocate(sizeof(double) * nTiles_);
+  partC_.allocate(sizeof(double) * nTiles_);
+  partE_.allocate(sizeof(double) * nSlices_);
+  partMax_.allocate(sizeof(float) * nTiles_);
+
+  // Terms in internal row indices.
+  std::vector<dk::Stretch32> st(M.stretch.size());
+  for (size_t k = 0; k < M.stretch.size(); ++k) {
+    const StretchTerm& t = M.stretch[k];
+    dk::Stretch32 g{};
+    for (int a = 0; a < 3; ++a) g.v[a] = rowOf_[t.v[a]];
+    for (int a = 0; a < 4; ++a) g.dmInv[a] = static_cast<float>(t.dmInv[a]);
+    g.area = static_cast<float>(t.area);
+    g.fabric = t.fabric;
+    st[k] = g;
+  }
+  upload(stretch_, st);
+  std::vector<dk::Seam32> sm(M.seams.size());
+  for (size_t k = 0; k < M.seams.size(); ++k)
+    sm[k] = dk::Seam32{rowOf_[M.seams[k].a], rowOf_[M.seams[k].b], M.seams[k].garment, M.seams[k].startGap};
+  upload(seams_, sm);
+  hinges_.allocate(sizeof(dk::Hinge32) * std::max<size_t>(M.hinges.size(), 1));
+  uploadHinges();
+  fabrics_.allocate(sizeof(dk::Fabric32) * std::max<size_t>(M.fabrics.size(), 1));
+  garments_.allocate(sizeof(dk::Garment32) * std::max<size_t>(M.garments.size(), 1));
+  uploadFabricsAndMass();
+  uploadGarments();
+
+  // Structure on the device.
+  const uint32_t S = static_cast<uint32_t>(M.stretch.size()), H = static_cast<uint32_t>(M.hinges.size()),
+                 K = static_cast<uint32_t>(M.seams.size());
+  incCount_.allocate(sizeof(uint32_t) * nRows_);
+  incStart_.allocate(sizeof(uint32_t) * (nRows_ + 1));
+  inc_.allocate(sizeof(uint32_t) * std::max<size_t>(3ull * S + 4ull * H + 2ull * K, 1));
+  rowLen_.allocate(sizeof(uint32_t) * nRows_);
+  rowStart_.allocate(sizeof(uint32_t) * (nRows_ + 1));
+  diagSlot_.allocate(sizeof(uint32_t) * nRows_);
+  stretchSlots_.allocate(sizeof(uint32_t) * 9 * std::max<uint32_t>(S, 1));
+  hingeSlots_.allocate(sizeof(uint32_t) * 16 * std::max<uint32_t>(H, 1));
+  seamSlots_.allocate(sizeof(uint32_t) * 4 * std::max<uint32_t>(K, 1));
+  sellK_.allocate(sizeof(uint32_t) * nSlices_);
+  sellBase_.allocate(sizeof(uint32_t) * (nSlices_ + 1));
+  buildError_.allocate(sizeof(uint32_t));
+  dk::BuildView b{};
+  b.nRows = nRows_;
+  b.nStretch = S;
+  b.nHinge = H;
+  b.nSeam = K;
+  b.nSlices = nSlices_;
+  b.stretch = stretch_.as<dk::Stretch32>();
+  b.hinges = hinges_.as<dk::Hinge32>();
+  b.seams = seams_.as<dk::Seam32>();
+  b.rowFlags = rowFlags_.as<uint32_t>();
+  b.incCount = incCount_.as<uint32_t>();
+  b.incStart = incStart_.as<uint32_t>();
+  b.inc = inc_.as<uint32_t>();
+  b.rowLen = rowLen_.as<uint32_t>();
+  b.rowStart = rowStart_.as<uint32_t>();
+  b.diagSlot = diagSlot_.as<uint32_t>();
+  b.stretchSlots = stretchSlots_.as<uint32_t>();
+  b.hingeSlots = hingeSlots_.as<uint32_t>();
+  b.seamSlots = seamSlots_.as<uint32_t>();
+  b.sellK = sellK_.as<uint32_t>();
+  b.sellSize = sellBase_.as<uint32_t>();
+  b.error = buildError_.as<uint32_t>();
+  uint32_t* colIndex = nullptr;
+  uint32_t* sellCol = nullptr;
+  uint32_t nBlocks = 0, maxK = 0;
+  uint64_t sellFloats = 0;
+  dk::launchBuildStructure(b, nBlocks, sellFloats, maxK, colIndex, sellCol, stream_);
+  colIndex_.adopt(colIndex, sizeof(uint32_t) * std::max<uint32_t>(nBlocks, 1));
+  sellCol_.adopt(sellCol, sizeof(uint32_t) * std::max<uint64_t>(sellFloats / 9, 1));
+  nBlocks_ = nBlocks;
+  sellFloats_ = sellFloats;
+  maxK_ = maxK;
+  if (maxK_ * 9ull * dk::kSlice * sizeof(float) > 64ull * 1024)
+    throw std::invalid_argument("hip: a 64-row slice has more than 28 blocks in one row");
+  sellVal_.allocate(sizeof(float) * std::max<uint64_t>(sellFloats, 1));
+
+  dk::View& v = view_;
+  v = dk::View{};
+  v.nRows = nRows_;
+  v.nTiles = nTiles_;
+  v.nSlices = nSlices_;
+  v.nSegments = static_cast<uint32_t>(M.segments.size());
+  v.nStretch = S;
+  v.nHinge = H;
+  v.nSeam = K;
+  v.rowFlags = rowFlags_.as<uint32_t>();
+  v.mass = mass_.as<float>();
+  v.rowFabric = rowFabric_.as<uint32_t>();
+  v.rowGarment = rowGarment_.as<uint32_t>();
+  v.fExt = fExt_.as<dk::F3>();
+  v.pinPos = pinPos_.as<dk::F3>();
+  v.x = x_.as<dk::F3>();
+  v.v = v_.as<dk::F3>();
+  v.xStart = xStart_.as<dk::F3>();
+  v.yRel = yRel_.as<dk::F3>();
+  v.u = u_.as<dk::F3>();
+  v.grad = grad_.as<dk::F3>();
+  v.step = step_.as<dk::F3>();
+  v.r = r_.as<dk::F3>();
+  v.z = z_.as<dk::F3>();
+  v.pA = pA_.as<dk::F3>();
+  v.pB = pB_.as<dk::F3>();
+  v.ap = ap_.as<dk::F3>();
+  v.warm = warm_.as<dk::F3>();
+  v.precond = precond_.as<float>();
+  v.cnormal = cnormal_.as<float4>();
+  v.cflags = cflags_.as<uint32_t>();
+  v.cforce = cforce_.as<float>();
+  v.clambda = clambda_.as<float>();
+  v.tileSeg = tileSeg_.as<uint32_t>();
+  v.seg = segParams_.as<dk::SegParams>();
+  v.state = segState_.as<dk::SegState>();
+  v.partA = partA_.as<double>();
+  v.partB = partB_.as<double>();
+  v.partC = partC_.as<double>();
+  v.partE = partE_.as<double>();
+  v.partMax = partMax_.as<float>();
+  v.stretch = stretch_.as<dk::Stretch32>();
+  v.hinges = hinges_.as<dk::Hinge32>();
+  v.seams = seams_.as<dk::Seam32>();
+  v.fabrics = fabrics_.as<dk::Fabric32>();
+  v.garments = garments_.as<dk::Garment32>();
+  v.incStart = incStart_.as<uint32_t>();
+  v.inc = inc_.as<uint32_t>();
+  v.rowStart = rowStart_.as<uint32_t>();
+  v.colIndex = colIndex_.as<uint32_t>();
+  v.diagSlot = diagSlot_.as<uint32_t>();
+  v.stretchSlots = stretchSlots_.as<uint32_t>();
+  v.hingeSlots = hingeSlots_.as<uint32_t>();
+  v.seamSlots = seamSlots_.as<uint32_t>();
+  v.sellK = sellK_.as<uint32_t>();
+  v.sellBase = sellBase_.as<uint32_t>();
+  v.sellCol = sellCol_.as<uint32_t>();
+  v.sellVal = sellVal_.as<float>();
+  v.fieldValues = fieldValues_.as<float>();
+  built_ = true;
+  syncStateToDevice();
+  check(hipStreamSynchronize(stream_), "build");
+}
+
+void HipSolver::uploadSegParams(double h) {
+  if (h == lastH_) return;
+  std::vector<dk::SegParams> params(model_.segments.size());
+  check(hipMemcpyAsync(params.data(), segParams_.as<void>(), sizeof(dk::SegParams) * params.size(),
+                       hipMemcpyDeviceToHost, stream_),
+        "copy");
+  check(hipStreamSynchronize(stream_), "sync");
+  for (auto& P : params) {
+    P.h = static_cast<float>(h);
+    P.invH2 = static_cast<float>(1.0 / (h * h));
+  }
+  check(hipMemcpyAsync(segParams_.as<void>(), params.data(), sizeof(dk::SegParams) * params.size(),
+                       hipMemcpyHostToDevice, stream_),
+        "copy");
+  check(hipStreamSynchronize(stream_), "sync");
+  lastH_ = h;
+}
+
+void HipSolver::uploadGarments() {
+  std::vector<dk::Garment32> g(model_.garments.size());
+  for (size_t k = 0; k < g.size(); ++k)
+    g[k] = dk::Garment32{model_.garments[k].phase == DRAPE_PHASE_SETTLING ? 1u : 0u,
+                         static_cast<float>(model_.garments[k].assemblyTime), {0, 0}};
+  if (!g.empty())
+    check(hipMemcpyAsync(garments_.as<void>(), g.data(), sizeof(dk::Garment32) * g.size(), hipMemcpyHostToDevice, stream_),
+          "copy");
+  check(hipStreamSynchronize(stream_), "sync");
+}
+
+void HipSolver::uploadFabricsAndMass() {
+  std::vector<dk::Fabric32> f(model_.fabrics.size());
+  for (size_t k = 0; k < f.size(); ++k) f[k] = fabric32(model_.fabrics[k]);
+  if (!f.empty())
+    check(hipMemcpy(fabrics_.as<void>(), f.data(), sizeof(dk::Fabric32) * f.size(), hipMemcpyHostToDevice), "copy");
+  std::vector<float> mass(nRows_, 1.0f);
+  for (uint32_t i = 0; i < model_.x.size(); ++i) mass[rowOf_[i]] = static_cast<float>(model_.mass[i]);
+  check(hipMemcpy(mass_.as<void>(), mass.data(), sizeof(float) * nRows_, hipMemcpyHostToDevice), "copy");
+}
+
+void HipSolver::uploadHinges() {
+  std::vector<dk::Hinge32> hg(model_.hinges.size());
+  for (size_t k = 0; k < hg.size(); ++k) {
+    const HingeTerm& h = model_.hinges[k];
+    dk::Hinge32 g{};
+    for (int a = 0; a < 4; ++a) g.v[a] = rowOf_[h.v[a]];
+    g.K = static_cast<float>(h.K);
+    g.restAngle = 0.0f;
+    hg[k] = g;
+  }
+  if (!hg.empty())
+    check(hipMemcpy(hinges_.as<void>(), hg.data(), sizeof(dk::Hinge32) * hg.size(), hipMemcpyHostToDevice), "copy");
+}
+
+void HipSolver::uploadSeams() {
+  std::vector<dk::Seam32> sm(model_.seams.size());
+  for (size_t k = 0; k < sm.size(); ++k)
+    sm[k] = dk::Seam32{rowOf_[model_.seams[k].a], rowOf_[model_.seams[k].b], model_.seams[k].garment,
+                       model_.seams[k].startGap};
+  if (!sm.empty())
+    check(hipMemcpy(seams_.as<void>(), sm.data(), sizeof(dk::Seam32) * sm.size(), hipMemcpyHostToDevice), "copy");
+}
+
+void HipSolver::syncStateFromDevice() const {
+  const size_t n = model_.x.size();
+  std::vector<dk::F3> xs(nRows_ + 1), vs(nRows_ + 1);
+  std::vector<float> cf(nRows_);
+  check(hipMemcpyAsync(xs.data(), x_.as<void>(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyDeviceToHost, stream_), "copy");
+  check(hipMemcpyAsync(vs.data(), v_.as<void>(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyDeviceToHost, stream_), "copy");
+  check(hipMemcpyAsync(cf.data(), cforce_.as<void>(), sizeof(float) * nRows_, hipMemcpyDeviceToHost, stream_), "copy");
+  check(hipStreamSynchronize(stream_), "sync");
+  Model& m = const_cast<Model&>(model_);
+  for (size_t i = 0; i < n; ++i) {
+    const dk::F3 a = xs[rowOf_[i]], b = vs[rowOf_[i]];
+    m.x[i] = {a.x, a.y, a.z};
+    m.v[i] = {b.x, b.y, b.z};
+    m.contactForce[i] = cf[rowOf_[i]];
+  }
+}
+
+void HipSolver::syncStateToDevice() {
+  std::vector<dk::F3> xs(nRows_ + 1, dk::F3{0, 0, 0}), vs(nRows_ + 1, dk::F3{0, 0, 0});
+  std::vector<float> cf(nRows_, 0.0f);
+  for (size_t i = 0; i < model_.x.size(); ++i) {
+    xs[rowOf_[i]] = f3(model_.x[i]);
+    vs[rowOf_[i]] = f3(model_.v[i]);
+    cf[rowOf_[i]] = static_cast<float>(model_.contactForce[i]);
+  }
+  check(hipMemcpyAsync(x_.as<void>(), xs.data(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyHostToDevice, stream_), "copy");
+  check(hipMemcpyAsync(v_.as<void>(), vs.data(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyHostToDevice, stream_), "copy");
+  check(hipMemcpyAsync(cforce_.as<void>(), cf.data(), sizeof(float) * nRows_, hipMemcpyHostToDevice, stream_), "copy");
+  check(hipStreamSynchronize(stream_), "sync");
+}
+
+void HipSolver::pollStates() const {
+  check(hipGetLastError(), "kernel launch");
+  check(hipMemcpyAsync(hostStates_, segState_.as<void>(), sizeof(dk::SegState) * model_.segments.size(),
+                       hipMemcpyDeviceToHost, stream_),
+        "copy");
+  check(hipStreamSynchronize(stream_), "step");
+}
+
+bool HipSolver::allSegments(bool (*pred)(const dk::SegState&)) const {
+  for (size_t s = 0; s < model_.segments.size(); ++s)
+    if (!pred(hostStates_[s])) return false;
+  return true;
+}
+
+void HipSolver::runPcg() {
+  dk::launchPcgInit(view_, stream_);
+  uint32_t maxCg = 0;
+  for (const auto& sg : model_.segments) maxCg = std::max<uint32_t>(maxCg, static_cast<uint32_t>(std::max(0, sg.settings.maxCg)));
+  // First chunk sized from the previous solve; later chunks short. Iterations past convergence are no-ops.
+  uint32_t k = 0;
+  uint32_t chunk = std::max<uint32_t>(8, lastPcgIterations_ * 3 / 4);
+  uint32_t total = 0;
+  while (k < maxCg) {
+    const uint32_t end = std::min(maxCg, k + chunk);
+    for (; k < end; ++k) dk::launchPcgIteration(view_, k, stream_);
+    pollStates();
+    if (allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; })) break;
+    chunk = 16;
+  }
+  for (size_t s = 0; s < model_.segments.size(); ++s) total = std::max(total, hostStates_[s].pcgIter);
+  lastPcgIterations_ = total;
+}
+
+void HipSolver::substep(double h, int iterations) {
+  uploadSegParams(h);
+  dk::launchPredictContact(view_, stream_);
+  for (int it = 0; it < iterations; ++it) {
+    dk::launchActivate(view_, stream_);
+    dk::launchAssemble(view_, maxK_, true, stream_);
+    dk::launchNewtonBegin(view_, stream_);
+    runPcg();
+    dk::launchContactForceSlope(view_, stream_);
+    dk::launchNewtonSlope(view_, stream_);
+    for (int ls = 0; ls < 20; ++ls) {
+      dk::launchEnergyTrial(view_, stream_);
+      dk::launchLineSearchDecide(view_, stream_);
+      pollStates();
+      if (allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.lsDone != 0u; })) break;
+    }
+    dk::launchApply(view_, stream_);
+    dk::launchNewtonEnd(view_, stream_);
+    pollStates();
+    if (allSegments([](const dk::SegState& s) { return s.newtonDone != 0u; })) break;
+  }
+  dk::launchFinish(view_, stream_);
+  bool assembling = false;
+  for (auto& g : model_.garments)
+    if (g.phase == DRAPE_PHASE_ASSEMBLING) {
+      g.assemblyTime += h;
+      assembling = true;
+    }
+  if (assembling) uploadGarments();
+}
+
+void HipSolver::step(double dt, int32_t substeps, int32_t iterations) {
+  requireBuilt();
+  if (model_.x.empty()) return;
+  const int ns = std::max(1, substeps);
+  const double h = dt / ns;
+  const int its = std::max(1, iterations);
+  for (int s = 0; s < ns; ++s) substep(h, its);
+  check(hipStreamSynchronize(stream_), "step");
+}
+
+void HipSolver::readPositions(float* out, size_t n) const {
+  requireBuilt();
+  if (n != model_.x.size()) throw std::invalid_argument("readPositions: span size must equal particleCount()");
+  std::vector<dk::F3> xs(nRows_ + 1);
+  check(hipMemcpyAsync(xs.data(), x_.as<void>(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyDeviceToHost, stream_), "copy");
+  check(hipStreamSynchronize(stream_), "sync");
+  for (size_t i = 0; i < n; ++i) {
+    const dk::F3 a = xs[rowOf_[i]];
+    out[3 * i] = a.x;
+    out[3 * i + 1] = a.y;
+    out[3 * i + 2] = a.z;
+  }
+}
+
+void HipSolver::readVelocities(float* out, size_t n) const {
+  requireBuilt();
+  if (n != model_.x.size()) throw std::invalid_argument("span size must equal particleCount()");
+  std::vector<dk::F3> vs(nRows_ + 1);
+  check(hipMemcpyAsync(vs.data(), v_.as<void>(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyDeviceToHost, stream_), "copy");
+  check(hipStreamSynchronize(stream_), "sync");
+  for (size_t i = 0; i < n; ++i) {
+    const dk::F3 a = vs[rowOf_[i]];
+    out[3 * i] = a.x;
+    out[3 * i + 1] = a.y;
+    out[3 * i + 2] = a.z;
+  }
+}
+
+// SimModel::stats (sim_model.cpp:222-241) on the host copy of x and v.
+drape_stats HipSolver::stats() const {
+  requireBuilt();
+  syncStateFromDevice();
+  const Model& m = model_;
+  drape_stats st{};
+  double sumV2 = 0;
+  size_t moving = 0;
+  for (size_t i = 0; i < m.x.size(); ++i) {
+    const Vec3f& x = m.x[i];
+    const Vec3f& v = m.v[i];
+    if (!std::isfinite(x.x) || !std::isfinite(x.y) || !std::isfinite(x.z) || !std::isfinite(v.x) ||
+        !std::isfinite(v.y) || !std::isfinite(v.z))
+      st.has_non_finite = 1;
+    if (m.pinned[i]) continue;
+    const double v2 = double(v.x) * v.x + double(v.y) * v.y + double(v.z) * v.z;
+    st.kinetic_energy += 0.5 * m.mass[i] * v2;
+    sumV2 += v2;
+    ++moving;
+  }
+  st.rms_speed = moving ? std::sqrt(sumV2 / static_cast<double>(moving)) : 0.0;
+  for (const auto& sp : m.seams) {
+    const Vec3f a = m.x[sp.a], b = m.x[sp.b];
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    st.max_seam_gap = std::max(st.max_seam_gap, static_cast<double>(std::sqrt(dx * dx + dy * dy + dz * dz)));
+  }
+  for (size_t i = 0; i < m.x.size(); ++i) {
+    const Field& body = m.segments[m.pSegment[i]].body;
+    if (!body.present()) continue;
+    st.max_body_penetration = std::max(st.max_body_penetration, -body.sample({m.x[i].x, m.x[i].y, m.x[i].z}));
+  }
+  return st;
+}
+
+// Snapshot blob: u64 particles, u64 garments, u64 seams; positions, velocities (3 floats each); phases (int32);
+// assemblyTime (double); seamStartGap (float); contactForce (double) — Snapshot (solver.hpp:76-82).
+size_t HipSolver::snapshotSize() const {
+  requireBuilt();
+  return 3 * sizeof(uint64_t) + model_.x.size() * 24 + model_.garments.size() * 12 + model_.seams.size() * 4 +
+         model_.x.size() * 8;
+}
+
+void HipSolver::snapshotSave(void* out, size_t bytes) const {
+  requireBuilt();
+  if (bytes != snapshotSize()) throw std::invalid_argument("snapshot buffer size mismatch");
+  syncStateFromDevice();
+  const Model& m = model_;
+  char* p = static_cast<char*>(out);
+  const uint64_t head[3] = {m.x.size(), m.garments.size(), m.seams.size()};
+  std::memcpy(p, head, sizeof(head));
+  p += sizeof(head);
+  std::memcpy(p, m.x.data(), m.x.size() * 12);
+  p += m.x.size() * 12;
+  std::memcpy(p, m.v.data(), m.v.size() * 12);
+  p += m.v.size() * 12;
+  for (const auto& g : m.garments) {
+    std::memcpy(p, &g.phase, 4);
+    p += 4;
+  }
+  for (const auto& g : m.garments) {
+    std::memcpy(p, &g.assemblyTime, 8);
+    p += 8;
+  }
+  for (const auto& s : m.seams) {
+    std::memcpy(p, &s.startGap, 4);
+    p += 4;
+  }
+  std::memcpy(p, m.contactForce.data(), m.contactForce.size() * 8);
+}
+
+void HipSolver::snapshotRestore(const void* in, size_t bytes) {
+  requireBuilt();
+  Model& m = model_;
+  const char* p = static_cast<const char*>(in);
+  if (bytes < 3 * sizeof(uint64_t)) throw std::invalid_argument("snapshot does not match this scene");
+  uint64_t head[3];
+  std::memcpy(head, p, sizeof(head));
+  p += sizeof(head);
+  const size_t expect = 3 * sizeof(uint64_t) + head[0] * 24 + head[1] * 12 + head[2] * 4 + head[0] * 8;
+  if (head[0] != m.x.size() || head[1] != m.garments.size() || bytes != expect)
+    throw std::invalid_argument("snapshot does not match this scene");
+  std::memcpy(m.x.data(), p, m.x.size() * 12);
+  p += m.x.size() * 12;
+  std::memcpy(m.v.data(), p, m.v.size() * 12);
+  p += m.v.size() * 12;
+  for (auto& g : m.garments) {
+    std::memcpy(&g.phase, p, 4);
+    p += 4;
+  }
+  for (auto& g : m.garments) {
+    std::memcpy(&g.assemblyTime, p, 8);
+    p += 8;
+  }
+  if (head[2] == m.seams.size()) {
+    for (auto& s : m.seams) {
+      std::memcpy(&s.startGap, p, 4);
+      p += 4;
+    }
+  } else {
+    p += head[2] * 4;
+  }
+  std::memcpy(m.contactForce.data(), p, m.contactForce.size() * 8);
+  syncStateToDevice();
+  uploadGarments();
+  uploadSeams();
+}
+
+void HipSolver::updateFabric(const std::string& garment, const std::vector<Fabric>& fabrics) {
+  requireBuilt();
+  Garment& g = model_.garments[model_.garmentIndex(garment)];
+  if (fabrics.size() != g.pieceCount) throw std::invalid_argument("one fabric per pattern piece required");
+  std::copy(fabrics.begin(), fabrics.end(), model_.fabrics.begin() + g.fabricOffset);
+  for (uint32_t i = g.offset; i < g.offset + g.count; ++i) model_.updateMass(i);
+  for (auto& h : model_.hinges) {
+    const uint32_t fi = h.side[0].fabric;
+    if (fi >= g.fabricOffset && fi < g.fabricOffset + g.pieceCount) h.K = model_.hingeStiffness(h);
+  }
+  check(hipStreamSynchronize(stream_), "sync");
+  uploadFabricsAndMass();
+  uploadHinges();
+}
+
+void HipSolver::setPhase(const std::string& garment, int32_t phase) {
+  requireBuilt();
+  const size_t gi = model_.garmentIndex(garment);
+  Garment& g = model_.garments[gi];
+  g.phase = phase;
+  if (phase == DRAPE_PHASE_ASSEMBLING) {
+    g.assemblyTime = 0;
+    syncStateFromDevice();
+    for (auto& sp : model_.seams) {
+      if (sp.garment != gi) continue;
+      const Vec3f a = model_.x[sp.a], b = model_.x[sp.b];
+      const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+      sp.startGap = std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    uploadSeams();
+  }
+  uploadGarments();
+}
+
+void HipSolver::range(const std::string& garment, uint32_t& offset, uint32_t& count) const {
+  requireBuilt();
+  const Garment& g = model_.garments[model_.garmentIndex(garment)];
+  offset = g.offset;
+  count = g.count;
+}
+
+void HipSolver::segmentRange(uint32_t s, uint32_t& offset, uint32_t& count) const {
+  requireBuilt();
+  if (s >= model_.segments.size()) throw std::out_of_range("segment index out of range");
+  offset = model_.segments[s].particleBegin;
+  count = model_.segments[s].particleCount;
+}
+
+drape_counters HipSolver::counters() const {
+  requireBuilt();
+  pollStates();
+  drape_counters c{};
+  for (size_t s = 0; s < model_.segments.size(); ++s) {
+    const dk::SegState& st = hostStates_[s];
+    c.substeps += st.cntSubsteps;
+    c.newton_iterations += st.cntNewton;
+    c.pcg_iterations += st.cntPcg;
+    c.line_search_trials += st.cntLs;
+    c.warm_start_fallbacks += st.cntWarmRejected;
+    c.warm_start_accepted += st.cntWarmAccepted;
+  }
+  return c;
+}
+
+void HipSolver::resetCounters() {
+  requireBuilt();
+  pollStates();
+  for (size_t s = 0; s < model_.segments.size(); ++s) {
+    dk::SegState& st = hostStates_[s];
+    st.cntSubsteps = st.cntNewton = st.cntPcg = st.cntLs = st.cntWarmRejected = st.cntWarmAccepted = 0;
+  }
+  check(hipMemcpyAsync(segState_.as<void>(), hostStates_, sizeof(dk::SegState) * model_.segments.size(),
+                       hipMemcpyHostToDevice, stream_),
+        "copy");
+  check(hipStreamSynchronize(stream_), "sync");
+}
+
+drape_structure_sizes HipSolver::structureSizes() const {
+  requireBuilt();
+  drape_structure_sizes s{};
+  s.n_particles = static_cast<uint32_t>(model_.x.size());
+  s.n_stretch = static_cast<uint32_t>(model_.stretch.size());
+  s.n_hinges = static_cast<uint32_t>(model_.hinges.size());
+  s.n_seams = static_cast<uint32_t>(model_.seams.size());
+  s.n_blocks = nBlocks_ - (nRows_ - static_cast<uint32_t>(model_.x.size()));  // padding rows hold one block each
+  return s;
+}
+
+// The device structure re-indexed to global particles (identical to the internal one for a single scene, whose
+// padding rows all follow the last particle).
+void HipSolver::globalStructure(std::vector<uint32_t>& rowStart, std::vector<uint32_t>& colIndex,
+                                std::vector<uint32_t>& diagSlot, std::vector<uint32_t>& internalToGlobalSlot) const {
+  std::vector<uint32_t> rs(nRows_ + 1), ci(nBlocks_);
+  check(hipMemcpy(rs.data(), rowStart_.as<void>(), sizeof(uint32_t) * (nRows_ + 1), hipMemcpyDeviceToHost), "copy");
+  check(hipMemcpy(ci.data(), colIndex_.as<void>(), sizeof(uint32_t) * nBlocks_, hipMemcpyDeviceToHost), "copy");
+  const size_t n = model_.x.size();
+  std::vector<uint32_t> particleOfRow(nRows_, UINT32_MAX);
+  for (uint32_t i = 0; i < n; ++i) particleOfRow[rowOf_[i]] = i;
+  rowStart.assign(n + 1, 0);
+  colIndex.clear();
+  diagSlot.assign(n, 0);
+  internalToGlobalSlot.assign(nBlocks_, UINT32_MAX);
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t r = rowOf_[i];
+    for (uint32_t s = rs[r]; s < rs[r + 1]; ++s) {
+      const uint32_t c = particleOfRow[ci[s]];
+      if (c == i) diagSlot[i] = static_cast<uint32_t>(colIndex.size());
+      internalToGlobalSlot[s] = static_cast<uint32_t>(colIndex.size());
+      colIndex.push_back(c);
+    }
+    rowStart[i + 1] = static_cast<uint32_t>(colIndex.size());
+  }
+}
+
+void HipSolver::readStructure(uint32_t* rowStart, uint32_t* colIndex, uint32_t* diagSlot, uint32_t* stretchSlots,
+                              uint32_t* hingeSlots, uint32_t* seamSlots) const {
+  requireBuilt();
+  check(hipStreamSynchronize(stream_), "sync");
+  std::vector<uint32_t> rs, ci, ds, map;
+  globalStructure(rs, ci, ds, map);
+  if (rowStart) std::copy(rs.begin(), rs.end(), rowStart);
+  if (colIndex) std::copy(ci.begin(), ci.end(), colIndex);
+  if (diagSlot) std::copy(ds.begin(), ds.end(), diagSlot);
+  auto slots = [&](const DeviceBuffer& b, size_t count, uint32_t* out) {
+    if (!out || count == 0) return;
+    std::vector<uint32_t> s(count);
+    check(hipMemcpy(s.data(), b.as<void>(), sizeof(uint32_t) * count, hipMemcpyDeviceToHost), "copy");
+    for (size_t k = 0; k < count; ++k) out[k] = map[s[k]];
+  };
+  slots(stretchSlots_, 9 * model_.stretch.size(), stretchSlots);
+  slots(hingeSlots_, 16 * model_.hinges.size(), hingeSlots);
+  slots(seamSlots_, 4 * model_.seams.size(), seamSlots);
+}
+
+void HipSolver::readTerms(uint32_t* stretchV, uint32_t* hingeV, double* hingeK, uint32_t* seamAB) const {
+  requireBuilt();
+  for (size_t k = 0; k < model_.stretch.size(); ++k)
+    if (stretchV)
+      for (int a = 0; a < 3; ++a) stretchV[3 * k + a] = model_.stretch[k].v[a];
+  for (size_t k = 0; k < model_.hinges.size(); ++k) {
+    if (hingeV)
+      for (int a = 0; a < 4; ++a) hingeV[4 * k + a] = model_.hinges[k].v[a];
+    if (hingeK) hingeK[k] = model_.hinges[k].K;
+  }
+  for (size_t k = 0; k < model_.seams.size(); ++k)
+    if (seamAB) {
+      seamAB[2 * k] = model_.seams[k].a;
+      seamAB[2 * k + 1] = model_.seams[k].b;
+    }
+}
+
+void HipSolver::debugAssemble(double h, const float* u, double* energy, float* grad, float* blocks) {
+  requireBuilt();
+  uploadSegParams(h);
+  // Keep the contact force the next step's friction reads.
+  std::vector<float> savedForce(nRows_);
+  check(hipMemcpy(savedForce.data(), cforce_.as<void>(), sizeof(float) * nRows_, hipMemcpyDeviceToHost), "copy");
+  dk::launchPredictContact(view_, stream_);
+  std::vector<dk::F3> us(nRows_ + 1, dk::F3{0, 0, 0});
+  for (size_t i = 0; i < model_.x.size(); ++i) us[rowOf_[i]] = dk::F3{u[3 * i], u[3 * i + 1], u[3 * i + 2]};
+  check(hipMemcpyAsync(u_.as<void>(), us.data(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyHostToDevice, stream_), "copy");
+  check(hipMemsetAsync(cflags_.as<void>(), 0, sizeof(uint32_t) * nRows_, stream_), "memset");
+  check(hipMemsetAsync(clambda_.as<void>(), 0, sizeof(float) * nRows_, stream_), "memset");
+  dk::launchAssemble(view_, maxK_, false, stream_);
+  dk::launchNewtonBegin(view_, stream_);
+  pollStates();
+  if (energy) *energy = hostStates_[0].e0;
+  for (size_t s = 1; s < model_.segments.size() && energy; ++s) *energy += hostStates_[s].e0;
+  if (grad) {
+    std::vector<dk::F3> g(nRows_ + 1);
+    check(hipMemcpy(g.data(), grad_.as<void>(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyDeviceToHost), "copy");
+    for (size_t i = 0; i < model_.x.size(); ++i) {
+      grad[3 * i] = g[rowOf_[i]].x;
+      grad[3 * i + 1] = g[rowOf_[i]].y;
+      grad[3 * i + 2] = g[rowOf_[i]].z;
+    }
+  }
+  if (blocks) {
+    DeviceBuffer tmp;
+    tmp.allocate(sizeof(float) * 9 * nBlocks_);
+    dk::launchSellToSlots(view_, tmp.as<float>(), stream_);
+    std::vector<float> internal(9 * size_t(nBlocks_));
+    check(hipMemcpyAsync(internal.data(), tmp.as<void>(), sizeof(float) * internal.size(), hipMemcpyDeviceToHost, stream_),
+          "copy");
+    check(hipStreamSynchronize(stream_), "sync");
+    std::vector<uint32_t> rs, ci, ds, map;
+    globalStructure(rs, ci, ds, map);
+    for (uint32_t s = 0; s < nBlocks_; ++s)
+      if (map[s] != UINT32_MAX) std::copy(&internal[9 * size_t(s)], &internal[9 * size_t(s)] + 9, blocks + 9 * size_t(map[s]));
+  }
+  check(hipMemcpy(cforce_.as<void>(), savedForce.data(), sizeof(float) * nRows_, hipMemcpyHostToDevice), "copy");
+}
+
+void HipSolver::debugSpmv(const float* x, float* y) {
+  requireBuilt();
+  std::vector<dk::F3> xs(nRows_ + 1, dk::F3{0, 0, 0}), ys(nRows_ + 1);
+  for (size_t i = 0; i < model_.x.size(); ++i) xs[rowOf_[i]] = dk::F3{x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+  check(hipMemcpyAsync(scratchF3_.as<void>(), xs.data(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyHostToDevice, stream_),
+        "copy");
+  dk::launchSpmvPlain(view_, scratchF3_.as<dk::F3>(), scratchF3b_.as<dk::F3>(), stream_);
+  check(hipMemcpyAsync(ys.data(), scratchF3b_.as<void>(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyDeviceToHost, stream_),
+        "copy");
+  check(hipStreamSynchronize(stream_), "sync");
+  for (size_t i = 0; i < model_.x.size(); ++i) {
+    y[3 * i] = ys[rowOf_[i]].x;
+    y[3 * i + 1] = ys[rowOf_[i]].y;
+    y[3 * i + 2] = ys[rowOf_[i]].z;
+  }
+}
+
+}  // namespace dh

@@ -1,0 +1,110 @@
+// solver.hpp — the HIP implementation of the reference's Solver surface (libs/sim/include/drape/sim/solver.hpp:85-98)
+// on one MI355X: device state, structure build and the Newton–PCG substep orchestration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "model.hpp"
+
+namespace dh {
+
+// Device allocation owned by the solver (zero-filled on allocation).
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() { release(); }
+  void allocate(size_t bytes);
+  void adopt(void* p, size_t bytes) {
+    release();
+    ptr_ = p;
+    bytes_ = bytes;
+  }
+  void release();
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(ptr_);
+  }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  void* ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+class HipSolver {
+ public:
+  HipSolver();
+  ~HipSolver();
+
+  void build(const drape_scene_desc* scenes, uint32_t nScenes);
+  void updateFabric(const std::string& garment, const std::vector<Fabric>& fabrics);
+  void setPhase(const std::string& garment, int32_t phase);
+  void step(double dt, int32_t substeps, int32_t iterations);
+  void readPositions(float* out, size_t n) const;
+  void readVelocities(float* out, size_t n) const;
+  drape_stats stats() const;
+  size_t snapshotSize() const;
+  void snapshotSave(void* out, size_t bytes) const;
+  void snapshotRestore(const void* in, size_t bytes);
+  void range(const std::string& garment, uint32_t& offset, uint32_t& count) const;
+  size_t particleCount() const { return model_.x.size(); }
+  uint32_t segmentCount() const { return static_cast<uint32_t>(model_.segments.size()); }
+  void segmentRange(uint32_t s, uint32_t& offset, uint32_t& count) const;
+  drape_counters counters() const;
+  void resetCounters();
+  void setGraph(bool enabled) { useGraph_ = enabled; }
+
+  drape_structure_sizes structureSizes() const;
+  void readStructure(uint32_t* rowStart, uint32_t* colIndex, uint32_t* diagSlot, uint32_t* stretchSlots,
+                     uint32_t* hingeSlots, uint32_t* seamSlots) const;
+  void readTerms(uint32_t* stretchV, uint32_t* hingeV, double* hingeK, uint32_t* seamAB) const;
+  void debugAssemble(double h, const float* u, double* energy, float* grad, float* blocks);
+  void debugSpmv(const float* x, float* y);
+
+ private:
+  void requireBuilt() const;
+  void uploadSegParams(double h);
+  void uploadGarments();
+  void uploadFabricsAndMass();
+  void uploadHinges();
+  void uploadSeams();
+  void syncStateFromDevice() const;   // x, v, contact force -> host model
+  void syncStateToDevice();           // host model -> x, v, contact force
+  void pollStates() const;            // device SegState -> hostStates_
+  bool allSegments(bool (*pred)(const dk::SegState&)) const;
+  void substep(double h, int iterations);
+  void runPcg();
+  void globalStructure(std::vector<uint32_t>& rowStart, std::vector<uint32_t>& colIndex,
+                       std::vector<uint32_t>& diagSlot, std::vector<uint32_t>& internalToGlobalSlot) const;
+
+  hipStream_t stream_ = nullptr;
+  Model model_;
+  bool built_ = false;
+  bool useGraph_ = true;
+  double lastH_ = -1.0;
+  uint32_t nRows_ = 0, nTiles_ = 0, nSlices_ = 0, nBlocks_ = 0, maxK_ = 0;
+  uint64_t sellFloats_ = 0;
+  uint32_t lastPcgIterations_ = 0;
+  std::vector<uint32_t> rowOf_;  // global particle -> internal row
+  dk::View view_{};
+
+  // Device buffers.
+  DeviceBuffer rowFlags_, mass_, rowFabric_, rowGarment_, fExt_, pinPos_;
+  DeviceBuffer x_, v_, xStart_, yRel_, u_, grad_, step_, r_, z_, pA_, pB_, ap_, warm_, precond_;
+  DeviceBuffer cnormal_, cflags_, cforce_, clambda_;
+  DeviceBuffer tileSeg_, segParams_, segState_, partA_, partB_, partC_, partE_, partMax_;
+  DeviceBuffer stretch_, hinges_, seams_, fabrics_, garments_;
+  DeviceBuffer incCount_, incStart_, inc_, rowLen_, rowStart_, colIndex_, diagSlot_;
+  DeviceBuffer stretchSlots_, hingeSlots_, seamSlots_, sellK_, sellBase_, sellCol_, sellVal_, buildError_;
+  DeviceBuffer fieldValues_, scratchF3_, scratchF3b_;
+  dk::SegState* hostStates_ = nullptr;  // pinned host copy
+};
+
+}  // namespace dh

@@ -1,0 +1,245 @@
+"""Python mirror of the reference's Solver surface (libs/sim/include/drape/sim/solver.hpp:85-98) over the HIP
+C-ABI (include/drape_hip.h).
+
+`HipSolver` is the `SolverBackend::Hip` of `make_solver` (solver_factory.hpp:12-20): build / update_fabric /
+set_phase / step / read_positions / stats / snapshot / restore / range / particle_count, with the reference's
+exception types mapped to ValueError (invalid_argument), IndexError (out_of_range), RuntimeError
+(runtime_error, naming "hip") and AssertionError (logic_error). It fails loudly when the library is not built
+or no gfx950 device is present; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import (Counters, SceneDesc, SceneHolder, Stats, StructureSizes, load_library, ptr)
+
+
+def _lib():
+    return load_library()
+
+
+def _check(code: int) -> None:
+    _ffi.check(code, _lib().drape_last_error)
+
+
+def hip_available() -> bool:
+    """isAvailable(SolverBackend::Hip): the library loads and device 0 is a gfx950."""
+    try:
+        return bool(_lib().drape_hip_available())
+    except (RuntimeError, OSError):
+        return False
+
+
+def default_settings() -> dict:
+    s = _ffi.Settings()
+    _check(_lib().drape_settings_default(C.byref(s)))
+    out = {n: getattr(s, n) for n, _ in s._fields_ if n != "pad0"}
+    out["gravity"] = tuple(s.gravity)
+    return out
+
+
+class HipSolver:
+    """One MI355X solver instance (one segment per scene of a batched build)."""
+
+    backend = "hip"
+
+    def __init__(self, graph: bool = True):
+        self._h = C.c_void_p()
+        _check(_lib().drape_solver_create(C.byref(self._h)))
+        self._n = 0
+        if not graph:
+            self.set_graph(False)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _ffi._lib is not None:
+            _ffi._lib.drape_solver_destroy(h)
+            self._h = None
+
+    # ---- Solver surface ----
+
+    def build(self, scene) -> None:
+        holder = SceneHolder(scene)
+        _check(_lib().drape_solver_build(self._h, C.byref(holder.desc)))
+        self._refresh_count()
+
+    def build_batch(self, scenes: list) -> None:
+        holders = [SceneHolder(s) for s in scenes]
+        arr = (SceneDesc * max(1, len(holders)))(*[h.desc for h in holders])
+        _check(_lib().drape_solver_build_batch(self._h, len(holders), arr))
+        self._refresh_count()
+
+    def _refresh_count(self):
+        n = C.c_size_t()
+        _check(_lib().drape_solver_particle_count(self._h, C.byref(n)))
+        self._n = n.value
+
+    def update_fabric(self, garment: str, fabrics: list) -> None:
+        arr = (_ffi.Fabric * max(1, len(fabrics)))(*[_ffi.fabric_struct(f) for f in fabrics])
+        _check(_lib().drape_solver_update_fabric(self._h, garment.encode(), len(fabrics), arr))
+
+    def set_phase(self, garment: str, phase: int) -> None:
+        _check(_lib().drape_solver_set_phase(self._h, garment.encode(), int(phase)))
+
+    def step(self, dt=1.0 / 60.0, substeps=2, iterations=2) -> None:
+        _check(_lib().drape_solver_step(self._h, float(dt), int(substeps), int(iterations)))
+
+    def particle_count(self) -> int:
+        return self._n
+
+    def range(self, garment: str):
+        off, cnt = C.c_uint32(), C.c_uint32()
+        _check(_lib().drape_solver_range(self._h, garment.encode(), C.byref(off), C.byref(cnt)))
+        return off.value, cnt.value
+
+    def segment_count(self) -> int:
+        n = C.c_uint32()
+        _check(_lib().drape_solver_segment_count(self._h, C.byref(n)))
+        return n.value
+
+    def segment_range(self, k: int):
+        off, cnt = C.c_uint32(), C.c_uint32()
+        _check(_lib().drape_solver_segment_range(self._h, int(k), C.byref(off), C.byref(cnt)))
+        return off.value, cnt.value
+
+    def read_positions(self) -> np.ndarray:
+        out = np.empty((self._n, 3), dtype=np.float32)
+        _check(_lib().drape_solver_read_positions(self._h, ptr(out, C.c_float), self._n))
+        return out
+
+    def read_velocities(self) -> np.ndarray:
+        out = np.empty((self._n, 3), dtype=np.float32)
+        _check(_lib().drape_solver_read_velocities(self._h, ptr(out, C.c_float), self._n))
+        return out
+
+    def stats(self) -> dict:
+        st = Stats()
+        _check(_lib().drape_solver_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def snapshot(self) -> bytes:
+        size = C.c_size_t()
+        _check(_lib().drape_solver_snapshot_size(self._h, C.byref(size)))
+        buf = C.create_string_buffer(size.value)
+        _check(_lib().drape_solver_snapshot_save(self._h, buf, size.value))
+        return buf.raw
+
+    def restore(self, blob: bytes) -> None:
+        buf = C.create_string_buffer(blob, len(blob))
+        _check(_lib().drape_solver_snapshot_restore(self._h, buf, len(blob)))
+
+    def counters(self) -> dict:
+        c = Counters()
+        _check(_lib().drape_solver_counters(self._h, C.byref(c)))
+        return c.as_dict()
+
+    def reset_counters(self) -> None:
+        _check(_lib().drape_solver_reset_counters(self._h))
+
+    def set_graph(self, enabled: bool) -> None:
+        _check(_lib().drape_solver_set_graph(self._h, 1 if enabled else 0))
+
+    # ---- structure / kernel access ----
+
+    def structure_sizes(self) -> dict:
+        s = StructureSizes()
+        _check(_lib().drape_solver_structure_sizes(self._h, C.byref(s)))
+        return {n: getattr(s, n) for n, _ in s._fields_}
+
+    def structure(self) -> dict:
+        sz = self.structure_sizes()
+        out = {
+            "row_start": np.empty(sz["n_particles"] + 1, np.uint32),
+            "col_index": np.empty(sz["n_blocks"], np.uint32),
+            "diag_slot": np.empty(sz["n_particles"], np.uint32),
+            "stretch_slots": np.empty((sz["n_stretch"], 9), np.uint32),
+            "hinge_slots": np.empty((sz["n_hinges"], 16), np.uint32),
+            "seam_slots": np.empty((sz["n_seams"], 4), np.uint32),
+        }
+        _check(_lib().drape_solver_read_structure(self._h, *[ptr(out[k], C.c_uint32) for k in (
+            "row_start", "col_index", "diag_slot", "stretch_slots", "hinge_slots", "seam_slots")]))
+        return out
+
+    def terms(self) -> dict:
+        sz = self.structure_sizes()
+        out = {
+            "stretch_v": np.empty((sz["n_stretch"], 3), np.uint32),
+            "hinge_v": np.empty((sz["n_hinges"], 4), np.uint32),
+            "hinge_k": np.empty(sz["n_hinges"], np.float64),
+            "seam_ab": np.empty((sz["n_seams"], 2), np.uint32),
+        }
+        _check(_lib().drape_solver_read_terms(self._h, ptr(out["stretch_v"], C.c_uint32),
+                                              ptr(out["hinge_v"], C.c_uint32), ptr(out["hinge_k"], C.c_double),
+                                              ptr(out["seam_ab"], C.c_uint32)))
+        return out
+
+    def debug_assemble(self, h: float, u: np.ndarray):
+        sz = self.structure_sizes()
+        ua = np.ascontiguousarray(u, dtype=np.float32).reshape(-1)
+        e = C.c_double()
+        grad = np.empty((self._n, 3), np.float32)
+        blocks = np.empty((sz["n_blocks"], 3, 3), np.float32)
+        _check(_lib().drape_solver_debug_assemble(self._h, float(h), ptr(ua, C.c_float), C.byref(e),
+                                                  ptr(grad, C.c_float), ptr(blocks, C.c_float)))
+        return e.value, grad, blocks
+
+    def debug_spmv(self, x: np.ndarray) -> np.ndarray:
+        xa = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+        out = np.empty((self._n, 3), np.float32)
+        _check(_lib().drape_solver_debug_spmv(self._h, ptr(xa, C.c_float), ptr(out, C.c_float)))
+        return out
+
+
+def make_solver(backend: str = "hip", **kw):
+    """makeSolver (solver_factory.cpp:38-48) for this library's backend."""
+    if backend != "hip":
+        raise RuntimeError(f"{backend}: not built in this library (only 'hip')")
+    return HipSolver(**kw)
+
+
+# ---- standalone kernels ----
+
+def bsr_spmv(row_start, col_index, blocks, pinned, x, reps: int = 1):
+    """y = A x on the device with the solver's SELL-64 SpMV kernel; returns (y, mean kernel microseconds)."""
+    n = len(row_start) - 1
+    rs = np.ascontiguousarray(row_start, np.uint32)
+    ci = np.ascontiguousarray(col_index, np.uint32)
+    bl = np.ascontiguousarray(blocks, np.float32).reshape(-1)
+    pn = None if pinned is None else np.ascontiguousarray(pinned, np.uint8)
+    xa = np.ascontiguousarray(x, np.float32).reshape(-1)
+    y = np.empty((n, 3), np.float32)
+    us = C.c_double()
+    _check(_lib().drape_bsr_spmv(n, len(ci), ptr(rs, C.c_uint32), ptr(ci, C.c_uint32), ptr(bl, C.c_float),
+                                 ptr(pn, C.c_uint8), ptr(xa, C.c_float), ptr(y, C.c_float), int(reps), C.byref(us)))
+    return y, us.value
+
+
+def eval_terms(kind: int, x: np.ndarray, params: np.ndarray, fabrics=None, limit_scale: float = 100.0):
+    """FP32 device term evaluation: kind 0 stretch, 1 hinge, 2 seam, 3 friction (see drape_eval_terms)."""
+    N = {0: 3, 1: 4, 2: 2, 3: 1}[kind]
+    xa = np.ascontiguousarray(x, np.float32).reshape(-1)
+    pa = np.ascontiguousarray(params, np.float32).reshape(-1)
+    n = len(xa) // (3 * N)
+    fab = None
+    if kind == 0:
+        fab = (_ffi.Fabric * n)(*[_ffi.fabric_struct(f) for f in fabrics])
+    e = np.empty(n, np.float32)
+    g = np.empty((n, N, 3), np.float32)
+    h = np.empty((n, N, N, 3, 3), np.float32)
+    _check(_lib().drape_eval_terms(int(kind), n, ptr(xa, C.c_float), ptr(pa, C.c_float),
+                                   fab if fab is not None else C.cast(None, C.POINTER(_ffi.Fabric)),
+                                   float(limit_scale), ptr(e, C.c_float), ptr(g, C.c_float), ptr(h, C.c_float)))
+    return e, g, h
+
+
+def field_query(field, points: np.ndarray):
+    keep: list = []
+    fd = _ffi.field_struct(field, keep)
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    phi = np.empty(len(p), np.float32)
+    nrm = np.empty((len(p), 3), np.float32)
+    _check(_lib().drape_field_query(C.byref(fd), len(p), ptr(p, C.c_float), ptr(phi, C.c_float), ptr(nrm, C.c_float)))
+    return phi, nrm

@@ -1,0 +1,268 @@
+"""GPU parity: the HIP path against the FP64 oracle on the same seeded inputs.
+
+Mirrors the reference's own checks: TermMath float-vs-double (libs/sim/tests/test_term_math.cpp:120-295), kernel
+unit tests on a random block-CSR rig (test_kernels.cpp:36-250), structure invariants (test_sim_model.cpp:38),
+CPU-vs-GPU trajectories (test_gpu_parity.cpp:29-82) and determinism (test_gpu_parity.cpp:87-114).
+"""
+import numpy as np
+import pytest
+
+from conftest import max_distance
+from drape_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _seamed_pair():
+    """Two 12x10 panels (one garment, two pieces) sewn along one side, with gravity: seams + seam hinges."""
+    a = scenes.grid_mesh(12, 10, 0.01)
+    b = scenes.grid_mesh(12, 10, 0.01)
+    mesh = scenes.join_meshes([a, b], [0.0, 0.2])
+    n1 = a.n
+    mesh.seam_pairs = np.asarray([(j * 12 + 11, n1 + j * 12, 0, j) for j in range(10)], np.uint32)
+    pos = np.stack([mesh.rest[:, 0], np.zeros(mesh.n), mesh.rest[:, 1]], axis=1)
+    pos[n1:, 0] -= 0.2 - 0.11 - 0.03  # 3 cm gap between the sewn edges
+    fab = scenes.test_jersey()
+    g = scenes.Garment(id="pair", mesh=mesh, initial_positions=pos, piece_fabric=[fab, fab])
+    return scenes.Scene(garments=[g], settings=scenes.settings(cg_tolerance=1e-6, max_cg_iterations=500))
+
+
+SCENES = {
+    "c1": lambda: scenes.c1_sheet(),
+    "hanging": lambda: scenes.hanging_sheet(size=0.2, spacing=0.01),
+    "seamed": _seamed_pair,
+}
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_structure_built_on_device_matches_oracle(hip, oracle, name):
+    sc = SCENES[name]()
+    g = hip.HipSolver()
+    g.build(sc)
+    o = oracle.OracleSolver()
+    o.build(sc)
+    print(f"case {name}: sizes {g.structure_sizes()}", flush=True)
+    assert g.structure_sizes() == o.structure_sizes()
+    gs, os_ = g.structure(), o.structure()
+    for key in gs:
+        np.testing.assert_array_equal(gs[key], os_[key], err_msg=key)
+    # invariant colIndex[diagSlot[i]] == i (test_sim_model.cpp:38)
+    assert np.all(gs["col_index"][gs["diag_slot"]] == np.arange(sc.n))
+    gt, ot = g.terms(), o.terms()
+    for key in ("stretch_v", "hinge_v", "seam_ab"):
+        np.testing.assert_array_equal(gt[key], ot[key], err_msg=key)
+    np.testing.assert_allclose(gt["hinge_k"], ot["hinge_k"], rtol=1e-15, atol=0)
+
+
+def _rand_stretch(rng):
+    la, lb, t = rng.uniform(0.002, 0.02), rng.uniform(0.002, 0.02), rng.uniform(0.6, 2.5)
+    r = np.array([[0.0, 0.0], [la, 0.0], [lb * np.cos(t), lb * np.sin(t)]])
+    dm = np.array([[r[1, 0] - r[0, 0], r[2, 0] - r[0, 0]], [r[1, 1] - r[0, 1], r[2, 1] - r[0, 1]]])
+    inv = np.linalg.inv(dm)
+    F = np.array([[1 + rng.uniform(-0.2, 0.3), rng.uniform(-0.2, 0.2)],
+                  [rng.uniform(-0.2, 0.2), 1 + rng.uniform(-0.2, 0.3)]])
+    ax = rng.normal(size=3)
+    ax /= np.linalg.norm(ax)
+    ang = 3 * rng.uniform(-1, 1)
+    K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+    R = np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+    min_edge = min(la, lb, np.linalg.norm(r[2] - r[1]))
+    x = np.array([R @ np.array([*(F @ r[k]), rng.uniform(-0.1, 0.1) * min_edge]) for k in range(3)])
+    x -= x[0]
+    return x, np.array([inv[0, 0], inv[1, 0], inv[0, 1], inv[1, 1]]), 0.5 * abs(np.linalg.det(dm)), min_edge
+
+
+def _close(fe, fg, fh, de, dg, dh, sE, sG, sH):
+    assert abs(fe - de) <= 1e-4 * abs(de) + 1e-6 * sE + 1e-12
+    assert np.linalg.norm(fg - dg) <= 1e-3 * np.linalg.norm(dg) + 1e-6 * sG + 1e-9
+    assert np.linalg.norm(fh - dh) <= 1e-3 * np.linalg.norm(dh) + 1e-6 * sH + 1e-9
+
+
+def test_stretch_fp32_matches_fp64(hip, oracle):
+    rng = np.random.default_rng(1234)
+    fabs = [scenes.test_jersey(), scenes.test_stiff(), scenes.membrane_fabric()]
+    samples = [_rand_stretch(rng) for _ in range(200)]
+    x = np.stack([s[0] for s in samples]).astype(np.float32)
+    params = np.array([[*s[1], s[2]] for s in samples], np.float32)
+    fl = [fabs[i % 3] for i in range(200)]
+    e, g, h = hip.eval_terms(0, x, params, fl, 100.0)
+    for i, (xs, d, area, me) in enumerate(samples):
+        de, dg, dh = oracle.eval_stretch(x[i].astype(np.float64), params[i, :4].astype(np.float64),
+                                         float(params[i, 4]), fl[i], 100.0)
+        f = fl[i]
+        kmax = max(f["stretch_warp"], f["stretch_weft"], f["stretch_bias"]) * area * 101
+        _close(e[i], g[i], h[i].transpose(0, 1, 2, 3), de, dg, dh, kmax, kmax / me, kmax / me ** 2)
+
+
+def test_hinge_fp32_matches_fp64(hip, oracle):
+    rng = np.random.default_rng(99)
+    xs, ps = [], []
+    for _ in range(200):
+        L = rng.uniform(0.002, 0.02)
+        e0 = np.zeros(3)
+        e1 = np.array([L, 0, 0])
+        a = np.array([rng.uniform(0.2, 0.8) * L, rng.uniform(0.3, 1.0) * L, 0])
+        ang = rng.uniform(-2.5, 2.5)
+        b2 = np.array([rng.uniform(0.2, 0.8) * L, -rng.uniform(0.3, 1.0) * L, 0])
+        b = np.array([b2[0], b2[1] * np.cos(ang), b2[1] * np.sin(ang)])
+        xs.append(np.stack([a, b, e0, e1]))
+        ps.append([rng.uniform(1e-6, 1e-3), 0.0])
+    x = np.stack(xs).astype(np.float32)
+    p = np.array(ps, np.float32)
+    e, g, h = hip.eval_terms(1, x, p)
+    for i in range(200):
+        de, dg, dh = oracle.eval_hinge(x[i].astype(np.float64), float(p[i, 0]))
+        L = np.linalg.norm(x[i, 3] - x[i, 2])
+        K = float(p[i, 0])
+        _close(e[i], g[i], h[i], de, dg, dh, K, K / L, K / L ** 2)
+
+
+def test_seam_and_friction_fp32_match_fp64(hip, oracle):
+    rng = np.random.default_rng(7)
+    xs = rng.uniform(-0.05, 0.05, size=(200, 2, 3)).astype(np.float32)
+    ps = np.stack([rng.uniform(0, 0.03, 200), rng.uniform(1e2, 1e4, 200)], axis=1).astype(np.float32)
+    e, g, h = hip.eval_terms(2, xs, ps)
+    for i in range(200):
+        de, dg, dh = oracle.eval_seam(xs[i].astype(np.float64), float(ps[i, 0]), float(ps[i, 1]))
+        K = float(ps[i, 1])
+        _close(e[i], g[i], h[i], de, dg, dh, K * 1e-4, K * 1e-2, K)
+    rng = np.random.default_rng(42)
+    eps = 3e-4 / 60
+    dx, pf = [], []
+    for _ in range(200):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        dx.append(rng.uniform(-3, 3, size=3) * eps)
+        pf.append([*n, rng.uniform(0.001, 0.1), eps])
+    dx = np.asarray(dx, np.float32).reshape(200, 1, 3)
+    pf = np.asarray(pf, np.float32)
+    e, g, h = hip.eval_terms(3, dx, pf)
+    for i in range(200):
+        de, dg, dh = oracle.eval_friction(dx[i, 0].astype(np.float64), np.zeros(3), pf[i, :3].astype(np.float64),
+                                          float(pf[i, 3]), float(pf[i, 4]))
+        mu = float(pf[i, 3])
+        _close(e[i], g[i], h[i], de, dg, dh, mu * eps, mu, mu / eps)
+
+
+def test_field_query_matches_fp64(hip, oracle):
+    sc = scenes.c1_sheet()
+    rng = np.random.default_rng(3)
+    dirs = rng.normal(size=(1000, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    pts = dirs * rng.uniform(0.185, 0.215, size=(1000, 1))
+    phi, nrm = hip.field_query(sc.body, pts)
+    dphi, dnrm = oracle.field_query(sc.body, pts.astype(np.float32).astype(np.float64))
+    assert np.max(np.abs(phi - dphi)) < 1e-6
+    ok = np.linalg.norm(dnrm, axis=1) > 0
+    cosang = np.sum(nrm[ok] * dnrm[ok], axis=1)
+    assert np.all(np.arccos(np.clip(cosang, -1, 1)) < 1e-3)
+    # outside the sampled box -> +band, zero gradient
+    far = np.array([[5.0, 5.0, 5.0], [np.nan, 0, 0]])
+    p2, n2 = hip.field_query(sc.body, far)
+    assert np.all(p2 == np.float32(sc.body.band)) and np.all(n2 == 0)
+
+
+@pytest.mark.parametrize("name", ["c1", "seamed"])
+def test_assembly_and_spmv_match_oracle(hip, oracle, name):
+    sc = SCENES[name]()
+    g = hip.HipSolver()
+    g.build(sc)
+    o = oracle.OracleSolver()
+    o.build(sc)
+    rng = np.random.default_rng(11)
+    u = rng.uniform(-2e-3, 2e-3, size=(sc.n, 3)).astype(np.float32)
+    h = 1.0 / 60.0
+    ge, gg, gb = g.debug_assemble(h, u)
+    oe, og, ob = o.debug_assemble(h, u.astype(np.float64))
+    print(f"case {name}: energy hip {ge:.12e} oracle {oe:.12e}", flush=True)
+    assert abs(ge - oe) <= 1e-5 * abs(oe)
+    assert np.linalg.norm(gg - og) <= 1e-4 * np.linalg.norm(og)
+    scale = np.max(np.abs(ob))
+    assert np.max(np.abs(gb - ob)) <= 1e-4 * scale
+    x = rng.uniform(-1, 1, size=(sc.n, 3)).astype(np.float32)
+    gy = g.debug_spmv(x)
+    oy = o.debug_spmv(x.astype(np.float64))
+    assert np.linalg.norm(gy - oy) <= 1e-5 * np.linalg.norm(oy)
+
+
+def _random_rig(n=40, pinned=(3, 17, 29), seed=77):
+    """Random block-CSR SPD system (test_kernels.cpp:36-116): each particle couples to itself and a few others."""
+    rng = np.random.default_rng(seed)
+    nb = [{i} for i in range(n)]
+    for i in range(n):
+        for _ in range(3):
+            j = int(rng.integers(0, n))
+            nb[i].add(j)
+            nb[j].add(i)
+    M = np.zeros((3 * n, 3 * n))
+    for i in range(n):
+        for j in sorted(nb[i]):
+            if j > i:
+                B = 0.3 * rng.uniform(-1, 1, size=(3, 3))
+                M[3 * i:3 * i + 3, 3 * j:3 * j + 3] = B
+                M[3 * j:3 * j + 3, 3 * i:3 * i + 3] = B.T
+        M[3 * i:3 * i + 3, 3 * i:3 * i + 3] = (6.0 + rng.uniform(-1, 1)) * np.eye(3)
+    rs, ci, bl = [0], [], []
+    for i in range(n):
+        for j in sorted(nb[i]):
+            ci.append(j)
+            bl.append(M[3 * i:3 * i + 3, 3 * j:3 * j + 3])
+        rs.append(len(ci))
+    pin = np.zeros(n, np.uint8)
+    pin[list(pinned)] = 1
+    A = M.copy()
+    for i in pinned:
+        A[3 * i:3 * i + 3, :] = 0
+        A[:, 3 * i:3 * i + 3] = 0
+    return np.array(rs), np.array(ci), np.array(bl), pin, A
+
+
+def test_bsr_spmv_matches_dense(hip):
+    rs, ci, bl, pin, A = _random_rig()
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, size=(40, 3))
+    y, _ = hip.bsr_spmv(rs, ci, bl, pin, x)
+    expect = A @ x.reshape(-1)
+    assert np.linalg.norm(y.reshape(-1) - expect) <= 1e-5 * np.linalg.norm(expect)
+
+
+def test_c1_sheet_60_steps_within_0p1mm_of_oracle(hip, oracle):
+    """Config 1 (FP64 oracle vs FP32 HIP with FP64 reductions): max particle distance after 60 steps."""
+    sc = scenes.c1_sheet()
+    g = hip.HipSolver()
+    g.build(sc)
+    o = oracle.OracleSolver()
+    o.build(sc)
+    for k in range(60):
+        g.step(1 / 60, 1, 5)
+        o.step(1 / 60, 1, 5)
+    d = max_distance(g.read_positions(), o.read_positions())
+    gs, os_ = g.stats(), o.stats()
+    print(f"c1 60 steps: max distance {d * 1e3:.4f} mm; hip counters {g.counters()}; oracle {o.counters()}", flush=True)
+    assert not gs["has_non_finite"]
+    assert d <= 1e-4
+    assert gs["max_body_penetration"] <= max(2e-3, os_["max_body_penetration"] + 1e-4)
+
+
+def test_two_runs_bit_identical_and_snapshot_replay(hip):
+    sc = scenes.c1_sheet()
+    runs = []
+    for _ in range(2):
+        g = hip.HipSolver()
+        g.build(sc)
+        for _ in range(10):
+            g.step(1 / 60, 1, 5)
+        runs.append(g.read_positions())
+    np.testing.assert_array_equal(runs[0], runs[1])
+    g = hip.HipSolver()
+    g.build(sc)
+    for _ in range(5):
+        g.step(1 / 60, 1, 5)
+    snap = g.snapshot()
+    for _ in range(5):
+        g.step(1 / 60, 1, 5)
+    first = g.read_positions()
+    g.restore(snap)
+    for _ in range(5):
+        g.step(1 / 60, 1, 5)
+    np.testing.assert_array_equal(first, g.read_positions())
