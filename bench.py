@@ -1,0 +1,139 @@
+"""Benchmark: implicit steps/s of the Newton–PCG cloth step on one MI355X.
+
+Workload (BASELINE metric "implicit steps/sec @65k particles"): the 256x256-vertex sheet (65,536 particles,
+130,050 stretch triangles, 194,565 hinges, 4 mm spacing) dropped from y = 0.4 m onto a sphere (r 0.25 m) plus a
+ground plane (y = -0.3 m), one implicit-Euler step per frame (dt = 1/60 s, 1 substep, Newton <= 5, block-Jacobi
+PCG to relative residual 1e-6), FP32 assembly/SpMV/PCG with FP64 reductions. A "step" is one Solver::step call;
+the timed region holds K consecutive steps of the drop after W warm-up steps, with every input resident in HBM.
+
+Printed as one JSON line with the measured roofline fraction of the dominant kernel (the fused PCG SpMV, timed
+with HIP events on the solver's stream inside the timed region) and the FP64 CPU oracle timed on the host as
+the CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "drape-2_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(make_scene, steps: int) -> dict:
+    """The FP64 oracle (this project's restatement of the reference's single-threaded CpuSolver) on one host
+    thread: `steps` implicit steps of the same scene from the same initial state."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle
+    pyoracle.set_threads(1)
+    s = pyoracle.OracleSolver()
+    s.build(make_scene())
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step(1 / 60, 1, 5)
+    dt = time.perf_counter() - t0
+    c = s.counters()
+    return {"value": steps / dt, "unit": "implicit steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} step(s) from rest of the same 65,536-particle drop, FP64 oracle, 1 thread "
+                      f"({c['newton_iterations']} Newton, {c['pcg_iterations']} PCG iterations), {cpu_model()}, "
+                      f"nproc {os.cpu_count()}"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256, help="sheet vertices per side (256 = the 65,536-particle sheet)")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sample-every", type=int, default=8, help="time every k-th PCG SpMV launch with HIP events")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              f"(the batched path shards by scene, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from drape_amd import HipSolver, scenes
+
+    def make_scene():
+        return scenes.c2_sheet(n=args.n)
+
+    sc = make_scene()
+    s = HipSolver()
+    s.build(sc)
+    s.set_spmv_sampling(args.sample_every)
+    for _ in range(args.warmup):
+        s.step(1 / 60, 1, 5)
+    s.reset_counters()
+    s.set_spmv_sampling(args.sample_every)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.step(1 / 60, 1, 5)  # returns after the device finished the step
+    elapsed = time.perf_counter() - t0
+    c = s.counters()
+    timing = s.spmv_timing()
+    st = s.stats()
+    value = args.steps / elapsed
+    achieved = timing["bytes_per_launch"] / (timing["mean_us"] * 1e-6) / 1e9 if timing["mean_us"] > 0 else None
+    line = {
+        "metric": "implicit steps/sec @65k particles",
+        "value": value,
+        "unit": "implicit steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 reductions)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"sheet {args.n}x{args.n} ({sc.n} particles) dropped from y=0.4 m on sphere r=0.25 m + "
+                        f"ground y=-0.3 m; steps {args.warmup + 1}..{args.warmup + args.steps} of the drop",
+            "particles": sc.n,
+            "dt": 1 / 60, "substeps": 1, "newton_max": 5, "pcg_tolerance": 1e-6,
+            "solver": "plain warm start (u = h v)",
+            "newton_per_step": c["newton_iterations"] / args.steps,
+            "pcg_per_step": c["pcg_iterations"] / args.steps,
+            "line_search_trials_per_step": c["line_search_trials"] / args.steps,
+            "final_stats": st,
+        },
+        "roofline": {
+            "kernel": "kPcgSpmv (fused p = z + beta p, SELL-64 block SpMV, contact filter, FP64 p.Ap partials)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": None,
+            "bytes_per_launch": timing["bytes_per_launch"],
+            "mean_launch_us": timing["mean_us"],
+            "timed_launches": timing["launches"],
+        },
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(make_scene, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

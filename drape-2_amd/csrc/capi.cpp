@@ -289,6 +289,32 @@ int drape_solver_set_graph(drape_solver* s, int32_t enabled) {
   });
 }
 
+int drape_solver_set_pcg_variant(drape_solver* s, int32_t variant) {
+  return guard([&] {
+    need(s, "solver");
+    s->impl.setPcgVariant(variant);
+  });
+}
+
+int drape_solver_set_spmv_sampling(drape_solver* s, int32_t every) {
+  return guard([&] {
+    need(s, "solver");
+    s->impl.setSpmvSampling(every);
+  });
+}
+
+int drape_solver_spmv_timing(const drape_solver* s, uint64_t* launches, double* mean_us, double* bytes_per_launch) {
+  return guard([&] {
+    need(s, "solver");
+    uint64_t l = 0;
+    double m = 0, b = 0;
+    s->impl.spmvTiming(l, m, b);
+    if (launches) *launches = l;
+    if (mean_us) *mean_us = m;
+    if (bytes_per_launch) *bytes_per_launch = b;
+  });
+}
+
 int drape_solver_structure_sizes(const drape_solver* s, drape_structure_sizes* out) {
   return guard([&] {
     need(s, "solver");

@@ -289,6 +289,26 @@ __global__ void __launch_bounds__(64) kAssemble(View v) {
   rowE = waveSum(rowE);
   if (lane == 0) v.partE[slice] = rowE;
   __syncthreads();
+  // Block-Jacobi preconditioner from the finished diagonal block: its inverse, the identity when it is not
+  // positive definite, zero for pinned rows (cpu_solver.cpp:171-178; kernels.h:237-248).
+  {
+    float M[9];
+    if (flags & kRowPinned) {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) M[c] = 0.0f;
+    } else {
+      const uint32_t kd = v.diagSlot[row] - v.rowStart[row];
+      float A[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) A[c] = acc[(kd * 9u + c) * kSlice + lane];
+      if (!invertSpd3(A, M)) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) M[c] = (c % 4 == 0) ? 1.0f : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 9; ++c) v.precond[9u * row + c] = M[c];
+  }
   float* out = v.sellVal + v.sellBase[slice];
   for (uint32_t i = lane; i < K * 9u * kSlice; i += kSlice) out[i] = acc[i];
 }
@@ -312,12 +332,6 @@ __global__ void __launch_bounds__(256) kNewtonBegin(View v) {
 // FP64 dot products. Per iteration two kernels; the search-direction update p = z + beta p is folded into the
 // next SpMV (each lane recomputes it for the columns it gathers), with p double-buffered by iteration parity.
 
-__device__ inline const float* diagBlock(const View& v, uint32_t row) {
-  const uint32_t slice = row / kSlice, lane = row % kSlice;
-  const uint32_t k = v.diagSlot[row] - v.rowStart[row];
-  return v.sellVal + v.sellBase[slice] + k * 9u * kSlice + lane;  // component c at [c * 64]
-}
-
 __global__ void __launch_bounds__(256) kPcgInit(View v) {
   __shared__ double lds[8];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
@@ -331,29 +345,12 @@ __global__ void __launch_bounds__(256) kPcgInit(View v) {
     S->pcgIter = 0u;
   }
   const uint32_t row = tile * kTile + threadIdx.x;
-  const uint32_t flags = v.rowFlags[row];
-  float M[9];
-  if (flags & kRowPinned) {
-#pragma unroll
-    for (int c = 0; c < 9; ++c) M[c] = 0.0f;
-  } else {
-    const float* d = diagBlock(v, row);
-    float A[9];
-#pragma unroll
-    for (int c = 0; c < 9; ++c) A[c] = d[c * kSlice];
-    if (!invertSpd3(A, M)) {
-#pragma unroll
-      for (int c = 0; c < 9; ++c) M[c] = (c % 4 == 0) ? 1.0f : 0.0f;
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < 9; ++c) v.precond[9u * row + c] = M[c];
   const uint32_t cf = v.cflags[row];
   const float4 cn = v.cnormal[row];
   const F3 ri = filterContact(cf, cn, -v.grad[row]);
   M3 Mm;
 #pragma unroll
-  for (int c = 0; c < 9; ++c) Mm.m[c] = M[c];
+  for (int c = 0; c < 9; ++c) Mm.m[c] = v.precond[9u * row + c];
   const F3 zi = filterContact(cf, cn, mul(Mm, ri));
   v.step[row] = f3(0, 0, 0);
   v.r[row] = ri;
@@ -484,6 +481,183 @@ __global__ void __launch_bounds__(256) kPcgUpdate(View v, uint32_t k, const F3* 
     v.partA[tile] = srz;
     v.partB[tile] = srr;
   }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// The same block-Jacobi PCG in Chronopoulos–Gear form: one kernel and one global reduction per iteration.
+// With u = S M r, w = S A u (S the contact-plane filter):
+//   gamma = r.u, delta = w.u, beta = gamma / gamma_prev, p.Ap = delta - beta gamma / alpha_prev, alpha = gamma / p.Ap
+//   p = u + beta p, s = w + beta s (= S A p), step += alpha p, r -= alpha s
+// which are the iterates of cpu_solver.cpp:169-213 in exact arithmetic (same stopping rules: r0 = 0, p.Ap <= 0,
+// |r| <= tol |r0| after an update, at most maxCg updates). Kernel k computes the next u and w = S A u itself:
+// for each gathered column it rebuilds the neighbour's new u from the previous r, w, s and its preconditioner,
+// so no second pass over the rows is needed.
+
+struct CgNext {
+  F3 r, s, u;
+};
+
+// Row j's s_k = w_k + beta s_{k-1}, r_{k+1} = r_k - alpha s_k, u_{k+1} = S M r_{k+1} (identical for the owner
+// and every neighbour that gathers it).
+__device__ inline CgNext cgAdvance(const View& v, const F3* __restrict__ r, const F3* __restrict__ w,
+                                   const F3* __restrict__ sOld, uint32_t j, float alpha, float beta) {
+  CgNext o;
+  o.s = w[j] + beta * sOld[j];
+  o.r = r[j] - alpha * o.s;
+  M3 M;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) M.m[c] = v.precond[9u * j + c];
+  o.u = filterContact(v.cflags[j], v.cnormal[j], mul(M, o.r));
+  return o;
+}
+
+// u0 = S M S(-grad) of row j.
+__device__ inline F3 cgU0(const View& v, uint32_t j) {
+  const uint32_t cf = v.cflags[j];
+  const float4 cn = v.cnormal[j];
+  const F3 r = filterContact(cf, cn, -v.grad[j]);
+  M3 M;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) M.m[c] = v.precond[9u * j + c];
+  return filterContact(cf, cn, mul(M, r));
+}
+
+__device__ inline void cgPartials(double* part, uint32_t tile, F3 r, F3 u, F3 w, double* lds) {
+  const double ru = double(r.x) * u.x + double(r.y) * u.y + double(r.z) * u.z;
+  const double wu = double(w.x) * u.x + double(w.y) * u.y + double(w.z) * u.z;
+  const double rr = double(r.x) * r.x + double(r.y) * r.y + double(r.z) * r.z;
+  const double a = blockSum256(ru, lds);
+  const double b = blockSum256(wu, lds);
+  const double c = blockSum256(rr, lds);
+  if (threadIdx.x == 0) {
+    part[3 * tile] = a;
+    part[3 * tile + 1] = b;
+    part[3 * tile + 2] = c;
+  }
+}
+
+__global__ void __launch_bounds__(256) kCgInit(View v) {
+  __shared__ double lds[8];
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t s = v.tileSeg[tile];
+  SegState* S = v.state + s;
+  if (S->newtonDone) return;
+  const SegParams& P = v.seg[s];
+  if (tile == P.tileBegin && threadIdx.x == 0) {
+    S->pcgConverged = 0u;
+    S->pcgIter = 0u;
+    S->kIter[0] = 0u;
+  }
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t slice = tile * kSlicesPerTile + wave;
+  const uint32_t row = slice * kSlice + lane;
+  const F3 r = filterContact(v.cflags[row], v.cnormal[row], -v.grad[row]);
+  const F3 u = cgU0(v, row);
+  const uint32_t K = v.sellK[slice];
+  const uint32_t base = v.sellBase[slice];
+  const float* __restrict__ val = v.sellVal + base + lane;
+  const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  for (uint32_t kk = 0; kk < K; ++kk) {
+    const uint32_t c = col[kk * kSlice];
+    const F3 uj = cgU0(v, c);
+    const float* bv = val + kk * 9u * kSlice;
+    a0 += bv[0 * kSlice] * uj.x + bv[1 * kSlice] * uj.y + bv[2 * kSlice] * uj.z;
+    a1 += bv[3 * kSlice] * uj.x + bv[4 * kSlice] * uj.y + bv[5 * kSlice] * uj.z;
+    a2 += bv[6 * kSlice] * uj.x + bv[7 * kSlice] * uj.y + bv[8 * kSlice] * uj.z;
+  }
+  const F3 w = filterContact(v.cflags[row], v.cnormal[row],
+                             (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : f3(a0, a1, a2));
+  v.cgR[0][row] = r;
+  v.cgW[0][row] = w;
+  v.cgS[1][row] = f3(0, 0, 0);
+  v.cgU[row] = u;
+  v.pA[row] = f3(0, 0, 0);
+  v.step[row] = f3(0, 0, 0);
+  cgPartials(v.cgPart[0], tile, r, u, w, lds);
+}
+
+__global__ void __launch_bounds__(256) kCgIteration(View v, uint32_t q) {
+  __shared__ double lds[8];
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t s = v.tileSeg[tile];
+  SegState* S = v.state + s;
+  const SegParams& P = v.seg[s];
+  const double* part = v.cgPart[q];
+  double ru = 0.0, wu = 0.0, rr = 0.0;
+  for (uint32_t t = P.tileBegin + threadIdx.x; t < P.tileEnd; t += 256u) {
+    ru += part[3 * t];
+    wu += part[3 * t + 1];
+    rr += part[3 * t + 2];
+  }
+  const double gamma = blockSum256(ru, lds);
+  const double delta = blockSum256(wu, lds);
+  const double rsum = blockSum256(rr, lds);
+  if (S->newtonDone) return;
+  const uint32_t k = S->kIter[q];
+  uint32_t conv = S->pcgConverged;
+  double beta = 0.0, pAp = delta, r0 = 0.0;
+  if (!conv) {
+    if (k == 0) {
+      r0 = sqrt(rsum);
+      if (!(r0 > 0.0)) conv = 1u;
+    } else if (sqrt(rsum) <= P.cgTolerance * S->r0) {
+      conv = 1u;
+    } else {
+      beta = gamma / S->gam[1u - q];
+      pAp = delta - beta * gamma / S->alph[1u - q];
+    }
+    if (!conv && k >= P.maxCg) conv = 1u;
+    if (!conv && !(pAp > 0.0)) conv = 1u;
+  }
+  const double alpha = gamma / pAp;
+  if (tile == P.tileBegin && threadIdx.x == 0) {
+    if (k == 0) S->r0 = r0;
+    S->kIter[1u - q] = k + 1u;
+    S->gam[q] = gamma;
+    S->alph[q] = alpha;
+    S->rr = rsum;
+    if (conv) {
+      S->pcgConverged = 1u;
+    } else {
+      S->pcgIter += 1u;
+      S->cntPcg += 1u;
+    }
+  }
+  if (conv) return;
+  const float a = float(alpha), b = float(beta);
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t slice = tile * kSlicesPerTile + wave;
+  const uint32_t row = slice * kSlice + lane;
+  const F3* __restrict__ rCur = v.cgR[q];
+  const F3* __restrict__ wCur = v.cgW[q];
+  const F3* __restrict__ sOld = v.cgS[1u - q];
+  const CgNext me = cgAdvance(v, rCur, wCur, sOld, row, a, b);
+  const F3 p = v.cgU[row] + b * v.pA[row];
+  v.pA[row] = p;
+  v.step[row] = v.step[row] + a * p;
+  v.cgS[q][row] = me.s;
+  v.cgR[1u - q][row] = me.r;
+  v.cgU[row] = me.u;
+  const uint32_t K = v.sellK[slice];
+  const uint32_t base = v.sellBase[slice];
+  const float* __restrict__ val = v.sellVal + base + lane;
+  const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  for (uint32_t kk = 0; kk < K; ++kk) {
+    const uint32_t c = col[kk * kSlice];
+    const F3 uj = cgAdvance(v, rCur, wCur, sOld, c, a, b).u;
+    const float* bv = val + kk * 9u * kSlice;
+    a0 += bv[0 * kSlice] * uj.x + bv[1 * kSlice] * uj.y + bv[2 * kSlice] * uj.z;
+    a1 += bv[3 * kSlice] * uj.x + bv[4 * kSlice] * uj.y + bv[5 * kSlice] * uj.z;
+    a2 += bv[6 * kSlice] * uj.x + bv[7 * kSlice] * uj.y + bv[8 * kSlice] * uj.z;
+  }
+  const F3 w = filterContact(v.cflags[row], v.cnormal[row],
+                             (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : f3(a0, a1, a2));
+  v.cgW[1u - q][row] = w;
+  cgPartials(v.cgPart[1u - q], tile, me.r, me.u, w, lds);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -976,11 +1150,21 @@ void launchAssemble(const View& v, uint32_t maxK, bool, hipStream_t st) {
 }
 void launchNewtonBegin(const View& v, hipStream_t st) { kNewtonBegin<<<v.nSegments, 256, 0, st>>>(v); }
 void launchPcgInit(const View& v, hipStream_t st) { kPcgInit<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
-void launchPcgIteration(const View& v, uint32_t k, hipStream_t st) {
+void launchPcgSpmv(const View& v, uint32_t k, hipStream_t st) {
   F3* pOld = (k & 1u) ? v.pB : v.pA;
   F3* pNew = (k & 1u) ? v.pA : v.pB;
   kPcgSpmv<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, k, pOld, pNew);
-  kPcgUpdate<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, k, pNew);
+}
+void launchPcgUpdate(const View& v, uint32_t k, hipStream_t st) {
+  kPcgUpdate<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, k, (k & 1u) ? v.pA : v.pB);
+}
+void launchPcgIteration(const View& v, uint32_t k, hipStream_t st) {
+  launchPcgSpmv(v, k, st);
+  launchPcgUpdate(v, k, st);
+}
+void launchCgInit(const View& v, hipStream_t st) { kCgInit<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
+void launchCgIteration(const View& v, uint32_t parity, hipStream_t st) {
+  kCgIteration<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, parity & 1u);
 }
 void launchContactForceSlope(const View& v, hipStream_t st) {
   kContactForceSlope<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);

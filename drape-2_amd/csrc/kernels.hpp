@@ -34,6 +34,13 @@ struct View {
   F3* pB;
   F3* ap;
   F3* warm;      // learned warm-start velocity change per row (zero when unused)
+  // Single-kernel PCG: r, w = S A u and s = S A p double-buffered by kernel parity (neighbours read the old
+  // copy while the owner writes the new one), u = S M r per row, and FP64 partials (r.u, w.u, r.r) per parity.
+  F3* cgR[2];
+  F3* cgW[2];
+  F3* cgS[2];
+  F3* cgU;
+  double* cgPart[2];  // 3 per tile: r.u, w.u, r.r
   float* precond;  // 9 per row, row-major
   float4* cnormal; // xyz normal, w = offset r - phi(xStart)
   uint32_t* cflags;
@@ -100,6 +107,10 @@ void launchAssemble(const View& v, uint32_t maxK, bool contact, hipStream_t st);
 void launchNewtonBegin(const View& v, hipStream_t st);
 void launchPcgInit(const View& v, hipStream_t st);
 void launchPcgIteration(const View& v, uint32_t k, hipStream_t st);
+void launchPcgSpmv(const View& v, uint32_t k, hipStream_t st);
+void launchPcgUpdate(const View& v, uint32_t k, hipStream_t st);
+void launchCgInit(const View& v, hipStream_t st);
+void launchCgIteration(const View& v, uint32_t parity, hipStream_t st);
 void launchContactForceSlope(const View& v, hipStream_t st);
 void launchNewtonSlope(const View& v, hipStream_t st);
 void launchEnergyTrial(const View& v, hipStream_t st);

@@ -123,6 +123,11 @@ struct SegState {
   uint32_t lsTrial;
   uint32_t active;        // 1 while this segment runs Newton iterations this substep
   uint32_t warmRejected;  // 1 when the learned warm start was rejected this substep
+  // Single-kernel PCG (Chronopoulos-Gear form): iteration index, r.u and alpha by kernel parity. A kernel of
+  // parity q reads slot 1-q (written by the previous kernel) and writes slot q; kIter[q] is read, kIter[1-q]
+  // written, so no workgroup ever reads a value another workgroup of the same launch writes.
+  uint32_t kIter[2];
+  double gam[2], alph[2];
   uint64_t cntNewton, cntPcg, cntLs, cntSubsteps, cntWarmRejected, cntWarmAccepted;
 };
 

@@ -76,6 +76,15 @@ HipSolver::HipSolver() {
 
 HipSolver::~HipSolver() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  releaseGraph();
+  for (auto& pr : eventPool_) {
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
+  for (auto& pr : eventPending_) {
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
   if (hostStates_) (void)hipHostFree(hostStates_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -94,6 +103,7 @@ static void upload(DeviceBuffer& b, const std::vector<T>& data, size_t count = 0
 void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   Model m = Model::build(scenes, nScenes);
   check(hipStreamSynchronize(stream_), "sync");
+  releaseGraph();
   model_ = std::move(m);
   built_ = false;
   lastH_ = -1.0;
@@ -128,9 +138,11 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   for (DeviceBuffer* b : {&x_, &v_, &xStart_, &yRel_, &u_, &grad_, &step_, &r_, &z_, &pA_, &pB_, &ap_, &warm_,
                           &scratchF3_, &scratchF3b_})
     b->allocate(vecBytes);
-  precond_.allocate(sizeof(float) * 9 * nRows_);
-  cnormal_.allocate(sizeof(float4) * nRows_);
-  cflags_.allocate(sizeof(uint32_t) * nRows_);
+  // Row arrays a gathered column may index hold the zero dummy entry nRows as well.
+  for (DeviceBuffer* b : {&cgR0_, &cgR1_, &cgW0_, &cgW1_, &cgS0_, &cgS1_, &cgU_}) b->allocate(vecBytes);
+  precond_.allocate(sizeof(float) * 9 * (nRows_ + 1));
+  cnormal_.allocate(sizeof(float4) * (nRows_ + 1));
+  cflags_.allocate(sizeof(uint32_t) * (nRows_ + 1));
   cforce_.allocate(sizeof(float) * nRows_);
   clambda_.allocate(sizeof(float) * nRows_);
 
@@ -190,6 +202,8 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   partC_.allocate(sizeof(double) * nTiles_);
   partE_.allocate(sizeof(double) * nSlices_);
   partMax_.allocate(sizeof(float) * nTiles_);
+  cgPart0_.allocate(sizeof(double) * 3 * nTiles_);
+  cgPart1_.allocate(sizeof(double) * 3 * nTiles_);
 
   // Terms in internal row indices.
   std::vector<dk::Stretch32> st(M.stretch.size());
@@ -293,6 +307,15 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   v.pB = pB_.as<dk::F3>();
   v.ap = ap_.as<dk::F3>();
   v.warm = warm_.as<dk::F3>();
+  v.cgR[0] = cgR0_.as<dk::F3>();
+  v.cgR[1] = cgR1_.as<dk::F3>();
+  v.cgW[0] = cgW0_.as<dk::F3>();
+  v.cgW[1] = cgW1_.as<dk::F3>();
+  v.cgS[0] = cgS0_.as<dk::F3>();
+  v.cgS[1] = cgS1_.as<dk::F3>();
+  v.cgU = cgU_.as<dk::F3>();
+  v.cgPart[0] = cgPart0_.as<double>();
+  v.cgPart[1] = cgPart1_.as<double>();
   v.precond = precond_.as<float>();
   v.cnormal = cnormal_.as<float4>();
   v.cflags = cflags_.as<uint32_t>();
@@ -428,6 +451,38 @@ void HipSolver::pollStates() const {
                        hipMemcpyDeviceToHost, stream_),
         "copy");
   check(hipStreamSynchronize(stream_), "step");
+  harvestEvents();
+}
+
+void HipSolver::setSpmvSampling(int every) {
+  spmvEvery_ = std::max(0, every);
+  spmvTimed_ = 0;
+  spmvTotalMs_ = 0.0;
+}
+
+// Reads the elapsed times of completed event pairs (the stream was synchronised) and recycles them.
+void HipSolver::harvestEvents() const {
+  auto* self = const_cast<HipSolver*>(this);
+  for (auto& pr : self->eventPending_) {
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, pr.first, pr.second), "event");
+    spmvTotalMs_ += ms;
+    ++spmvTimed_;
+    self->eventPool_.push_back(pr);
+  }
+  self->eventPending_.clear();
+}
+
+void HipSolver::spmvTiming(uint64_t& launches, double& meanUs, double& bytesPerLaunch) const {
+  launches = spmvTimed_;
+  meanUs = spmvTimed_ ? 1000.0 * spmvTotalMs_ / double(spmvTimed_) : 0.0;
+  // Algorithmic bytes of one fused PCG SpMV launch: block values and column index (40 B per block), and per
+  // particle the row and contact flags (8 B), z and p_old gathered once (24 B), p_new and ap written (24 B).
+  const double n = double(model_.x.size());
+  const double blocks = double(nBlocks_) - (double(nRows_) - n);
+  // The single-kernel iteration also streams, per particle, r, w, s_old, u, p, step (72 B read), the
+  // preconditioner (36 B) and writes s, r, u, p, step, w (72 B): 40 B per block + 204 B per particle.
+  bytesPerLaunch = 40.0 * blocks + (pcgVariant_ == 1 ? 204.0 : 56.0) * n;
 }
 
 bool HipSolver::allSegments(bool (*pred)(const dk::SegState&)) const {
@@ -436,21 +491,117 @@ bool HipSolver::allSegments(bool (*pred)(const dk::SegState&)) const {
   return true;
 }
 
+void HipSolver::releaseGraph() {
+  if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graphExec_ = nullptr;
+  graph_ = nullptr;
+}
+
+void HipSolver::setPcgVariant(int variant) {
+  if (variant != 0 && variant != 1) throw std::invalid_argument("pcg variant must be 0 or 1");
+  pcgVariant_ = variant;
+}
+
 void HipSolver::runPcg() {
-  dk::launchPcgInit(view_, stream_);
+  if (pcgVariant_ == 1)
+    runPcgSingleKernel();
+  else
+    runPcgStandard();
+}
+
+static uint32_t maxCgOf(const Model& m) {
   uint32_t maxCg = 0;
-  for (const auto& sg : model_.segments) maxCg = std::max<uint32_t>(maxCg, static_cast<uint32_t>(std::max(0, sg.settings.maxCg)));
+  for (const auto& sg : m.segments) maxCg = std::max<uint32_t>(maxCg, static_cast<uint32_t>(std::max(0, sg.settings.maxCg)));
+  return maxCg;
+}
+
+void HipSolver::runPcgStandard() {
+  dk::launchPcgInit(view_, stream_);
+  const uint32_t maxCg = maxCgOf(model_);
   // First chunk sized from the previous solve; later chunks short. Iterations past convergence are no-ops.
   uint32_t k = 0;
   uint32_t chunk = std::max<uint32_t>(8, lastPcgIterations_ * 3 / 4);
-  uint32_t total = 0;
   while (k < maxCg) {
     const uint32_t end = std::min(maxCg, k + chunk);
-    for (; k < end; ++k) dk::launchPcgIteration(view_, k, stream_);
+    for (; k < end; ++k) {
+      if (spmvEvery_ > 0 && k % uint32_t(spmvEvery_) == 0) {
+        if (eventPool_.empty()) {
+          hipEvent_t a, b;
+          check(hipEventCreate(&a), "event");
+          check(hipEventCreate(&b), "event");
+          eventPool_.push_back({a, b});
+        }
+        const auto pr = eventPool_.back();
+        eventPool_.pop_back();
+        check(hipEventRecord(pr.first, stream_), "event");
+        dk::launchPcgSpmv(view_, k, stream_);
+        check(hipEventRecord(pr.second, stream_), "event");
+        eventPending_.push_back(pr);
+        dk::launchPcgUpdate(view_, k, stream_);
+      } else {
+        dk::launchPcgIteration(view_, k, stream_);
+      }
+    }
     pollStates();
     if (allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; })) break;
     chunk = 16;
   }
+  uint32_t total = 0;
+  for (size_t s = 0; s < model_.segments.size(); ++s) total = std::max(total, hostStates_[s].pcgIter);
+  lastPcgIterations_ = total;
+}
+
+void HipSolver::timedCgIteration(uint32_t parity) {
+  if (eventPool_.empty()) {
+    hipEvent_t a, b;
+    check(hipEventCreate(&a), "event");
+    check(hipEventCreate(&b), "event");
+    eventPool_.push_back({a, b});
+  }
+  const auto pr = eventPool_.back();
+  eventPool_.pop_back();
+  check(hipEventRecord(pr.first, stream_), "event");
+  dk::launchCgIteration(view_, parity, stream_);
+  check(hipEventRecord(pr.second, stream_), "event");
+  eventPending_.push_back(pr);
+}
+
+// Chunks of kGraphChunk single-kernel iterations, replayed from a captured hipGraph (or launched eagerly). The
+// iteration index lives on the device, so one graph serves every chunk; iterations after convergence return at
+// once. The host reads the converged flags after the first batch (sized from the previous solve) and then after
+// every chunk.
+void HipSolver::runPcgSingleKernel() {
+  dk::launchCgInit(view_, stream_);
+  const uint32_t maxCg = maxCgOf(model_);
+  const bool sampling = spmvEvery_ > 0;
+  if (useGraph_ && !sampling && !graphExec_) {
+    check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
+    for (uint32_t i = 0; i < kGraphChunk; ++i) dk::launchCgIteration(view_, i & 1u, stream_);
+    check(hipStreamEndCapture(stream_, &graph_), "capture");
+    check(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0), "graph instantiate");
+  }
+  uint32_t launched = 0;
+  uint32_t chunks = std::max<uint32_t>(1, (lastPcgIterations_ * 7 / 8) / kGraphChunk);
+  while (launched < maxCg + 1) {
+    for (uint32_t c = 0; c < chunks; ++c) {
+      if (useGraph_ && !sampling) {
+        check(hipGraphLaunch(graphExec_, stream_), "graph launch");
+      } else {
+        for (uint32_t i = 0; i < kGraphChunk; ++i) {
+          if (sampling && (launched + i) % uint32_t(spmvEvery_) == 0)
+            timedCgIteration(i & 1u);
+          else
+            dk::launchCgIteration(view_, i & 1u, stream_);
+        }
+      }
+      launched += kGraphChunk;
+    }
+    pollStates();
+    if (allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; })) break;
+    chunks = 1;
+  }
+  uint32_t total = 0;
   for (size_t s = 0; s < model_.segments.size(); ++s) total = std::max(total, hostStates_[s].pcgIter);
   lastPcgIterations_ = total;
 }

@@ -60,6 +60,11 @@ class HipSolver {
   drape_counters counters() const;
   void resetCounters();
   void setGraph(bool enabled) { useGraph_ = enabled; }
+  // 1: single-kernel PCG (Chronopoulos-Gear form, default); 0: two kernels per iteration (standard form).
+  void setPcgVariant(int variant);
+  // Every `every`-th PCG SpMV launch of the step is bracketed by HIP events (0 = off).
+  void setSpmvSampling(int every);
+  void spmvTiming(uint64_t& launches, double& meanUs, double& bytesPerLaunch) const;
 
   drape_structure_sizes structureSizes() const;
   void readStructure(uint32_t* rowStart, uint32_t* colIndex, uint32_t* diagSlot, uint32_t* stretchSlots,
@@ -81,6 +86,10 @@ class HipSolver {
   bool allSegments(bool (*pred)(const dk::SegState&)) const;
   void substep(double h, int iterations);
   void runPcg();
+  void runPcgStandard();
+  void runPcgSingleKernel();
+  void releaseGraph();
+  void timedCgIteration(uint32_t parity);
   void globalStructure(std::vector<uint32_t>& rowStart, std::vector<uint32_t>& colIndex,
                        std::vector<uint32_t>& diagSlot, std::vector<uint32_t>& internalToGlobalSlot) const;
 
@@ -104,7 +113,17 @@ class HipSolver {
   DeviceBuffer incCount_, incStart_, inc_, rowLen_, rowStart_, colIndex_, diagSlot_;
   DeviceBuffer stretchSlots_, hingeSlots_, seamSlots_, sellK_, sellBase_, sellCol_, sellVal_, buildError_;
   DeviceBuffer fieldValues_, scratchF3_, scratchF3b_;
+  DeviceBuffer cgR0_, cgR1_, cgW0_, cgW1_, cgS0_, cgS1_, cgU_, cgPart0_, cgPart1_;
+  int pcgVariant_ = 1;
+  static constexpr uint32_t kGraphChunk = 16;  // PCG iterations per captured graph (even: parity pattern repeats)
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graphExec_ = nullptr;
   dk::SegState* hostStates_ = nullptr;  // pinned host copy
+  int spmvEvery_ = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> eventPool_, eventPending_;
+  mutable uint64_t spmvTimed_ = 0;
+  mutable double spmvTotalMs_ = 0.0;
+  void harvestEvents() const;
 };
 
 }  // namespace dh

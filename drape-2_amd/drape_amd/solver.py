@@ -46,12 +46,14 @@ class HipSolver:
 
     backend = "hip"
 
-    def __init__(self, graph: bool = True):
+    def __init__(self, graph: bool = True, pcg_variant: int = 1):
         self._h = C.c_void_p()
         _check(_lib().drape_solver_create(C.byref(self._h)))
         self._n = 0
         if not graph:
             self.set_graph(False)
+        if pcg_variant != 1:
+            self.set_pcg_variant(pcg_variant)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -141,6 +143,18 @@ class HipSolver:
 
     def set_graph(self, enabled: bool) -> None:
         _check(_lib().drape_solver_set_graph(self._h, 1 if enabled else 0))
+
+    def set_pcg_variant(self, variant: int) -> None:
+        """1: single-kernel PCG (default); 0: the two-kernel standard form."""
+        _check(_lib().drape_solver_set_pcg_variant(self._h, int(variant)))
+
+    def set_spmv_sampling(self, every: int) -> None:
+        _check(_lib().drape_solver_set_spmv_sampling(self._h, int(every)))
+
+    def spmv_timing(self) -> dict:
+        n, us, b = C.c_uint64(), C.c_double(), C.c_double()
+        _check(_lib().drape_solver_spmv_timing(self._h, C.byref(n), C.byref(us), C.byref(b)))
+        return {"launches": n.value, "mean_us": us.value, "bytes_per_launch": b.value}
 
     # ---- structure / kernel access ----
 

@@ -189,6 +189,18 @@ int drape_solver_reset_counters(drape_solver* s);
  * launch eagerly (0). Results are bit-identical either way. */
 int drape_solver_set_graph(drape_solver* s, int32_t enabled);
 
+/* PCG form: 1 (default) one kernel and one global reduction per iteration (Chronopoulos-Gear form of the same
+ * preconditioned CG: identical iterates in exact arithmetic, same stopping rules); 0 the two-kernel standard form
+ * (cpu_solver.cpp:169-213 step by step). */
+int drape_solver_set_pcg_variant(drape_solver* s, int32_t variant);
+
+/* Kernel timing inside the step: every `every`-th PCG SpMV launch (0 = off) is bracketed by HIP events on the
+ * solver's stream. drape_solver_spmv_timing reports how many launches were timed since sampling was set, their
+ * mean duration in microseconds and the algorithmic bytes one launch moves (40 B per block for values and column
+ * index, plus 56 B per particle: row and contact flags, z and p_old gathered, p_new and ap written). */
+int drape_solver_set_spmv_sampling(drape_solver* s, int32_t every);
+int drape_solver_spmv_timing(const drape_solver* s, uint64_t* launches, double* mean_us, double* bytes_per_launch);
+
 /* ---- Structure and kernel access (parity tests and benchmarks; same kernels as the step) ---- */
 
 /* Block-CSR pattern and term slot tables built on the device (sim_model.cpp:41-74). Sizes first (any pointer may

@@ -1,0 +1,54 @@
+"""Batched uneven solves: independent scenes stepped together, one segment each.
+
+Each segment keeps its own line search and convergence (the reference decides these per scene,
+cpu_solver.cpp:264-284), so a batched segment reproduces its scene's single-scene HIP run bit for bit, and
+stays within the single-scene tolerance of the oracle.
+"""
+import numpy as np
+import pytest
+
+from conftest import max_distance
+from drape_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batched_segments_match_single_scene_runs_bit_for_bit(hip, oracle):
+    batch = scenes.c4_batch(count=4, lo=300, hi=3000, tolerance=1e-6, max_cg=1000)
+    g = hip.HipSolver()
+    g.build_batch(batch)
+    assert g.segment_count() == 4
+    steps = 12
+    for _ in range(steps):
+        g.step(1 / 60, 1, 5)
+    xb = g.read_positions()
+    for k, sc in enumerate(batch):
+        off, cnt = g.segment_range(k)
+        assert cnt == sc.n
+        single = hip.HipSolver()
+        single.build(sc)
+        o = oracle.OracleSolver()
+        o.build(sc)
+        for _ in range(steps):
+            single.step(1 / 60, 1, 5)
+            o.step(1 / 60, 1, 5)
+        print(f"segment {k} ({sc.name}, {sc.n} particles): oracle distance "
+              f"{max_distance(xb[off:off + cnt], o.read_positions()) * 1e3:.4f} mm", flush=True)
+        np.testing.assert_array_equal(xb[off:off + cnt], single.read_positions())
+        assert max_distance(xb[off:off + cnt], o.read_positions()) <= 1e-4
+
+
+def test_batch_counters_sum_segments(hip):
+    batch = scenes.c4_batch(count=3, lo=300, hi=1500, tolerance=1e-6, max_cg=1000)
+    g = hip.HipSolver()
+    g.build_batch(batch)
+    g.step(1 / 60, 1, 5)
+    c = g.counters()
+    assert c["substeps"] == 3
+    singles = 0
+    for sc in batch:
+        s = hip.HipSolver()
+        s.build(sc)
+        s.step(1 / 60, 1, 5)
+        singles += s.counters()["pcg_iterations"]
+    assert c["pcg_iterations"] == singles

@@ -226,10 +226,11 @@ def test_bsr_spmv_matches_dense(hip):
     assert np.linalg.norm(y.reshape(-1) - expect) <= 1e-5 * np.linalg.norm(expect)
 
 
-def test_c1_sheet_60_steps_within_0p1mm_of_oracle(hip, oracle):
+@pytest.mark.parametrize("variant,graph", [(1, True), (0, False)], ids=["single-kernel-graph", "two-kernel-eager"])
+def test_c1_sheet_60_steps_within_0p1mm_of_oracle(hip, oracle, variant, graph):
     """Config 1 (FP64 oracle vs FP32 HIP with FP64 reductions): max particle distance after 60 steps."""
     sc = scenes.c1_sheet()
-    g = hip.HipSolver()
+    g = hip.HipSolver(graph=graph, pcg_variant=variant)
     g.build(sc)
     o = oracle.OracleSolver()
     o.build(sc)
@@ -242,6 +243,20 @@ def test_c1_sheet_60_steps_within_0p1mm_of_oracle(hip, oracle):
     assert not gs["has_non_finite"]
     assert d <= 1e-4
     assert gs["max_body_penetration"] <= max(2e-3, os_["max_body_penetration"] + 1e-4)
+
+
+def test_graph_replay_matches_eager_launches_bit_for_bit(hip):
+    """The captured PCG chunks run the same kernels in the same order as eager launches."""
+    sc = scenes.c1_sheet()
+    out = []
+    for graph in (True, False):
+        g = hip.HipSolver(graph=graph)
+        g.build(sc)
+        for _ in range(8):
+            g.step(1 / 60, 1, 5)
+        out.append((g.read_positions(), g.counters()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
 
 
 def test_two_runs_bit_identical_and_snapshot_replay(hip):

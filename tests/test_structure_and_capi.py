@@ -128,3 +128,11 @@ def test_errors_map_to_reference_exception_types():
     assert lib.drape_last_error().startswith(b"hip:")
     with pytest.raises(RuntimeError, match="hip"):
         drape_amd.HipSolver()
+
+
+def test_cpp_wrapper_header_compiles():
+    """include/drape/sim/hip_solver.hpp (the C++ Solver-surface wrapper) compiles with the host compiler."""
+    src = '#include "drape/sim/hip_solver.hpp"\nint main() { return drape::sim::hip::isAvailable() ? 0 : 1; }\n'
+    p = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(REPO, "include"), "-x", "c++", "-"],
+                       input=src, capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
