@@ -63,7 +63,10 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256, help="sheet vertices per side (256 = the 65,536-particle sheet)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sample-every", type=int, default=8, help="time every k-th PCG SpMV launch with HIP events")
+    ap.add_argument("--sample-every", type=int, default=8,
+                    help="event-time PCG iteration launches (graph path: one per poll; eager: every k-th)")
+    ap.add_argument("--pcg-variant", type=int, default=1, help="1 single-kernel PCG (default), 0 two-kernel")
+    ap.add_argument("--no-graph", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
@@ -76,7 +79,7 @@ def main() -> int:
         return scenes.c2_sheet(n=args.n)
 
     sc = make_scene()
-    s = HipSolver()
+    s = HipSolver(graph=not args.no_graph, pcg_variant=args.pcg_variant)
     s.build(sc)
     s.set_spmv_sampling(args.sample_every)
     for _ in range(args.warmup):
@@ -111,13 +114,17 @@ def main() -> int:
             "particles": sc.n,
             "dt": 1 / 60, "substeps": 1, "newton_max": 5, "pcg_tolerance": 1e-6,
             "solver": "plain warm start (u = h v)",
+            "pcg_form": "single-kernel" if args.pcg_variant == 1 else "two-kernel",
+            "graph": not args.no_graph,
             "newton_per_step": c["newton_iterations"] / args.steps,
             "pcg_per_step": c["pcg_iterations"] / args.steps,
             "line_search_trials_per_step": c["line_search_trials"] / args.steps,
             "final_stats": st,
         },
         "roofline": {
-            "kernel": "kPcgSpmv (fused p = z + beta p, SELL-64 block SpMV, contact filter, FP64 p.Ap partials)",
+            "kernel": ("kCgIteration (single-kernel PCG iteration: vector updates, SELL-64 block SpMV with on-the-fly "
+                       "neighbour preconditioning, contact filter, FP64 partials)") if args.pcg_variant == 1 else
+                      "kPcgSpmv (fused p = z + beta p, SELL-64 block SpMV, contact filter, FP64 p.Ap partials)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

@@ -81,6 +81,57 @@ __device__ inline double waveSum(double v) {
   return v;  // valid in lane 0
 }
 
+// Three sums over a 256-thread workgroup in one pass (fixed order), returned to every thread. lds >= 16 doubles.
+__device__ inline void blockSum3x256(double& a, double& b, double& c, double* lds) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+    c += __shfl_down(c, off, 64);
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) {
+    lds[w] = a;
+    lds[4 + w] = b;
+    lds[8 + w] = c;
+  }
+  __syncthreads();
+  a = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+  b = ((lds[4] + lds[5]) + lds[6]) + lds[7];
+  c = ((lds[8] + lds[9]) + lds[10]) + lds[11];
+}
+
+// Row product of a SELL-64 slice with a gathered vector: sum_k B(row, k) x(col(row, k)). `gather(c)` returns x_c
+// (the dummy column nRows gives zero). Columns are taken eight at a time: the eight index loads, then the eight
+// gathers, then the block products, so a row costs two or three memory round trips instead of one per block.
+template <class Gather>
+__device__ inline F3 sellRowProduct(const View& v, uint32_t slice, uint32_t lane, Gather gather) {
+  const uint32_t K = v.sellK[slice];
+  const uint32_t base = v.sellBase[slice];
+  const float* __restrict__ val = v.sellVal + base + lane;
+  const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  for (uint32_t k0 = 0; k0 < K; k0 += 8u) {
+    uint32_t c[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) c[t] = (k0 + t < K) ? col[(k0 + t) * kSlice] : v.nRows;
+    F3 x[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) x[t] = gather(c[t]);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      if (k0 + t < K) {
+        const float* bv = val + (k0 + t) * 9u * kSlice;
+        a0 += bv[0 * kSlice] * x[t].x + bv[1 * kSlice] * x[t].y + bv[2 * kSlice] * x[t].z;
+        a1 += bv[3 * kSlice] * x[t].x + bv[4 * kSlice] * x[t].y + bv[5 * kSlice] * x[t].z;
+        a2 += bv[6 * kSlice] * x[t].x + bv[7 * kSlice] * x[t].y + bv[8 * kSlice] * x[t].z;
+      }
+    }
+  }
+  return f3(a0, a1, a2);
+}
+
 __device__ inline float contactRadius(const SegParams& P, const Fabric32& f) {
   return P.collisionOffset + 0.5f * f.thickness;
 }
@@ -415,22 +466,10 @@ __global__ void __launch_bounds__(256) kPcgSpmv(View v, uint32_t k, const F3* __
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   const uint32_t slice = tile * kSlicesPerTile + wave;
   const uint32_t row = slice * kSlice + lane;
-  const uint32_t K = v.sellK[slice];
-  const uint32_t base = v.sellBase[slice];
-  const float* __restrict__ val = v.sellVal + base + lane;
-  const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
   const F3* __restrict__ z = v.z;
-  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-  for (uint32_t kk = 0; kk < K; ++kk) {
-    const uint32_t c = col[kk * kSlice];
-    const F3 pj = z[c] + b * pOld[c];
-    const float* bv = val + kk * 9u * kSlice;
-    a0 += bv[0 * kSlice] * pj.x + bv[1 * kSlice] * pj.y + bv[2 * kSlice] * pj.z;
-    a1 += bv[3 * kSlice] * pj.x + bv[4 * kSlice] * pj.y + bv[5 * kSlice] * pj.z;
-    a2 += bv[6 * kSlice] * pj.x + bv[7 * kSlice] * pj.y + bv[8 * kSlice] * pj.z;
-  }
+  const F3 prod = sellRowProduct(v, slice, lane, [&](uint32_t c) { return z[c] + b * pOld[c]; });
   const uint32_t flags = v.rowFlags[row];
-  F3 acc = (flags & kRowPinned) ? f3(0, 0, 0) : f3(a0, a1, a2);
+  F3 acc = (flags & kRowPinned) ? f3(0, 0, 0) : prod;
   const F3 pi = z[row] + b * pOld[row];
   pNew[row] = pi;
   const F3 api = filterContact(v.cflags[row], v.cnormal[row], acc);
@@ -484,60 +523,60 @@ __global__ void __launch_bounds__(256) kPcgUpdate(View v, uint32_t k, const F3* 
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// The same block-Jacobi PCG in Chronopoulos–Gear form: one kernel and one global reduction per iteration.
-// With u = S M r, w = S A u (S the contact-plane filter):
+// The same block-Jacobi PCG in Chronopoulos–Gear form with preconditioned recurrences: one kernel and one global
+// reduction per iteration. With u = S M r, w = S A u, y = S M w, s = S A p, q = S M s (S the contact-plane filter):
 //   gamma = r.u, delta = w.u, beta = gamma / gamma_prev, p.Ap = delta - beta gamma / alpha_prev, alpha = gamma / p.Ap
-//   p = u + beta p, s = w + beta s (= S A p), step += alpha p, r -= alpha s
+//   p = u + beta p, s = w + beta s, q = y + beta q, step += alpha p, r -= alpha s, u -= alpha q
 // which are the iterates of cpu_solver.cpp:169-213 in exact arithmetic (same stopping rules: r0 = 0, p.Ap <= 0,
-// |r| <= tol |r0| after an update, at most maxCg updates). Kernel k computes the next u and w = S A u itself:
-// for each gathered column it rebuilds the neighbour's new u from the previous r, w, s and its preconditioner,
-// so no second pass over the rows is needed.
+// |r| <= tol |r0| after an update, at most maxCg updates). Kernel k computes the next w = S A u itself: each
+// gathered column's new u is rebuilt from the neighbour's data, so no second pass over the rows is needed. In FP32
+// the carried u and q drift from S M r and S M s; a direct iteration, which rebuilds them from r and s, opens
+// every chunk of iterations (residual replacement).
 
-struct CgNext {
-  F3 r, s, u;
-};
-
-// Row j's s_k = w_k + beta s_{k-1}, r_{k+1} = r_k - alpha s_k, u_{k+1} = S M r_{k+1} (identical for the owner
-// and every neighbour that gathers it).
-__device__ inline CgNext cgAdvance(const View& v, const F3* __restrict__ r, const F3* __restrict__ w,
-                                   const F3* __restrict__ sOld, uint32_t j, float alpha, float beta) {
-  CgNext o;
-  o.s = w[j] + beta * sOld[j];
-  o.r = r[j] - alpha * o.s;
+__device__ inline F3 precondApply(const View& v, uint32_t row, F3 x) {
   M3 M;
 #pragma unroll
-  for (int c = 0; c < 9; ++c) M.m[c] = v.precond[9u * j + c];
-  o.u = filterContact(v.cflags[j], v.cnormal[j], mul(M, o.r));
-  return o;
+  for (int c = 0; c < 9; ++c) M.m[c] = v.precond[9u * row + c];
+  return filterContact(v.cflags[row], v.cnormal[row], mul(M, x));
 }
 
-// u0 = S M S(-grad) of row j.
-__device__ inline F3 cgU0(const View& v, uint32_t j) {
-  const uint32_t cf = v.cflags[j];
-  const float4 cn = v.cnormal[j];
-  const F3 r = filterContact(cf, cn, -v.grad[j]);
-  M3 M;
-#pragma unroll
-  for (int c = 0; c < 9; ++c) M.m[c] = v.precond[9u * j + c];
-  return filterContact(cf, cn, mul(M, r));
+// The iteration's inputs (slot q) and outputs (slot 1-q, or q for s and q) of parity q.
+struct CgSlots {
+  const F3* __restrict__ r;
+  const F3* __restrict__ w;
+  const F3* __restrict__ u;
+  const F3* __restrict__ y;
+  const F3* __restrict__ sOld;
+  const F3* __restrict__ qOld;
+};
+
+// Row j's u_{k+1}. Pipelined: u_k - alpha (y_k + beta q_{k-1}). Direct (residual replacement):
+// S M (r_k - alpha (w_k + beta s_{k-1})). The owner and every neighbour gathering row j evaluate the same
+// expression, so all of them see the same value.
+template <bool Direct>
+__device__ inline F3 cgNextU(const View& v, const CgSlots& in, uint32_t j, float alpha, float beta) {
+  if (Direct) {
+    const F3 s = in.w[j] + beta * in.sOld[j];
+    return precondApply(v, j, in.r[j] - alpha * s);
+  }
+  const F3 q = in.y[j] + beta * in.qOld[j];
+  return in.u[j] - alpha * q;
 }
 
 __device__ inline void cgPartials(double* part, uint32_t tile, F3 r, F3 u, F3 w, double* lds) {
-  const double ru = double(r.x) * u.x + double(r.y) * u.y + double(r.z) * u.z;
-  const double wu = double(w.x) * u.x + double(w.y) * u.y + double(w.z) * u.z;
-  const double rr = double(r.x) * r.x + double(r.y) * r.y + double(r.z) * r.z;
-  const double a = blockSum256(ru, lds);
-  const double b = blockSum256(wu, lds);
-  const double c = blockSum256(rr, lds);
+  double ru = double(r.x) * u.x + double(r.y) * u.y + double(r.z) * u.z;
+  double wu = double(w.x) * u.x + double(w.y) * u.y + double(w.z) * u.z;
+  double rr = double(r.x) * r.x + double(r.y) * r.y + double(r.z) * r.z;
+  blockSum3x256(ru, wu, rr, lds);
   if (threadIdx.x == 0) {
-    part[3 * tile] = a;
-    part[3 * tile + 1] = b;
-    part[3 * tile + 2] = c;
+    part[3 * tile] = ru;
+    part[3 * tile + 1] = wu;
+    part[3 * tile + 2] = rr;
   }
 }
 
-__global__ void __launch_bounds__(256) kCgInit(View v) {
-  __shared__ double lds[8];
+// Init, part 1 (per row): r0 = S(-grad), u0 = S M r0; p = step = 0, s_{-1} = q_{-1} = 0.
+__global__ void __launch_bounds__(256) kCgInitU(View v) {
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
   const uint32_t s = v.tileSeg[tile];
@@ -549,52 +588,53 @@ __global__ void __launch_bounds__(256) kCgInit(View v) {
     S->pcgIter = 0u;
     S->kIter[0] = 0u;
   }
+  const uint32_t row = tile * kTile + threadIdx.x;
+  const F3 r = filterContact(v.cflags[row], v.cnormal[row], -v.grad[row]);
+  v.cgR[0][row] = r;
+  v.cgU[0][row] = precondApply(v, row, r);
+  v.cgQ[1][row] = f3(0, 0, 0);
+  v.cgS[1][row] = f3(0, 0, 0);
+  v.pA[row] = f3(0, 0, 0);
+  v.step[row] = f3(0, 0, 0);
+}
+
+// Init, part 2: w0 = S A u0, y0 = S M w0; partials r.u, w.u, r.r.
+__global__ void __launch_bounds__(256) kCgInitW(View v) {
+  __shared__ double lds[16];
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  if (v.state[v.tileSeg[tile]].newtonDone) return;
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   const uint32_t slice = tile * kSlicesPerTile + wave;
   const uint32_t row = slice * kSlice + lane;
-  const F3 r = filterContact(v.cflags[row], v.cnormal[row], -v.grad[row]);
-  const F3 u = cgU0(v, row);
-  const uint32_t K = v.sellK[slice];
-  const uint32_t base = v.sellBase[slice];
-  const float* __restrict__ val = v.sellVal + base + lane;
-  const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
-  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-  for (uint32_t kk = 0; kk < K; ++kk) {
-    const uint32_t c = col[kk * kSlice];
-    const F3 uj = cgU0(v, c);
-    const float* bv = val + kk * 9u * kSlice;
-    a0 += bv[0 * kSlice] * uj.x + bv[1 * kSlice] * uj.y + bv[2 * kSlice] * uj.z;
-    a1 += bv[3 * kSlice] * uj.x + bv[4 * kSlice] * uj.y + bv[5 * kSlice] * uj.z;
-    a2 += bv[6 * kSlice] * uj.x + bv[7 * kSlice] * uj.y + bv[8 * kSlice] * uj.z;
-  }
-  const F3 w = filterContact(v.cflags[row], v.cnormal[row],
-                             (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : f3(a0, a1, a2));
-  v.cgR[0][row] = r;
+  const F3* __restrict__ u0 = v.cgU[0];
+  const F3 au = sellRowProduct(v, slice, lane, [&](uint32_t c) { return u0[c]; });
+  const F3 w = filterContact(v.cflags[row], v.cnormal[row], (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : au);
   v.cgW[0][row] = w;
-  v.cgS[1][row] = f3(0, 0, 0);
-  v.cgU[row] = u;
-  v.pA[row] = f3(0, 0, 0);
-  v.step[row] = f3(0, 0, 0);
-  cgPartials(v.cgPart[0], tile, r, u, w, lds);
+  v.cgY[0][row] = precondApply(v, row, w);
+  cgPartials(v.cgPart[0], tile, v.cgR[0][row], u0[row], w, lds);
 }
 
+// Iteration of parity q: reads r, w, u, y (slot q) and s_{k-1}, q_{k-1} (slot 1-q); writes r, w, u, y (slot 1-q)
+// and s_k, q_k (slot q). Direct iterations rebuild u and q from r and s (residual replacement); pipelined ones
+// carry them by recurrence and gather three vectors per column instead of the residual, preconditioner and
+// contact state.
+template <bool Direct>
 __global__ void __launch_bounds__(256) kCgIteration(View v, uint32_t q) {
-  __shared__ double lds[8];
+  __shared__ double lds[16];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
   const uint32_t s = v.tileSeg[tile];
   SegState* S = v.state + s;
   const SegParams& P = v.seg[s];
   const double* part = v.cgPart[q];
-  double ru = 0.0, wu = 0.0, rr = 0.0;
+  double gamma = 0.0, delta = 0.0, rsum = 0.0;
   for (uint32_t t = P.tileBegin + threadIdx.x; t < P.tileEnd; t += 256u) {
-    ru += part[3 * t];
-    wu += part[3 * t + 1];
-    rr += part[3 * t + 2];
+    gamma += part[3 * t];
+    delta += part[3 * t + 1];
+    rsum += part[3 * t + 2];
   }
-  const double gamma = blockSum256(ru, lds);
-  const double delta = blockSum256(wu, lds);
-  const double rsum = blockSum256(rr, lds);
+  blockSum3x256(gamma, delta, rsum, lds);
   if (S->newtonDone) return;
   const uint32_t k = S->kIter[q];
   uint32_t conv = S->pcgConverged;
@@ -631,33 +671,26 @@ __global__ void __launch_bounds__(256) kCgIteration(View v, uint32_t q) {
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   const uint32_t slice = tile * kSlicesPerTile + wave;
   const uint32_t row = slice * kSlice + lane;
-  const F3* __restrict__ rCur = v.cgR[q];
-  const F3* __restrict__ wCur = v.cgW[q];
-  const F3* __restrict__ sOld = v.cgS[1u - q];
-  const CgNext me = cgAdvance(v, rCur, wCur, sOld, row, a, b);
-  const F3 p = v.cgU[row] + b * v.pA[row];
-  v.pA[row] = p;
-  v.step[row] = v.step[row] + a * p;
-  v.cgS[q][row] = me.s;
-  v.cgR[1u - q][row] = me.r;
-  v.cgU[row] = me.u;
-  const uint32_t K = v.sellK[slice];
-  const uint32_t base = v.sellBase[slice];
-  const float* __restrict__ val = v.sellVal + base + lane;
-  const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
-  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-  for (uint32_t kk = 0; kk < K; ++kk) {
-    const uint32_t c = col[kk * kSlice];
-    const F3 uj = cgAdvance(v, rCur, wCur, sOld, c, a, b).u;
-    const float* bv = val + kk * 9u * kSlice;
-    a0 += bv[0 * kSlice] * uj.x + bv[1 * kSlice] * uj.y + bv[2 * kSlice] * uj.z;
-    a1 += bv[3 * kSlice] * uj.x + bv[4 * kSlice] * uj.y + bv[5 * kSlice] * uj.z;
-    a2 += bv[6 * kSlice] * uj.x + bv[7 * kSlice] * uj.y + bv[8 * kSlice] * uj.z;
-  }
-  const F3 w = filterContact(v.cflags[row], v.cnormal[row],
-                             (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : f3(a0, a1, a2));
-  v.cgW[1u - q][row] = w;
-  cgPartials(v.cgPart[1u - q], tile, me.r, me.u, w, lds);
+  const uint32_t o = 1u - q;
+  const CgSlots in{v.cgR[q], v.cgW[q], v.cgU[q], v.cgY[q], v.cgS[o], v.cgQ[o]};
+  // Own row: s_k, q_k, p_k, step, r_{k+1}, u_{k+1}.
+  const F3 sk = in.w[row] + b * in.sOld[row];
+  const F3 qk = Direct ? precondApply(v, row, sk) : in.y[row] + b * in.qOld[row];
+  const F3 pk = in.u[row] + b * v.pA[row];
+  const F3 rNext = in.r[row] - a * sk;
+  const F3 uNext = cgNextU<Direct>(v, in, row, a, b);
+  v.cgS[q][row] = sk;
+  v.cgQ[q][row] = qk;
+  v.pA[row] = pk;
+  v.step[row] = v.step[row] + a * pk;
+  v.cgR[o][row] = rNext;
+  v.cgU[o][row] = uNext;
+  // w_{k+1} = S A u_{k+1}, each gathered u_{k+1} rebuilt from the neighbour's slot-q data.
+  const F3 au = sellRowProduct(v, slice, lane, [&](uint32_t c) { return cgNextU<Direct>(v, in, c, a, b); });
+  const F3 w = filterContact(v.cflags[row], v.cnormal[row], (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : au);
+  v.cgW[o][row] = w;
+  v.cgY[o][row] = precondApply(v, row, w);
+  cgPartials(v.cgPart[o], tile, rNext, uNext, w, lds);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -678,20 +711,9 @@ __global__ void __launch_bounds__(256) kContactForceSlope(View v) {
   const F3 si = v.step[row];
   float force = 0.0f;
   if (cf & kContactActive) {
-    const uint32_t K = v.sellK[slice];
-    const uint32_t base = v.sellBase[slice];
-    const float* val = v.sellVal + base + lane;
-    const uint32_t* col = v.sellCol + base / 9u + lane;
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-    for (uint32_t kk = 0; kk < K; ++kk) {
-      const uint32_t c = col[kk * kSlice];
-      const F3 pj = v.step[c];
-      const float* bv = val + kk * 9u * kSlice;
-      a0 += bv[0 * kSlice] * pj.x + bv[1 * kSlice] * pj.y + bv[2 * kSlice] * pj.z;
-      a1 += bv[3 * kSlice] * pj.x + bv[4 * kSlice] * pj.y + bv[5 * kSlice] * pj.z;
-      a2 += bv[6 * kSlice] * pj.x + bv[7 * kSlice] * pj.y + bv[8 * kSlice] * pj.z;
-    }
-    const F3 resid = -gi - f3(a0, a1, a2);
+    const F3* __restrict__ stp = v.step;
+    const F3 as = sellRowProduct(v, slice, lane, [&](uint32_t c) { return stp[c]; });
+    const F3 resid = -gi - as;
     const float4 cn = v.cnormal[row];
     force = fmaxf(0.0f, -dot(resid, f3(cn.x, cn.y, cn.z)));
   }
@@ -854,20 +876,8 @@ __global__ void __launch_bounds__(256) kSpmvPlain(View v, const F3* __restrict__
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   const uint32_t slice = tile * kSlicesPerTile + wave;
   const uint32_t row = slice * kSlice + lane;
-  const uint32_t K = v.sellK[slice];
-  const uint32_t base = v.sellBase[slice];
-  const float* __restrict__ val = v.sellVal + base + lane;
-  const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
-  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-  for (uint32_t kk = 0; kk < K; ++kk) {
-    const uint32_t c = col[kk * kSlice];
-    const F3 pj = in[c];
-    const float* bv = val + kk * 9u * kSlice;
-    a0 += bv[0 * kSlice] * pj.x + bv[1 * kSlice] * pj.y + bv[2 * kSlice] * pj.z;
-    a1 += bv[3 * kSlice] * pj.x + bv[4 * kSlice] * pj.y + bv[5 * kSlice] * pj.z;
-    a2 += bv[6 * kSlice] * pj.x + bv[7 * kSlice] * pj.y + bv[8 * kSlice] * pj.z;
-  }
-  out[row] = (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : f3(a0, a1, a2);
+  const F3 prod = sellRowProduct(v, slice, lane, [&](uint32_t c) { return in[c]; });
+  out[row] = (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : prod;
 }
 
 // SELL values -> canonical slot order (9 floats per slot), for parity checks of the assembled matrix.
@@ -1162,9 +1172,15 @@ void launchPcgIteration(const View& v, uint32_t k, hipStream_t st) {
   launchPcgSpmv(v, k, st);
   launchPcgUpdate(v, k, st);
 }
-void launchCgInit(const View& v, hipStream_t st) { kCgInit<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
-void launchCgIteration(const View& v, uint32_t parity, hipStream_t st) {
-  kCgIteration<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, parity & 1u);
+void launchCgInit(const View& v, hipStream_t st) {
+  kCgInitU<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);
+  kCgInitW<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);
+}
+void launchCgIteration(const View& v, uint32_t parity, bool direct, hipStream_t st) {
+  if (direct)
+    kCgIteration<true><<<xcdGrid(v.nTiles), 256, 0, st>>>(v, parity & 1u);
+  else
+    kCgIteration<false><<<xcdGrid(v.nTiles), 256, 0, st>>>(v, parity & 1u);
 }
 void launchContactForceSlope(const View& v, hipStream_t st) {
   kContactForceSlope<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);

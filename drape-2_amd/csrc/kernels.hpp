@@ -34,12 +34,15 @@ struct View {
   F3* pB;
   F3* ap;
   F3* warm;      // learned warm-start velocity change per row (zero when unused)
-  // Single-kernel PCG: r, w = S A u and s = S A p double-buffered by kernel parity (neighbours read the old
-  // copy while the owner writes the new one), u = S M r per row, and FP64 partials (r.u, w.u, r.r) per parity.
+  // Single-kernel PCG: r, w = S A u, u = S M r, y = S M w, s = S A p and q = S M s, each double-buffered by kernel
+  // parity (neighbours read the old copy while the owner writes the new one); p lives in `pA`; FP64 partials
+  // (r.u, w.u, r.r) per parity.
   F3* cgR[2];
   F3* cgW[2];
+  F3* cgU[2];
+  F3* cgY[2];
   F3* cgS[2];
-  F3* cgU;
+  F3* cgQ[2];
   double* cgPart[2];  // 3 per tile: r.u, w.u, r.r
   float* precond;  // 9 per row, row-major
   float4* cnormal; // xyz normal, w = offset r - phi(xStart)
@@ -110,7 +113,7 @@ void launchPcgIteration(const View& v, uint32_t k, hipStream_t st);
 void launchPcgSpmv(const View& v, uint32_t k, hipStream_t st);
 void launchPcgUpdate(const View& v, uint32_t k, hipStream_t st);
 void launchCgInit(const View& v, hipStream_t st);
-void launchCgIteration(const View& v, uint32_t parity, hipStream_t st);
+void launchCgIteration(const View& v, uint32_t parity, bool direct, hipStream_t st);
 void launchContactForceSlope(const View& v, hipStream_t st);
 void launchNewtonSlope(const View& v, hipStream_t st);
 void launchEnergyTrial(const View& v, hipStream_t st);

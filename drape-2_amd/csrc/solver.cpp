@@ -139,7 +139,7 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
                           &scratchF3_, &scratchF3b_})
     b->allocate(vecBytes);
   // Row arrays a gathered column may index hold the zero dummy entry nRows as well.
-  for (DeviceBuffer* b : {&cgR0_, &cgR1_, &cgW0_, &cgW1_, &cgS0_, &cgS1_, &cgU_}) b->allocate(vecBytes);
+  for (DeviceBuffer& b : cgBuf_) b.allocate(vecBytes);
   precond_.allocate(sizeof(float) * 9 * (nRows_ + 1));
   cnormal_.allocate(sizeof(float4) * (nRows_ + 1));
   cflags_.allocate(sizeof(uint32_t) * (nRows_ + 1));
@@ -307,13 +307,14 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   v.pB = pB_.as<dk::F3>();
   v.ap = ap_.as<dk::F3>();
   v.warm = warm_.as<dk::F3>();
-  v.cgR[0] = cgR0_.as<dk::F3>();
-  v.cgR[1] = cgR1_.as<dk::F3>();
-  v.cgW[0] = cgW0_.as<dk::F3>();
-  v.cgW[1] = cgW1_.as<dk::F3>();
-  v.cgS[0] = cgS0_.as<dk::F3>();
-  v.cgS[1] = cgS1_.as<dk::F3>();
-  v.cgU = cgU_.as<dk::F3>();
+  for (int b = 0; b < 2; ++b) {
+    v.cgR[b] = cgBuf_[0 + b].as<dk::F3>();
+    v.cgW[b] = cgBuf_[2 + b].as<dk::F3>();
+    v.cgU[b] = cgBuf_[4 + b].as<dk::F3>();
+    v.cgY[b] = cgBuf_[6 + b].as<dk::F3>();
+    v.cgS[b] = cgBuf_[8 + b].as<dk::F3>();
+    v.cgQ[b] = cgBuf_[10 + b].as<dk::F3>();
+  }
   v.cgPart[0] = cgPart0_.as<double>();
   v.cgPart[1] = cgPart1_.as<double>();
   v.precond = precond_.as<float>();
@@ -480,9 +481,10 @@ void HipSolver::spmvTiming(uint64_t& launches, double& meanUs, double& bytesPerL
   // particle the row and contact flags (8 B), z and p_old gathered once (24 B), p_new and ap written (24 B).
   const double n = double(model_.x.size());
   const double blocks = double(nBlocks_) - (double(nRows_) - n);
-  // The single-kernel iteration also streams, per particle, r, w, s_old, u, p, step (72 B read), the
-  // preconditioner (36 B) and writes s, r, u, p, step, w (72 B): 40 B per block + 204 B per particle.
-  bytesPerLaunch = 40.0 * blocks + (pcgVariant_ == 1 ? 204.0 : 56.0) * n;
+  // The (pipelined) single-kernel iteration streams, per particle, r, w, s, p, step, u, y, q_old (96 B read), the
+  // preconditioner and contact state (56 B), the row flags (4 B), and writes q, s, p, step, r, u, w, y (96 B):
+  // 40 B per block + 252 B per particle.
+  bytesPerLaunch = 40.0 * blocks + (pcgVariant_ == 1 ? 252.0 : 56.0) * n;
 }
 
 bool HipSolver::allSegments(bool (*pred)(const dk::SegState&)) const {
@@ -552,7 +554,7 @@ void HipSolver::runPcgStandard() {
   lastPcgIterations_ = total;
 }
 
-void HipSolver::timedCgIteration(uint32_t parity) {
+void HipSolver::timedCgIteration(uint32_t parity, bool direct) {
   if (eventPool_.empty()) {
     hipEvent_t a, b;
     check(hipEventCreate(&a), "event");
@@ -562,7 +564,7 @@ void HipSolver::timedCgIteration(uint32_t parity) {
   const auto pr = eventPool_.back();
   eventPool_.pop_back();
   check(hipEventRecord(pr.first, stream_), "event");
-  dk::launchCgIteration(view_, parity, stream_);
+  dk::launchCgIteration(view_, parity, direct, stream_);
   check(hipEventRecord(pr.second, stream_), "event");
   eventPending_.push_back(pr);
 }
@@ -575,24 +577,35 @@ void HipSolver::runPcgSingleKernel() {
   dk::launchCgInit(view_, stream_);
   const uint32_t maxCg = maxCgOf(model_);
   const bool sampling = spmvEvery_ > 0;
-  if (useGraph_ && !sampling && !graphExec_) {
+  if (useGraph_ && !graphExec_) {
     check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
-    for (uint32_t i = 0; i < kGraphChunk; ++i) dk::launchCgIteration(view_, i & 1u, stream_);
+    for (uint32_t i = 0; i < kGraphChunk; ++i) dk::launchCgIteration(view_, i & 1u, i == 0, stream_);
     check(hipStreamEndCapture(stream_, &graph_), "capture");
     check(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0), "graph instantiate");
   }
+  auto iterationsDone = [&] {
+    uint32_t total = 0;
+    for (size_t s = 0; s < model_.segments.size(); ++s) total = std::max(total, hostStates_[s].pcgIter);
+    return total;
+  };
   uint32_t launched = 0;
   uint32_t chunks = std::max<uint32_t>(1, (lastPcgIterations_ * 7 / 8) / kGraphChunk);
   while (launched < maxCg + 1) {
+    if (useGraph_ && sampling) {
+      // One event-timed iteration (and its partner, keeping the graph's parity pattern) opens each batch.
+      timedCgIteration(0u, false);
+      dk::launchCgIteration(view_, 1u, false, stream_);
+      launched += 2;
+    }
     for (uint32_t c = 0; c < chunks; ++c) {
-      if (useGraph_ && !sampling) {
+      if (useGraph_) {
         check(hipGraphLaunch(graphExec_, stream_), "graph launch");
       } else {
         for (uint32_t i = 0; i < kGraphChunk; ++i) {
-          if (sampling && (launched + i) % uint32_t(spmvEvery_) == 0)
-            timedCgIteration(i & 1u);
+          if (sampling && (launched + i) % uint32_t(spmvEvery_) == 1)
+            timedCgIteration(i & 1u, i == 0);
           else
-            dk::launchCgIteration(view_, i & 1u, stream_);
+            dk::launchCgIteration(view_, i & 1u, i == 0, stream_);
         }
       }
       launched += kGraphChunk;
@@ -601,9 +614,7 @@ void HipSolver::runPcgSingleKernel() {
     if (allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; })) break;
     chunks = 1;
   }
-  uint32_t total = 0;
-  for (size_t s = 0; s < model_.segments.size(); ++s) total = std::max(total, hostStates_[s].pcgIter);
-  lastPcgIterations_ = total;
+  lastPcgIterations_ = iterationsDone();
 }
 
 void HipSolver::substep(double h, int iterations) {

@@ -89,7 +89,7 @@ class HipSolver {
   void runPcgStandard();
   void runPcgSingleKernel();
   void releaseGraph();
-  void timedCgIteration(uint32_t parity);
+  void timedCgIteration(uint32_t parity, bool direct);
   void globalStructure(std::vector<uint32_t>& rowStart, std::vector<uint32_t>& colIndex,
                        std::vector<uint32_t>& diagSlot, std::vector<uint32_t>& internalToGlobalSlot) const;
 
@@ -113,7 +113,7 @@ class HipSolver {
   DeviceBuffer incCount_, incStart_, inc_, rowLen_, rowStart_, colIndex_, diagSlot_;
   DeviceBuffer stretchSlots_, hingeSlots_, seamSlots_, sellK_, sellBase_, sellCol_, sellVal_, buildError_;
   DeviceBuffer fieldValues_, scratchF3_, scratchF3b_;
-  DeviceBuffer cgR0_, cgR1_, cgW0_, cgW1_, cgS0_, cgS1_, cgU_, cgPart0_, cgPart1_;
+  DeviceBuffer cgBuf_[12], cgPart0_, cgPart1_;
   int pcgVariant_ = 1;
   static constexpr uint32_t kGraphChunk = 16;  // PCG iterations per captured graph (even: parity pattern repeats)
   hipGraph_t graph_ = nullptr;
