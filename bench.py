@@ -67,6 +67,8 @@ def main() -> int:
                     help="event-time PCG iteration launches (graph path: one per poll; eager: every k-th)")
     ap.add_argument("--pcg-variant", type=int, default=1, help="1 single-kernel PCG (default), 0 two-kernel")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--pcg-waves", type=int, default=0,
+                    help="wavefronts per slice in the one-kernel PCG iteration (1, 2, 4; 0 = tune during warm-up)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
@@ -79,7 +81,7 @@ def main() -> int:
         return scenes.c2_sheet(n=args.n)
 
     sc = make_scene()
-    s = HipSolver(graph=not args.no_graph, pcg_variant=args.pcg_variant)
+    s = HipSolver(graph=not args.no_graph, pcg_variant=args.pcg_variant, pcg_waves=args.pcg_waves)
     s.build(sc)
     s.set_spmv_sampling(args.sample_every)
     for _ in range(args.warmup):
@@ -114,7 +116,8 @@ def main() -> int:
             "particles": sc.n,
             "dt": 1 / 60, "substeps": 1, "newton_max": 5, "pcg_tolerance": 1e-6,
             "solver": "plain warm start (u = h v)",
-            "pcg_form": "single-kernel" if args.pcg_variant == 1 else "two-kernel",
+            "pcg_form": "one-kernel" if args.pcg_variant == 1 else "two-kernel",
+            "pcg_waves_per_slice": s.pcg_waves() if args.pcg_variant == 1 else None,
             "graph": not args.no_graph,
             "newton_per_step": c["newton_iterations"] / args.steps,
             "pcg_per_step": c["pcg_iterations"] / args.steps,
@@ -122,8 +125,9 @@ def main() -> int:
             "final_stats": st,
         },
         "roofline": {
-            "kernel": ("kCgIteration (single-kernel PCG iteration: vector updates, SELL-64 block SpMV with on-the-fly "
-                       "neighbour preconditioning, contact filter, FP64 partials)") if args.pcg_variant == 1 else
+            "kernel": ("kPcgStep (one-kernel PCG iteration: update, convergence test, SELL-64 block SpMV with the "
+                       "search direction rebuilt per gathered column, contact filter, FP64 partials)")
+                      if args.pcg_variant == 1 else
                       "kPcgSpmv (fused p = z + beta p, SELL-64 block SpMV, contact filter, FP64 p.Ap partials)",
             "bound": "hbm",
             "achieved": achieved,

@@ -296,6 +296,21 @@ int drape_solver_set_pcg_variant(drape_solver* s, int32_t variant) {
   });
 }
 
+int drape_solver_set_pcg_waves(drape_solver* s, int32_t waves) {
+  return guard([&] {
+    need(s, "solver");
+    s->impl.setPcgWaves(waves);
+  });
+}
+
+int drape_solver_pcg_waves(const drape_solver* s, int32_t* out) {
+  return guard([&] {
+    need(s, "solver");
+    need(out, "output");
+    *out = s->impl.pcgWaves();
+  });
+}
+
 int drape_solver_set_spmv_sampling(drape_solver* s, int32_t every) {
   return guard([&] {
     need(s, "solver");
@@ -355,6 +370,14 @@ int drape_solver_debug_spmv(drape_solver* s, const float* x, float* y) {
     need(x, "input");
     need(y, "output");
     s->impl.debugSpmv(x, y);
+  });
+}
+
+int drape_solver_debug_system(drape_solver* s, float* blocks, float* grad, float* precond, uint32_t* contact_flags,
+                              float* contact_normals) {
+  return guard([&] {
+    need(s, "solver");
+    s->impl.debugSystem(blocks, grad, precond, contact_flags, contact_normals);
   });
 }
 

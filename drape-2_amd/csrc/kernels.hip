@@ -333,15 +333,17 @@ __global__ void __launch_bounds__(64) kAssemble(View v) {
   if (flags & kRowPinned) g = f3(0, 0, 0);
   v.grad[row] = g;
   uint32_t cf = v.cflags[row];
-  if (cf & kContactActive) {
-    const float4 cn = v.cnormal[row];
-    if (dot(g, f3(cn.x, cn.y, cn.z)) < 0.0f) v.cflags[row] = cf & ~kContactActive;
+  const float4 cn = v.cnormal[row];
+  if ((cf & kContactActive) && dot(g, f3(cn.x, cn.y, cn.z)) < 0.0f) {
+    cf &= ~kContactActive;
+    v.cflags[row] = cf;
   }
   rowE = waveSum(rowE);
   if (lane == 0) v.partE[slice] = rowE;
   __syncthreads();
   // Block-Jacobi preconditioner from the finished diagonal block: its inverse, the identity when it is not
-  // positive definite, zero for pinned rows (cpu_solver.cpp:171-178; kernels.h:237-248).
+  // positive definite, zero for pinned rows (cpu_solver.cpp:171-178; kernels.h:237-248). Stored with the contact
+  // filter applied, S M, since the solve only ever uses filter(M x) (cpu_solver.cpp:179-181).
   {
     float M[9];
     if (flags & kRowPinned) {
@@ -355,6 +357,16 @@ __global__ void __launch_bounds__(64) kAssemble(View v) {
       if (!invertSpd3(A, M)) {
 #pragma unroll
         for (int c = 0; c < 9; ++c) M[c] = (c % 4 == 0) ? 1.0f : 0.0f;
+      }
+    }
+    if (cf & kContactActive) {
+      const F3 n = f3(cn.x, cn.y, cn.z);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float nm = n.x * M[c] + n.y * M[3 + c] + n.z * M[6 + c];
+        M[c] -= n.x * nm;
+        M[3 + c] -= n.y * nm;
+        M[6 + c] -= n.z * nm;
       }
     }
 #pragma unroll
@@ -402,7 +414,7 @@ __global__ void __launch_bounds__(256) kPcgInit(View v) {
   M3 Mm;
 #pragma unroll
   for (int c = 0; c < 9; ++c) Mm.m[c] = v.precond[9u * row + c];
-  const F3 zi = filterContact(cf, cn, mul(Mm, ri));
+  const F3 zi = mul(Mm, ri);  // the stored preconditioner already carries the contact filter
   v.step[row] = f3(0, 0, 0);
   v.r[row] = ri;
   v.z[row] = zi;
@@ -510,7 +522,7 @@ __global__ void __launch_bounds__(256) kPcgUpdate(View v, uint32_t k, const F3* 
   M3 M;
 #pragma unroll
   for (int c = 0; c < 9; ++c) M.m[c] = v.precond[9u * row + c];
-  const F3 zi = filterContact(v.cflags[row], v.cnormal[row], mul(M, ri));
+  const F3 zi = mul(M, ri);
   v.z[row] = zi;
   const double rr = double(ri.x) * ri.x + double(ri.y) * ri.y + double(ri.z) * ri.z;
   const double rz = double(ri.x) * zi.x + double(ri.y) * zi.y + double(ri.z) * zi.z;
@@ -523,60 +535,68 @@ __global__ void __launch_bounds__(256) kPcgUpdate(View v, uint32_t k, const F3* 
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// The same block-Jacobi PCG in Chronopoulos–Gear form with preconditioned recurrences: one kernel and one global
-// reduction per iteration. With u = S M r, w = S A u, y = S M w, s = S A p, q = S M s (S the contact-plane filter):
-//   gamma = r.u, delta = w.u, beta = gamma / gamma_prev, p.Ap = delta - beta gamma / alpha_prev, alpha = gamma / p.Ap
-//   p = u + beta p, s = w + beta s, q = y + beta q, step += alpha p, r -= alpha s, u -= alpha q
-// which are the iterates of cpu_solver.cpp:169-213 in exact arithmetic (same stopping rules: r0 = 0, p.Ap <= 0,
-// |r| <= tol |r0| after an update, at most maxCg updates). Kernel k computes the next w = S A u itself: each
-// gathered column's new u is rebuilt from the neighbour's data, so no second pass over the rows is needed. In FP32
-// the carried u and q drift from S M r and S M s; a direct iteration, which rebuilds them from r and s, opens
-// every chunk of iterations (residual replacement).
+// The same block-Jacobi PCG with one kernel and one global reduction per iteration.
+//
+// Kernel k applies update k of the standard iteration (step += alpha_{k-1} p_{k-1}, r_k = r_{k-1} -
+// alpha_{k-1} ap_{k-1}), tests convergence, forms p_k = z_k + beta_k p_{k-1} and ap_k = S A p_k, and writes the
+// partial dot products the next kernel needs. Standard PCG needs a second global reduction after its update, for
+// r_k.z_k (beta) and r_k.r_k (the stopping test); here both are expanded over the previous iteration's vectors,
+//   r_k.z_k = r.zr - 2 alpha r.t + alpha^2 ap.t,    r_k.r_k = r.r - 2 alpha r.ap + alpha^2 ap.ap,
+// with zr = Mf r and t = Mf ap computed directly by each row's owner (Mf = S M, the contact-filtered block-Jacobi
+// inverse). Every term is a dot product of stored vectors, so no scalar is carried by recurrence and no
+// orthogonality is assumed; p.ap is reduced directly, as in the standard form. A gathered column's z_k is rebuilt
+// from its owner's zr and t (z_k = zr - alpha t = Mf r_k). Stopping rules of cpu_solver.cpp:188-213: r0 = 0,
+// p.Ap <= 0 before an update, |r| <= tol |r0| after one, at most maxCg updates.
+//
+// Matrix rows: W wavefronts share each 64-row SELL slice (column t goes to wavefront t mod W) and combine their
+// row sums through LDS in a fixed order that does not depend on W, so one 256-row tile runs as 4 W wavefronts. Every load that does not
+// depend on the scalars (column indices, block values and the gathered zr, t, p of the first batch of columns) is
+// issued before the segment reduction.
 
-__device__ inline F3 precondApply(const View& v, uint32_t row, F3 x) {
-  M3 M;
+namespace {
+constexpr int kPcgParts = 7;  // per tile: p.ap, r.zr, r.t, ap.t, r.ap, ap.ap, r.r
+enum : int { kPpAp = 0, kPrZ, kPrT, kPapT, kPrAp, kPapAp, kPrR };
+}  // namespace
+
+// N sums over a workgroup of NW wavefronts (fixed order), returned to every thread. lds >= N * NW doubles.
+template <int N, int NW>
+__device__ inline void blockSumN(double (&x)[N], double* lds) {
 #pragma unroll
-  for (int c = 0; c < 9; ++c) M.m[c] = v.precond[9u * row + c];
-  return filterContact(v.cflags[row], v.cnormal[row], mul(M, x));
-}
-
-// The iteration's inputs (slot q) and outputs (slot 1-q, or q for s and q) of parity q.
-struct CgSlots {
-  const F3* __restrict__ r;
-  const F3* __restrict__ w;
-  const F3* __restrict__ u;
-  const F3* __restrict__ y;
-  const F3* __restrict__ sOld;
-  const F3* __restrict__ qOld;
-};
-
-// Row j's u_{k+1}. Pipelined: u_k - alpha (y_k + beta q_{k-1}). Direct (residual replacement):
-// S M (r_k - alpha (w_k + beta s_{k-1})). The owner and every neighbour gathering row j evaluate the same
-// expression, so all of them see the same value.
-template <bool Direct>
-__device__ inline F3 cgNextU(const View& v, const CgSlots& in, uint32_t j, float alpha, float beta) {
-  if (Direct) {
-    const F3 s = in.w[j] + beta * in.sOld[j];
-    return precondApply(v, j, in.r[j] - alpha * s);
-  }
-  const F3 q = in.y[j] + beta * in.qOld[j];
-  return in.u[j] - alpha * q;
-}
-
-__device__ inline void cgPartials(double* part, uint32_t tile, F3 r, F3 u, F3 w, double* lds) {
-  double ru = double(r.x) * u.x + double(r.y) * u.y + double(r.z) * u.z;
-  double wu = double(w.x) * u.x + double(w.y) * u.y + double(w.z) * u.z;
-  double rr = double(r.x) * r.x + double(r.y) * r.y + double(r.z) * r.z;
-  blockSum3x256(ru, wu, rr, lds);
-  if (threadIdx.x == 0) {
-    part[3 * tile] = ru;
-    part[3 * tile + 1] = wu;
-    part[3 * tile + 2] = rr;
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] += __shfl_down(x[i], off, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0)
+#pragma unroll
+    for (int i = 0; i < N; ++i) lds[i * NW + w] = x[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) s += lds[i * NW + j];
+    x[i] = s;
   }
 }
 
-// Init, part 1 (per row): r0 = S(-grad), u0 = S M r0; p = step = 0, s_{-1} = q_{-1} = 0.
-__global__ void __launch_bounds__(256) kCgInitU(View v) {
+// The search direction of a row from what its owner stored: p_k = (zr - alpha t) + beta p_{k-1}. The owner and
+// every row gathering it evaluate this same expression.
+__device__ inline F3 pcgDirection(F3 zr, F3 t, F3 p, float alpha, float beta) {
+  return (zr - alpha * t) + beta * p;
+}
+
+__device__ inline F3 mulRow9(const float* m, F3 x) {
+  return f3(m[0] * x.x + m[1] * x.y + m[2] * x.z, m[3] * x.x + m[4] * x.y + m[5] * x.z,
+            m[6] * x.x + m[7] * x.y + m[8] * x.z);
+}
+
+__device__ inline double dot64(F3 a, F3 b) { return double(a.x) * b.x + double(a.y) * b.y + double(a.z) * b.z; }
+
+// Init: r_0 = S(-grad), zr_0 = Mf r_0, written as the outputs of a kernel "-1" (slot 1) with ap = p = t = 0;
+// partials r.zr and r.r.
+__global__ void __launch_bounds__(256) kPcgStepInit(View v) {
+  __shared__ double lds[kPcgParts * 4];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
   const uint32_t s = v.tileSeg[tile];
@@ -587,110 +607,240 @@ __global__ void __launch_bounds__(256) kCgInitU(View v) {
     S->pcgConverged = 0u;
     S->pcgIter = 0u;
     S->kIter[0] = 0u;
+    S->convPar[0] = 0u;
   }
   const uint32_t row = tile * kTile + threadIdx.x;
   const F3 r = filterContact(v.cflags[row], v.cnormal[row], -v.grad[row]);
-  v.cgR[0][row] = r;
-  v.cgU[0][row] = precondApply(v, row, r);
-  v.cgQ[1][row] = f3(0, 0, 0);
-  v.cgS[1][row] = f3(0, 0, 0);
-  v.pA[row] = f3(0, 0, 0);
-  v.step[row] = f3(0, 0, 0);
+  float m[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) m[e] = v.precond[9u * row + e];
+  const F3 zr = mulRow9(m, r);
+  const F3 zero = f3(0, 0, 0);
+  v.cgR[1][row] = r;
+  v.cgAp[1][row] = zero;
+  v.cgP[1][row] = zero;
+  v.cgZr[1][row] = zr;
+  v.cgT[1][row] = zero;
+  v.step[row] = zero;
+  double d[kPcgParts] = {0.0, dot64(r, zr), 0.0, 0.0, 0.0, 0.0, dot64(r, r)};
+  blockSumN<kPcgParts, 4>(d, lds);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < kPcgParts; ++i) v.cgPart[1][kPcgParts * tile + i] = d[i];
 }
 
-// Init, part 2: w0 = S A u0, y0 = S M w0; partials r.u, w.u, r.r.
-__global__ void __launch_bounds__(256) kCgInitW(View v) {
-  __shared__ double lds[16];
-  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
-  if (tile >= v.nTiles) return;
-  if (v.state[v.tileSeg[tile]].newtonDone) return;
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  const uint32_t slice = tile * kSlicesPerTile + wave;
-  const uint32_t row = slice * kSlice + lane;
-  const F3* __restrict__ u0 = v.cgU[0];
-  const F3 au = sellRowProduct(v, slice, lane, [&](uint32_t c) { return u0[c]; });
-  const F3 w = filterContact(v.cflags[row], v.cnormal[row], (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : au);
-  v.cgW[0][row] = w;
-  v.cgY[0][row] = precondApply(v, row, w);
-  cgPartials(v.cgPart[0], tile, v.cgR[0][row], u0[row], w, lds);
-}
-
-// Iteration of parity q: reads r, w, u, y (slot q) and s_{k-1}, q_{k-1} (slot 1-q); writes r, w, u, y (slot 1-q)
-// and s_k, q_k (slot q). Direct iterations rebuild u and q from r and s (residual replacement); pipelined ones
-// carry them by recurrence and gather three vectors per column instead of the residual, preconditioner and
-// contact state.
-template <bool Direct>
-__global__ void __launch_bounds__(256) kCgIteration(View v, uint32_t q) {
-  __shared__ double lds[16];
+template <int W>
+__global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
+  constexpr int NW = 4 * W;
+  constexpr uint32_t B = W == 1 ? 8u : 16u / W;  // columns per wavefront in one batch
+  __shared__ double red[kPcgParts * NW];
+  __shared__ float comb[(W > 1 ? 4 : 1) * kSlicesPerTile * 3 * kSlice];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
   const uint32_t s = v.tileSeg[tile];
   SegState* S = v.state + s;
   const SegParams& P = v.seg[s];
-  const double* part = v.cgPart[q];
-  double gamma = 0.0, delta = 0.0, rsum = 0.0;
-  for (uint32_t t = P.tileBegin + threadIdx.x; t < P.tileEnd; t += 256u) {
-    gamma += part[3 * t];
-    delta += part[3 * t + 1];
-    rsum += part[3 * t + 2];
-  }
-  blockSum3x256(gamma, delta, rsum, lds);
+  const uint32_t o = 1u - q;
+  // A solve that an earlier kernel ended: pass the flag on to the next parity slot and load nothing.
   if (S->newtonDone) return;
-  const uint32_t k = S->kIter[q];
-  uint32_t conv = S->pcgConverged;
-  double beta = 0.0, pAp = delta, r0 = 0.0;
-  if (!conv) {
-    if (k == 0) {
-      r0 = sqrt(rsum);
-      if (!(r0 > 0.0)) conv = 1u;
-    } else if (sqrt(rsum) <= P.cgTolerance * S->r0) {
-      conv = 1u;
-    } else {
-      beta = gamma / S->gam[1u - q];
-      pAp = delta - beta * gamma / S->alph[1u - q];
+  if (S->convPar[q]) {
+    if (tile == P.tileBegin && threadIdx.x == 0) {
+      S->convPar[o] = 1u;
+      S->kIter[o] = S->kIter[q] + 1u;
     }
-    if (!conv && k >= P.maxCg) conv = 1u;
-    if (!conv && !(pAp > 0.0)) conv = 1u;
+    return;
   }
-  const double alpha = gamma / pAp;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t sl = wave & 3u, part = wave >> 2;
+  // Slice-uniform values through the scalar cache.
+  const uint32_t slice = __builtin_amdgcn_readfirstlane(tile * kSlicesPerTile + sl);
+  const uint32_t row = slice * kSlice + lane;
+  const F3* __restrict__ Zr = v.cgZr[o];
+  const F3* __restrict__ T = v.cgT[o];
+  const F3* __restrict__ Pp = v.cgP[o];
+  const uint32_t K = v.sellK[slice];
+  const uint32_t base = v.sellBase[slice];
+
+  // Round trip 1: the previous kernel's partial sums, the owner row's data and the column indices of the first
+  // batch. Nothing here depends on the scalars.
+  const double* __restrict__ prev = v.cgPart[o];
+  uint32_t ti = P.tileBegin + threadIdx.x;
+  double sum[kPcgParts];
+#pragma unroll
+  for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < P.tileEnd ? prev[kPcgParts * ti + n] : 0.0;
+  F3 oR, oAp, oZ, oT, oP, oStep;
+  float m[9];
+  uint32_t oFlags = 0u, oCf = 0u;
+  float4 oCn = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (part == 0) {
+    oR = v.cgR[o][row];
+    oAp = v.cgAp[o][row];
+    oZ = Zr[row];
+    oT = T[row];
+    oP = Pp[row];
+    oStep = v.step[row];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) m[e] = v.precond[9u * row + e];
+    oFlags = v.rowFlags[row];
+    oCf = v.cflags[row];
+    oCn = v.cnormal[row];
+  }
+  const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
+  const float* __restrict__ val = v.sellVal + base + lane;
+  uint32_t c[B];
+#pragma unroll
+  for (uint32_t j = 0; j < B; ++j) {
+    const uint32_t t = part + W * j;
+    c[j] = t < K ? col[t * kSlice] : v.nRows;
+  }
+  // Round trip 2: gathered (zr, t, p) and block values of the first batch.
+  F3 gz[B], gt[B], gp[B];
+#pragma unroll
+  for (uint32_t j = 0; j < B; ++j) {
+    gz[j] = Zr[c[j]];
+    gt[j] = T[c[j]];
+    gp[j] = Pp[c[j]];
+  }
+  float bv[B][9];
+#pragma unroll
+  for (uint32_t j = 0; j < B; ++j) {
+    const uint32_t t = part + W * j;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) bv[j][e] = t < K ? val[(t * 9u + e) * kSlice] : 0.0f;
+  }
+
+  // Scalars of this kernel (identical in every workgroup of the segment), while the gathers are in flight.
+  for (ti += 256u * W; ti < P.tileEnd; ti += 256u * W)
+#pragma unroll
+    for (int n = 0; n < kPcgParts; ++n) sum[n] += prev[kPcgParts * ti + n];
+  blockSumN<kPcgParts, NW>(sum, red);
+  const uint32_t k = S->kIter[q];
+  uint32_t conv = 0u;
+  bool update = false;
+  double alpha = 0.0, beta = 0.0, r0 = 0.0, rrNext = sum[kPrR];
+  if (k == 0) {
+    r0 = sqrt(sum[kPrR]);
+    if (!(r0 > 0.0) || P.maxCg == 0u) conv = 1u;
+  } else if (!(sum[kPpAp] > 0.0)) {
+    conv = 1u;
+  } else {
+    update = true;
+    alpha = sum[kPrZ] / sum[kPpAp];
+    rrNext = sum[kPrR] - 2.0 * alpha * sum[kPrAp] + alpha * alpha * sum[kPapAp];
+    const double rzNext = sum[kPrZ] - 2.0 * alpha * sum[kPrT] + alpha * alpha * sum[kPapT];
+    if (sqrt(fmax(rrNext, 0.0)) <= P.cgTolerance * S->r0 || k >= P.maxCg)
+      conv = 1u;
+    else
+      beta = rzNext / sum[kPrZ];
+  }
   if (tile == P.tileBegin && threadIdx.x == 0) {
     if (k == 0) S->r0 = r0;
-    S->kIter[1u - q] = k + 1u;
-    S->gam[q] = gamma;
-    S->alph[q] = alpha;
-    S->rr = rsum;
-    if (conv) {
-      S->pcgConverged = 1u;
-    } else {
+    S->kIter[o] = k + 1u;
+    S->convPar[o] = conv;
+    if (update) {
+      S->alpha = alpha;
+      S->rr = rrNext;
       S->pcgIter += 1u;
       S->cntPcg += 1u;
     }
+    if (conv) S->pcgConverged = 1u;
   }
-  if (conv) return;
+  if (!update && conv) return;
   const float a = float(alpha), b = float(beta);
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  const uint32_t slice = tile * kSlicesPerTile + wave;
-  const uint32_t row = slice * kSlice + lane;
-  const uint32_t o = 1u - q;
-  const CgSlots in{v.cgR[q], v.cgW[q], v.cgU[q], v.cgY[q], v.cgS[o], v.cgQ[o]};
-  // Own row: s_k, q_k, p_k, step, r_{k+1}, u_{k+1}.
-  const F3 sk = in.w[row] + b * in.sOld[row];
-  const F3 qk = Direct ? precondApply(v, row, sk) : in.y[row] + b * in.qOld[row];
-  const F3 pk = in.u[row] + b * v.pA[row];
-  const F3 rNext = in.r[row] - a * sk;
-  const F3 uNext = cgNextU<Direct>(v, in, row, a, b);
-  v.cgS[q][row] = sk;
-  v.cgQ[q][row] = qk;
-  v.pA[row] = pk;
-  v.step[row] = v.step[row] + a * pk;
-  v.cgR[o][row] = rNext;
-  v.cgU[o][row] = uNext;
-  // w_{k+1} = S A u_{k+1}, each gathered u_{k+1} rebuilt from the neighbour's slot-q data.
-  const F3 au = sellRowProduct(v, slice, lane, [&](uint32_t c) { return cgNextU<Direct>(v, in, c, a, b); });
-  const F3 w = filterContact(v.cflags[row], v.cnormal[row], (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : au);
-  v.cgW[o][row] = w;
-  v.cgY[o][row] = precondApply(v, row, w);
-  cgPartials(v.cgPart[o], tile, rNext, uNext, w, lds);
+  if (part == 0 && update) v.step[row] = oStep + a * oP;
+  if (conv) return;
+
+  // Row sums of A p_k. Columns are summed in four groups (column index mod 4), each in increasing column order,
+  // and the groups are added as ((g0 + g1) + g2) + g3 whatever W is: every W gives bit-identical results.
+  constexpr uint32_t G = 4u / W;  // groups held by one wavefront: group part + W lg
+  float acc[G][3];
+#pragma unroll
+  for (uint32_t lg = 0; lg < G; ++lg) acc[lg][0] = acc[lg][1] = acc[lg][2] = 0.0f;
+#pragma unroll
+  for (uint32_t j = 0; j < B; ++j) {
+    const F3 x = pcgDirection(gz[j], gt[j], gp[j], a, b);
+    float* g = acc[j % G];
+    g[0] += bv[j][0] * x.x + bv[j][1] * x.y + bv[j][2] * x.z;
+    g[1] += bv[j][3] * x.x + bv[j][4] * x.y + bv[j][5] * x.z;
+    g[2] += bv[j][6] * x.x + bv[j][7] * x.y + bv[j][8] * x.z;
+  }
+  for (uint32_t j0 = B; part + W * j0 < K; j0 += B) {  // rows with more than B W blocks
+#pragma unroll
+    for (uint32_t j = 0; j < B; ++j) {
+      const uint32_t t = part + W * (j0 + j);
+      c[j] = t < K ? col[t * kSlice] : v.nRows;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < B; ++j) {
+      const uint32_t t = part + W * (j0 + j);
+      const F3 x = pcgDirection(Zr[c[j]], T[c[j]], Pp[c[j]], a, b);
+      if (t < K) {
+        const float* bj = val + t * 9u * kSlice;
+        float* g = acc[j % G];
+        g[0] += bj[0 * kSlice] * x.x + bj[1 * kSlice] * x.y + bj[2 * kSlice] * x.z;
+        g[1] += bj[3 * kSlice] * x.x + bj[4 * kSlice] * x.y + bj[5 * kSlice] * x.z;
+        g[2] += bj[6 * kSlice] * x.x + bj[7 * kSlice] * x.y + bj[8 * kSlice] * x.z;
+      }
+    }
+  }
+  if constexpr (W > 1) {
+    if (part > 0) {
+#pragma unroll
+      for (uint32_t lg = 0; lg < G; ++lg) {
+        float* cb = comb + ((part + W * lg) * kSlicesPerTile + sl) * 3u * kSlice + lane;
+        cb[0] = acc[lg][0];
+        cb[kSlice] = acc[lg][1];
+        cb[2 * kSlice] = acc[lg][2];
+      }
+    }
+    __syncthreads();
+  }
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  if (part == 0) {
+    float grp[4][3];
+#pragma unroll
+    for (uint32_t g = 0; g < 4; ++g) {
+      if (g % W == 0) {
+        grp[g][0] = acc[g / W][0];
+        grp[g][1] = acc[g / W][1];
+        grp[g][2] = acc[g / W][2];
+      } else {
+        const float* cb = comb + (g * kSlicesPerTile + sl) * 3u * kSlice + lane;
+        grp[g][0] = cb[0];
+        grp[g][1] = cb[kSlice];
+        grp[g][2] = cb[2 * kSlice];
+      }
+    }
+    a0 = ((grp[0][0] + grp[1][0]) + grp[2][0]) + grp[3][0];
+    a1 = ((grp[0][1] + grp[1][1]) + grp[2][1]) + grp[3][1];
+    a2 = ((grp[0][2] + grp[1][2]) + grp[2][2]) + grp[3][2];
+  }
+
+  // Owner rows: r_k, p_k, ap_k, zr_k = Mf r_k, t_k = Mf ap_k; partials for the next kernel.
+  double d[kPcgParts] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (part == 0) {
+    const F3 rk = oR - a * oAp;
+    const F3 pk = pcgDirection(oZ, oT, oP, a, b);
+    const F3 apk = (oFlags & kRowPinned) ? f3(0, 0, 0) : filterContact(oCf, oCn, f3(a0, a1, a2));
+    const F3 zr = mulRow9(m, rk);
+    const F3 tk = mulRow9(m, apk);
+    v.cgR[q][row] = rk;
+    v.cgAp[q][row] = apk;
+    v.cgP[q][row] = pk;
+    v.cgZr[q][row] = zr;
+    v.cgT[q][row] = tk;
+    d[kPpAp] = dot64(pk, apk);
+    d[kPrZ] = dot64(rk, zr);
+    d[kPrT] = dot64(rk, tk);
+    d[kPapT] = dot64(apk, tk);
+    d[kPrAp] = dot64(rk, apk);
+    d[kPapAp] = dot64(apk, apk);
+    d[kPrR] = dot64(rk, rk);
+  }
+  blockSumN<kPcgParts, NW>(d, red);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < kPcgParts; ++i) v.cgPart[q][kPcgParts * tile + i] = d[i];
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -1172,15 +1322,14 @@ void launchPcgIteration(const View& v, uint32_t k, hipStream_t st) {
   launchPcgSpmv(v, k, st);
   launchPcgUpdate(v, k, st);
 }
-void launchCgInit(const View& v, hipStream_t st) {
-  kCgInitU<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);
-  kCgInitW<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);
-}
-void launchCgIteration(const View& v, uint32_t parity, bool direct, hipStream_t st) {
-  if (direct)
-    kCgIteration<true><<<xcdGrid(v.nTiles), 256, 0, st>>>(v, parity & 1u);
-  else
-    kCgIteration<false><<<xcdGrid(v.nTiles), 256, 0, st>>>(v, parity & 1u);
+void launchPcgStepInit(const View& v, hipStream_t st) { kPcgStepInit<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
+void launchPcgStep(const View& v, uint32_t parity, uint32_t wavesPerSlice, hipStream_t st) {
+  switch (wavesPerSlice) {
+    case 1: kPcgStep<1><<<xcdGrid(v.nTiles), 256, 0, st>>>(v, parity & 1u); break;
+    case 2: kPcgStep<2><<<xcdGrid(v.nTiles), 512, 0, st>>>(v, parity & 1u); break;
+    case 4: kPcgStep<4><<<xcdGrid(v.nTiles), 1024, 0, st>>>(v, parity & 1u); break;
+    default: throw std::invalid_argument("PCG wavefronts per slice must be 1, 2 or 4");
+  }
 }
 void launchContactForceSlope(const View& v, hipStream_t st) {
   kContactForceSlope<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);

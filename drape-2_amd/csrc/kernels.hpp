@@ -34,17 +34,16 @@ struct View {
   F3* pB;
   F3* ap;
   F3* warm;      // learned warm-start velocity change per row (zero when unused)
-  // Single-kernel PCG: r, w = S A u, u = S M r, y = S M w, s = S A p and q = S M s, each double-buffered by kernel
-  // parity (neighbours read the old copy while the owner writes the new one); p lives in `pA`; FP64 partials
-  // (r.u, w.u, r.r) per parity.
+  // One-reduction PCG (kPcgStep): r, ap = S A p, p, zr = Mf r and t = Mf ap, each double-buffered by kernel
+  // parity (neighbours read the previous kernel's copy while the owner writes the new one); FP64 partials per
+  // tile (7 per parity).
   F3* cgR[2];
-  F3* cgW[2];
-  F3* cgU[2];
-  F3* cgY[2];
-  F3* cgS[2];
-  F3* cgQ[2];
-  double* cgPart[2];  // 3 per tile: r.u, w.u, r.r
-  float* precond;  // 9 per row, row-major
+  F3* cgAp[2];
+  F3* cgP[2];
+  F3* cgZr[2];
+  F3* cgT[2];
+  double* cgPart[2];
+  float* precond;  // 9 per row, row-major: S M, the block-Jacobi inverse with the contact filter applied
   float4* cnormal; // xyz normal, w = offset r - phi(xStart)
   uint32_t* cflags;
   float* cforce;
@@ -112,8 +111,9 @@ void launchPcgInit(const View& v, hipStream_t st);
 void launchPcgIteration(const View& v, uint32_t k, hipStream_t st);
 void launchPcgSpmv(const View& v, uint32_t k, hipStream_t st);
 void launchPcgUpdate(const View& v, uint32_t k, hipStream_t st);
-void launchCgInit(const View& v, hipStream_t st);
-void launchCgIteration(const View& v, uint32_t parity, bool direct, hipStream_t st);
+void launchPcgStepInit(const View& v, hipStream_t st);
+// One PCG iteration of the one-reduction form; wavesPerSlice (1, 2 or 4) wavefronts share each 64-row slice.
+void launchPcgStep(const View& v, uint32_t parity, uint32_t wavesPerSlice, hipStream_t st);
 void launchContactForceSlope(const View& v, hipStream_t st);
 void launchNewtonSlope(const View& v, hipStream_t st);
 void launchEnergyTrial(const View& v, hipStream_t st);

@@ -5,7 +5,8 @@
 //   Newton iteration (up to `iterations`, stopping a segment once its slope is not negative or its largest move
 //   is below 1e-9 m):
 //     activate contacts; assemble energy / gradient / block-CSR Hessian; e0
-//     block-Jacobi PCG (two kernels per iteration; the host polls the per-segment converged flags in chunks)
+//     block-Jacobi PCG (one kernel per iteration, replayed from hipGraph chunks, or the two-kernel form; the host
+//     polls the per-segment converged flags after each batch of chunks)
 //     contact forces from the unfiltered residual; slope
 //     Armijo line search (one trial kernel + one decision kernel per trial)
 //     apply; largest move
@@ -202,8 +203,8 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   partC_.allocate(sizeof(double) * nTiles_);
   partE_.allocate(sizeof(double) * nSlices_);
   partMax_.allocate(sizeof(float) * nTiles_);
-  cgPart0_.allocate(sizeof(double) * 3 * nTiles_);
-  cgPart1_.allocate(sizeof(double) * 3 * nTiles_);
+  cgPart0_.allocate(sizeof(double) * 7 * nTiles_);
+  cgPart1_.allocate(sizeof(double) * 7 * nTiles_);
 
   // Terms in internal row indices.
   std::vector<dk::Stretch32> st(M.stretch.size());
@@ -309,11 +310,10 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   v.warm = warm_.as<dk::F3>();
   for (int b = 0; b < 2; ++b) {
     v.cgR[b] = cgBuf_[0 + b].as<dk::F3>();
-    v.cgW[b] = cgBuf_[2 + b].as<dk::F3>();
-    v.cgU[b] = cgBuf_[4 + b].as<dk::F3>();
-    v.cgY[b] = cgBuf_[6 + b].as<dk::F3>();
-    v.cgS[b] = cgBuf_[8 + b].as<dk::F3>();
-    v.cgQ[b] = cgBuf_[10 + b].as<dk::F3>();
+    v.cgAp[b] = cgBuf_[2 + b].as<dk::F3>();
+    v.cgP[b] = cgBuf_[4 + b].as<dk::F3>();
+    v.cgZr[b] = cgBuf_[6 + b].as<dk::F3>();
+    v.cgT[b] = cgBuf_[8 + b].as<dk::F3>();
   }
   v.cgPart[0] = cgPart0_.as<double>();
   v.cgPart[1] = cgPart1_.as<double>();
@@ -477,14 +477,15 @@ void HipSolver::harvestEvents() const {
 void HipSolver::spmvTiming(uint64_t& launches, double& meanUs, double& bytesPerLaunch) const {
   launches = spmvTimed_;
   meanUs = spmvTimed_ ? 1000.0 * spmvTotalMs_ / double(spmvTimed_) : 0.0;
-  // Algorithmic bytes of one fused PCG SpMV launch: block values and column index (40 B per block), and per
-  // particle the row and contact flags (8 B), z and p_old gathered once (24 B), p_new and ap written (24 B).
   const double n = double(model_.x.size());
   const double blocks = double(nBlocks_) - (double(nRows_) - n);
-  // The (pipelined) single-kernel iteration streams, per particle, r, w, s, p, step, u, y, q_old (96 B read), the
-  // preconditioner and contact state (56 B), the row flags (4 B), and writes q, s, p, step, r, u, w, y (96 B):
-  // 40 B per block + 252 B per particle.
-  bytesPerLaunch = 40.0 * blocks + (pcgVariant_ == 1 ? 252.0 : 56.0) * n;
+  // Algorithmic bytes of one PCG matrix launch: block values and column index (40 B per block), and per particle
+  // - two-kernel form (kPcgSpmv): row and contact flags (8 B), z and p_old gathered once (24 B), p_new and ap
+  //   written (24 B): 56 B;
+  // - one-kernel form (kPcgStep): zr, t, p gathered once (36 B), own r and ap (24 B), step (12 B), the
+  //   preconditioner (36 B), contact flags and normal (20 B), row flags (4 B) read; r, ap, p, zr, t and step
+  //   written (72 B): 204 B.
+  bytesPerLaunch = 40.0 * blocks + (pcgVariant_ == 1 ? 204.0 : 56.0) * n;
 }
 
 bool HipSolver::allSegments(bool (*pred)(const dk::SegState&)) const {
@@ -503,6 +504,12 @@ void HipSolver::releaseGraph() {
 void HipSolver::setPcgVariant(int variant) {
   if (variant != 0 && variant != 1) throw std::invalid_argument("pcg variant must be 0 or 1");
   pcgVariant_ = variant;
+}
+
+void HipSolver::setPcgWaves(int waves) {
+  if (waves != 0 && waves != 1 && waves != 2 && waves != 4)
+    throw std::invalid_argument("PCG wavefronts per slice must be 0 (tune), 1, 2 or 4");
+  pcgWaves_ = waves;
 }
 
 void HipSolver::runPcg() {
@@ -528,14 +535,7 @@ void HipSolver::runPcgStandard() {
     const uint32_t end = std::min(maxCg, k + chunk);
     for (; k < end; ++k) {
       if (spmvEvery_ > 0 && k % uint32_t(spmvEvery_) == 0) {
-        if (eventPool_.empty()) {
-          hipEvent_t a, b;
-          check(hipEventCreate(&a), "event");
-          check(hipEventCreate(&b), "event");
-          eventPool_.push_back({a, b});
-        }
-        const auto pr = eventPool_.back();
-        eventPool_.pop_back();
+        const auto pr = takeEvents();
         check(hipEventRecord(pr.first, stream_), "event");
         dk::launchPcgSpmv(view_, k, stream_);
         check(hipEventRecord(pr.second, stream_), "event");
@@ -554,7 +554,7 @@ void HipSolver::runPcgStandard() {
   lastPcgIterations_ = total;
 }
 
-void HipSolver::timedCgIteration(uint32_t parity, bool direct) {
+std::pair<hipEvent_t, hipEvent_t> HipSolver::takeEvents() {
   if (eventPool_.empty()) {
     hipEvent_t a, b;
     check(hipEventCreate(&a), "event");
@@ -563,38 +563,84 @@ void HipSolver::timedCgIteration(uint32_t parity, bool direct) {
   }
   const auto pr = eventPool_.back();
   eventPool_.pop_back();
+  return pr;
+}
+
+void HipSolver::timedPcgStep(uint32_t parity, uint32_t waves) {
+  const auto pr = takeEvents();
   check(hipEventRecord(pr.first, stream_), "event");
-  dk::launchCgIteration(view_, parity, direct, stream_);
+  dk::launchPcgStep(view_, parity, waves, stream_);
   check(hipEventRecord(pr.second, stream_), "event");
   eventPending_.push_back(pr);
 }
 
-// Chunks of kGraphChunk single-kernel iterations, replayed from a captured hipGraph (or launched eagerly). The
+// Solves run before a wave count is chosen: chunks of kGraphChunk launches rotate over W = 1, 2, 4 (the option
+// with the fewest samples goes next), each bracketed by HIP events and followed by a poll. A chunk counts only if
+// the solve was still running at its end, so every timed launch did a full iteration. Once every option has four
+// such chunks the fastest mean is kept; W only changes the launch shape, never the results.
+void HipSolver::runPcgTuning(uint32_t maxCg) {
+  static const uint32_t kOptions[3] = {1u, 2u, 4u};
+  uint32_t launched = 0;
+  while (launched < maxCg + 1) {
+    uint32_t o = 0;
+    for (uint32_t i = 1; i < 3; ++i)
+      if (tuneLaunches_[i] < tuneLaunches_[o]) o = i;
+    const uint32_t waves = tunedWaves_ ? uint32_t(tunedWaves_) : kOptions[o];
+    const auto pr = takeEvents();
+    check(hipEventRecord(pr.first, stream_), "event");
+    for (uint32_t i = 0; i < kGraphChunk; ++i) dk::launchPcgStep(view_, i & 1u, waves, stream_);
+    check(hipEventRecord(pr.second, stream_), "event");
+    launched += kGraphChunk;
+    pollStates();
+    const bool done = allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; });
+    if (!done && !tunedWaves_) {
+      float ms = 0.0f;
+      check(hipEventElapsedTime(&ms, pr.first, pr.second), "event time");
+      tuneMs_[o] += ms;
+      tuneLaunches_[o] += kGraphChunk;
+      if (*std::min_element(tuneLaunches_, tuneLaunches_ + 3) >= 4u * kGraphChunk) {
+        uint32_t best = 0;
+        for (uint32_t i = 1; i < 3; ++i)
+          if (tuneMs_[i] / tuneLaunches_[i] < tuneMs_[best] / tuneLaunches_[best]) best = i;
+        tunedWaves_ = int(kOptions[best]);
+      }
+    }
+    eventPool_.push_back(pr);
+    if (done) break;
+  }
+  uint32_t total = 0;
+  for (size_t s = 0; s < model_.segments.size(); ++s) total = std::max(total, hostStates_[s].pcgIter);
+  lastPcgIterations_ = total;
+}
+
+// Chunks of kGraphChunk one-kernel iterations, replayed from a captured hipGraph (or launched eagerly). The
 // iteration index lives on the device, so one graph serves every chunk; iterations after convergence return at
 // once. The host reads the converged flags after the first batch (sized from the previous solve) and then after
 // every chunk.
 void HipSolver::runPcgSingleKernel() {
-  dk::launchCgInit(view_, stream_);
+  dk::launchPcgStepInit(view_, stream_);
   const uint32_t maxCg = maxCgOf(model_);
+  const uint32_t waves = uint32_t(pcgWaves());
+  if (waves == 0) {
+    runPcgTuning(maxCg);
+    return;
+  }
   const bool sampling = spmvEvery_ > 0;
-  if (useGraph_ && !graphExec_) {
+  if (useGraph_ && (!graphExec_ || graphWaves_ != waves)) {
+    releaseGraph();
     check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
-    for (uint32_t i = 0; i < kGraphChunk; ++i) dk::launchCgIteration(view_, i & 1u, i == 0, stream_);
+    for (uint32_t i = 0; i < kGraphChunk; ++i) dk::launchPcgStep(view_, i & 1u, waves, stream_);
     check(hipStreamEndCapture(stream_, &graph_), "capture");
     check(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0), "graph instantiate");
+    graphWaves_ = waves;
   }
-  auto iterationsDone = [&] {
-    uint32_t total = 0;
-    for (size_t s = 0; s < model_.segments.size(); ++s) total = std::max(total, hostStates_[s].pcgIter);
-    return total;
-  };
   uint32_t launched = 0;
   uint32_t chunks = std::max<uint32_t>(1, (lastPcgIterations_ * 7 / 8) / kGraphChunk);
   while (launched < maxCg + 1) {
     if (useGraph_ && sampling) {
       // One event-timed iteration (and its partner, keeping the graph's parity pattern) opens each batch.
-      timedCgIteration(0u, false);
-      dk::launchCgIteration(view_, 1u, false, stream_);
+      timedPcgStep(0u, waves);
+      dk::launchPcgStep(view_, 1u, waves, stream_);
       launched += 2;
     }
     for (uint32_t c = 0; c < chunks; ++c) {
@@ -603,9 +649,9 @@ void HipSolver::runPcgSingleKernel() {
       } else {
         for (uint32_t i = 0; i < kGraphChunk; ++i) {
           if (sampling && (launched + i) % uint32_t(spmvEvery_) == 1)
-            timedCgIteration(i & 1u, i == 0);
+            timedPcgStep(i & 1u, waves);
           else
-            dk::launchCgIteration(view_, i & 1u, i == 0, stream_);
+            dk::launchPcgStep(view_, i & 1u, waves, stream_);
         }
       }
       launched += kGraphChunk;
@@ -614,7 +660,9 @@ void HipSolver::runPcgSingleKernel() {
     if (allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; })) break;
     chunks = 1;
   }
-  lastPcgIterations_ = iterationsDone();
+  uint32_t total = 0;
+  for (size_t s = 0; s < model_.segments.size(); ++s) total = std::max(total, hostStates_[s].pcgIter);
+  lastPcgIterations_ = total;
 }
 
 void HipSolver::substep(double h, int iterations) {
@@ -984,6 +1032,50 @@ void HipSolver::debugAssemble(double h, const float* u, double* energy, float* g
       if (map[s] != UINT32_MAX) std::copy(&internal[9 * size_t(s)], &internal[9 * size_t(s)] + 9, blocks + 9 * size_t(map[s]));
   }
   check(hipMemcpy(cforce_.as<void>(), savedForce.data(), sizeof(float) * nRows_, hipMemcpyHostToDevice), "copy");
+}
+
+void HipSolver::debugSystem(float* blocks, float* grad, float* precond, uint32_t* contactFlags,
+                            float* contactNormals) {
+  requireBuilt();
+  check(hipStreamSynchronize(stream_), "sync");
+  const size_t n = model_.x.size();
+  if (blocks) {
+    DeviceBuffer tmp;
+    tmp.allocate(sizeof(float) * 9 * nBlocks_);
+    dk::launchSellToSlots(view_, tmp.as<float>(), stream_);
+    std::vector<float> internal(9 * size_t(nBlocks_));
+    check(hipMemcpyAsync(internal.data(), tmp.as<void>(), sizeof(float) * internal.size(), hipMemcpyDeviceToHost, stream_),
+          "copy");
+    check(hipStreamSynchronize(stream_), "sync");
+    std::vector<uint32_t> rs, ci, ds, map;
+    globalStructure(rs, ci, ds, map);
+    for (uint32_t s = 0; s < nBlocks_; ++s)
+      if (map[s] != UINT32_MAX) std::copy(&internal[9 * size_t(s)], &internal[9 * size_t(s)] + 9, blocks + 9 * size_t(map[s]));
+  }
+  if (grad) {
+    std::vector<dk::F3> g(nRows_ + 1);
+    check(hipMemcpy(g.data(), grad_.as<void>(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyDeviceToHost), "copy");
+    for (size_t i = 0; i < n; ++i) {
+      grad[3 * i] = g[rowOf_[i]].x;
+      grad[3 * i + 1] = g[rowOf_[i]].y;
+      grad[3 * i + 2] = g[rowOf_[i]].z;
+    }
+  }
+  if (precond) {
+    std::vector<float> m(9 * size_t(nRows_ + 1));
+    check(hipMemcpy(m.data(), precond_.as<void>(), sizeof(float) * m.size(), hipMemcpyDeviceToHost), "copy");
+    for (size_t i = 0; i < n; ++i) std::copy(&m[9 * size_t(rowOf_[i])], &m[9 * size_t(rowOf_[i])] + 9, precond + 9 * i);
+  }
+  if (contactFlags) {
+    std::vector<uint32_t> f(nRows_ + 1);
+    check(hipMemcpy(f.data(), cflags_.as<void>(), sizeof(uint32_t) * (nRows_ + 1), hipMemcpyDeviceToHost), "copy");
+    for (size_t i = 0; i < n; ++i) contactFlags[i] = f[rowOf_[i]];
+  }
+  if (contactNormals) {
+    std::vector<float> c(4 * size_t(nRows_ + 1));
+    check(hipMemcpy(c.data(), cnormal_.as<void>(), sizeof(float) * c.size(), hipMemcpyDeviceToHost), "copy");
+    for (size_t i = 0; i < n; ++i) std::copy(&c[4 * size_t(rowOf_[i])], &c[4 * size_t(rowOf_[i])] + 4, contactNormals + 4 * i);
+  }
 }
 
 void HipSolver::debugSpmv(const float* x, float* y) {

@@ -60,8 +60,12 @@ class HipSolver {
   drape_counters counters() const;
   void resetCounters();
   void setGraph(bool enabled) { useGraph_ = enabled; }
-  // 1: single-kernel PCG (Chronopoulos-Gear form, default); 0: two kernels per iteration (standard form).
+  // 1: one kernel and one reduction per PCG iteration (default); 0: two kernels per iteration.
   void setPcgVariant(int variant);
+  // Wavefronts per 64-row slice in the one-kernel PCG iteration: 1, 2 or 4, or 0 to time all three during the
+  // first solves and keep the fastest (results are bit-identical for every choice).
+  void setPcgWaves(int waves);
+  int pcgWaves() const { return pcgWaves_ ? pcgWaves_ : tunedWaves_; }
   // Every `every`-th PCG SpMV launch of the step is bracketed by HIP events (0 = off).
   void setSpmvSampling(int every);
   void spmvTiming(uint64_t& launches, double& meanUs, double& bytesPerLaunch) const;
@@ -72,6 +76,7 @@ class HipSolver {
   void readTerms(uint32_t* stretchV, uint32_t* hingeV, double* hingeK, uint32_t* seamAB) const;
   void debugAssemble(double h, const float* u, double* energy, float* grad, float* blocks);
   void debugSpmv(const float* x, float* y);
+  void debugSystem(float* blocks, float* grad, float* precond, uint32_t* contactFlags, float* contactNormals);
 
  private:
   void requireBuilt() const;
@@ -89,7 +94,9 @@ class HipSolver {
   void runPcgStandard();
   void runPcgSingleKernel();
   void releaseGraph();
-  void timedCgIteration(uint32_t parity, bool direct);
+  void timedPcgStep(uint32_t parity, uint32_t waves);
+  std::pair<hipEvent_t, hipEvent_t> takeEvents();
+  void runPcgTuning(uint32_t maxCg);
   void globalStructure(std::vector<uint32_t>& rowStart, std::vector<uint32_t>& colIndex,
                        std::vector<uint32_t>& diagSlot, std::vector<uint32_t>& internalToGlobalSlot) const;
 
@@ -113,11 +120,16 @@ class HipSolver {
   DeviceBuffer incCount_, incStart_, inc_, rowLen_, rowStart_, colIndex_, diagSlot_;
   DeviceBuffer stretchSlots_, hingeSlots_, seamSlots_, sellK_, sellBase_, sellCol_, sellVal_, buildError_;
   DeviceBuffer fieldValues_, scratchF3_, scratchF3b_;
-  DeviceBuffer cgBuf_[12], cgPart0_, cgPart1_;
+  DeviceBuffer cgBuf_[10], cgPart0_, cgPart1_;
   int pcgVariant_ = 1;
+  int pcgWaves_ = 0;      // requested (0 = tune)
+  int tunedWaves_ = 0;    // chosen by the tuner (0 until it has measured every option)
+  double tuneMs_[3] = {0.0, 0.0, 0.0};
+  uint32_t tuneLaunches_[3] = {0u, 0u, 0u};
   static constexpr uint32_t kGraphChunk = 16;  // PCG iterations per captured graph (even: parity pattern repeats)
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graphExec_ = nullptr;
+  uint32_t graphWaves_ = 0;  // W the captured graph launches
   dk::SegState* hostStates_ = nullptr;  // pinned host copy
   int spmvEvery_ = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> eventPool_, eventPending_;

@@ -283,12 +283,15 @@ def _declare(lib):
         "drape_solver_set_graph": (C.c_int, [S, C.c_int32]),
         "drape_solver_set_spmv_sampling": (C.c_int, [S, C.c_int32]),
         "drape_solver_set_pcg_variant": (C.c_int, [S, C.c_int32]),
+        "drape_solver_set_pcg_waves": (C.c_int, [S, C.c_int32]),
+        "drape_solver_pcg_waves": (C.c_int, [S, C.POINTER(C.c_int32)]),
         "drape_solver_spmv_timing": (C.c_int, [S, C.POINTER(C.c_uint64), f64p, f64p]),
         "drape_solver_structure_sizes": (C.c_int, [S, C.POINTER(StructureSizes)]),
         "drape_solver_read_structure": (C.c_int, [S, u32p, u32p, u32p, u32p, u32p, u32p]),
         "drape_solver_read_terms": (C.c_int, [S, u32p, u32p, f64p, u32p]),
         "drape_solver_debug_assemble": (C.c_int, [S, C.c_double, f32p, f64p, f32p, f32p]),
         "drape_solver_debug_spmv": (C.c_int, [S, f32p, f32p]),
+        "drape_solver_debug_system": (C.c_int, [S, f32p, f32p, f32p, u32p, f32p]),
         "drape_bsr_spmv": (C.c_int, [C.c_uint32, C.c_uint32, u32p, u32p, f32p, u8p, f32p, f32p, C.c_int32,
                                      f64p]),
         "drape_eval_terms": (C.c_int, [C.c_int32, C.c_uint32, f32p, f32p, C.POINTER(Fabric), C.c_float, f32p,

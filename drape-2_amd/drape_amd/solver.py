@@ -46,7 +46,7 @@ class HipSolver:
 
     backend = "hip"
 
-    def __init__(self, graph: bool = True, pcg_variant: int = 1):
+    def __init__(self, graph: bool = True, pcg_variant: int = 1, pcg_waves: int = 0):
         self._h = C.c_void_p()
         _check(_lib().drape_solver_create(C.byref(self._h)))
         self._n = 0
@@ -54,6 +54,8 @@ class HipSolver:
             self.set_graph(False)
         if pcg_variant != 1:
             self.set_pcg_variant(pcg_variant)
+        if pcg_waves:
+            self.set_pcg_waves(pcg_waves)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -145,8 +147,17 @@ class HipSolver:
         _check(_lib().drape_solver_set_graph(self._h, 1 if enabled else 0))
 
     def set_pcg_variant(self, variant: int) -> None:
-        """1: single-kernel PCG (default); 0: the two-kernel standard form."""
+        """1: one kernel per PCG iteration (default); 0: the two-kernel standard form."""
         _check(_lib().drape_solver_set_pcg_variant(self._h, int(variant)))
+
+    def set_pcg_waves(self, waves: int) -> None:
+        """Wavefronts per 64-row slice in the one-kernel PCG iteration: 1, 2, 4, or 0 to tune (default)."""
+        _check(_lib().drape_solver_set_pcg_waves(self._h, int(waves)))
+
+    def pcg_waves(self) -> int:
+        out = C.c_int32()
+        _check(_lib().drape_solver_pcg_waves(self._h, C.byref(out)))
+        return out.value
 
     def set_spmv_sampling(self, every: int) -> None:
         _check(_lib().drape_solver_set_spmv_sampling(self._h, int(every)))
@@ -199,6 +210,23 @@ class HipSolver:
         _check(_lib().drape_solver_debug_assemble(self._h, float(h), ptr(ua, C.c_float), C.byref(e),
                                                   ptr(grad, C.c_float), ptr(blocks, C.c_float)))
         return e.value, grad, blocks
+
+    def debug_system(self) -> dict:
+        """The last Newton iteration's system: blocks (slot order), grad, block-Jacobi inverse, contact flags and
+        contact normals (xyz, offset)."""
+        sz = self.structure_sizes()
+        out = {
+            "blocks": np.empty((sz["n_blocks"], 3, 3), np.float32),
+            "grad": np.empty((self._n, 3), np.float32),
+            "precond": np.empty((self._n, 3, 3), np.float32),
+            "contact_flags": np.empty(self._n, np.uint32),
+            "contact_normals": np.empty((self._n, 4), np.float32),
+        }
+        _check(_lib().drape_solver_debug_system(self._h, ptr(out["blocks"], C.c_float), ptr(out["grad"], C.c_float),
+                                                ptr(out["precond"], C.c_float),
+                                                ptr(out["contact_flags"], C.c_uint32),
+                                                ptr(out["contact_normals"], C.c_float)))
+        return out
 
     def debug_spmv(self, x: np.ndarray) -> np.ndarray:
         xa = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)

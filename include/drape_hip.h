@@ -189,15 +189,23 @@ int drape_solver_reset_counters(drape_solver* s);
  * launch eagerly (0). Results are bit-identical either way. */
 int drape_solver_set_graph(drape_solver* s, int32_t enabled);
 
-/* PCG form: 1 (default) one kernel and one global reduction per iteration (Chronopoulos-Gear form of the same
- * preconditioned CG: identical iterates in exact arithmetic, same stopping rules); 0 the two-kernel standard form
- * (cpu_solver.cpp:169-213 step by step). */
+/* PCG form: 1 (default) one kernel and one global reduction per iteration (the standard preconditioned CG with
+ * p.Ap reduced directly and r.z, r.r expanded exactly over the previous iteration's vectors: the same iterates and
+ * stopping rules); 0 the two-kernel standard form (cpu_solver.cpp:169-213 step by step). */
 int drape_solver_set_pcg_variant(drape_solver* s, int32_t variant);
+/* Launch shape of the one-kernel PCG iteration: 1, 2 or 4 wavefronts per 64-row slice, or 0 (default) to time
+ * all three during the first solves and keep the fastest. Results are bit-identical for every value.
+ * drape_solver_pcg_waves reports the value in use (0 while still tuning). No reference counterpart: the
+ * reference's GPU path has one thread per row (kernels.h:272-289). */
+int drape_solver_set_pcg_waves(drape_solver* s, int32_t waves);
+int drape_solver_pcg_waves(const drape_solver* s, int32_t* out);
 
 /* Kernel timing inside the step: every `every`-th PCG SpMV launch (0 = off) is bracketed by HIP events on the
  * solver's stream. drape_solver_spmv_timing reports how many launches were timed since sampling was set, their
- * mean duration in microseconds and the algorithmic bytes one launch moves (40 B per block for values and column
- * index, plus 56 B per particle: row and contact flags, z and p_old gathered, p_new and ap written). */
+ * mean duration in microseconds and the algorithmic bytes one launch moves: 40 B per block for values and column
+ * index, plus per particle 204 B for the one-kernel form (zr, t, p gathered; r, ap, step, preconditioner, contact
+ * and row flags read; r, ap, p, zr, t, step written) or 56 B for the two-kernel SpMV (row and contact flags, z and
+ * p_old gathered, p_new and ap written). */
 int drape_solver_set_spmv_sampling(drape_solver* s, int32_t every);
 int drape_solver_spmv_timing(const drape_solver* s, uint64_t* launches, double* mean_us, double* bytes_per_launch);
 
@@ -223,6 +231,12 @@ int drape_solver_debug_assemble(drape_solver* s, double h, const float* u, doubl
                                 float* blocks);
 /* y = A x over the last assembled matrix, pinned rows/columns excluded (cpu_solver.cpp:147-159). */
 int drape_solver_debug_spmv(drape_solver* s, const float* x, float* y);
+/* The linear system of the last Newton iteration as the device holds it (cpu_solver.cpp:163-213): blocks in BSR
+ * slot order (9 per block), gradient (3 per particle), block-Jacobi inverse (9 per particle, zero when pinned),
+ * contact flags (bit 0 near, bit 1 active) and contact normals (4 per particle: normal, offset). Any output may be
+ * NULL. */
+int drape_solver_debug_system(drape_solver* s, float* blocks, float* grad, float* precond, uint32_t* contact_flags,
+                              float* contact_normals);
 
 /* Standalone block-CSR SpMV on caller arrays (host memory): copies in, runs the solver's SpMV kernel `reps`
  * times, copies y out, and reports the mean kernel time in microseconds (HIP events). pinned may be NULL. */

@@ -1630,9 +1630,9 @@ int oracle_invert_spd3(const double* a9, double* inv9) {
   return ok ? 1 : 0;
 }
 
-int oracle_bsr_pcg(uint32_t n, const uint32_t* row_start, const uint32_t* col_index, const double* blocks,
-                   const uint8_t* pinned, const double* rhs, double tol, int32_t max_iter, double* x,
-                   int32_t* iterations) {
+int oracle_bsr_pcg_contact(uint32_t n, const uint32_t* row_start, const uint32_t* col_index, const double* blocks,
+                           const uint8_t* pinned, const uint8_t* active, const double* normals, const double* rhs,
+                           double tol, int32_t max_iter, double* x, int32_t* iterations, double* contact_force) {
   return guard([&] {
     orc::Solver so;
     so.m.x.assign(n, orc::V3f{});
@@ -1646,6 +1646,13 @@ int oracle_bsr_pcg(uint32_t n, const uint32_t* row_start, const uint32_t* col_in
     so.m.settings.cgTolerance = tol;
     so.m.settings.maxCgIterations = max_iter;
     so.allocate();
+    if (active) {
+      if (!normals) throw std::invalid_argument("contact normals required with active contacts");
+      for (uint32_t i = 0; i < n; ++i) {
+        so.contactActive[i] = active[i] ? 1 : 0;
+        so.contactNormal[i] = {normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]};
+      }
+    }
     for (size_t k = 0; k < so.blocks.size(); ++k) std::memcpy(so.blocks[k].a, blocks + 9 * k, sizeof(double) * 9);
     for (uint32_t i = 0; i < n; ++i) so.grad[i] = {-rhs[3 * i], -rhs[3 * i + 1], -rhs[3 * i + 2]};
     so.solveLinear();
@@ -1655,7 +1662,16 @@ int oracle_bsr_pcg(uint32_t n, const uint32_t* row_start, const uint32_t* col_in
       x[3 * i + 2] = so.step[i].z;
     }
     if (iterations) *iterations = static_cast<int32_t>(so.counters.pcg_iterations);
+    if (contact_force)
+      for (uint32_t i = 0; i < n; ++i) contact_force[i] = so.m.contactForce[i];
   });
+}
+
+int oracle_bsr_pcg(uint32_t n, const uint32_t* row_start, const uint32_t* col_index, const double* blocks,
+                   const uint8_t* pinned, const double* rhs, double tol, int32_t max_iter, double* x,
+                   int32_t* iterations) {
+  return oracle_bsr_pcg_contact(n, row_start, col_index, blocks, pinned, nullptr, nullptr, rhs, tol, max_iter, x,
+                                iterations, nullptr);
 }
 
 }  // extern "C"

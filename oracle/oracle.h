@@ -83,6 +83,12 @@ int oracle_invert_spd3(const double* a9, double* inv9);
 int oracle_bsr_pcg(uint32_t n, const uint32_t* row_start, const uint32_t* col_index, const double* blocks,
                    const uint8_t* pinned, const double* rhs, double tol, int32_t max_iter, double* x,
                    int32_t* iterations);
+/* The same with held contacts: rows with active[i] != 0 solve in the plane normal to normals[3i..3i+2]
+ * (filterContact, cpu_solver.cpp:163-167). contact_force (may be NULL) receives each held row's normal force
+ * from the unfiltered residual (cpu_solver.cpp:210-212). */
+int oracle_bsr_pcg_contact(uint32_t n, const uint32_t* row_start, const uint32_t* col_index, const double* blocks,
+                           const uint8_t* pinned, const uint8_t* active, const double* normals, const double* rhs,
+                           double tol, int32_t max_iter, double* x, int32_t* iterations, double* contact_force);
 
 #ifdef __cplusplus
 }

@@ -82,6 +82,8 @@ def lib():
         "oracle_invert_spd3": (C.c_int, [f64p, f64p]),
         "oracle_bsr_pcg": (C.c_int, [C.c_uint32, u32p, u32p, f64p, u8p, f64p, C.c_double, C.c_int32, f64p,
                                      C.POINTER(C.c_int32)]),
+        "oracle_bsr_pcg_contact": (C.c_int, [C.c_uint32, u32p, u32p, f64p, u8p, u8p, f64p, f64p, C.c_double,
+                                             C.c_int32, f64p, C.POINTER(C.c_int32), f64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -367,3 +369,23 @@ def bsr_pcg(row_start, col_index, blocks, pinned, rhs, tol, max_iter):
                                ptr(pn, C.c_uint8), ptr(b, C.c_double), float(tol), int(max_iter),
                                ptr(x, C.c_double), C.byref(it)))
     return x, it.value
+
+
+def bsr_pcg_contact(row_start, col_index, blocks, pinned, active, normals, rhs, tol, max_iter):
+    """solveLinear with held contacts: (step, PCG iterations, contact force per row)."""
+    n = len(row_start) - 1
+    rs = np.ascontiguousarray(row_start, np.uint32)
+    ci = np.ascontiguousarray(col_index, np.uint32)
+    bl = _d(blocks).reshape(-1)
+    pn = None if pinned is None else np.ascontiguousarray(pinned, np.uint8)
+    ac = np.ascontiguousarray(active, np.uint8)
+    nm = _d(normals).reshape(-1)
+    b = _d(rhs).reshape(-1)
+    x = np.empty((n, 3))
+    force = np.empty(n)
+    it = C.c_int32()
+    check(lib().oracle_bsr_pcg_contact(n, ptr(rs, C.c_uint32), ptr(ci, C.c_uint32), ptr(bl, C.c_double),
+                                       ptr(pn, C.c_uint8), ptr(ac, C.c_uint8), ptr(nm, C.c_double),
+                                       ptr(b, C.c_double), float(tol), int(max_iter), ptr(x, C.c_double),
+                                       C.byref(it), ptr(force, C.c_double)))
+    return x, it.value, force
