@@ -61,7 +61,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256, help="sheet vertices per side (256 = the 65,536-particle sheet)")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sample-every", type=int, default=8,
                     help="event-time PCG iteration launches (graph path: one per poll; eager: every k-th)")

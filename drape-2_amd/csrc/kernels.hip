@@ -10,6 +10,7 @@
 // shuffle tree, and every workgroup that needs a segment total re-sums the same partials in the same order.
 // Integer atomics appear only in the structure builder's counting pass, whose results are order independent (the
 // lists they fill are sorted afterwards).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -1310,10 +1311,11 @@ void launchAssemble(const View& v, uint32_t maxK, bool, hipStream_t st) {
 }
 void launchNewtonBegin(const View& v, hipStream_t st) { kNewtonBegin<<<v.nSegments, 256, 0, st>>>(v); }
 void launchPcgInit(const View& v, hipStream_t st) { kPcgInit<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
-void launchPcgSpmv(const View& v, uint32_t k, hipStream_t st) {
+void launchPcgSpmv(const View& v, uint32_t k, hipStream_t st, hipEvent_t start, hipEvent_t stop) {
   F3* pOld = (k & 1u) ? v.pB : v.pA;
   F3* pNew = (k & 1u) ? v.pA : v.pB;
-  kPcgSpmv<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, k, pOld, pNew);
+  hipExtLaunchKernelGGL(kPcgSpmv, dim3(xcdGrid(v.nTiles)), dim3(256), 0, st, start, stop, 0, v, k,
+                        static_cast<const F3*>(pOld), pNew);
 }
 void launchPcgUpdate(const View& v, uint32_t k, hipStream_t st) {
   kPcgUpdate<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, k, (k & 1u) ? v.pA : v.pB);
@@ -1323,11 +1325,13 @@ void launchPcgIteration(const View& v, uint32_t k, hipStream_t st) {
   launchPcgUpdate(v, k, st);
 }
 void launchPcgStepInit(const View& v, hipStream_t st) { kPcgStepInit<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
-void launchPcgStep(const View& v, uint32_t parity, uint32_t wavesPerSlice, hipStream_t st) {
+void launchPcgStep(const View& v, uint32_t parity, uint32_t wavesPerSlice, hipStream_t st, hipEvent_t start,
+                   hipEvent_t stop) {
+  const dim3 grid(xcdGrid(v.nTiles));
   switch (wavesPerSlice) {
-    case 1: kPcgStep<1><<<xcdGrid(v.nTiles), 256, 0, st>>>(v, parity & 1u); break;
-    case 2: kPcgStep<2><<<xcdGrid(v.nTiles), 512, 0, st>>>(v, parity & 1u); break;
-    case 4: kPcgStep<4><<<xcdGrid(v.nTiles), 1024, 0, st>>>(v, parity & 1u); break;
+    case 1: hipExtLaunchKernelGGL(kPcgStep<1>, grid, dim3(256), 0, st, start, stop, 0, v, parity & 1u); break;
+    case 2: hipExtLaunchKernelGGL(kPcgStep<2>, grid, dim3(512), 0, st, start, stop, 0, v, parity & 1u); break;
+    case 4: hipExtLaunchKernelGGL(kPcgStep<4>, grid, dim3(1024), 0, st, start, stop, 0, v, parity & 1u); break;
     default: throw std::invalid_argument("PCG wavefronts per slice must be 1, 2 or 4");
   }
 }

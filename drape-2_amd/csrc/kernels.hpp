@@ -109,11 +109,13 @@ void launchAssemble(const View& v, uint32_t maxK, bool contact, hipStream_t st);
 void launchNewtonBegin(const View& v, hipStream_t st);
 void launchPcgInit(const View& v, hipStream_t st);
 void launchPcgIteration(const View& v, uint32_t k, hipStream_t st);
-void launchPcgSpmv(const View& v, uint32_t k, hipStream_t st);
+// start/stop (optional): events the launch records at kernel start and end (hipExtLaunchKernel).
+void launchPcgSpmv(const View& v, uint32_t k, hipStream_t st, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void launchPcgUpdate(const View& v, uint32_t k, hipStream_t st);
 void launchPcgStepInit(const View& v, hipStream_t st);
 // One PCG iteration of the one-reduction form; wavesPerSlice (1, 2 or 4) wavefronts share each 64-row slice.
-void launchPcgStep(const View& v, uint32_t parity, uint32_t wavesPerSlice, hipStream_t st);
+void launchPcgStep(const View& v, uint32_t parity, uint32_t wavesPerSlice, hipStream_t st, hipEvent_t start = nullptr,
+                   hipEvent_t stop = nullptr);
 void launchContactForceSlope(const View& v, hipStream_t st);
 void launchNewtonSlope(const View& v, hipStream_t st);
 void launchEnergyTrial(const View& v, hipStream_t st);

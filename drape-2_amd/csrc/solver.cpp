@@ -536,9 +536,7 @@ void HipSolver::runPcgStandard() {
     for (; k < end; ++k) {
       if (spmvEvery_ > 0 && k % uint32_t(spmvEvery_) == 0) {
         const auto pr = takeEvents();
-        check(hipEventRecord(pr.first, stream_), "event");
-        dk::launchPcgSpmv(view_, k, stream_);
-        check(hipEventRecord(pr.second, stream_), "event");
+        dk::launchPcgSpmv(view_, k, stream_, pr.first, pr.second);
         eventPending_.push_back(pr);
         dk::launchPcgUpdate(view_, k, stream_);
       } else {
@@ -568,9 +566,7 @@ std::pair<hipEvent_t, hipEvent_t> HipSolver::takeEvents() {
 
 void HipSolver::timedPcgStep(uint32_t parity, uint32_t waves) {
   const auto pr = takeEvents();
-  check(hipEventRecord(pr.first, stream_), "event");
-  dk::launchPcgStep(view_, parity, waves, stream_);
-  check(hipEventRecord(pr.second, stream_), "event");
+  dk::launchPcgStep(view_, parity, waves, stream_, pr.first, pr.second);
   eventPending_.push_back(pr);
 }
 
@@ -638,9 +634,10 @@ void HipSolver::runPcgSingleKernel() {
   uint32_t chunks = std::max<uint32_t>(1, (lastPcgIterations_ * 7 / 8) / kGraphChunk);
   while (launched < maxCg + 1) {
     if (useGraph_ && sampling) {
-      // One event-timed iteration (and its partner, keeping the graph's parity pattern) opens each batch.
-      timedPcgStep(0u, waves);
-      dk::launchPcgStep(view_, 1u, waves, stream_);
+      // Two eager iterations (keeping the graph's parity pattern) open each batch; the second, which starts
+      // right behind the first as graph nodes do, is event-timed.
+      dk::launchPcgStep(view_, 0u, waves, stream_);
+      timedPcgStep(1u, waves);
       launched += 2;
     }
     for (uint32_t c = 0; c < chunks; ++c) {
