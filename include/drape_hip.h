@@ -4,7 +4,7 @@
  * (libs/sim/include/drape/sim/solver.hpp:85-98) and its backend switch
  * (libs/sim/include/drape/sim/solver_factory.hpp:12-20). Every entry point is plain C:
  * opaque handles, plain pointers and sizes, an int status code, and a thread-local last-error
- * string. A C++ caller uses the header-only `drape::sim::HipSolver` wrapper
+ * string. A C++ caller uses the header-only `drape::sim::hip::HipSolver` wrapper
  * (include/drape/sim/hip_solver.hpp), which maps status codes back to the reference's
  * exception types.
  *
@@ -176,8 +176,14 @@ int drape_solver_read_velocities(const drape_solver* s, float* out, size_t n);
 int drape_solver_stats(const drape_solver* s, drape_stats* out);
 
 /* Solver::snapshot / restore (solver.hpp:94-95) as one flat byte blob: query the size, save into a caller
- * buffer, restore from one. Layout: positions, velocities (float3 each), phases, assemblyTime, seamStartGap,
- * contactForce (sim_model.cpp:243-274). */
+ * buffer, restore from one (sim_model.cpp:243-274). Little-endian layout, no padding:
+ *   uint64 n_particles, uint64 n_garments, uint64 n_seams
+ *   float  positions[n_particles][3], velocities[n_particles][3]
+ *   int32  phase[n_garments]            (DRAPE_PHASE_*)
+ *   double assembly_time[n_garments]
+ *   float  seam_start_gap[n_seams]      (seam springs in build order)
+ *   double contact_force[n_particles]
+ * restore rejects a blob whose counts differ from the built scene (std::invalid_argument). */
 int drape_solver_snapshot_size(const drape_solver* s, size_t* bytes);
 int drape_solver_snapshot_save(const drape_solver* s, void* out, size_t bytes);
 int drape_solver_snapshot_restore(drape_solver* s, const void* in, size_t bytes);
