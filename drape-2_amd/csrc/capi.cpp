@@ -419,7 +419,7 @@ int drape_bsr_spmv(uint32_t n, uint32_t n_blocks, const uint32_t* row_start, con
         const uint32_t k = s - row_start[r];
         const uint32_t c = col_index[s];
         sellCol[sellBase[t] / 9 + k * dk::kSlice + l] = (pinned && pinned[c]) ? nRows : c;
-        for (int q = 0; q < 9; ++q) sellVal[sellBase[t] + (k * 9 + q) * dk::kSlice + l] = blocks[9 * size_t(s) + q];
+        for (int q = 0; q < 9; ++q) sellVal[sellBase[t] + (k * dk::kSlice + l) * 9 + q] = blocks[9 * size_t(s) + q];
       }
     }
     Stream st;

@@ -38,6 +38,14 @@ __device__ inline uint32_t xcdMap(uint32_t bid, uint32_t count) {
   return (bid & 7u) * per + (bid >> 3);
 }
 
+// Load of a workgroup-uniform value through the scalar cache (constant address space). Only for data no
+// workgroup of the running kernel writes: the scalar cache is invalidated at every kernel start, so values
+// written by earlier kernels are seen.
+template <class T>
+__device__ inline T ldUniform(const T* p) {
+  return *reinterpret_cast<const __attribute__((address_space(4))) T*>(reinterpret_cast<uintptr_t>(p));
+}
+
 // Sum over a 256-thread workgroup, fixed order; the result is returned to every thread.
 __device__ inline double blockSum256(double v, double* lds) {
 #pragma unroll
@@ -110,7 +118,7 @@ template <class Gather>
 __device__ inline F3 sellRowProduct(const View& v, uint32_t slice, uint32_t lane, Gather gather) {
   const uint32_t K = v.sellK[slice];
   const uint32_t base = v.sellBase[slice];
-  const float* __restrict__ val = v.sellVal + base + lane;
+  const float* __restrict__ val = v.sellVal + base + 9u * lane;
   const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
   float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
   for (uint32_t k0 = 0; k0 < K; k0 += 8u) {
@@ -124,9 +132,9 @@ __device__ inline F3 sellRowProduct(const View& v, uint32_t slice, uint32_t lane
     for (int t = 0; t < 8; ++t) {
       if (k0 + t < K) {
         const float* bv = val + (k0 + t) * 9u * kSlice;
-        a0 += bv[0 * kSlice] * x[t].x + bv[1 * kSlice] * x[t].y + bv[2 * kSlice] * x[t].z;
-        a1 += bv[3 * kSlice] * x[t].x + bv[4 * kSlice] * x[t].y + bv[5 * kSlice] * x[t].z;
-        a2 += bv[6 * kSlice] * x[t].x + bv[7 * kSlice] * x[t].y + bv[8 * kSlice] * x[t].z;
+        a0 += bv[0] * x[t].x + bv[1] * x[t].y + bv[2] * x[t].z;
+        a1 += bv[3] * x[t].x + bv[4] * x[t].y + bv[5] * x[t].z;
+        a2 += bv[6] * x[t].x + bv[7] * x[t].y + bv[8] * x[t].z;
       }
     }
   }
@@ -312,7 +320,7 @@ __global__ void __launch_bounds__(64) kAssemble(View v) {
     {
       const uint32_t kd = v.diagSlot[row] - rs;
 #pragma unroll
-      for (int c = 0; c < 9; ++c) acc[(kd * 9u + c) * kSlice + lane] += Hown.m[c];
+      for (int c = 0; c < 9; ++c) acc[(kd * kSlice + lane) * 9u + c] += Hown.m[c];
     }
     const uint32_t i1 = v.incStart[row + 1];
     for (uint32_t i = v.incStart[row]; i < i1; ++i) {
@@ -327,7 +335,7 @@ __global__ void __launch_bounds__(64) kAssemble(View v) {
       for (uint32_t b = 0; b < m; ++b) {
         const uint32_t kk = slots[b] - rs;
 #pragma unroll
-        for (int c = 0; c < 9; ++c) acc[(kk * 9u + c) * kSlice + lane] += H[b].m[c];
+        for (int c = 0; c < 9; ++c) acc[(kk * kSlice + lane) * 9u + c] += H[b].m[c];
       }
     }
   }
@@ -354,7 +362,7 @@ __global__ void __launch_bounds__(64) kAssemble(View v) {
       const uint32_t kd = v.diagSlot[row] - v.rowStart[row];
       float A[9];
 #pragma unroll
-      for (int c = 0; c < 9; ++c) A[c] = acc[(kd * 9u + c) * kSlice + lane];
+      for (int c = 0; c < 9; ++c) A[c] = acc[(kd * kSlice + lane) * 9u + c];
       if (!invertSpd3(A, M)) {
 #pragma unroll
         for (int c = 0; c < 9; ++c) M[c] = (c % 4 == 0) ? 1.0f : 0.0f;
@@ -633,42 +641,49 @@ __global__ void __launch_bounds__(256) kPcgStepInit(View v) {
 template <int W>
 __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
   constexpr int NW = 4 * W;
-  constexpr uint32_t B = W == 1 ? 8u : 16u / W;  // columns per wavefront in one batch
+  constexpr uint32_t B = 16u / W;  // columns per wavefront in one batch
   __shared__ double red[kPcgParts * NW];
   __shared__ float comb[(W > 1 ? 4 : 1) * kSlicesPerTile * 3 * kSlice];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
-  const uint32_t s = v.tileSeg[tile];
+  // Per-segment scalars through the scalar cache: none of them is written by this kernel (it writes only the
+  // other parity slot of kIter / convPar, and r0 when k == 0, where it does not read it).
+  const uint32_t s = ldUniform(v.tileSeg + tile);
   SegState* S = v.state + s;
-  const SegParams& P = v.seg[s];
+  const SegParams* P = v.seg + s;
+  const uint32_t tileBegin = ldUniform(&P->tileBegin), tileEnd = ldUniform(&P->tileEnd);
+  const uint32_t maxCg = ldUniform(&P->maxCg);
+  const double cgTolerance = ldUniform(&P->cgTolerance);
   const uint32_t o = 1u - q;
+  const uint32_t k = ldUniform(&S->kIter[q]);
   // A solve that an earlier kernel ended: pass the flag on to the next parity slot and load nothing.
-  if (S->newtonDone) return;
-  if (S->convPar[q]) {
-    if (tile == P.tileBegin && threadIdx.x == 0) {
+  if (ldUniform(&S->newtonDone)) return;
+  if (ldUniform(&S->convPar[q])) {
+    if (tile == tileBegin && threadIdx.x == 0) {
       S->convPar[o] = 1u;
-      S->kIter[o] = S->kIter[q] + 1u;
+      S->kIter[o] = k + 1u;
     }
     return;
   }
+  const double r0Prev = ldUniform(&S->r0);
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  const uint32_t sl = wave & 3u, part = wave >> 2;
+  const uint32_t sl = wave & 3u, part = W == 1 ? 0u : wave >> 2;
   // Slice-uniform values through the scalar cache.
   const uint32_t slice = __builtin_amdgcn_readfirstlane(tile * kSlicesPerTile + sl);
   const uint32_t row = slice * kSlice + lane;
   const F3* __restrict__ Zr = v.cgZr[o];
   const F3* __restrict__ T = v.cgT[o];
   const F3* __restrict__ Pp = v.cgP[o];
-  const uint32_t K = v.sellK[slice];
-  const uint32_t base = v.sellBase[slice];
+  const uint32_t K = ldUniform(v.sellK + slice);
+  const uint32_t base = ldUniform(v.sellBase + slice);
 
   // Round trip 1: the previous kernel's partial sums, the owner row's data and the column indices of the first
   // batch. Nothing here depends on the scalars.
   const double* __restrict__ prev = v.cgPart[o];
-  uint32_t ti = P.tileBegin + threadIdx.x;
+  uint32_t ti = tileBegin + threadIdx.x;
   double sum[kPcgParts];
 #pragma unroll
-  for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < P.tileEnd ? prev[kPcgParts * ti + n] : 0.0;
+  for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < tileEnd ? prev[kPcgParts * ti + n] : 0.0;
   F3 oR, oAp, oZ, oT, oP, oStep;
   float m[9];
   uint32_t oFlags = 0u, oCf = 0u;
@@ -687,14 +702,25 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
     oCn = v.cnormal[row];
   }
   const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
-  const float* __restrict__ val = v.sellVal + base + lane;
+  const float* __restrict__ val = v.sellVal + base + 9u * lane;
   uint32_t c[B];
 #pragma unroll
   for (uint32_t j = 0; j < B; ++j) {
     const uint32_t t = part + W * j;
     c[j] = t < K ? col[t * kSlice] : v.nRows;
   }
-  // Round trip 2: gathered (zr, t, p) and block values of the first batch.
+  // The first batch's block values (they need no column index, so they share the first round trip). Slots past
+  // K are read unconditionally — the value buffer carries kSellSlack zero floats past its end — because their
+  // column is the dummy one, whose gathered vectors are zero, so they add exactly nothing.
+  float bv[B][9];
+#pragma unroll
+  for (uint32_t j = 0; j < B; ++j) {
+    const uint32_t t = part + W * j;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) bv[j][e] = val[t * 9u * kSlice + e];
+  }
+  __builtin_amdgcn_sched_barrier(0);  // keep the block loads ahead of the wait for the column indices
+  // Round trip 2: gathered (zr, t, p) of the first batch.
   F3 gz[B], gt[B], gp[B];
 #pragma unroll
   for (uint32_t j = 0; j < B; ++j) {
@@ -702,26 +728,18 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
     gt[j] = T[c[j]];
     gp[j] = Pp[c[j]];
   }
-  float bv[B][9];
-#pragma unroll
-  for (uint32_t j = 0; j < B; ++j) {
-    const uint32_t t = part + W * j;
-#pragma unroll
-    for (int e = 0; e < 9; ++e) bv[j][e] = t < K ? val[(t * 9u + e) * kSlice] : 0.0f;
-  }
 
   // Scalars of this kernel (identical in every workgroup of the segment), while the gathers are in flight.
-  for (ti += 256u * W; ti < P.tileEnd; ti += 256u * W)
+  for (ti += 256u * W; ti < tileEnd; ti += 256u * W)
 #pragma unroll
     for (int n = 0; n < kPcgParts; ++n) sum[n] += prev[kPcgParts * ti + n];
   blockSumN<kPcgParts, NW>(sum, red);
-  const uint32_t k = S->kIter[q];
   uint32_t conv = 0u;
   bool update = false;
   double alpha = 0.0, beta = 0.0, r0 = 0.0, rrNext = sum[kPrR];
   if (k == 0) {
     r0 = sqrt(sum[kPrR]);
-    if (!(r0 > 0.0) || P.maxCg == 0u) conv = 1u;
+    if (!(r0 > 0.0) || maxCg == 0u) conv = 1u;
   } else if (!(sum[kPpAp] > 0.0)) {
     conv = 1u;
   } else {
@@ -729,12 +747,12 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
     alpha = sum[kPrZ] / sum[kPpAp];
     rrNext = sum[kPrR] - 2.0 * alpha * sum[kPrAp] + alpha * alpha * sum[kPapAp];
     const double rzNext = sum[kPrZ] - 2.0 * alpha * sum[kPrT] + alpha * alpha * sum[kPapT];
-    if (sqrt(fmax(rrNext, 0.0)) <= P.cgTolerance * S->r0 || k >= P.maxCg)
+    if (sqrt(fmax(rrNext, 0.0)) <= cgTolerance * r0Prev || k >= maxCg)
       conv = 1u;
     else
       beta = rzNext / sum[kPrZ];
   }
-  if (tile == P.tileBegin && threadIdx.x == 0) {
+  if (tile == tileBegin && threadIdx.x == 0) {
     if (k == 0) S->r0 = r0;
     S->kIter[o] = k + 1u;
     S->convPar[o] = conv;
@@ -778,9 +796,9 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
       if (t < K) {
         const float* bj = val + t * 9u * kSlice;
         float* g = acc[j % G];
-        g[0] += bj[0 * kSlice] * x.x + bj[1 * kSlice] * x.y + bj[2 * kSlice] * x.z;
-        g[1] += bj[3 * kSlice] * x.x + bj[4 * kSlice] * x.y + bj[5 * kSlice] * x.z;
-        g[2] += bj[6 * kSlice] * x.x + bj[7 * kSlice] * x.y + bj[8 * kSlice] * x.z;
+        g[0] += bj[0] * x.x + bj[1] * x.y + bj[2] * x.z;
+        g[1] += bj[3] * x.x + bj[4] * x.y + bj[5] * x.z;
+        g[2] += bj[6] * x.x + bj[7] * x.y + bj[8] * x.z;
       }
     }
   }
@@ -1037,9 +1055,9 @@ __global__ void __launch_bounds__(256) kSellToSlots(View v, float* out) {
   if (row >= v.nRows) return;
   const uint32_t slice = row / kSlice, lane = row % kSlice;
   const uint32_t rs = v.rowStart[row], re = v.rowStart[row + 1];
-  const float* base = v.sellVal + v.sellBase[slice] + lane;
+  const float* base = v.sellVal + v.sellBase[slice] + 9u * lane;
   for (uint32_t sl = rs; sl < re; ++sl)
-    for (int c = 0; c < 9; ++c) out[9u * sl + c] = base[((sl - rs) * 9u + c) * kSlice];
+    for (int c = 0; c < 9; ++c) out[9u * sl + c] = base[(sl - rs) * 9u * kSlice + c];
 }
 
 // Batch term evaluation (full gradient and Hessian from the row kernels, a = 0..N-1).

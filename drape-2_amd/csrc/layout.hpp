@@ -9,8 +9,10 @@
 // The Hessian is block-CSR over 3x3 blocks with the reference's pattern and slot numbering
 // (libs/sim/src/sim_model.cpp:41-74): rowStart/colIndex/diagSlot. The block values live in a sliced-ELL layout
 // (SELL-64) so that one lane per row streams them coalesced: slice t holds K_t = max row length of its 64 rows;
-// value (slot k of lane l, component c) is at sellBase[t] + (k * 9 + c) * 64 + l, column index at
-// sellBase[t] / 9 + k * 64 + l. Components are row-major (c = 3 * row + col). Padding entries carry the
+// value (slot k of lane l, component c) is at sellBase[t] + (k * 64 + l) * 9 + c — a lane's block is 9
+// consecutive floats and the 64 lanes' blocks of one slot are consecutive, so a wavefront reads 2,304
+// contiguous bytes per slot — and the column index at sellBase[t] / 9 + k * 64 + l. Components are row-major
+// (c = 3 * row + col). Padding entries carry the
 // dummy column `nRows` (a zero entry of every gathered vector) and zero values; so do pinned columns, which the
 // product excludes (cpu_solver.cpp:147-159).
 #pragma once
@@ -29,6 +31,9 @@ constexpr uint32_t kTile = 256;   // rows per tile = threads per workgroup
 constexpr uint32_t kSlicesPerTile = kTile / kSlice;
 constexpr uint32_t kMaxIncidence = 96;  // term incidences per particle the structure builder accepts
 constexpr uint32_t kMaxRow = 64;        // blocks per row the structure builder accepts
+// Zero floats after the last slice's block values: a kernel may read a whole batch of 16 slots of any slice
+// without testing each slot against K.
+constexpr uint32_t kSellSlack = 16u * 9u * 64u;
 
 // Row flags.
 constexpr uint32_t kRowPinned = 1u;

@@ -278,7 +278,7 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   maxK_ = maxK;
   if (maxK_ * 9ull * dk::kSlice * sizeof(float) > 64ull * 1024)
     throw std::invalid_argument("hip: a 64-row slice has more than 28 blocks in one row");
-  sellVal_.allocate(sizeof(float) * std::max<uint64_t>(sellFloats, 1));
+  sellVal_.allocate(sizeof(float) * (sellFloats + dk::kSellSlack));  // zero slack: see kSellSlack
 
   dk::View& v = view_;
   v = dk::View{};
