@@ -3,6 +3,7 @@
 // message in a thread-local string (drape_last_error).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -11,9 +12,33 @@
 #include "drape_hip.h"
 #include "kernels.hpp"
 #include "solver.hpp"
+#include "warmstart.hpp"
 
 struct drape_solver {
   dh::HipSolver impl;
+};
+
+// A network with the device buffers its host-array entry points stage through.
+struct drape_net {
+  explicit drape_net(uint64_t seed) : impl(seed) {
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
+      throw std::runtime_error("hip: stream creation failed");
+  }
+  ~drape_net() {
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+  }
+  ws::Net impl;
+  hipStream_t stream = nullptr;
+  dh::DeviceBuffer x, v, dv, pn, target, out;
+  uint32_t cap = 0;
+};
+
+struct drape_graph {
+  drape_graph(uint32_t n, const uint32_t* rs, const uint32_t* nbr, const double* rest2) : impl(n, rs, nbr, rest2) {}
+  ws::Graph impl;
 };
 
 namespace {
@@ -532,6 +557,163 @@ int drape_field_query(const drape_field_desc* field, uint32_t n, const float* p,
     check(hipStreamSynchronize(st.s), "query");
     check(hipMemcpy(phi, dphi.p, sizeof(float) * n, hipMemcpyDeviceToHost), "copy");
     check(hipMemcpy(normal, dn.p, sizeof(float) * 3 * size_t(n), hipMemcpyDeviceToHost), "copy");
+  });
+}
+
+int drape_net_create(drape_net** out, uint64_t seed) {
+  return guard([&] {
+    need(out, "output");
+    requireDevice();
+    *out = new drape_net(seed);
+  });
+}
+
+void drape_net_destroy(drape_net* net) { delete net; }
+
+int drape_net_param_count(const drape_net* net, size_t* out) {
+  return guard([&] {
+    need(net, "network");
+    need(out, "output");
+    *out = net->impl.paramCount();
+  });
+}
+
+int drape_net_get_params(const drape_net* net, float* out, size_t n) {
+  return guard([&] {
+    need(net, "network");
+    need(out, "output");
+    if (n != net->impl.paramCount()) throw std::invalid_argument("parameter count mismatch");
+    net->impl.getParams(out);
+  });
+}
+
+int drape_net_set_params(drape_net* net, const float* in, size_t n) {
+  return guard([&] {
+    need(net, "network");
+    need(in, "parameters");
+    if (n != net->impl.paramCount()) throw std::invalid_argument("parameter count mismatch");
+    net->impl.setParams(in);
+  });
+}
+
+int drape_net_reset_optimizer(drape_net* net) {
+  return guard([&] {
+    need(net, "network");
+    net->impl.resetOptimizer();
+  });
+}
+
+int drape_graph_create(drape_graph** out, uint32_t n, const uint32_t* row_start, const uint32_t* nbr,
+                       const double* rest2) {
+  return guard([&] {
+    need(out, "output");
+    need(row_start, "row_start");
+    need(rest2, "rest2");
+    if (row_start[n] && !nbr) throw std::invalid_argument("null nbr");
+    for (uint32_t i = 0; i < n; ++i)
+      if (row_start[i + 1] < row_start[i]) throw std::invalid_argument("row_start must be non-decreasing");
+    requireDevice();
+    *out = new drape_graph(n, row_start, nbr, rest2);
+  });
+}
+
+void drape_graph_destroy(drape_graph* g) { delete g; }
+
+namespace {
+
+// Stage a frame of host arrays (n vertices) into the network's device buffers (graph rows, zero padded).
+ws::FrameView stageFrame(drape_net* net, const drape_graph* g, const float* x, const float* v, const float* dvPrev,
+                         const float* phiNormal, const float* target) {
+  const uint32_t rows = g->impl.rows(), n = g->impl.particles();
+  if (rows > net->cap) {
+    for (dh::DeviceBuffer* b : {&net->x, &net->v, &net->dv, &net->target, &net->out}) b->allocate(sizeof(dk::F3) * rows);
+    net->pn.allocate(sizeof(float4) * rows);
+    net->cap = rows;
+  }
+  auto put = [&](dh::DeviceBuffer& b, const float* src, size_t floatsPerRow) {
+    check(hipMemsetAsync(b.as<void>(), 0, sizeof(float) * floatsPerRow * rows, net->stream), "memset");
+    if (src)
+      check(hipMemcpyAsync(b.as<void>(), src, sizeof(float) * floatsPerRow * n, hipMemcpyHostToDevice, net->stream),
+            "copy");
+  };
+  put(net->x, x, 3);
+  put(net->v, v, 3);
+  put(net->dv, dvPrev, 3);
+  put(net->pn, phiNormal, 4);
+  put(net->target, target, 3);
+  return ws::FrameView{net->x.as<dk::F3>(), net->v.as<dk::F3>(), net->dv.as<dk::F3>(), net->pn.as<float4>()};
+}
+
+}  // namespace
+
+int drape_net_predict(drape_net* net, const drape_graph* g, const float* x, const float* v, const float* dv_prev,
+                      const float* phi_normal, float* dv_out) {
+  return guard([&] {
+    need(net, "network");
+    need(g, "graph");
+    need(x, "x");
+    need(v, "v");
+    need(dv_prev, "dv_prev");
+    need(phi_normal, "phi_normal");
+    need(dv_out, "output");
+    const ws::FrameView f = stageFrame(net, g, x, v, dv_prev, phi_normal, nullptr);
+    net->impl.predict(g->impl.view(), f, net->out.as<dk::F3>(), net->stream);
+    check(hipMemcpyAsync(dv_out, net->out.as<void>(), sizeof(float) * 3 * g->impl.particles(), hipMemcpyDeviceToHost,
+                         net->stream),
+          "copy");
+    check(hipStreamSynchronize(net->stream), "sync");
+  });
+}
+
+int drape_net_train_step(drape_net* net, const drape_graph* g, const float* x, const float* v, const float* dv_prev,
+                         const float* phi_normal, const float* target, float lr, double* loss, float* grad) {
+  return guard([&] {
+    need(net, "network");
+    need(g, "graph");
+    need(x, "x");
+    need(v, "v");
+    need(dv_prev, "dv_prev");
+    need(phi_normal, "phi_normal");
+    need(target, "target");
+    if (!(lr >= 0.0f)) throw std::invalid_argument("learning rate must be >= 0");
+    const ws::FrameView f = stageFrame(net, g, x, v, dv_prev, phi_normal, target);
+    const double l = net->impl.lossAndGradient(g->impl.view(), f, net->target.as<dk::F3>(), lr, net->stream);
+    if (loss) *loss = l;
+    if (grad) net->impl.getGradient(grad);
+  });
+}
+
+int drape_solver_set_warm_start(drape_solver* s, drape_net* net) {
+  return guard([&] {
+    need(s, "solver");
+    s->impl.setWarmStart(net ? &net->impl : nullptr);
+  });
+}
+
+int drape_solver_frame_inputs(drape_solver* s, float* x, float* v, float* dv_prev, float* phi_normal) {
+  return guard([&] {
+    need(s, "solver");
+    s->impl.frameInputs(x, v, dv_prev, phi_normal);
+  });
+}
+
+int drape_solver_mesh_graph_size(const drape_solver* s, size_t* n_entries) {
+  return guard([&] {
+    need(s, "solver");
+    need(n_entries, "output");
+    std::vector<uint32_t> rs, nb;
+    s->impl.meshGraph(rs, nb);
+    *n_entries = nb.size();
+  });
+}
+
+int drape_solver_mesh_graph(const drape_solver* s, uint32_t* row_start, uint32_t* nbr) {
+  return guard([&] {
+    need(s, "solver");
+    std::vector<uint32_t> rs, nb;
+    s->impl.meshGraph(rs, nb);
+    if (row_start) std::copy(rs.begin(), rs.end(), row_start);
+    if (nbr) std::copy(nb.begin(), nb.end(), nbr);
   });
 }
 

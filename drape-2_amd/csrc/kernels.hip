@@ -1026,8 +1026,83 @@ __global__ void __launch_bounds__(256) kFinish(View v) {
   }
   float c = P.dampingScale * fab.damping;
   if (v.garments[v.rowGarment[row]].phase == 0u) c += P.assemblyExtraDamping;
+  const F3 vOld = v.v[row];
+  v.dvLast[row] = (flags & kRowPinned) ? f3(0, 0, 0) : v.u[row] / P.h - vOld;
   v.x[row] = xn;
   v.v[row] = ((xn - xs) / P.h) * fmaxf(0.0f, 1.0f - c * P.h);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Learned warm start support.
+
+// SDF distance and normal at the current position (distance = band and zero normal without a body).
+__global__ void __launch_bounds__(256) kWarmFeatures(View v) {
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const SegParams& P = v.seg[v.tileSeg[tile]];
+  const uint32_t row = tile * kTile + threadIdx.x;
+  float4 pn = make_float4(0.f, 0.f, 0.f, P.field.present ? P.field.band : 0.02f);
+  if (!(v.rowFlags[row] & kRowPad) && P.field.present) {
+    const F3 x = v.x[row];
+    const F3 n = fieldGradient(v.fieldValues, P.field, x);
+    pn = make_float4(n.x, n.y, n.z, fieldSample(v.fieldValues, P.field, x));
+  }
+  v.phiN[row] = pn;
+}
+
+// Per row: step = the move from the learned guess to the previous-velocity guess (-h dv); each segment evaluates
+// its energy at the learned guess first (lsAlpha = 0).
+__global__ void __launch_bounds__(256) kWarmPrepare(View v) {
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t s = v.tileSeg[tile];
+  const SegParams& P = v.seg[s];
+  if (tile == P.tileBegin && threadIdx.x == 0) v.state[s].lsAlpha = 0.0;
+  const uint32_t row = tile * kTile + threadIdx.x;
+  v.step[row] = (v.rowFlags[row] & (kRowPinned | kRowPad)) ? f3(0, 0, 0) : -P.h * v.warm[row];
+}
+
+// Per segment: energy of the learned guess into eTrial, then evaluate the previous-velocity guess (lsAlpha = 1).
+__global__ void __launch_bounds__(256) kWarmLearnedEnergy(View v) {
+  __shared__ double lds[8];
+  const uint32_t s = blockIdx.x;
+  SegState* S = v.state + s;
+  const SegParams& P = v.seg[s];
+  const double e = rangeSum256(v.partC, P.tileBegin, P.tileEnd, lds);
+  if (threadIdx.x == 0) {
+    S->eTrial = e;
+    S->lsAlpha = 1.0;
+  }
+}
+
+// Per segment: keep the learned guess unless its energy is above the previous-velocity guess's.
+__global__ void __launch_bounds__(256) kWarmDecide(View v) {
+  __shared__ double lds[8];
+  const uint32_t s = blockIdx.x;
+  SegState* S = v.state + s;
+  const SegParams& P = v.seg[s];
+  const double ePlain = rangeSum256(v.partC, P.tileBegin, P.tileEnd, lds);
+  if (threadIdx.x == 0) {
+    const bool reject = !(S->eTrial <= ePlain);
+    S->warmRejected = reject ? 1u : 0u;
+    if (reject)
+      S->cntWarmRejected += 1u;
+    else
+      S->cntWarmAccepted += 1u;
+    S->lsAlpha = 1.0;
+  }
+}
+
+// Rows of a segment that rejected the learned guess go back to u = h v, computed as the predict kernel computes
+// it with dv = 0, so a rejected guess leaves no trace in the trajectory.
+__global__ void __launch_bounds__(256) kWarmApply(View v) {
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t s = v.tileSeg[tile];
+  if (!v.state[s].warmRejected) return;
+  const uint32_t row = tile * kTile + threadIdx.x;
+  const F3 zero = f3(0, 0, 0);
+  v.u[row] = (v.rowFlags[row] & (kRowPinned | kRowPad)) ? zero : v.seg[s].h * (v.v[row] + zero);
 }
 
 __global__ void kFill(uint32_t* p, uint32_t value, uint64_t count) {
@@ -1362,6 +1437,15 @@ void launchLineSearchDecide(const View& v, hipStream_t st) { kLineSearchDecide<<
 void launchApply(const View& v, hipStream_t st) { kApply<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
 void launchNewtonEnd(const View& v, hipStream_t st) { kNewtonEnd<<<v.nSegments, 256, 0, st>>>(v); }
 void launchFinish(const View& v, hipStream_t st) { kFinish<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
+void launchWarmFeatures(const View& v, hipStream_t st) { kWarmFeatures<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
+void launchWarmCheck(const View& v, hipStream_t st) {
+  kWarmPrepare<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);
+  kEnergyTrial<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);
+  kWarmLearnedEnergy<<<v.nSegments, 256, 0, st>>>(v);
+  kEnergyTrial<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);
+  kWarmDecide<<<v.nSegments, 256, 0, st>>>(v);
+  kWarmApply<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);
+}
 void launchFill(uint32_t* p, uint32_t value, uint64_t count, hipStream_t st) {
   if (count == 0) return;
   kFill<<<uint32_t((count + 255) / 256), 256, 0, st>>>(p, value, count);

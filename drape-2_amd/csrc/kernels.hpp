@@ -34,6 +34,8 @@ struct View {
   F3* pB;
   F3* ap;
   F3* warm;      // learned warm-start velocity change per row (zero when unused)
+  F3* dvLast;    // velocity change the last substep realised: u / h - v at its start (zero for pinned rows)
+  float4* phiN;  // SDF distance (w) and unit normal (xyz) at x, for the warm-start features
   // One-reduction PCG (kPcgStep): r, ap = S A p, p, zr = Mf r and t = Mf ap, each double-buffered by kernel
   // parity (neighbours read the previous kernel's copy while the owner writes the new one); FP64 partials per
   // tile (7 per parity).
@@ -123,6 +125,10 @@ void launchLineSearchDecide(const View& v, hipStream_t st);
 void launchApply(const View& v, hipStream_t st);
 void launchNewtonEnd(const View& v, hipStream_t st);
 void launchFinish(const View& v, hipStream_t st);
+// Learned warm start: SDF features at x; the energy check of the learned guess against the previous-velocity
+// guess (two energy-only evaluations, then per segment keep the lower; cpu_solver.cpp:264-274 energy).
+void launchWarmFeatures(const View& v, hipStream_t st);
+void launchWarmCheck(const View& v, hipStream_t st);
 void launchFill(uint32_t* p, uint32_t value, uint64_t count, hipStream_t st);
 
 // Debug / standalone paths.

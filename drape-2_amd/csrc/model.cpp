@@ -168,6 +168,8 @@ Model Model::build(const drape_scene_desc* scenes, uint32_t nScenes) {
         const double* p = sg.initial_positions + 3 * size_t(vtx);
         m.x.push_back({static_cast<float>(p[0]), static_cast<float>(p[1]), static_cast<float>(p[2])});
         m.vertexArea.push_back(mesh.vertex_area[vtx]);
+        m.rest.push_back(mesh.rest[2 * size_t(vtx)]);
+        m.rest.push_back(mesh.rest[2 * size_t(vtx) + 1]);
         m.pFabric.push_back(g.fabricOffset + mesh.vertex_piece[vtx]);
         m.pGarment.push_back(gi);
         m.pSegment.push_back(segIndex);

@@ -91,6 +91,7 @@ struct Model {
   std::vector<Fabric> fabrics;
   std::vector<Vec3f> x, v;
   std::vector<double> mass, vertexArea, contactForce;
+  std::vector<double> rest;  // pattern-space rest position, 2 per particle
   std::vector<Vec3> fExt;
   std::vector<uint32_t> pFabric, pGarment, pSegment;
   std::vector<uint8_t> pinned;

@@ -137,8 +137,33 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   upload(pinPos_, pins);
   const size_t vecBytes = sizeof(dk::F3) * (nRows_ + 1);
   for (DeviceBuffer* b : {&x_, &v_, &xStart_, &yRel_, &u_, &grad_, &step_, &r_, &z_, &pA_, &pB_, &ap_, &warm_,
-                          &scratchF3_, &scratchF3b_})
+                          &scratchF3_, &scratchF3b_, &dvLast_})
     b->allocate(vecBytes);
+  phiN_.allocate(sizeof(float4) * (nRows_ + 1));
+  {
+    // Mesh edge graph over internal rows (triangle edges, sorted unique neighbours) for the learned warm start.
+    std::vector<std::vector<uint32_t>> adj(nRows_);
+    for (const StretchTerm& t : M.stretch)
+      for (int a = 0; a < 3; ++a) {
+        const uint32_t i = rowOf_[t.v[a]], j = rowOf_[t.v[(a + 1) % 3]];
+        adj[i].push_back(j);
+        adj[j].push_back(i);
+      }
+    std::vector<uint32_t> rs(nRows_ + 1, 0), nb, mask(nRows_, 0);
+    std::vector<double> rest(2 * size_t(nRows_), 0.0);
+    for (uint32_t r = 0; r < nRows_; ++r) {
+      std::sort(adj[r].begin(), adj[r].end());
+      adj[r].erase(std::unique(adj[r].begin(), adj[r].end()), adj[r].end());
+      nb.insert(nb.end(), adj[r].begin(), adj[r].end());
+      rs[r + 1] = static_cast<uint32_t>(nb.size());
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+      mask[rowOf_[i]] = 1u;
+      rest[2 * size_t(rowOf_[i])] = M.rest[2 * size_t(i)];
+      rest[2 * size_t(rowOf_[i]) + 1] = M.rest[2 * size_t(i) + 1];
+    }
+    meshGraph_ = std::make_unique<ws::Graph>(nRows_, rs.data(), nb.data(), rest.data(), mask.data());
+  }
   // Row arrays a gathered column may index hold the zero dummy entry nRows as well.
   for (DeviceBuffer& b : cgBuf_) b.allocate(vecBytes);
   precond_.allocate(sizeof(float) * 9 * (nRows_ + 1));
@@ -308,6 +333,8 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   v.pB = pB_.as<dk::F3>();
   v.ap = ap_.as<dk::F3>();
   v.warm = warm_.as<dk::F3>();
+  v.dvLast = dvLast_.as<dk::F3>();
+  v.phiN = phiN_.as<float4>();
   for (int b = 0; b < 2; ++b) {
     v.cgR[b] = cgBuf_[0 + b].as<dk::F3>();
     v.cgAp[b] = cgBuf_[2 + b].as<dk::F3>();
@@ -664,7 +691,13 @@ void HipSolver::runPcgSingleKernel() {
 
 void HipSolver::substep(double h, int iterations) {
   uploadSegParams(h);
+  if (net_) {
+    dk::launchWarmFeatures(view_, stream_);
+    const ws::FrameView f{view_.x, view_.v, view_.dvLast, view_.phiN};
+    net_->predict(meshGraph_->view(), f, warm_.as<dk::F3>(), stream_);
+  }
   dk::launchPredictContact(view_, stream_);
+  if (net_) dk::launchWarmCheck(view_, stream_);
   for (int it = 0; it < iterations; ++it) {
     dk::launchActivate(view_, stream_);
     dk::launchAssemble(view_, maxK_, true, stream_);
@@ -836,6 +869,8 @@ void HipSolver::snapshotRestore(const void* in, size_t bytes) {
   syncStateToDevice();
   uploadGarments();
   uploadSeams();
+  // The realised dv is not part of the reference's snapshot: the learned warm start restarts from dv = 0.
+  check(hipMemset(dvLast_.as<void>(), 0, dvLast_.bytes()), "memset");
 }
 
 void HipSolver::updateFabric(const std::string& garment, const std::vector<Fabric>& fabrics) {
@@ -1072,6 +1107,64 @@ void HipSolver::debugSystem(float* blocks, float* grad, float* precond, uint32_t
     std::vector<float> c(4 * size_t(nRows_ + 1));
     check(hipMemcpy(c.data(), cnormal_.as<void>(), sizeof(float) * c.size(), hipMemcpyDeviceToHost), "copy");
     for (size_t i = 0; i < n; ++i) std::copy(&c[4 * size_t(rowOf_[i])], &c[4 * size_t(rowOf_[i])] + 4, contactNormals + 4 * i);
+  }
+}
+
+void HipSolver::setWarmStart(ws::Net* net) {
+  requireBuilt();
+  check(hipStreamSynchronize(stream_), "sync");
+  net_ = net;
+  if (!net_) check(hipMemset(warm_.as<void>(), 0, warm_.bytes()), "memset");
+}
+
+void HipSolver::frameInputs(float* x, float* v, float* dvPrev, float* phiNormal) {
+  requireBuilt();
+  dk::launchWarmFeatures(view_, stream_);
+  check(hipStreamSynchronize(stream_), "sync");
+  const size_t n = model_.x.size();
+  auto copyF3 = [&](const DeviceBuffer& b, float* out) {
+    if (!out) return;
+    std::vector<dk::F3> h(nRows_ + 1);
+    check(hipMemcpy(h.data(), b.as<void>(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyDeviceToHost), "copy");
+    for (size_t i = 0; i < n; ++i) {
+      const dk::F3 q = h[rowOf_[i]];
+      out[3 * i] = q.x;
+      out[3 * i + 1] = q.y;
+      out[3 * i + 2] = q.z;
+    }
+  };
+  copyF3(x_, x);
+  copyF3(v_, v);
+  copyF3(dvLast_, dvPrev);
+  if (phiNormal) {
+    std::vector<float4> h(nRows_ + 1);
+    check(hipMemcpy(h.data(), phiN_.as<void>(), sizeof(float4) * (nRows_ + 1), hipMemcpyDeviceToHost), "copy");
+    for (size_t i = 0; i < n; ++i) {
+      const float4 q = h[rowOf_[i]];
+      phiNormal[4 * i] = q.x;
+      phiNormal[4 * i + 1] = q.y;
+      phiNormal[4 * i + 2] = q.z;
+      phiNormal[4 * i + 3] = q.w;
+    }
+  }
+}
+
+void HipSolver::meshGraph(std::vector<uint32_t>& rowStart, std::vector<uint32_t>& nbr) const {
+  requireBuilt();
+  const size_t n = model_.x.size();
+  std::vector<std::vector<uint32_t>> adj(n);
+  for (const StretchTerm& t : model_.stretch)
+    for (int a = 0; a < 3; ++a) {
+      adj[t.v[a]].push_back(t.v[(a + 1) % 3]);
+      adj[t.v[(a + 1) % 3]].push_back(t.v[a]);
+    }
+  rowStart.assign(n + 1, 0);
+  nbr.clear();
+  for (size_t i = 0; i < n; ++i) {
+    std::sort(adj[i].begin(), adj[i].end());
+    adj[i].erase(std::unique(adj[i].begin(), adj[i].end()), adj[i].end());
+    nbr.insert(nbr.end(), adj[i].begin(), adj[i].end());
+    rowStart[i + 1] = static_cast<uint32_t>(nbr.size());
   }
 }
 

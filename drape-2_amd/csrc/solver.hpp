@@ -8,35 +8,12 @@
 #include <string>
 #include <vector>
 
+#include "device_buffer.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
+#include "warmstart.hpp"
 
 namespace dh {
-
-// Device allocation owned by the solver (zero-filled on allocation).
-class DeviceBuffer {
- public:
-  DeviceBuffer() = default;
-  DeviceBuffer(const DeviceBuffer&) = delete;
-  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-  ~DeviceBuffer() { release(); }
-  void allocate(size_t bytes);
-  void adopt(void* p, size_t bytes) {
-    release();
-    ptr_ = p;
-    bytes_ = bytes;
-  }
-  void release();
-  template <class T>
-  T* as() const {
-    return static_cast<T*>(ptr_);
-  }
-  size_t bytes() const { return bytes_; }
-
- private:
-  void* ptr_ = nullptr;
-  size_t bytes_ = 0;
-};
 
 class HipSolver {
  public:
@@ -78,6 +55,15 @@ class HipSolver {
   void debugSpmv(const float* x, float* y);
   void debugSystem(float* blocks, float* grad, float* precond, uint32_t* contactFlags, float* contactNormals);
 
+  // Learned warm start: the network predicts each step's dv; null restores the previous-velocity guess. The
+  // solver does not own the network.
+  void setWarmStart(ws::Net* net);
+  // Network inputs of the current state in particle order: x, v, the last substep's realised dv (3 each) and the
+  // SDF distance and normal (4: normal, distance). Any output may be null.
+  void frameInputs(float* x, float* v, float* dvPrev, float* phiNormal);
+  // The mesh edge graph the network runs on (rows = internal rows), as host CSR in particle order.
+  void meshGraph(std::vector<uint32_t>& rowStart, std::vector<uint32_t>& nbr) const;
+
  private:
   void requireBuilt() const;
   void uploadSegParams(double h);
@@ -115,6 +101,9 @@ class HipSolver {
   DeviceBuffer rowFlags_, mass_, rowFabric_, rowGarment_, fExt_, pinPos_;
   DeviceBuffer x_, v_, xStart_, yRel_, u_, grad_, step_, r_, z_, pA_, pB_, ap_, warm_, precond_;
   DeviceBuffer cnormal_, cflags_, cforce_, clambda_;
+  DeviceBuffer dvLast_, phiN_;
+  std::unique_ptr<ws::Graph> meshGraph_;
+  ws::Net* net_ = nullptr;
   DeviceBuffer tileSeg_, segParams_, segState_, partA_, partB_, partC_, partE_, partMax_;
   DeviceBuffer stretch_, hinges_, seams_, fabrics_, garments_;
   DeviceBuffer incCount_, incStart_, inc_, rowLen_, rowStart_, colIndex_, diagSlot_;

@@ -292,6 +292,20 @@ def _declare(lib):
         "drape_solver_debug_assemble": (C.c_int, [S, C.c_double, f32p, f64p, f32p, f32p]),
         "drape_solver_debug_spmv": (C.c_int, [S, f32p, f32p]),
         "drape_solver_debug_system": (C.c_int, [S, f32p, f32p, f32p, u32p, f32p]),
+        "drape_net_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint64]),
+        "drape_net_destroy": (None, [S]),
+        "drape_net_param_count": (C.c_int, [S, C.POINTER(C.c_size_t)]),
+        "drape_net_get_params": (C.c_int, [S, f32p, C.c_size_t]),
+        "drape_net_set_params": (C.c_int, [S, f32p, C.c_size_t]),
+        "drape_net_reset_optimizer": (C.c_int, [S]),
+        "drape_graph_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, u32p, u32p, f64p]),
+        "drape_graph_destroy": (None, [S]),
+        "drape_net_predict": (C.c_int, [S, S, f32p, f32p, f32p, f32p, f32p]),
+        "drape_net_train_step": (C.c_int, [S, S, f32p, f32p, f32p, f32p, f32p, C.c_float, f64p, f32p]),
+        "drape_solver_set_warm_start": (C.c_int, [S, S]),
+        "drape_solver_frame_inputs": (C.c_int, [S, f32p, f32p, f32p, f32p]),
+        "drape_solver_mesh_graph_size": (C.c_int, [S, C.POINTER(C.c_size_t)]),
+        "drape_solver_mesh_graph": (C.c_int, [S, u32p, u32p]),
         "drape_bsr_spmv": (C.c_int, [C.c_uint32, C.c_uint32, u32p, u32p, f32p, u8p, f32p, f32p, C.c_int32,
                                      f64p]),
         "drape_eval_terms": (C.c_int, [C.c_int32, C.c_uint32, f32p, f32p, C.POINTER(Fabric), C.c_float, f32p,

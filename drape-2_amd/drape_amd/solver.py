@@ -211,6 +211,30 @@ class HipSolver:
                                                   ptr(grad, C.c_float), ptr(blocks, C.c_float)))
         return e.value, grad, blocks
 
+    # ---- learned warm start ----
+
+    def set_warm_start(self, net) -> None:
+        """Attach a `learn.WarmStartNet` (None restores the previous-velocity guess). Keep `net` alive."""
+        self._warm_net = net
+        _check(_lib().drape_solver_set_warm_start(self._h, net.handle if net is not None else None))
+
+    def frame_inputs(self) -> dict:
+        """Network inputs of the current state: x, v, dv_prev (the last step's realised dv), phi_normal."""
+        out = {"x": np.empty((self._n, 3), np.float32), "v": np.empty((self._n, 3), np.float32),
+               "dv_prev": np.empty((self._n, 3), np.float32), "phi_normal": np.empty((self._n, 4), np.float32)}
+        _check(_lib().drape_solver_frame_inputs(self._h, ptr(out["x"], C.c_float), ptr(out["v"], C.c_float),
+                                                ptr(out["dv_prev"], C.c_float), ptr(out["phi_normal"], C.c_float)))
+        return out
+
+    def mesh_graph(self):
+        """The triangle-edge graph in particle order: (row_start, nbr)."""
+        size = C.c_size_t()
+        _check(_lib().drape_solver_mesh_graph_size(self._h, C.byref(size)))
+        rs = np.empty(self._n + 1, np.uint32)
+        nb = np.empty(max(size.value, 1), np.uint32)
+        _check(_lib().drape_solver_mesh_graph(self._h, ptr(rs, C.c_uint32), ptr(nb, C.c_uint32)))
+        return rs, nb[:size.value]
+
     def debug_system(self) -> dict:
         """The last Newton iteration's system: blocks (slot order), grad, block-Jacobi inverse, contact flags and
         contact normals (xyz, offset)."""
