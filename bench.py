@@ -19,6 +19,8 @@ import platform
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "drape-2_amd"))
 
@@ -55,6 +57,57 @@ def cpu_baseline(make_scene, steps: int) -> dict:
                       f"nproc {os.cpu_count()}"}
 
 
+WEIGHTS = os.path.join(REPO, "drape-2_amd", "drape_amd", "weights", "warmstart.npy")
+
+
+def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 5), judge_iterations: int = 15,
+                  sample_every: int = 4) -> dict:
+    """Learned warm start against the previous-velocity guess. Each method steps the drop with Newton capped at
+    each of `caps`; every `sample_every`-th timed step a tightly converged plain solve (Newton <= judge_iterations,
+    same PCG) from that method's own state judges its result (max particle distance, mm). Reports steps/s over the
+    timed steps (the judge excluded), iteration counts and how often the energy check kept the learned guess."""
+    from drape_amd import HipSolver, learn
+    net = learn.WarmStartNet()
+    net.load(net_path)
+    sc = make_scene()
+    judge = HipSolver()
+    judge.build(sc)
+    runs = []
+    for learned in (False, True):
+        for k in caps:
+            s = HipSolver()
+            s.build(sc)
+            if learned:
+                s.set_warm_start(net)
+            errs, t = [], 0.0
+            for i in range(warmup + steps):
+                check = i >= warmup and (i - warmup) % sample_every == 0
+                if check:
+                    judge.restore(s.snapshot())
+                    judge.step(1 / 60, 1, judge_iterations)
+                    xj = judge.read_positions()
+                if i == warmup:
+                    s.reset_counters()
+                t0 = time.perf_counter()
+                s.step(1 / 60, 1, k)
+                if i >= warmup:
+                    t += time.perf_counter() - t0
+                if check:
+                    d = np.linalg.norm(s.read_positions().astype(np.float64) - xj, axis=1)
+                    errs.append(float(d.max()) * 1e3)
+            c = s.counters()
+            runs.append({"guess": "learned" if learned else "previous velocity", "newton_cap": k,
+                         "steps_per_s": steps / t, "newton_per_step": c["newton_iterations"] / steps,
+                         "pcg_per_step": c["pcg_iterations"] / steps,
+                         "max_step_error_mm": max(errs), "median_step_error_mm": float(np.median(errs)),
+                         "steps_within_0p1mm": int(sum(e <= 0.1 for e in errs)), "judged_steps": len(errs),
+                         "learned_guess_kept": c["warm_start_accepted"],
+                         "learned_guess_rejected": c["warm_start_fallbacks"]})
+    return {"judge": f"plain solve with Newton <= {judge_iterations} from the method's own state, "
+                     f"every {sample_every} timed steps",
+            "weights": os.path.relpath(net_path, REPO), "runs": runs}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +120,9 @@ def main() -> int:
                     help="event-time PCG iteration launches (graph path: one per poll; eager: every k-th)")
     ap.add_argument("--pcg-variant", type=int, default=1, help="1 single-kernel PCG (default), 0 two-kernel")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-warm-start-ab", action="store_true",
+                    help="skip the learned-vs-plain warm start comparison")
+    ap.add_argument("--warm-start-weights", default=WEIGHTS)
     ap.add_argument("--pcg-waves", type=int, default=0,
                     help="wavefronts per slice in the one-kernel PCG iteration (1, 2, 4; 0 = tune during warm-up)")
     args = ap.parse_args()
@@ -140,6 +196,8 @@ def main() -> int:
             "timed_launches": timing["launches"],
         },
     }
+    if not args.no_warm_start_ab and os.path.exists(args.warm_start_weights):
+        line["warm_start_ab"] = warm_start_ab(make_scene, args.warmup, args.steps, args.warm_start_weights)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(make_scene, args.cpu_steps)
     print(json.dumps(line), flush=True)

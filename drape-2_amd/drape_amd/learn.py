@@ -158,14 +158,37 @@ class Sample:
     target: np.ndarray
 
 
-def collect(scene, steps: int, dt: float = 1 / 60, iterations: int = 5, every: int = 1, solver=None) -> list:
+_replicas: dict = {}
+
+
+def replicate(graph: MeshGraph, k: int) -> MeshGraph:
+    """k disjoint copies of a graph (block-diagonal), cached: the graph of k frames of one mesh stacked."""
+    key = (id(graph), k)
+    if key not in _replicas:
+        n, e = graph.n, len(graph.nbr)
+        rs = np.concatenate([graph.row_start[:-1].astype(np.int64) + i * e for i in range(k)] + [[k * e]])
+        nbr = np.concatenate([graph.nbr.astype(np.int64) + i * n for i in range(k)])
+        _replicas[key] = (graph, MeshGraph(rs, nbr, np.tile(graph.rest2, (k, 1))))
+    return _replicas[key][1]
+
+
+def stack(samples: list) -> Sample:
+    """Samples of one mesh as one sample over the replicated graph."""
+    g = replicate(samples[0].graph, len(samples))
+    frame = {key: np.concatenate([s.frame[key] for s in samples]) for key in samples[0].frame}
+    return Sample(g, frame, np.concatenate([s.target for s in samples]))
+
+
+def collect(scene, steps: int, dt: float = 1 / 60, iterations: int = 5, every: int = 1, solver=None,
+            graph: MeshGraph | None = None) -> list:
     """Solver-generated training pairs: the inputs of the state before a step and the dv the step realised.
-    The plain (previous-velocity) guess is used; every `every`-th step is kept."""
+    The plain (previous-velocity) guess is used; every `every`-th step is kept. Pass `graph` to share one
+    graph between scenes of the same mesh (so that their frames can be stacked)."""
     from .solver import HipSolver
     s = solver or HipSolver()
     if solver is None:
         s.build(scene)
-    graph = MeshGraph.of_scene(scene)
+    graph = graph or MeshGraph.of_scene(scene)
     out = []
     for k in range(steps):
         frame = s.frame_inputs() if k % every == 0 else None
@@ -175,17 +198,32 @@ def collect(scene, steps: int, dt: float = 1 / 60, iterations: int = 5, every: i
     return out
 
 
-def train(net: WarmStartNet, samples: list, epochs: int, lr: float = 1e-3, seed: int = 0, log=None) -> list:
-    """Adam over the samples in a seeded random order per epoch; returns the mean loss of each epoch."""
+def train(net: WarmStartNet, samples: list, epochs: int, lr: float = 1e-3, seed: int = 0, log=None,
+          batch_vertices: int = 0, lr_final: float | None = None) -> list:
+    """Adam over the samples in a seeded random order per epoch; returns the mean loss of each epoch.
+    With batch_vertices > 0, samples of the same mesh are stacked into batches of about that many vertices.
+    The learning rate decays geometrically to lr_final (default: lr) over the epochs."""
     rng = np.random.default_rng(seed)
     history = []
     for e in range(epochs):
-        order = rng.permutation(len(samples))
+        rate = lr if lr_final is None or epochs == 1 else lr * (lr_final / lr) ** (e / (epochs - 1))
+        order = list(rng.permutation(len(samples)))
+        batches = []
+        if batch_vertices > 0:
+            by_graph: dict = {}
+            for i in order:
+                by_graph.setdefault(id(samples[i].graph), []).append(samples[i])
+            for group in by_graph.values():
+                k = max(1, batch_vertices // group[0].graph.n)
+                batches += [group[j:j + k] for j in range(0, len(group), k)]
+            batches = [batches[i] for i in rng.permutation(len(batches))]
+        else:
+            batches = [[samples[i]] for i in order]
         tot = 0.0
-        for i in order:
-            s = samples[i]
-            tot += net.step(s.graph, s.frame, s.target, lr)
+        for b in batches:
+            s = b[0] if len(b) == 1 else stack(b)
+            tot += net.step(s.graph, s.frame, s.target, rate) * len(b)
         history.append(tot / max(len(samples), 1))
         if log:
-            log(f"epoch {e}: loss {history[-1]:.5f}")
+            log(f"epoch {e}: lr {rate:.2e} loss {history[-1]:.5f}")
     return history
