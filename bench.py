@@ -25,6 +25,9 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "drape-2_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# HBM bytes per kPcgStep launch on the 256x256 sheet (one wavefront per slice), from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE scaled by a plain copy's readings (x2.0, x1.0): profiles/kPcgStep_sheet65k_kernel_trace.md.
+PCG_STEP_TRAFFIC_BYTES = 50.49e6
 
 
 def cpu_model() -> str:
@@ -190,7 +193,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None,
+            "traffic": (PCG_STEP_TRAFFIC_BYTES if args.pcg_variant == 1 and args.n == 256 and s.pcg_waves() == 1
+                        else None),
             "bytes_per_launch": timing["bytes_per_launch"],
             "mean_launch_us": timing["mean_us"],
             "timed_launches": timing["launches"],
