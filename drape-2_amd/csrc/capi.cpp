@@ -436,8 +436,10 @@ int drape_bsr_spmv(uint32_t n, uint32_t n_blocks, const uint32_t* row_start, con
       sellK[t] = K;
       sellBase[t + 1] = sellBase[t] + K * 9u * dk::kSlice;
     }
+    uint32_t maxK = 0;
+    for (uint32_t t = 0; t < nSlices; ++t) maxK = std::max(maxK, sellK[t]);
     std::vector<uint32_t> sellCol(std::max<size_t>(sellBase[nSlices] / 9, 1), nRows);
-    std::vector<float> sellVal(std::max<size_t>(sellBase[nSlices], 1), 0.0f);
+    std::vector<float> sellVal(size_t(sellBase[nSlices]) + dk::kSellSlack, 0.0f);  // zero slack: see kSellSlack
     for (uint32_t r = 0; r < n; ++r) {
       const uint32_t t = r / dk::kSlice, l = r % dk::kSlice;
       for (uint32_t s = row_start[r]; s < row_start[r + 1]; ++s) {
@@ -467,13 +469,13 @@ int drape_bsr_spmv(uint32_t n, uint32_t n_blocks, const uint32_t* row_start, con
     v.sellBase = dBase.as<uint32_t>();
     v.sellCol = dCol.as<uint32_t>();
     v.sellVal = dVal.as<float>();
-    dk::launchSpmvPlain(v, dx.as<dk::F3>(), dy.as<dk::F3>(), st.s);  // warm-up
+    dk::launchSpmvPlain(v, maxK, dx.as<dk::F3>(), dy.as<dk::F3>(), st.s);  // warm-up
     const int R = std::max(1, reps);
     hipEvent_t e0, e1;
     check(hipEventCreate(&e0), "event");
     check(hipEventCreate(&e1), "event");
     check(hipEventRecord(e0, st.s), "event");
-    for (int r = 0; r < R; ++r) dk::launchSpmvPlain(v, dx.as<dk::F3>(), dy.as<dk::F3>(), st.s);
+    for (int r = 0; r < R; ++r) dk::launchSpmvPlain(v, maxK, dx.as<dk::F3>(), dy.as<dk::F3>(), st.s);
     check(hipEventRecord(e1, st.s), "event");
     check(hipEventSynchronize(e1), "event");
     float ms = 0;

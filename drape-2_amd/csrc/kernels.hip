@@ -1124,6 +1124,47 @@ __global__ void __launch_bounds__(256) kSpmvPlain(View v, const F3* __restrict__
   out[row] = (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : prod;
 }
 
+// The same product for matrices whose rows have at most 16 blocks (every regular-grid garment): column indices
+// and block values of all 16 slots in one memory round trip — slots past K are read unconditionally (the value
+// buffer carries kSellSlack zero floats past its end) and point at the dummy column, so they add nothing — then
+// the gathered x. Row sums in the kPcgStep order (column index mod 4 groups).
+__global__ void __launch_bounds__(256) kSpmv16(View v, const F3* __restrict__ in, F3* out) {
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t slice = __builtin_amdgcn_readfirstlane(tile * kSlicesPerTile + wave);
+  const uint32_t row = slice * kSlice + lane;
+  const uint32_t K = ldUniform(v.sellK + slice);
+  const uint32_t base = ldUniform(v.sellBase + slice);
+  const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
+  const float* __restrict__ val = v.sellVal + base + 9u * lane;
+  uint32_t c[16];
+#pragma unroll
+  for (uint32_t t = 0; t < 16; ++t) c[t] = t < K ? col[t * kSlice] : v.nRows;
+  float bv[16][9];
+#pragma unroll
+  for (uint32_t t = 0; t < 16; ++t)
+#pragma unroll
+    for (int e = 0; e < 9; ++e) bv[t][e] = val[t * 9u * kSlice + e];
+  __builtin_amdgcn_sched_barrier(0);
+  F3 x[16];
+#pragma unroll
+  for (uint32_t t = 0; t < 16; ++t) x[t] = in[c[t]];
+  float g[4][3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) g[q][0] = g[q][1] = g[q][2] = 0.0f;
+#pragma unroll
+  for (uint32_t t = 0; t < 16; ++t) {
+    float* a = g[t & 3u];
+    a[0] += bv[t][0] * x[t].x + bv[t][1] * x[t].y + bv[t][2] * x[t].z;
+    a[1] += bv[t][3] * x[t].x + bv[t][4] * x[t].y + bv[t][5] * x[t].z;
+    a[2] += bv[t][6] * x[t].x + bv[t][7] * x[t].y + bv[t][8] * x[t].z;
+  }
+  const F3 prod = f3(((g[0][0] + g[1][0]) + g[2][0]) + g[3][0], ((g[0][1] + g[1][1]) + g[2][1]) + g[3][1],
+                     ((g[0][2] + g[1][2]) + g[2][2]) + g[3][2]);
+  out[row] = (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : prod;
+}
+
 // SELL values -> canonical slot order (9 floats per slot), for parity checks of the assembled matrix.
 __global__ void __launch_bounds__(256) kSellToSlots(View v, float* out) {
   const uint32_t row = blockIdx.x * 256u + threadIdx.x;
@@ -1450,8 +1491,11 @@ void launchFill(uint32_t* p, uint32_t value, uint64_t count, hipStream_t st) {
   if (count == 0) return;
   kFill<<<uint32_t((count + 255) / 256), 256, 0, st>>>(p, value, count);
 }
-void launchSpmvPlain(const View& v, const F3* in, F3* out, hipStream_t st) {
-  kSpmvPlain<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, in, out);
+void launchSpmvPlain(const View& v, uint32_t maxK, const F3* in, F3* out, hipStream_t st) {
+  if (maxK <= 16u)
+    kSpmv16<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, in, out);
+  else
+    kSpmvPlain<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, in, out);
 }
 void launchSellToSlots(const View& v, float* out, hipStream_t st) {
   kSellToSlots<<<(v.nRows + 255) / 256, 256, 0, st>>>(v, out);

@@ -132,7 +132,9 @@ void launchWarmCheck(const View& v, hipStream_t st);
 void launchFill(uint32_t* p, uint32_t value, uint64_t count, hipStream_t st);
 
 // Debug / standalone paths.
-void launchSpmvPlain(const View& v, const F3* in, F3* out, hipStream_t st);
+// y = A x (pinned rows and columns excluded); maxK = the largest row length (<= 16: the one-batch kernel; the
+// value buffer must then carry kSellSlack zero floats past its end).
+void launchSpmvPlain(const View& v, uint32_t maxK, const F3* in, F3* out, hipStream_t st);
 void launchSellToSlots(const View& v, float* out /* 9 per slot */, hipStream_t st);
 void launchEvalTerms(int kind, uint32_t n, const float* x, const float* params, const Fabric32* fabrics,
                      float limitScale, float* energy, float* grad, float* hess, hipStream_t st);

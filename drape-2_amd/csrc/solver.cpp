@@ -1174,7 +1174,7 @@ void HipSolver::debugSpmv(const float* x, float* y) {
   for (size_t i = 0; i < model_.x.size(); ++i) xs[rowOf_[i]] = dk::F3{x[3 * i], x[3 * i + 1], x[3 * i + 2]};
   check(hipMemcpyAsync(scratchF3_.as<void>(), xs.data(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyHostToDevice, stream_),
         "copy");
-  dk::launchSpmvPlain(view_, scratchF3_.as<dk::F3>(), scratchF3b_.as<dk::F3>(), stream_);
+  dk::launchSpmvPlain(view_, maxK_, scratchF3_.as<dk::F3>(), scratchF3b_.as<dk::F3>(), stream_);
   check(hipMemcpyAsync(ys.data(), scratchF3b_.as<void>(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyDeviceToHost, stream_),
         "copy");
   check(hipStreamSynchronize(stream_), "sync");
