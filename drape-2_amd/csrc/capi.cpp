@@ -268,6 +268,14 @@ int drape_solver_stats(const drape_solver* s, drape_stats* out) {
   });
 }
 
+int drape_solver_segment_stats(const drape_solver* s, uint32_t first, uint32_t count, drape_stats* out) {
+  return guard([&] {
+    need(s, "solver");
+    if (count) need(out, "output");
+    s->impl.segmentStats(first, count, out);
+  });
+}
+
 int drape_solver_snapshot_size(const drape_solver* s, size_t* bytes) {
   return guard([&] {
     need(s, "solver");

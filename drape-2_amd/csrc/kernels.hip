@@ -1033,6 +1033,126 @@ __global__ void __launch_bounds__(256) kFinish(View v) {
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// Stats on the device (SimModel::stats, sim_model.cpp:222-241): per-tile partials, then one workgroup per segment
+// sums its tiles in a fixed order. Only the per-segment results leave the device.
+
+// CollisionField::sample (collision_field.cpp:8-27) in FP64 from the body's FP64 origin and cell.
+__device__ inline double fieldSampleF64(const float* values, const SegParams& P, F3 p) {
+  const Field32& f = P.field;
+  const double l[3] = {(double(p.x) - P.fieldOrigin[0]) / P.fieldCell, (double(p.y) - P.fieldOrigin[1]) / P.fieldCell,
+                       (double(p.z) - P.fieldOrigin[2]) / P.fieldCell};
+  const uint32_t dims[3] = {f.dx, f.dy, f.dz};
+  int i[3];
+  double t[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (!(l[a] >= 0.0 && l[a] <= double(dims[a]) - 1.0)) return P.fieldBand;
+    const int fl = int(floor(l[a]));
+    const int top = int(dims[a]) - 2;
+    i[a] = fl < top ? fl : top;
+    t[a] = l[a] - double(i[a]);
+  }
+  const float* v = values + f.offset;
+  double r = 0.0;
+#pragma unroll
+  for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 2; ++dx) {
+        const double w = (dx ? t[0] : 1.0 - t[0]) * (dy ? t[1] : 1.0 - t[1]) * (dz ? t[2] : 1.0 - t[2]);
+        r += w * double(v[(size_t(uint32_t(i[2] + dz)) * f.dy + uint32_t(i[1] + dy)) * f.dx + uint32_t(i[0] + dx)]);
+      }
+  return r;
+}
+
+__device__ inline bool finite3(F3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
+
+namespace {
+// Per tile: kinetic energy, sum of v^2 and moving count (sums); non-finite flag, penetration and seam gap (maxima).
+constexpr int kStatSums = 3, kStatMaxes = 3, kStatParts = kStatSums + kStatMaxes;
+}  // namespace
+
+template <int N>
+__device__ inline void blockMaxN256(double (&x)[N], double* lds) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = fmax(x[i], __shfl_down(x[i], off, 64));
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0)
+#pragma unroll
+    for (int i = 0; i < N; ++i) lds[i * 4 + w] = x[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < N; ++i) x[i] = fmax(fmax(lds[i * 4], lds[i * 4 + 1]), fmax(lds[i * 4 + 2], lds[i * 4 + 3]));
+}
+
+__global__ void __launch_bounds__(256) kStatsTile(View v, const double* massF64, double* part) {
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  __shared__ double lds[kStatParts * 4];
+  const SegParams& P = v.seg[v.tileSeg[tile]];
+  const uint32_t row = tile * kTile + threadIdx.x;
+  const uint32_t flags = v.rowFlags[row];
+  double sum[kStatSums] = {0.0, 0.0, 0.0};
+  double mx[kStatMaxes] = {0.0, 0.0, 0.0};
+  if (!(flags & kRowPad)) {
+    const F3 x = v.x[row], vel = v.v[row];
+    if (!finite3(x) || !finite3(vel)) mx[0] = 1.0;
+    if (!(flags & kRowPinned)) {
+      const double v2 = double(vel.x) * vel.x + double(vel.y) * vel.y + double(vel.z) * vel.z;
+      sum[0] = 0.5 * massF64[row] * v2;
+      sum[1] = v2;
+      sum[2] = 1.0;
+    }
+    if (P.field.present) {
+      const double pen = -fieldSampleF64(v.fieldValues, P, x);
+      if (pen > mx[1]) mx[1] = pen;
+    }
+    // Seams whose first vertex is this row: seam entries end the row's sorted incidence list.
+    for (uint32_t k = v.incStart[row + 1]; k > v.incStart[row]; --k) {
+      const uint32_t e = v.inc[k - 1];
+      if ((e >> 30) != kKindSeam) break;
+      if ((e & 3u) != 0u) continue;
+      const Seam32 sp = v.seams[(e >> 2) & 0x0FFFFFFFu];
+      const F3 d = v.x[sp.a] - v.x[sp.b];
+      const float gap = __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(d.x, d.x), __fmul_rn(d.y, d.y)), __fmul_rn(d.z, d.z)));
+      if (double(gap) > mx[2]) mx[2] = gap;  // a NaN gap never wins, as with std::max
+    }
+  }
+  blockSumN<kStatSums, 4>(sum, lds);
+  blockMaxN256<kStatMaxes>(mx, lds);
+  if (threadIdx.x == 0) {
+    double* o = part + size_t(tile) * kStatParts;
+    for (int i = 0; i < kStatSums; ++i) o[i] = sum[i];
+    for (int i = 0; i < kStatMaxes; ++i) o[kStatSums + i] = mx[i];
+  }
+}
+
+// One workgroup per segment: its tiles' partials in tile order -> out[segment * 6].
+__global__ void __launch_bounds__(256) kStatsSegment(View v, const double* part, double* out) {
+  const uint32_t s = blockIdx.x;
+  const SegParams& P = v.seg[s];
+  __shared__ double lds[kStatParts * 4];
+  double sum[kStatSums] = {0.0, 0.0, 0.0};
+  double mx[kStatMaxes] = {0.0, 0.0, 0.0};
+  for (uint32_t t = P.tileBegin + threadIdx.x; t < P.tileEnd; t += 256u) {
+    const double* o = part + size_t(t) * kStatParts;
+    for (int i = 0; i < kStatSums; ++i) sum[i] += o[i];
+    for (int i = 0; i < kStatMaxes; ++i) mx[i] = fmax(mx[i], o[kStatSums + i]);
+  }
+  blockSumN<kStatSums, 4>(sum, lds);
+  blockMaxN256<kStatMaxes>(mx, lds);
+  if (threadIdx.x == 0) {
+    double* o = out + size_t(s) * kStatParts;
+    for (int i = 0; i < kStatSums; ++i) o[i] = sum[i];
+    for (int i = 0; i < kStatMaxes; ++i) o[kStatSums + i] = mx[i];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // Learned warm start support.
 
 // SDF distance and normal at the current position (distance = band and zero normal without a body).
@@ -1478,6 +1598,10 @@ void launchLineSearchDecide(const View& v, hipStream_t st) { kLineSearchDecide<<
 void launchApply(const View& v, hipStream_t st) { kApply<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
 void launchNewtonEnd(const View& v, hipStream_t st) { kNewtonEnd<<<v.nSegments, 256, 0, st>>>(v); }
 void launchFinish(const View& v, hipStream_t st) { kFinish<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
+void launchStats(const View& v, const double* massF64, double* tilePart, double* segOut, hipStream_t st) {
+  kStatsTile<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, massF64, tilePart);
+  kStatsSegment<<<v.nSegments, 256, 0, st>>>(v, tilePart, segOut);
+}
 void launchWarmFeatures(const View& v, hipStream_t st) { kWarmFeatures<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
 void launchWarmCheck(const View& v, hipStream_t st) {
   kWarmPrepare<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);

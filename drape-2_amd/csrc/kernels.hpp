@@ -115,6 +115,10 @@ void launchPcgIteration(const View& v, uint32_t k, hipStream_t st);
 void launchPcgSpmv(const View& v, uint32_t k, hipStream_t st, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void launchPcgUpdate(const View& v, uint32_t k, hipStream_t st);
 void launchPcgStepInit(const View& v, hipStream_t st);
+// SimModel::stats per segment: segOut[6 s + i] = kinetic energy, sum of |v|^2 and count of unpinned rows, then
+// the non-finite flag (1.0), the maximum body penetration and the maximum seam gap. tilePart: 6 per tile.
+constexpr uint32_t kStatsPerSegment = 6;
+void launchStats(const View& v, const double* massF64, double* tilePart, double* segOut, hipStream_t st);
 // One PCG iteration of the one-reduction form; wavesPerSlice (1, 2 or 4) wavefronts share each 64-row slice.
 void launchPcgStep(const View& v, uint32_t parity, uint32_t wavesPerSlice, hipStream_t st, hipEvent_t start = nullptr,
                    hipEvent_t stop = nullptr);

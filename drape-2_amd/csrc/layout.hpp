@@ -112,6 +112,7 @@ struct SegParams {
   uint32_t rowBegin, rowEnd;    // internal rows [rowBegin, rowEnd) (rowEnd padded to a tile)
   uint32_t pad2;
   Field32 field;
+  double fieldOrigin[3], fieldCell, fieldBand;  // the body grid's FP64 geometry (the FP64 sample of the stats)
 };
 
 // Per-segment solver state on the device. Written by the first workgroup of the segment (or by the per-segment

@@ -131,6 +131,9 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   for (const auto& [i, pos] : M.pins) pins[rowOf_[i]] = f3(pos);
   upload(rowFlags_, flags);
   upload(mass_, mass);
+  std::vector<double> massF64(nRows_, 0.0);
+  for (uint32_t i = 0; i < n; ++i) massF64[rowOf_[i]] = M.mass[i];
+  upload(massF64_, massF64);
   upload(rowFabric_, fab);
   upload(rowGarment_, gar);
   upload(fExt_, fext);
@@ -212,6 +215,9 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
       fieldValues.insert(fieldValues.end(), sg.body.values.begin(), sg.body.values.end());
     }
     P.field.band = static_cast<float>(sg.body.band);
+    for (int a = 0; a < 3; ++a) P.fieldOrigin[a] = sg.body.origin[a];
+    P.fieldCell = sg.body.cell;
+    P.fieldBand = sg.body.band;
   }
   upload(tileSeg_, tileSeg);
   upload(segParams_, params);
@@ -228,6 +234,8 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   partC_.allocate(sizeof(double) * nTiles_);
   partE_.allocate(sizeof(double) * nSlices_);
   partMax_.allocate(sizeof(float) * nTiles_);
+  statsTile_.allocate(sizeof(double) * dk::kStatsPerSegment * nTiles_);
+  statsSeg_.allocate(sizeof(double) * dk::kStatsPerSegment * std::max<size_t>(M.segments.size(), 1));
   cgPart0_.allocate(sizeof(double) * 7 * nTiles_);
   cgPart1_.allocate(sizeof(double) * 7 * nTiles_);
 
@@ -415,8 +423,13 @@ void HipSolver::uploadFabricsAndMass() {
   if (!f.empty())
     check(hipMemcpy(fabrics_.as<void>(), f.data(), sizeof(dk::Fabric32) * f.size(), hipMemcpyHostToDevice), "copy");
   std::vector<float> mass(nRows_, 1.0f);
-  for (uint32_t i = 0; i < model_.x.size(); ++i) mass[rowOf_[i]] = static_cast<float>(model_.mass[i]);
+  std::vector<double> massF64(nRows_, 0.0);
+  for (uint32_t i = 0; i < model_.x.size(); ++i) {
+    mass[rowOf_[i]] = static_cast<float>(model_.mass[i]);
+    massF64[rowOf_[i]] = model_.mass[i];
+  }
   check(hipMemcpy(mass_.as<void>(), mass.data(), sizeof(float) * nRows_, hipMemcpyHostToDevice), "copy");
+  check(hipMemcpy(massF64_.as<void>(), massF64.data(), sizeof(double) * nRows_, hipMemcpyHostToDevice), "copy");
 }
 
 void HipSolver::uploadHinges() {
@@ -764,38 +777,46 @@ void HipSolver::readVelocities(float* out, size_t n) const {
   }
 }
 
-// SimModel::stats (sim_model.cpp:222-241) on the host copy of x and v.
+// SimModel::stats (sim_model.cpp:222-241) reduced on the device: per-tile FP64 partials, summed per segment in a
+// fixed order (kStatsTile, kStatsSegment); only kStatsPerSegment doubles per segment are copied back.
+std::vector<double> HipSolver::segmentPartials() const {
+  const dk::View& v = view_;
+  dk::launchStats(v, massF64_.as<double>(), statsTile_.as<double>(), statsSeg_.as<double>(), stream_);
+  std::vector<double> out(dk::kStatsPerSegment * model_.segments.size());
+  check(hipMemcpyAsync(out.data(), statsSeg_.as<void>(), sizeof(double) * out.size(), hipMemcpyDeviceToHost, stream_),
+        "copy");
+  check(hipStreamSynchronize(stream_), "sync");
+  return out;
+}
+
+namespace {
+drape_stats combineSegments(const double* p, uint32_t count) {
+  drape_stats st{};
+  double sumV2 = 0.0, moving = 0.0;
+  for (uint32_t s = 0; s < count; ++s, p += dk::kStatsPerSegment) {
+    st.kinetic_energy += p[0];
+    sumV2 += p[1];
+    moving += p[2];
+    if (p[3] != 0.0) st.has_non_finite = 1;
+    st.max_body_penetration = std::max(st.max_body_penetration, p[4]);
+    st.max_seam_gap = std::max(st.max_seam_gap, p[5]);
+  }
+  st.rms_speed = moving > 0.0 ? std::sqrt(sumV2 / moving) : 0.0;
+  return st;
+}
+}  // namespace
+
 drape_stats HipSolver::stats() const {
   requireBuilt();
-  syncStateFromDevice();
-  const Model& m = model_;
-  drape_stats st{};
-  double sumV2 = 0;
-  size_t moving = 0;
-  for (size_t i = 0; i < m.x.size(); ++i) {
-    const Vec3f& x = m.x[i];
-    const Vec3f& v = m.v[i];
-    if (!std::isfinite(x.x) || !std::isfinite(x.y) || !std::isfinite(x.z) || !std::isfinite(v.x) ||
-        !std::isfinite(v.y) || !std::isfinite(v.z))
-      st.has_non_finite = 1;
-    if (m.pinned[i]) continue;
-    const double v2 = double(v.x) * v.x + double(v.y) * v.y + double(v.z) * v.z;
-    st.kinetic_energy += 0.5 * m.mass[i] * v2;
-    sumV2 += v2;
-    ++moving;
-  }
-  st.rms_speed = moving ? std::sqrt(sumV2 / static_cast<double>(moving)) : 0.0;
-  for (const auto& sp : m.seams) {
-    const Vec3f a = m.x[sp.a], b = m.x[sp.b];
-    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-    st.max_seam_gap = std::max(st.max_seam_gap, static_cast<double>(std::sqrt(dx * dx + dy * dy + dz * dz)));
-  }
-  for (size_t i = 0; i < m.x.size(); ++i) {
-    const Field& body = m.segments[m.pSegment[i]].body;
-    if (!body.present()) continue;
-    st.max_body_penetration = std::max(st.max_body_penetration, -body.sample({m.x[i].x, m.x[i].y, m.x[i].z}));
-  }
-  return st;
+  const std::vector<double> p = segmentPartials();
+  return combineSegments(p.data(), static_cast<uint32_t>(model_.segments.size()));
+}
+
+void HipSolver::segmentStats(uint32_t first, uint32_t count, drape_stats* out) const {
+  requireBuilt();
+  if (uint64_t(first) + count > model_.segments.size()) throw std::out_of_range("segment range out of range");
+  const std::vector<double> p = segmentPartials();
+  for (uint32_t s = 0; s < count; ++s) out[s] = combineSegments(p.data() + size_t(first + s) * dk::kStatsPerSegment, 1);
 }
 
 // Snapshot blob: u64 particles, u64 garments, u64 seams; positions, velocities (3 floats each); phases (int32);

@@ -27,6 +27,8 @@ class HipSolver {
   void readPositions(float* out, size_t n) const;
   void readVelocities(float* out, size_t n) const;
   drape_stats stats() const;
+  // Stats of segments [first, first + count) (one scene of a batched build each), from one device reduction.
+  void segmentStats(uint32_t first, uint32_t count, drape_stats* out) const;
   size_t snapshotSize() const;
   void snapshotSave(void* out, size_t bytes) const;
   void snapshotRestore(const void* in, size_t bytes);
@@ -74,6 +76,7 @@ class HipSolver {
   void syncStateFromDevice() const;   // x, v, contact force -> host model
   void syncStateToDevice();           // host model -> x, v, contact force
   void pollStates() const;            // device SegState -> hostStates_
+  std::vector<double> segmentPartials() const;  // launchStats -> host, kStatsPerSegment per segment
   bool allSegments(bool (*pred)(const dk::SegState&)) const;
   void substep(double h, int iterations);
   void runPcg();
@@ -102,6 +105,7 @@ class HipSolver {
   DeviceBuffer x_, v_, xStart_, yRel_, u_, grad_, step_, r_, z_, pA_, pB_, ap_, warm_, precond_;
   DeviceBuffer cnormal_, cflags_, cforce_, clambda_;
   DeviceBuffer dvLast_, phiN_;
+  DeviceBuffer massF64_, statsTile_, statsSeg_;  // device stats: FP64 masses, per-tile and per-segment partials
   std::unique_ptr<ws::Graph> meshGraph_;
   ws::Net* net_ = nullptr;
   DeviceBuffer tileSeg_, segParams_, segState_, partA_, partB_, partC_, partE_, partMax_;

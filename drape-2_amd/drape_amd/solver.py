@@ -124,6 +124,14 @@ class HipSolver:
         _check(_lib().drape_solver_stats(self._h, C.byref(st)))
         return st.as_dict()
 
+    def segment_stats(self, first: int = 0, count: int | None = None) -> list:
+        """Stats of each scene of a batched build (segments [first, first + count)), one device reduction."""
+        if count is None:
+            count = self.segment_count() - first
+        arr = (Stats * max(count, 1))()
+        _check(_lib().drape_solver_segment_stats(self._h, first, count, arr))
+        return [arr[i].as_dict() for i in range(count)]
+
     def snapshot(self) -> bytes:
         size = C.c_size_t()
         _check(_lib().drape_solver_snapshot_size(self._h, C.byref(size)))

@@ -53,6 +53,11 @@ class HipSolver {
     check(drape_solver_stats(s_, &st));
     return st;
   }
+  std::vector<drape_stats> segmentStats(uint32_t first, uint32_t count) const {
+    std::vector<drape_stats> st(count);
+    check(drape_solver_segment_stats(s_, first, count, st.data()));
+    return st;
+  }
   std::vector<unsigned char> snapshot() const {
     std::size_t bytes = 0;
     check(drape_solver_snapshot_size(s_, &bytes));

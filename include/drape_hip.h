@@ -172,8 +172,16 @@ int drape_solver_step(drape_solver* s, double dt, int32_t substeps, int32_t iter
 int drape_solver_read_positions(const drape_solver* s, float* out, size_t n);
 int drape_solver_read_velocities(const drape_solver* s, float* out, size_t n);
 
-/* Solver::stats (solver.hpp:93), computed on the device (FP64 reductions). */
+/* Solver::stats (solver.hpp:93; SimModel::stats, sim_model.cpp:222-241), reduced on the device in FP64 (per-tile
+ * partials summed per segment in a fixed order; the body is sampled in FP64): no positions or velocities cross
+ * PCIe, 48 bytes per segment do. Over all segments of a batched build. */
 int drape_solver_stats(const drape_solver* s, drape_stats* out);
+
+/* Stats of segments [first, first + count) of a batched build, one drape_stats per segment (each as
+ * drape_solver_stats of that scene built alone), from one device reduction: the per-frame check of a settle
+ * loop (drape_session.cpp:32-95 calls Solver::stats every frame) for many scenes at once. out_of_range when the
+ * range exceeds drape_solver_segment_count. No counterpart in the reference's Solver. */
+int drape_solver_segment_stats(const drape_solver* s, uint32_t first, uint32_t count, drape_stats* out);
 
 /* Solver::snapshot / restore (solver.hpp:94-95) as one flat byte blob: query the size, save into a caller
  * buffer, restore from one (sim_model.cpp:243-274). Little-endian layout, no padding:

@@ -328,3 +328,81 @@ def test_sheet_drop_contact_phase_stays_bounded(hip):
           f"{np.median(dist) * 1e3:.3f} mm, 99th percentile {np.percentile(dist, 99) * 1e3:.3f} mm", flush=True)
     ke1, ke0 = res[1][1]["kinetic_energy"], res[0][1]["kinetic_energy"]
     assert abs(ke1 - ke0) <= 0.5 * max(ke1, ke0)
+
+
+def _planted(blob: bytes, n: int, x=None, v=None) -> bytes:
+    """A snapshot blob with its positions and / or velocities replaced (layout of drape_solver_snapshot_save)."""
+    b = bytearray(blob)
+    head = 24
+    if x is not None:
+        b[head:head + 12 * n] = np.ascontiguousarray(x, np.float32).tobytes()
+    if v is not None:
+        b[head + 12 * n:head + 24 * n] = np.ascontiguousarray(v, np.float32).tobytes()
+    return bytes(b)
+
+
+def _stats_close(g, o, label):
+    gs, os_ = g.stats(), o.stats()
+    print(f"{label}: device {gs}\n{' ' * len(label)}  oracle {os_}", flush=True)
+    assert gs["has_non_finite"] == os_["has_non_finite"]
+    assert gs["kinetic_energy"] == pytest.approx(os_["kinetic_energy"], rel=1e-12, abs=1e-300)
+    assert gs["rms_speed"] == pytest.approx(os_["rms_speed"], rel=1e-12, abs=1e-300)
+    assert gs["max_body_penetration"] == pytest.approx(os_["max_body_penetration"], rel=1e-12, abs=1e-12)
+    assert gs["max_seam_gap"] == pytest.approx(os_["max_seam_gap"], rel=3e-7, abs=1e-12)
+    return gs
+
+
+def test_device_stats_match_oracle_stats(hip, oracle):
+    """stats() reduced on the device (FP64 partials per tile, FP64 SDF sample) against the oracle's
+    SimModel::stats on the same state: the sheet on its sphere after 12 steps, a seamed pair after 6 steps, and a
+    planted state (a NaN velocity on a pinned particle, one particle 6 mm inside the sphere)."""
+    for sc, steps in ((scenes.c1_sheet(), 12), (_seamed_pair(), 6)):
+        g = hip.HipSolver()
+        g.build(sc)
+        for _ in range(steps):
+            g.step(1 / 60, 1, 5)
+        o = oracle.OracleSolver()
+        o.build(sc)
+        o.restore(g.snapshot())
+        st = _stats_close(g, o, sc.name or "seamed pair")
+        assert st["kinetic_energy"] > 0.0
+    sc = scenes.c1_sheet()
+    g = hip.HipSolver()
+    g.build(sc)
+    g.step(1 / 60, 1, 5)
+    n = g.particle_count()
+    x, v = g.read_positions(), g.read_velocities()
+    x[100] = (0.0, 0.2 - 0.006, 0.0)
+    v[sc.pins[1][0]] = np.nan
+    blob = _planted(g.snapshot(), n, x, v)
+    g.restore(blob)
+    o = oracle.OracleSolver()
+    o.build(sc)
+    o.restore(blob)
+    st = _stats_close(g, o, "planted")
+    assert st["has_non_finite"] and st["max_body_penetration"] > 5e-3
+
+
+def test_segment_stats_equal_single_scene_stats(hip):
+    """Per-scene stats of a batched build equal, bit for bit, the stats of each scene stepped alone (segments start
+    on tile boundaries, so each one's tile partials and their sum are the same); stats() of the batch combines
+    them (sums, maxima, rms over all unpinned particles)."""
+    batch = [scenes.c1_sheet(), _seamed_pair(), scenes.c1_sheet(seed=5)]
+    g = hip.HipSolver()
+    g.build_batch(batch)
+    for _ in range(4):
+        g.step(1 / 60, 1, 5)
+    per = g.segment_stats()
+    assert len(per) == 3
+    for k, sc in enumerate(batch):
+        s = hip.HipSolver()
+        s.build(sc)
+        for _ in range(4):
+            s.step(1 / 60, 1, 5)
+        assert per[k] == s.stats(), k
+    whole = g.stats()
+    assert whole["kinetic_energy"] == pytest.approx(sum(p["kinetic_energy"] for p in per), rel=1e-15)
+    assert whole["max_seam_gap"] == max(p["max_seam_gap"] for p in per)
+    assert whole["max_body_penetration"] == max(p["max_body_penetration"] for p in per)
+    with pytest.raises(IndexError):
+        g.segment_stats(2, 2)
