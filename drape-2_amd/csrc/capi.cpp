@@ -446,7 +446,7 @@ int drape_bsr_spmv(uint32_t n, uint32_t n_blocks, const uint32_t* row_start, con
     }
     uint32_t maxK = 0;
     for (uint32_t t = 0; t < nSlices; ++t) maxK = std::max(maxK, sellK[t]);
-    std::vector<uint32_t> sellCol(std::max<size_t>(sellBase[nSlices] / 9, 1), nRows);
+    std::vector<uint32_t> sellCol(size_t(sellBase[nSlices] / 9) + dk::kSellColSlack, nRows);  // see kSellColSlack
     std::vector<float> sellVal(size_t(sellBase[nSlices]) + dk::kSellSlack, 0.0f);  // zero slack: see kSellSlack
     for (uint32_t r = 0; r < n; ++r) {
       const uint32_t t = r / dk::kSlice, l = r % dk::kSlice;

@@ -114,6 +114,15 @@ __device__ inline void blockSum3x256(double& a, double& b, double& c, double* ld
 // Row product of a SELL-64 slice with a gathered vector: sum_k B(row, k) x(col(row, k)). `gather(c)` returns x_c
 // (the dummy column nRows gives zero). Columns are taken eight at a time: the eight index loads, then the eight
 // gathers, then the block products, so a row costs two or three memory round trips instead of one per block.
+// Column of slot t of a lane's SELL row (col points at the lane's slot 0), the dummy column past K. The load is
+// unconditional — the column buffer carries kSellColSlack entries past its end — and the select acts on the loaded
+// value: a load of a selected address (this entry or the dummy index in the kernel arguments) would be a generic
+// (flat) load, which retires out of order with global loads and makes every later wait a wait for all of them.
+__device__ inline uint32_t sellColumn(const uint32_t* __restrict__ col, uint32_t t, uint32_t K, uint32_t dummy) {
+  const uint32_t c = col[t * kSlice];
+  return t < K ? c : dummy;
+}
+
 template <class Gather>
 __device__ inline F3 sellRowProduct(const View& v, uint32_t slice, uint32_t lane, Gather gather) {
   const uint32_t K = v.sellK[slice];
@@ -124,7 +133,7 @@ __device__ inline F3 sellRowProduct(const View& v, uint32_t slice, uint32_t lane
   for (uint32_t k0 = 0; k0 < K; k0 += 8u) {
     uint32_t c[8];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) c[t] = (k0 + t < K) ? col[(k0 + t) * kSlice] : v.nRows;
+    for (int t = 0; t < 8; ++t) c[t] = sellColumn(col, k0 + t, K, v.nRows);
     F3 x[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) x[t] = gather(c[t]);
@@ -646,44 +655,41 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
   __shared__ float comb[(W > 1 ? 4 : 1) * kSlicesPerTile * 3 * kSlice];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
-  // Per-segment scalars through the scalar cache: none of them is written by this kernel (it writes only the
-  // other parity slot of kIter / convPar, and r0 when k == 0, where it does not read it).
-  const uint32_t s = ldUniform(v.tileSeg + tile);
-  SegState* S = v.state + s;
-  const SegParams* P = v.seg + s;
-  const uint32_t tileBegin = ldUniform(&P->tileBegin), tileEnd = ldUniform(&P->tileEnd);
-  const uint32_t maxCg = ldUniform(&P->maxCg);
-  const double cgTolerance = ldUniform(&P->cgTolerance);
   const uint32_t o = 1u - q;
-  const uint32_t k = ldUniform(&S->kIter[q]);
-  // A solve that an earlier kernel ended: pass the flag on to the next parity slot and load nothing.
-  if (ldUniform(&S->newtonDone)) return;
-  if (ldUniform(&S->convPar[q])) {
-    if (tile == tileBegin && threadIdx.x == 0) {
-      S->convPar[o] = 1u;
-      S->kIter[o] = k + 1u;
-    }
-    return;
-  }
-  const double r0Prev = ldUniform(&S->r0);
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   const uint32_t sl = wave & 3u, part = W == 1 ? 0u : wave >> 2;
-  // Slice-uniform values through the scalar cache.
+  // Slice-uniform values through the scalar cache, issued with the segment index: neither depends on the other.
   const uint32_t slice = __builtin_amdgcn_readfirstlane(tile * kSlicesPerTile + sl);
   const uint32_t row = slice * kSlice + lane;
+  const uint32_t K = ldUniform(v.sellK + slice);
+  const uint32_t base = ldUniform(v.sellBase + slice);
+  const uint32_t s = ldUniform(v.tileSeg + tile);
   const F3* __restrict__ Zr = v.cgZr[o];
   const F3* __restrict__ T = v.cgT[o];
   const F3* __restrict__ Pp = v.cgP[o];
-  const uint32_t K = ldUniform(v.sellK + slice);
-  const uint32_t base = ldUniform(v.sellBase + slice);
 
-  // Round trip 1: the previous kernel's partial sums, the owner row's data and the column indices of the first
-  // batch. Nothing here depends on the scalars.
-  const double* __restrict__ prev = v.cgPart[o];
-  uint32_t ti = tileBegin + threadIdx.x;
-  double sum[kPcgParts];
+  // Round trip 1, issued before the segment's flags are known (everything here depends only on the slice and the
+  // parity): the column indices and block values of the first batch, then the owner row's data. A launch whose
+  // solve has already ended drops them unused; one that iterates has them in flight while the flags arrive.
+  // Column loads first (unconditional, see sellColumn), block loads after, and nothing that uses a column before
+  // both are issued: the wait before the gathers is then one for the columns only (the memory counter retires in
+  // issue order), not for the 36 B per slot of block values still in flight.
+  const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
+  const float* __restrict__ val = v.sellVal + base + 9u * lane;
+  uint32_t c[B];
 #pragma unroll
-  for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < tileEnd ? prev[kPcgParts * ti + n] : 0.0;
+  for (uint32_t j = 0; j < B; ++j) c[j] = col[(part + W * j) * kSlice];
+  __builtin_amdgcn_sched_barrier(0);
+  // The first batch's block values. Slots past K are read unconditionally — the value buffer carries kSellSlack
+  // zero floats past its end — because their column is the dummy one, whose gathered vectors are zero, so they add
+  // exactly nothing.
+  float bv[B][9];
+#pragma unroll
+  for (uint32_t j = 0; j < B; ++j) {
+    const uint32_t t = part + W * j;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) bv[j][e] = val[t * 9u * kSlice + e];
+  }
   F3 oR, oAp, oZ, oT, oP, oStep;
   float m[9];
   uint32_t oFlags = 0u, oCf = 0u;
@@ -701,27 +707,36 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
     oCf = v.cflags[row];
     oCn = v.cnormal[row];
   }
-  const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
-  const float* __restrict__ val = v.sellVal + base + 9u * lane;
-  uint32_t c[B];
-#pragma unroll
-  for (uint32_t j = 0; j < B; ++j) {
-    const uint32_t t = part + W * j;
-    c[j] = t < K ? col[t * kSlice] : v.nRows;
+  __builtin_amdgcn_sched_barrier(0);
+
+  // Per-segment scalars through the scalar cache: none of them is written by this kernel (it writes only the
+  // other parity slot of kIter / convPar, and r0 when k == 0, where it does not read it).
+  SegState* S = v.state + s;
+  const SegParams* P = v.seg + s;
+  const uint32_t tileBegin = ldUniform(&P->tileBegin), tileEnd = ldUniform(&P->tileEnd);
+  const uint32_t maxCg = ldUniform(&P->maxCg);
+  const double cgTolerance = ldUniform(&P->cgTolerance);
+  const uint32_t k = ldUniform(&S->kIter[q]);
+  // A solve that an earlier kernel ended: pass the flag on to the next parity slot.
+  if (ldUniform(&S->newtonDone)) return;
+  if (ldUniform(&S->convPar[q])) {
+    if (tile == tileBegin && threadIdx.x == 0) {
+      S->convPar[o] = 1u;
+      S->kIter[o] = k + 1u;
+    }
+    return;
   }
-  // The first batch's block values (they need no column index, so they share the first round trip). Slots past
-  // K are read unconditionally — the value buffer carries kSellSlack zero floats past its end — because their
-  // column is the dummy one, whose gathered vectors are zero, so they add exactly nothing.
-  float bv[B][9];
+  const double r0Prev = ldUniform(&S->r0);
+  // The previous kernel's partial sums.
+  const double* __restrict__ prev = v.cgPart[o];
+  uint32_t ti = tileBegin + threadIdx.x;
+  double sum[kPcgParts];
 #pragma unroll
-  for (uint32_t j = 0; j < B; ++j) {
-    const uint32_t t = part + W * j;
-#pragma unroll
-    for (int e = 0; e < 9; ++e) bv[j][e] = val[t * 9u * kSlice + e];
-  }
-  __builtin_amdgcn_sched_barrier(0);  // keep the block loads ahead of the wait for the column indices
+  for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < tileEnd ? prev[kPcgParts * ti + n] : 0.0;
   // Round trip 2: gathered (zr, t, p) of the first batch.
   F3 gz[B], gt[B], gp[B];
+#pragma unroll
+  for (uint32_t j = 0; j < B; ++j) c[j] = part + W * j < K ? c[j] : v.nRows;
 #pragma unroll
   for (uint32_t j = 0; j < B; ++j) {
     gz[j] = Zr[c[j]];
@@ -787,7 +802,7 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
 #pragma unroll
     for (uint32_t j = 0; j < B; ++j) {
       const uint32_t t = part + W * (j0 + j);
-      c[j] = t < K ? col[t * kSlice] : v.nRows;
+      c[j] = sellColumn(col, t, K, v.nRows);
     }
 #pragma unroll
     for (uint32_t j = 0; j < B; ++j) {
@@ -1248,7 +1263,10 @@ __global__ void __launch_bounds__(256) kSpmvPlain(View v, const F3* __restrict__
 // and block values of all 16 slots in one memory round trip — slots past K are read unconditionally (the value
 // buffer carries kSellSlack zero floats past its end) and point at the dummy column, so they add nothing — then
 // the gathered x. Row sums in the kPcgStep order (column index mod 4 groups).
-__global__ void __launch_bounds__(256) kSpmv16(View v, const F3* __restrict__ in, F3* out) {
+// One wavefront per SIMD is all a tile-per-workgroup launch of this size gets, so the kernel may use the whole
+// register file: the 16 columns, 144 block values and 48 gathered values of a lane are all in flight at once.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+kSpmv16(View v, const F3* __restrict__ in, F3* out) {
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -1258,9 +1276,12 @@ __global__ void __launch_bounds__(256) kSpmv16(View v, const F3* __restrict__ in
   const uint32_t base = ldUniform(v.sellBase + slice);
   const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
   const float* __restrict__ val = v.sellVal + base + 9u * lane;
+  // Columns (unconditional loads, see sellColumn) issued before the block values and used only after them, so the
+  // gathers wait for the columns alone.
   uint32_t c[16];
 #pragma unroll
-  for (uint32_t t = 0; t < 16; ++t) c[t] = t < K ? col[t * kSlice] : v.nRows;
+  for (uint32_t t = 0; t < 16; ++t) c[t] = col[t * kSlice];
+  __builtin_amdgcn_sched_barrier(0);
   float bv[16][9];
 #pragma unroll
   for (uint32_t t = 0; t < 16; ++t)
@@ -1269,7 +1290,8 @@ __global__ void __launch_bounds__(256) kSpmv16(View v, const F3* __restrict__ in
   __builtin_amdgcn_sched_barrier(0);
   F3 x[16];
 #pragma unroll
-  for (uint32_t t = 0; t < 16; ++t) x[t] = in[c[t]];
+  for (uint32_t t = 0; t < 16; ++t) x[t] = in[t < K ? c[t] : v.nRows];
+  __builtin_amdgcn_sched_barrier(0);  // every gather issued before the first product waits on memory
   float g[4][3];
 #pragma unroll
   for (int q = 0; q < 4; ++q) g[q][0] = g[q][1] = g[q][2] = 0.0f;
@@ -1667,8 +1689,9 @@ void launchBuildStructure(const BuildView& bIn, uint32_t& nBlocks, uint64_t& sel
   hipCheck(hipMemcpyAsync(&hostSell, b.sellSize + b.nSlices, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "copy");
   hipCheck(hipStreamSynchronize(st), "sync");
   sellFloats = hostSell;
-  hipCheck(hipMalloc(&b.sellCol, sizeof(uint32_t) * std::max<uint64_t>(hostSell / 9u, 1)), "malloc sellCol");
+  hipCheck(hipMalloc(&b.sellCol, sizeof(uint32_t) * (uint64_t(hostSell / 9u) + kSellColSlack)), "malloc sellCol");
   kSellCol<<<(nRows + 255) / 256, 256, 0, st>>>(b, b.sellSize);
+  launchFill(b.sellCol + hostSell / 9u, nRows, kSellColSlack, st);
   // maxK: copy sellK back (small) to size the assembly kernel's LDS.
   std::string err;
   {

@@ -34,6 +34,9 @@ constexpr uint32_t kMaxRow = 64;        // blocks per row the structure builder 
 // Zero floats after the last slice's block values: a kernel may read a whole batch of 16 slots of any slice
 // without testing each slot against K.
 constexpr uint32_t kSellSlack = 16u * 9u * 64u;
+// Column entries after the last slice's columns: a kernel may load the column indices of 16 slots past any slice's
+// K (and discards them).
+constexpr uint32_t kSellColSlack = 16u * 64u;
 
 // Row flags.
 constexpr uint32_t kRowPinned = 1u;

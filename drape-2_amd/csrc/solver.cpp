@@ -305,7 +305,7 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   uint64_t sellFloats = 0;
   dk::launchBuildStructure(b, nBlocks, sellFloats, maxK, colIndex, sellCol, stream_);
   colIndex_.adopt(colIndex, sizeof(uint32_t) * std::max<uint32_t>(nBlocks, 1));
-  sellCol_.adopt(sellCol, sizeof(uint32_t) * std::max<uint64_t>(sellFloats / 9, 1));
+  sellCol_.adopt(sellCol, sizeof(uint32_t) * (sellFloats / 9 + dk::kSellColSlack));
   nBlocks_ = nBlocks;
   sellFloats_ = sellFloats;
   maxK_ = maxK;
