@@ -13,9 +13,11 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "kernels.hpp"
 #include "term_math.hpp"
@@ -111,11 +113,8 @@ __device__ inline void blockSum3x256(double& a, double& b, double& c, double* ld
   c = ((lds[8] + lds[9]) + lds[10]) + lds[11];
 }
 
-// Row product of a SELL-64 slice with a gathered vector: sum_k B(row, k) x(col(row, k)). `gather(c)` returns x_c
-// (the dummy column nRows gives zero). Columns are taken eight at a time: the eight index loads, then the eight
-// gathers, then the block products, so a row costs two or three memory round trips instead of one per block.
 // Column of slot t of a lane's SELL row (col points at the lane's slot 0), the dummy column past K. The load is
-// unconditional — the column buffer carries kSellColSlack entries past its end — and the select acts on the loaded
+// unconditional — the column buffers carry kSellColSlack entries past their ends — and the select acts on the loaded
 // value: a load of a selected address (this entry or the dummy index in the kernel arguments) would be a generic
 // (flat) load, which retires out of order with global loads and makes every later wait a wait for all of them.
 __device__ inline uint32_t sellColumn(const uint32_t* __restrict__ col, uint32_t t, uint32_t K, uint32_t dummy) {
@@ -123,13 +122,47 @@ __device__ inline uint32_t sellColumn(const uint32_t* __restrict__ col, uint32_t
   return t < K ? c : dummy;
 }
 
+// Block times vector, and the transposed block times vector (a lower slot's block is stored as its transpose).
+__device__ inline void blockMulAdd(const float* b, F3 x, float& a0, float& a1, float& a2) {
+  a0 += b[0] * x.x + b[1] * x.y + b[2] * x.z;
+  a1 += b[3] * x.x + b[4] * x.y + b[5] * x.z;
+  a2 += b[6] * x.x + b[7] * x.y + b[8] * x.z;
+}
+__device__ inline void blockTMulAdd(const float* b, F3 x, float& a0, float& a1, float& a2) {
+  a0 += b[0] * x.x + b[3] * x.y + b[6] * x.z;
+  a1 += b[1] * x.x + b[4] * x.y + b[7] * x.z;
+  a2 += b[2] * x.x + b[5] * x.y + b[8] * x.z;
+}
+
+// Row product of a slice of the symmetric SELL-64 matrix with a gathered vector: the lower slots (transposed
+// blocks found through lowBlk) then the upper slots. `gather(c)` returns x_c (the dummy column nRows gives zero).
+// Slots are taken eight at a time: the eight index loads, then the eight gathers, then the block products, so a
+// row costs a few memory round trips instead of one per block.
 template <class Gather>
 __device__ inline F3 sellRowProduct(const View& v, uint32_t slice, uint32_t lane, Gather gather) {
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  if (v.lowK) {
+    const uint32_t KL = v.lowK[slice];
+    const uint32_t lb = v.lowBase[slice] + lane;
+    for (uint32_t k0 = 0; k0 < KL; k0 += 8u) {
+      uint32_t c[8], blk[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        c[t] = sellColumn(v.lowCol + lb, k0 + t, KL, v.nRows);
+        blk[t] = sellColumn(v.lowBlk + lb, k0 + t, KL, v.zeroBlock);
+      }
+      F3 x[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) x[t] = gather(c[t]);
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (k0 + t < KL) blockTMulAdd(v.sellVal + size_t(blk[t]) * 9u, x[t], a0, a1, a2);
+    }
+  }
   const uint32_t K = v.sellK[slice];
   const uint32_t base = v.sellBase[slice];
   const float* __restrict__ val = v.sellVal + base + 9u * lane;
   const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
-  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
   for (uint32_t k0 = 0; k0 < K; k0 += 8u) {
     uint32_t c[8];
 #pragma unroll
@@ -138,14 +171,8 @@ __device__ inline F3 sellRowProduct(const View& v, uint32_t slice, uint32_t lane
 #pragma unroll
     for (int t = 0; t < 8; ++t) x[t] = gather(c[t]);
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      if (k0 + t < K) {
-        const float* bv = val + (k0 + t) * 9u * kSlice;
-        a0 += bv[0] * x[t].x + bv[1] * x[t].y + bv[2] * x[t].z;
-        a1 += bv[3] * x[t].x + bv[4] * x[t].y + bv[5] * x[t].z;
-        a2 += bv[6] * x[t].x + bv[7] * x[t].y + bv[8] * x[t].z;
-      }
-    }
+    for (int t = 0; t < 8; ++t)
+      if (k0 + t < K) blockMulAdd(val + (k0 + t) * 9u * kSlice, x[t], a0, a1, a2);
   }
   return f3(a0, a1, a2);
 }
@@ -302,9 +329,10 @@ __device__ inline float incidence(const View& v, const SegParams& P, uint32_t e,
 
 // ---------------------------------------------------------------------------------------------------------------
 // Assembly: energy, gradient and the block-CSR Hessian, atomic-free. One wavefront per 64-row slice; each lane
-// owns a row, evaluates every term incident to it (kernels.h:104-224 semantics) and accumulates its blocks in LDS
-// in the slice's SELL layout; the slice is then written out with coalesced stores. Contact release follows
-// (kernels.h:227-231).
+// owns a row, evaluates every term incident to it (kernels.h:104-224 semantics) and accumulates its upper blocks
+// (the diagonal and the columns above it) in LDS in the slice's SELL layout; the slice is then written out with
+// coalesced stores. A block below the diagonal is the transpose of its column's upper block and is not stored.
+// Contact release follows (kernels.h:227-231).
 
 __global__ void __launch_bounds__(64) kAssemble(View v) {
   extern __shared__ float acc[];  // K * 9 * 64
@@ -323,14 +351,11 @@ __global__ void __launch_bounds__(64) kAssemble(View v) {
   double rowE = 0.0;
   F3 g = f3(0, 0, 0);
   if (!(flags & kRowPad)) {
-    const uint32_t rs = v.rowStart[row];
+    const uint32_t kd = v.diagSlot[row];  // upper slot k of the row is BSR slot kd + k
     M3 Hown = zeroM();
     rowE += double(ownTerm<true>(v, P, row, flags, 0.0f, g, Hown));
-    {
-      const uint32_t kd = v.diagSlot[row] - rs;
 #pragma unroll
-      for (int c = 0; c < 9; ++c) acc[(kd * kSlice + lane) * 9u + c] += Hown.m[c];
-    }
+    for (int c = 0; c < 9; ++c) acc[lane * 9u + c] += Hown.m[c];
     const uint32_t i1 = v.incStart[row + 1];
     for (uint32_t i = v.incStart[row]; i < i1; ++i) {
       const uint32_t e = v.inc[i];
@@ -342,7 +367,8 @@ __global__ void __launch_bounds__(64) kAssemble(View v) {
       if (incLocal(e) == 0u) rowE += double(E);
       g += gt;
       for (uint32_t b = 0; b < m; ++b) {
-        const uint32_t kk = slots[b] - rs;
+        if (slots[b] < kd) continue;
+        const uint32_t kk = slots[b] - kd;
 #pragma unroll
         for (int c = 0; c < 9; ++c) acc[(kk * kSlice + lane) * 9u + c] += H[b].m[c];
       }
@@ -368,10 +394,9 @@ __global__ void __launch_bounds__(64) kAssemble(View v) {
 #pragma unroll
       for (int c = 0; c < 9; ++c) M[c] = 0.0f;
     } else {
-      const uint32_t kd = v.diagSlot[row] - v.rowStart[row];
       float A[9];
 #pragma unroll
-      for (int c = 0; c < 9; ++c) A[c] = acc[(kd * kSlice + lane) * 9u + c];
+      for (int c = 0; c < 9; ++c) A[c] = acc[lane * 9u + c];
       if (!invertSpd3(A, M)) {
 #pragma unroll
         for (int c = 0; c < 9; ++c) M[c] = (c % 4 == 0) ? 1.0f : 0.0f;
@@ -650,7 +675,7 @@ __global__ void __launch_bounds__(256) kPcgStepInit(View v) {
 template <int W>
 __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
   constexpr int NW = 4 * W;
-  constexpr uint32_t B = 16u / W;  // columns per wavefront in one batch
+  constexpr uint32_t B = 8u / W;  // lower slots, and upper slots, per wavefront in one batch of eight
   __shared__ double red[kPcgParts * NW];
   __shared__ float comb[(W > 1 ? 4 : 1) * kSlicesPerTile * 3 * kSlice];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
@@ -663,26 +688,35 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
   const uint32_t row = slice * kSlice + lane;
   const uint32_t K = ldUniform(v.sellK + slice);
   const uint32_t base = ldUniform(v.sellBase + slice);
+  const uint32_t KL = ldUniform(v.lowK + slice);
+  const uint32_t lbase = ldUniform(v.lowBase + slice);
   const uint32_t s = ldUniform(v.tileSeg + tile);
   const F3* __restrict__ Zr = v.cgZr[o];
   const F3* __restrict__ T = v.cgT[o];
   const F3* __restrict__ Pp = v.cgP[o];
 
   // Round trip 1, issued before the segment's flags are known (everything here depends only on the slice and the
-  // parity): the column indices and block values of the first batch, then the owner row's data. A launch whose
-  // solve has already ended drops them unused; one that iterates has them in flight while the flags arrive.
-  // Column loads first (unconditional, see sellColumn), block loads after, and nothing that uses a column before
-  // both are issued: the wait before the gathers is then one for the columns only (the memory counter retires in
-  // issue order), not for the 36 B per slot of block values still in flight.
+  // parity): the first batch's columns (lower and upper), lower block indices and upper block values, then the
+  // owner row's data. A launch whose solve has already ended drops them unused; one that iterates has them in
+  // flight while the flags arrive. Index loads first (unconditional, see sellColumn), block loads after, and
+  // nothing that uses an index before both are issued: the wait before the gathers is then one for the indices
+  // only (the memory counter retires in issue order), not for the 36 B per slot of block values still in flight.
   const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
   const float* __restrict__ val = v.sellVal + base + 9u * lane;
-  uint32_t c[B];
+  const uint32_t* __restrict__ lcol = v.lowCol + lbase + lane;
+  const uint32_t* __restrict__ lblk = v.lowBlk + lbase + lane;
+  uint32_t c[B], cl[B], bl[B];
+#pragma unroll
+  for (uint32_t j = 0; j < B; ++j) {
+    cl[j] = lcol[(part + W * j) * kSlice];
+    bl[j] = lblk[(part + W * j) * kSlice];
+  }
 #pragma unroll
   for (uint32_t j = 0; j < B; ++j) c[j] = col[(part + W * j) * kSlice];
   __builtin_amdgcn_sched_barrier(0);
-  // The first batch's block values. Slots past K are read unconditionally — the value buffer carries kSellSlack
-  // zero floats past its end — because their column is the dummy one, whose gathered vectors are zero, so they add
-  // exactly nothing.
+  // The first batch's upper block values. Slots past K are read unconditionally — the value buffer carries
+  // kSellSlack zero floats past its end — because their column is the dummy one, whose gathered vectors are zero,
+  // so they add exactly nothing.
   float bv[B][9];
 #pragma unroll
   for (uint32_t j = 0; j < B; ++j) {
@@ -734,9 +768,26 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
 #pragma unroll
   for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < tileEnd ? prev[kPcgParts * ti + n] : 0.0;
   // Round trip 2: gathered (zr, t, p) of the first batch.
-  F3 gz[B], gt[B], gp[B];
+  // Round trip 2: the transposed lower blocks, and the gathered (zr, t, p) of the lower and upper columns.
 #pragma unroll
-  for (uint32_t j = 0; j < B; ++j) c[j] = part + W * j < K ? c[j] : v.nRows;
+  for (uint32_t j = 0; j < B; ++j) {
+    const uint32_t t = part + W * j;
+    cl[j] = t < KL ? cl[j] : v.nRows;
+    bl[j] = t < KL ? bl[j] : v.zeroBlock;
+    c[j] = t < K ? c[j] : v.nRows;
+  }
+  float bt[B][9];
+#pragma unroll
+  for (uint32_t j = 0; j < B; ++j)
+#pragma unroll
+    for (int e = 0; e < 9; ++e) bt[j][e] = v.sellVal[size_t(bl[j]) * 9u + e];  // 64-bit index: one wide load
+  F3 lz[B], lt[B], lp[B], gz[B], gt[B], gp[B];
+#pragma unroll
+  for (uint32_t j = 0; j < B; ++j) {
+    lz[j] = Zr[cl[j]];
+    lt[j] = T[cl[j]];
+    lp[j] = Pp[cl[j]];
+  }
 #pragma unroll
   for (uint32_t j = 0; j < B; ++j) {
     gz[j] = Zr[c[j]];
@@ -784,37 +835,40 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
   if (part == 0 && update) v.step[row] = oStep + a * oP;
   if (conv) return;
 
-  // Row sums of A p_k. Columns are summed in four groups (column index mod 4), each in increasing column order,
-  // and the groups are added as ((g0 + g1) + g2) + g3 whatever W is: every W gives bit-identical results.
+  // Row sums of A p_k. Slots are summed in four groups (slot index mod 4); each group takes, batch by batch of
+  // eight slots, its lower slots then its upper slots in increasing slot order, and the groups are added as
+  // ((g0 + g1) + g2) + g3 whatever W is: every W gives bit-identical results.
   constexpr uint32_t G = 4u / W;  // groups held by one wavefront: group part + W lg
   float acc[G][3];
 #pragma unroll
   for (uint32_t lg = 0; lg < G; ++lg) acc[lg][0] = acc[lg][1] = acc[lg][2] = 0.0f;
 #pragma unroll
-  for (uint32_t j = 0; j < B; ++j) {
-    const F3 x = pcgDirection(gz[j], gt[j], gp[j], a, b);
-    float* g = acc[j % G];
-    g[0] += bv[j][0] * x.x + bv[j][1] * x.y + bv[j][2] * x.z;
-    g[1] += bv[j][3] * x.x + bv[j][4] * x.y + bv[j][5] * x.z;
-    g[2] += bv[j][6] * x.x + bv[j][7] * x.y + bv[j][8] * x.z;
-  }
-  for (uint32_t j0 = B; part + W * j0 < K; j0 += B) {  // rows with more than B W blocks
+  for (uint32_t j = 0; j < B; ++j)
+    blockTMulAdd(bt[j], pcgDirection(lz[j], lt[j], lp[j], a, b), acc[j % G][0], acc[j % G][1], acc[j % G][2]);
+#pragma unroll
+  for (uint32_t j = 0; j < B; ++j)
+    blockMulAdd(bv[j], pcgDirection(gz[j], gt[j], gp[j], a, b), acc[j % G][0], acc[j % G][1], acc[j % G][2]);
+  const uint32_t Kmax = K > KL ? K : KL;
+  for (uint32_t j0 = B; part + W * j0 < Kmax; j0 += B) {  // rows with more than eight lower or upper slots
 #pragma unroll
     for (uint32_t j = 0; j < B; ++j) {
       const uint32_t t = part + W * (j0 + j);
-      c[j] = sellColumn(col, t, K, v.nRows);
+      cl[j] = sellColumn(lcol, t, KL, v.nRows);
+      bl[j] = sellColumn(lblk, t, KL, v.zeroBlock);
     }
 #pragma unroll
     for (uint32_t j = 0; j < B; ++j) {
       const uint32_t t = part + W * (j0 + j);
+      const F3 x = pcgDirection(Zr[cl[j]], T[cl[j]], Pp[cl[j]], a, b);
+      if (t < KL) blockTMulAdd(v.sellVal + size_t(bl[j]) * 9u, x, acc[j % G][0], acc[j % G][1], acc[j % G][2]);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < B; ++j) c[j] = sellColumn(col, part + W * (j0 + j), K, v.nRows);
+#pragma unroll
+    for (uint32_t j = 0; j < B; ++j) {
+      const uint32_t t = part + W * (j0 + j);
       const F3 x = pcgDirection(Zr[c[j]], T[c[j]], Pp[c[j]], a, b);
-      if (t < K) {
-        const float* bj = val + t * 9u * kSlice;
-        float* g = acc[j % G];
-        g[0] += bj[0] * x.x + bj[1] * x.y + bj[2] * x.z;
-        g[1] += bj[3] * x.x + bj[4] * x.y + bj[5] * x.z;
-        g[2] += bj[6] * x.x + bj[7] * x.y + bj[8] * x.z;
-      }
+      if (t < K) blockMulAdd(val + t * 9u * kSlice, x, acc[j % G][0], acc[j % G][1], acc[j % G][2]);
     }
   }
   if constexpr (W > 1) {
@@ -1259,10 +1313,10 @@ __global__ void __launch_bounds__(256) kSpmvPlain(View v, const F3* __restrict__
   out[row] = (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : prod;
 }
 
-// The same product for matrices whose rows have at most 16 blocks (every regular-grid garment): column indices
-// and block values of all 16 slots in one memory round trip — slots past K are read unconditionally (the value
-// buffer carries kSellSlack zero floats past its end) and point at the dummy column, so they add nothing — then
-// the gathered x. Row sums in the kPcgStep order (column index mod 4 groups).
+// The same product for a general matrix (every block stored, lowK null) whose rows have at most 16 blocks: column
+// indices and block values of all 16 slots in one memory round trip — slots past K are read unconditionally (the
+// value buffer carries kSellSlack zero floats past its end) and point at the dummy column, so they add nothing —
+// then the gathered x. Row sums in four groups (slot index mod 4).
 // One wavefront per SIMD is all a tile-per-workgroup launch of this size gets, so the kernel may use the whole
 // register file: the 16 columns, 144 block values and 48 gathered values of a lane are all in flight at once.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
@@ -1312,10 +1366,14 @@ __global__ void __launch_bounds__(256) kSellToSlots(View v, float* out) {
   const uint32_t row = blockIdx.x * 256u + threadIdx.x;
   if (row >= v.nRows) return;
   const uint32_t slice = row / kSlice, lane = row % kSlice;
-  const uint32_t rs = v.rowStart[row], re = v.rowStart[row + 1];
+  const uint32_t rs = v.rowStart[row], re = v.rowStart[row + 1], kd = v.diagSlot[row];
   const float* base = v.sellVal + v.sellBase[slice] + 9u * lane;
-  for (uint32_t sl = rs; sl < re; ++sl)
-    for (int c = 0; c < 9; ++c) out[9u * sl + c] = base[(sl - rs) * 9u * kSlice + c];
+  for (uint32_t sl = rs; sl < kd; ++sl) {  // below the diagonal: the transpose of the column's upper block
+    const float* b = v.sellVal + 9u * v.lowBlk[v.lowBase[slice] + (sl - rs) * kSlice + lane];
+    for (int c = 0; c < 9; ++c) out[9u * sl + c] = b[(c % 3) * 3 + c / 3];
+  }
+  for (uint32_t sl = kd; sl < re; ++sl)
+    for (int c = 0; c < 9; ++c) out[9u * sl + c] = base[(sl - kd) * 9u * kSlice + c];
 }
 
 // Batch term evaluation (full gradient and Hessian from the row kernels, a = 0..N-1).
@@ -1549,29 +1607,52 @@ __global__ void kSlots(BuildView b) {
       out[a * m + c] = lowerBound(b.colIndex, b.rowStart[vs[a]], b.rowStart[vs[a] + 1], vs[c]);
 }
 
+// Slots of each slice: upper (the diagonal and the columns above it) and lower (the columns below it). Every row
+// holds its diagonal, so a row's lower count is its diagonal's position in its sorted column list.
 __global__ void kSellK(BuildView b) {
   const uint32_t slice = blockIdx.x * blockDim.x + threadIdx.x;
   if (slice >= b.nSlices) return;
-  uint32_t K = 0;
-  for (uint32_t l = 0; l < kSlice; ++l) K = max(K, b.rowLen[slice * kSlice + l]);
+  uint32_t K = 0, KL = 0;
+  for (uint32_t l = 0; l < kSlice; ++l) {
+    const uint32_t row = slice * kSlice + l;
+    const uint32_t lower = b.diagSlot[row] - b.rowStart[row];
+    K = max(K, b.rowLen[row] - lower);
+    KL = max(KL, lower);
+  }
   b.sellK[slice] = K;
   b.sellSize[slice] = K * 9u * kSlice;
+  b.lowK[slice] = KL;
+  b.lowSize[slice] = KL * kSlice;
 }
 
-__global__ void kSellCol(BuildView b, const uint32_t* sellBase) {
+// Upper slot k of a row: column colIndex[diag + k]; lower slot k: column colIndex[rowStart + k] and the block of
+// (that column, this row) in the column's upper slots. Pinned columns point at the dummy column nRows; padding
+// slots at the dummy column and the zero block.
+__global__ void kSellCol(BuildView b, const uint32_t* sellBase, const uint32_t* lowBase, uint32_t zeroBlock) {
   const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= b.nRows) return;
   const uint32_t slice = row / kSlice, lane = row % kSlice;
-  const uint32_t K = b.sellK[slice];
-  const uint32_t base = sellBase[slice] / 9u;
-  const uint32_t rs = b.rowStart[row], len = b.rowStart[row + 1] - rs;
+  const uint32_t K = b.sellK[slice], KL = b.lowK[slice];
+  const uint32_t base = sellBase[slice] / 9u, lbase = lowBase[slice];
+  const uint32_t rs = b.rowStart[row], re = b.rowStart[row + 1], kd = b.diagSlot[row];
   for (uint32_t k = 0; k < K; ++k) {
     uint32_t c = b.nRows;
-    if (k < len) {
-      c = b.colIndex[rs + k];
+    if (kd + k < re) {
+      c = b.colIndex[kd + k];
       if (b.rowFlags[c] & kRowPinned) c = b.nRows;
     }
     b.sellCol[base + k * kSlice + lane] = c;
+  }
+  for (uint32_t k = 0; k < KL; ++k) {
+    uint32_t c = b.nRows, blk = zeroBlock;
+    if (rs + k < kd) {
+      const uint32_t j = b.colIndex[rs + k];
+      const uint32_t u = lowerBound(b.colIndex, b.rowStart[j], b.rowStart[j + 1], row) - b.diagSlot[j];
+      blk = sellBase[j / kSlice] / 9u + u * kSlice + j % kSlice;
+      c = (b.rowFlags[j] & kRowPinned) ? b.nRows : j;
+    }
+    b.lowCol[lbase + k * kSlice + lane] = c;
+    b.lowBlk[lbase + k * kSlice + lane] = blk;
   }
 }
 
@@ -1638,7 +1719,7 @@ void launchFill(uint32_t* p, uint32_t value, uint64_t count, hipStream_t st) {
   kFill<<<uint32_t((count + 255) / 256), 256, 0, st>>>(p, value, count);
 }
 void launchSpmvPlain(const View& v, uint32_t maxK, const F3* in, F3* out, hipStream_t st) {
-  if (maxK <= 16u)
+  if (!v.lowK && maxK <= 16u)
     kSpmv16<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, in, out);
   else
     kSpmvPlain<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, in, out);
@@ -1657,8 +1738,7 @@ void launchFieldQuery(const float* values, Field32 field, uint32_t n, const floa
   kFieldQuery<<<(n + 255) / 256, 256, 0, st>>>(values, field, n, p, phi, normal);
 }
 
-void launchBuildStructure(const BuildView& bIn, uint32_t& nBlocks, uint64_t& sellFloats, uint32_t& maxK,
-                          uint32_t*& colIndexOut, uint32_t*& sellColOut, hipStream_t st) {
+void launchBuildStructure(const BuildView& bIn, BuiltStructure& out, hipStream_t st) {
   BuildView b = bIn;
   const uint32_t items = 3u * b.nStretch + 4u * b.nHinge + 2u * b.nSeam;
   const uint32_t nRows = b.nRows;
@@ -1678,32 +1758,45 @@ void launchBuildStructure(const BuildView& bIn, uint32_t& nBlocks, uint64_t& sel
     throw std::invalid_argument(hostErr == 1u ? "hip: a particle has more than 96 incident terms"
                                               : "hip: a particle couples to more than 64 particles");
   }
-  nBlocks = hostTotal;
-  hipCheck(hipMalloc(&b.colIndex, sizeof(uint32_t) * std::max<uint32_t>(nBlocks, 1)), "malloc colIndex");
+  out.nBlocks = hostTotal;
+  hipCheck(hipMalloc(&b.colIndex, sizeof(uint32_t) * std::max<uint32_t>(out.nBlocks, 1)), "malloc colIndex");
   kPatternFill<<<(nRows + 127) / 128, 128, 0, st>>>(b);
   const uint32_t terms = b.nStretch + b.nHinge + b.nSeam;
   if (terms) kSlots<<<(terms + 255) / 256, 256, 0, st>>>(b);
   kSellK<<<(b.nSlices + 255) / 256, 256, 0, st>>>(b);
   kScan<<<1, 1024, 0, st>>>(b.sellSize, b.sellSize, b.nSlices);
-  uint32_t hostSell = 0;
+  kScan<<<1, 1024, 0, st>>>(b.lowSize, b.lowSize, b.nSlices);
+  uint32_t hostSell = 0, hostLow = 0;
   hipCheck(hipMemcpyAsync(&hostSell, b.sellSize + b.nSlices, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "copy");
+  hipCheck(hipMemcpyAsync(&hostLow, b.lowSize + b.nSlices, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "copy");
   hipCheck(hipStreamSynchronize(st), "sync");
-  sellFloats = hostSell;
+  out.sellFloats = hostSell;
+  out.lowEntries = hostLow;
+  const uint32_t zeroBlock = hostSell / 9u;
   hipCheck(hipMalloc(&b.sellCol, sizeof(uint32_t) * (uint64_t(hostSell / 9u) + kSellColSlack)), "malloc sellCol");
-  kSellCol<<<(nRows + 255) / 256, 256, 0, st>>>(b, b.sellSize);
+  hipCheck(hipMalloc(&b.lowCol, sizeof(uint32_t) * (uint64_t(hostLow) + kSellColSlack)), "malloc lowCol");
+  hipCheck(hipMalloc(&b.lowBlk, sizeof(uint32_t) * (uint64_t(hostLow) + kSellColSlack)), "malloc lowBlk");
+  kSellCol<<<(nRows + 255) / 256, 256, 0, st>>>(b, b.sellSize, b.lowSize, zeroBlock);
   launchFill(b.sellCol + hostSell / 9u, nRows, kSellColSlack, st);
-  // maxK: copy sellK back (small) to size the assembly kernel's LDS.
-  std::string err;
+  launchFill(b.lowCol + hostLow, nRows, kSellColSlack, st);
+  launchFill(b.lowBlk + hostLow, zeroBlock, kSellColSlack, st);
+  // The largest slot counts: copied back (small) to size the assembly kernel's LDS and pick kernels.
   {
-    uint32_t* hk = static_cast<uint32_t*>(malloc(sizeof(uint32_t) * b.nSlices));
-    hipCheck(hipMemcpyAsync(hk, b.sellK, sizeof(uint32_t) * b.nSlices, hipMemcpyDeviceToHost, st), "copy");
+    std::vector<uint32_t> hk(b.nSlices), hl(b.nSlices);
+    hipCheck(hipMemcpyAsync(hk.data(), b.sellK, sizeof(uint32_t) * b.nSlices, hipMemcpyDeviceToHost, st), "copy");
+    hipCheck(hipMemcpyAsync(hl.data(), b.lowK, sizeof(uint32_t) * b.nSlices, hipMemcpyDeviceToHost, st), "copy");
     hipCheck(hipStreamSynchronize(st), "sync");
-    maxK = 0;
-    for (uint32_t i = 0; i < b.nSlices; ++i) maxK = std::max(maxK, hk[i]);
-    free(hk);
+    out.maxK = 0;
+    out.maxLowK = 0;
+    for (uint32_t i = 0; i < b.nSlices; ++i) {
+      out.maxK = std::max(out.maxK, hk[i]);
+      out.maxLowK = std::max(out.maxLowK, hl[i]);
+    }
   }
-  colIndexOut = b.colIndex;
-  sellColOut = b.sellCol;
+  out.colIndex = b.colIndex;
+  out.sellCol = b.sellCol;
+  out.lowCol = b.lowCol;
+  out.lowBlk = b.lowBlk;
 }
 
 }  // namespace dk

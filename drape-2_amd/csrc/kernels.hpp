@@ -13,7 +13,8 @@ namespace dk {
 // "dummy column" that padding and pinned-column SELL entries point at.
 struct View {
   uint32_t nRows, nTiles, nSlices, nSegments;
-  uint32_t nStretch, nHinge, nSeam, pad0;
+  uint32_t nStretch, nHinge, nSeam;
+  uint32_t zeroBlock;  // index (in blocks) of an all-zero block after the last slice's values
   // per row
   const uint32_t* rowFlags;
   const float* mass;
@@ -74,10 +75,17 @@ struct View {
   const uint32_t* stretchSlots;
   const uint32_t* hingeSlots;
   const uint32_t* seamSlots;
-  const uint32_t* sellK;     // per slice
+  // Symmetric SELL-64: the upper part (diagonal first, then the columns above it) holds block values; the lower part
+  // holds, per slot, the column and the index of the transposed block in that column's upper part. lowK == nullptr
+  // means a general matrix stored whole in the upper arrays.
+  const uint32_t* sellK;     // per slice: upper slots
   const uint32_t* sellBase;  // per slice, in floats (column base = sellBase / 9)
   const uint32_t* sellCol;
   float* sellVal;
+  const uint32_t* lowK;      // per slice: lower slots
+  const uint32_t* lowBase;   // per slice, in entries
+  const uint32_t* lowCol;
+  const uint32_t* lowBlk;    // block index (float offset / 9) of block (column, row) in sellVal
   const float* fieldValues;
 };
 
@@ -98,9 +106,13 @@ struct BuildView {
   uint32_t* stretchSlots;
   uint32_t* hingeSlots;
   uint32_t* seamSlots;
-  uint32_t* sellK;      // nSlices
+  uint32_t* sellK;      // nSlices: upper slots
   uint32_t* sellSize;   // nSlices + 1 (scan of K * 576)
   uint32_t* sellCol;
+  uint32_t* lowK;       // nSlices: lower slots
+  uint32_t* lowSize;    // nSlices + 1 (scan of lowK * 64)
+  uint32_t* lowCol;
+  uint32_t* lowBlk;
   uint32_t* error;      // nonzero: a row exceeded kMaxIncidence or kMaxRow
 };
 
@@ -136,8 +148,8 @@ void launchWarmCheck(const View& v, hipStream_t st);
 void launchFill(uint32_t* p, uint32_t value, uint64_t count, hipStream_t st);
 
 // Debug / standalone paths.
-// y = A x (pinned rows and columns excluded); maxK = the largest row length (<= 16: the one-batch kernel; the
-// value buffer must then carry kSellSlack zero floats past its end).
+// y = A x (pinned rows and columns excluded); maxK = the most upper slots of a slice (a general matrix, lowK null,
+// with maxK <= 16 takes the one-batch kernel; the value buffer must then carry kSellSlack zero floats past its end).
 void launchSpmvPlain(const View& v, uint32_t maxK, const F3* in, F3* out, hipStream_t st);
 void launchSellToSlots(const View& v, float* out /* 9 per slot */, hipStream_t st);
 void launchEvalTerms(int kind, uint32_t n, const float* x, const float* params, const Fabric32* fabrics,
@@ -146,7 +158,18 @@ void launchFieldQuery(const float* values, Field32 field, uint32_t n, const floa
                       hipStream_t st);
 
 // Structure builder (all on the device).
-void launchBuildStructure(const BuildView& b, uint32_t& nBlocks, uint64_t& sellFloats, uint32_t& maxK,
-                          uint32_t*& colIndexOut, uint32_t*& sellColOut, hipStream_t st);
+// Device arrays it allocates are handed to the caller, which frees them.
+struct BuiltStructure {
+  uint32_t nBlocks = 0;
+  uint32_t maxK = 0;          // upper slots of the largest slice
+  uint32_t maxLowK = 0;       // lower slots of the largest slice
+  uint64_t sellFloats = 0;    // upper block values (the value buffer adds kSellSlack zero floats)
+  uint64_t lowEntries = 0;    // lower slots times 64
+  uint32_t* colIndex = nullptr;  // nBlocks
+  uint32_t* sellCol = nullptr;   // sellFloats / 9 + kSellColSlack
+  uint32_t* lowCol = nullptr;    // lowEntries + kSellColSlack
+  uint32_t* lowBlk = nullptr;    // lowEntries + kSellColSlack
+};
+void launchBuildStructure(const BuildView& b, BuiltStructure& out, hipStream_t st);
 
 }  // namespace dk

@@ -276,6 +276,8 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   seamSlots_.allocate(sizeof(uint32_t) * 4 * std::max<uint32_t>(K, 1));
   sellK_.allocate(sizeof(uint32_t) * nSlices_);
   sellBase_.allocate(sizeof(uint32_t) * (nSlices_ + 1));
+  lowK_.allocate(sizeof(uint32_t) * nSlices_);
+  lowBase_.allocate(sizeof(uint32_t) * (nSlices_ + 1));
   buildError_.allocate(sizeof(uint32_t));
   dk::BuildView b{};
   b.nRows = nRows_;
@@ -298,20 +300,22 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   b.seamSlots = seamSlots_.as<uint32_t>();
   b.sellK = sellK_.as<uint32_t>();
   b.sellSize = sellBase_.as<uint32_t>();
+  b.lowK = lowK_.as<uint32_t>();
+  b.lowSize = lowBase_.as<uint32_t>();
   b.error = buildError_.as<uint32_t>();
-  uint32_t* colIndex = nullptr;
-  uint32_t* sellCol = nullptr;
-  uint32_t nBlocks = 0, maxK = 0;
-  uint64_t sellFloats = 0;
-  dk::launchBuildStructure(b, nBlocks, sellFloats, maxK, colIndex, sellCol, stream_);
-  colIndex_.adopt(colIndex, sizeof(uint32_t) * std::max<uint32_t>(nBlocks, 1));
-  sellCol_.adopt(sellCol, sizeof(uint32_t) * (sellFloats / 9 + dk::kSellColSlack));
-  nBlocks_ = nBlocks;
-  sellFloats_ = sellFloats;
-  maxK_ = maxK;
+  dk::BuiltStructure bs;
+  dk::launchBuildStructure(b, bs, stream_);
+  colIndex_.adopt(bs.colIndex, sizeof(uint32_t) * std::max<uint32_t>(bs.nBlocks, 1));
+  sellCol_.adopt(bs.sellCol, sizeof(uint32_t) * (bs.sellFloats / 9 + dk::kSellColSlack));
+  lowCol_.adopt(bs.lowCol, sizeof(uint32_t) * (bs.lowEntries + dk::kSellColSlack));
+  lowBlk_.adopt(bs.lowBlk, sizeof(uint32_t) * (bs.lowEntries + dk::kSellColSlack));
+  nBlocks_ = bs.nBlocks;
+  sellFloats_ = bs.sellFloats;
+  lowEntries_ = bs.lowEntries;
+  maxK_ = bs.maxK;
   if (maxK_ * 9ull * dk::kSlice * sizeof(float) > 64ull * 1024)
-    throw std::invalid_argument("hip: a 64-row slice has more than 28 blocks in one row");
-  sellVal_.allocate(sizeof(float) * (sellFloats + dk::kSellSlack));  // zero slack: see kSellSlack
+    throw std::invalid_argument("hip: a 64-row slice has more than 28 blocks on or above the diagonal in one row");
+  sellVal_.allocate(sizeof(float) * (sellFloats_ + dk::kSellSlack));  // zero slack: see kSellSlack
 
   dk::View& v = view_;
   v = dk::View{};
@@ -322,6 +326,7 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   v.nStretch = S;
   v.nHinge = H;
   v.nSeam = K;
+  v.zeroBlock = static_cast<uint32_t>(sellFloats_ / 9);
   v.rowFlags = rowFlags_.as<uint32_t>();
   v.mass = mass_.as<float>();
   v.rowFabric = rowFabric_.as<uint32_t>();
@@ -382,6 +387,10 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   v.sellBase = sellBase_.as<uint32_t>();
   v.sellCol = sellCol_.as<uint32_t>();
   v.sellVal = sellVal_.as<float>();
+  v.lowK = lowK_.as<uint32_t>();
+  v.lowBase = lowBase_.as<uint32_t>();
+  v.lowCol = lowCol_.as<uint32_t>();
+  v.lowBlk = lowBlk_.as<uint32_t>();
   v.fieldValues = fieldValues_.as<float>();
   built_ = true;
   syncStateToDevice();
@@ -519,13 +528,15 @@ void HipSolver::spmvTiming(uint64_t& launches, double& meanUs, double& bytesPerL
   meanUs = spmvTimed_ ? 1000.0 * spmvTotalMs_ / double(spmvTimed_) : 0.0;
   const double n = double(model_.x.size());
   const double blocks = double(nBlocks_) - (double(nRows_) - n);
-  // Algorithmic bytes of one PCG matrix launch: block values and column index (40 B per block), and per particle
+  // Algorithmic bytes of one PCG matrix launch. The matrix is stored symmetric: every block on or above the
+  // diagonal once with its column (40 B; (blocks + n) / 2 of them), every slot below it as a column and the index
+  // of its transposed block (8 B; (blocks - n) / 2). Per particle
   // - two-kernel form (kPcgSpmv): row and contact flags (8 B), z and p_old gathered once (24 B), p_new and ap
   //   written (24 B): 56 B;
   // - one-kernel form (kPcgStep): zr, t, p gathered once (36 B), own r and ap (24 B), step (12 B), the
   //   preconditioner (36 B), contact flags and normal (20 B), row flags (4 B) read; r, ap, p, zr, t and step
   //   written (72 B): 204 B.
-  bytesPerLaunch = 40.0 * blocks + (pcgVariant_ == 1 ? 204.0 : 56.0) * n;
+  bytesPerLaunch = 40.0 * 0.5 * (blocks + n) + 8.0 * 0.5 * (blocks - n) + (pcgVariant_ == 1 ? 204.0 : 56.0) * n;
 }
 
 bool HipSolver::allSegments(bool (*pred)(const dk::SegState&)) const {

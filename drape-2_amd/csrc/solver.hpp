@@ -95,7 +95,7 @@ class HipSolver {
   bool useGraph_ = true;
   double lastH_ = -1.0;
   uint32_t nRows_ = 0, nTiles_ = 0, nSlices_ = 0, nBlocks_ = 0, maxK_ = 0;
-  uint64_t sellFloats_ = 0;
+  uint64_t sellFloats_ = 0, lowEntries_ = 0;
   uint32_t lastPcgIterations_ = 0;
   std::vector<uint32_t> rowOf_;  // global particle -> internal row
   dk::View view_{};
@@ -112,6 +112,7 @@ class HipSolver {
   DeviceBuffer stretch_, hinges_, seams_, fabrics_, garments_;
   DeviceBuffer incCount_, incStart_, inc_, rowLen_, rowStart_, colIndex_, diagSlot_;
   DeviceBuffer stretchSlots_, hingeSlots_, seamSlots_, sellK_, sellBase_, sellCol_, sellVal_, buildError_;
+  DeviceBuffer lowK_, lowBase_, lowCol_, lowBlk_;
   DeviceBuffer fieldValues_, scratchF3_, scratchF3b_;
   DeviceBuffer cgBuf_[10], cgPart0_, cgPart1_;
   int pcgVariant_ = 1;
