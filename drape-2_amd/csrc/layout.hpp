@@ -7,14 +7,19 @@
 // (libs/sim/include/drape/sim/solver.hpp:84); only padding rows follow the last particle.
 //
 // The Hessian is block-CSR over 3x3 blocks with the reference's pattern and slot numbering
-// (libs/sim/src/sim_model.cpp:41-74): rowStart/colIndex/diagSlot. The block values live in a sliced-ELL layout
-// (SELL-64) so that one lane per row streams them coalesced: slice t holds K_t = max row length of its 64 rows;
-// value (slot k of lane l, component c) is at sellBase[t] + (k * 64 + l) * 9 + c — a lane's block is 9
-// consecutive floats and the 64 lanes' blocks of one slot are consecutive, so a wavefront reads 2,304
-// contiguous bytes per slot — and the column index at sellBase[t] / 9 + k * 64 + l. Components are row-major
-// (c = 3 * row + col). Padding entries carry the
-// dummy column `nRows` (a zero entry of every gathered vector) and zero values; so do pinned columns, which the
-// product excludes (cpu_solver.cpp:147-159).
+// (libs/sim/src/sim_model.cpp:41-74): rowStart/colIndex/diagSlot. It is symmetric, and the device stores each
+// off-diagonal block once, in a sliced-ELL layout (SELL-64) that one lane per row streams coalesced:
+//  - upper part: a row's diagonal block and the blocks of the columns above it (upper slot 0 is the diagonal).
+//    Slice t holds K_t = the most such blocks of its 64 rows; value (slot k of lane l, component c) is at
+//    sellBase[t] + (k * 64 + l) * 9 + c — a lane's block is 9 consecutive floats and the 64 lanes' blocks of one
+//    slot are consecutive, so a wavefront reads 2,304 contiguous bytes per slot — and the column index at
+//    sellBase[t] / 9 + k * 64 + l. Components are row-major (c = 3 * row + col).
+//  - lower part: for each column below the diagonal, the column (lowCol) and the block index (lowBlk, in units
+//    of 9 floats) of block (column, row) in that column's upper part, whose transpose is block (row, column);
+//    slice t holds KL_t slots at lowBase[t] + k * 64 + l.
+// Padding entries carry the dummy column `nRows` (a zero entry of every gathered vector), zero values and, in the
+// lower part, the index of a zero block after the last slice; pinned columns also carry the dummy column, which
+// excludes them from the product (cpu_solver.cpp:147-159).
 #pragma once
 
 #include <cstdint>
@@ -31,11 +36,11 @@ constexpr uint32_t kTile = 256;   // rows per tile = threads per workgroup
 constexpr uint32_t kSlicesPerTile = kTile / kSlice;
 constexpr uint32_t kMaxIncidence = 96;  // term incidences per particle the structure builder accepts
 constexpr uint32_t kMaxRow = 64;        // blocks per row the structure builder accepts
-// Zero floats after the last slice's block values: a kernel may read a whole batch of 16 slots of any slice
-// without testing each slot against K.
+// Zero floats after the last slice's block values (the first zero block is the lower part's padding block): a
+// kernel may read a whole batch of up to 16 slots of any slice without testing each slot against K.
 constexpr uint32_t kSellSlack = 16u * 9u * 64u;
-// Column entries after the last slice's columns: a kernel may load the column indices of 16 slots past any slice's
-// K (and discards them).
+// Entries after the last slice's upper columns, lower columns and lower block indices: a kernel may load the
+// indices of 16 slots past any slice's K (and discards them).
 constexpr uint32_t kSellColSlack = 16u * 64u;
 
 // Row flags.
