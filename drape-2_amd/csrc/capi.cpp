@@ -462,12 +462,12 @@ int drape_bsr_spmv(uint32_t n, uint32_t n_blocks, const uint32_t* row_start, con
         dVal(4 * sellVal.size()), dx(sizeof(dk::F3) * (nRows + 1)), dy(sizeof(dk::F3) * (nRows + 1));
     std::vector<dk::F3> xs(nRows + 1, dk::F3{0, 0, 0});
     for (uint32_t i = 0; i < n; ++i) xs[i] = dk::F3{x[3 * i], x[3 * i + 1], x[3 * i + 2]};
-    check(hipMemcpy(dFlags.p, flags.data(), 4 * nRows, hipMemcpyHostToDevice), "copy");
-    check(hipMemcpy(dK.p, sellK.data(), 4 * nSlices, hipMemcpyHostToDevice), "copy");
-    check(hipMemcpy(dBase.p, sellBase.data(), 4 * (nSlices + 1), hipMemcpyHostToDevice), "copy");
-    check(hipMemcpy(dCol.p, sellCol.data(), 4 * sellCol.size(), hipMemcpyHostToDevice), "copy");
-    check(hipMemcpy(dVal.p, sellVal.data(), 4 * sellVal.size(), hipMemcpyHostToDevice), "copy");
-    check(hipMemcpy(dx.p, xs.data(), sizeof(dk::F3) * (nRows + 1), hipMemcpyHostToDevice), "copy");
+    check(hipMemcpyAsync(dFlags.p, flags.data(), 4 * nRows, hipMemcpyHostToDevice, st.s), "copy");
+    check(hipMemcpyAsync(dK.p, sellK.data(), 4 * nSlices, hipMemcpyHostToDevice, st.s), "copy");
+    check(hipMemcpyAsync(dBase.p, sellBase.data(), 4 * (nSlices + 1), hipMemcpyHostToDevice, st.s), "copy");
+    check(hipMemcpyAsync(dCol.p, sellCol.data(), 4 * sellCol.size(), hipMemcpyHostToDevice, st.s), "copy");
+    check(hipMemcpyAsync(dVal.p, sellVal.data(), 4 * sellVal.size(), hipMemcpyHostToDevice, st.s), "copy");
+    check(hipMemcpyAsync(dx.p, xs.data(), sizeof(dk::F3) * (nRows + 1), hipMemcpyHostToDevice, st.s), "copy");
     dk::View v{};
     v.nRows = nRows;
     v.nTiles = nTiles;
@@ -518,12 +518,12 @@ int drape_eval_terms(int32_t kind, uint32_t n, const float* x, const float* para
     Stream st;
     Dev dx(sizeof(float) * 3 * N * size_t(n)), dp(sizeof(float) * np * size_t(n)), df(sizeof(dk::Fabric32) * size_t(n)),
         de(sizeof(float) * n), dg(sizeof(float) * 3 * N * size_t(n)), dh(sizeof(float) * 9 * N * N * size_t(n));
-    check(hipMemcpy(dx.p, x, sizeof(float) * 3 * N * size_t(n), hipMemcpyHostToDevice), "copy");
-    check(hipMemcpy(dp.p, params, sizeof(float) * np * size_t(n), hipMemcpyHostToDevice), "copy");
+    check(hipMemcpyAsync(dx.p, x, sizeof(float) * 3 * N * size_t(n), hipMemcpyHostToDevice, st.s), "copy");
+    check(hipMemcpyAsync(dp.p, params, sizeof(float) * np * size_t(n), hipMemcpyHostToDevice, st.s), "copy");
     if (kind == 0) {
       std::vector<dk::Fabric32> f(n);
       for (uint32_t i = 0; i < n; ++i) f[i] = fabric32(fabrics[i]);
-      check(hipMemcpy(df.p, f.data(), sizeof(dk::Fabric32) * n, hipMemcpyHostToDevice), "copy");
+      check(hipMemcpyAsync(df.p, f.data(), sizeof(dk::Fabric32) * n, hipMemcpyHostToDevice, st.s), "copy");
     }
     dk::launchEvalTerms(kind, n, dx.as<float>(), dp.as<float>(), df.as<dk::Fabric32>(), limit_scale, de.as<float>(),
                         dg.as<float>(), dh.as<float>(), st.s);
@@ -561,8 +561,8 @@ int drape_field_query(const drape_field_desc* field, uint32_t n, const float* p,
     Stream st;
     Dev dv(sizeof(float) * std::max<size_t>(count, 1)), dp(sizeof(float) * 3 * size_t(n)), dphi(sizeof(float) * n),
         dn(sizeof(float) * 3 * size_t(n));
-    if (count) check(hipMemcpy(dv.p, field->values, sizeof(float) * count, hipMemcpyHostToDevice), "copy");
-    check(hipMemcpy(dp.p, p, sizeof(float) * 3 * size_t(n), hipMemcpyHostToDevice), "copy");
+    if (count) check(hipMemcpyAsync(dv.p, field->values, sizeof(float) * count, hipMemcpyHostToDevice, st.s), "copy");
+    check(hipMemcpyAsync(dp.p, p, sizeof(float) * 3 * size_t(n), hipMemcpyHostToDevice, st.s), "copy");
     dk::launchFieldQuery(dv.as<float>(), f, n, dp.as<float>(), dphi.as<float>(), dn.as<float>(), st.s);
     check(hipStreamSynchronize(st.s), "query");
     check(hipMemcpy(phi, dphi.p, sizeof(float) * n, hipMemcpyDeviceToHost), "copy");

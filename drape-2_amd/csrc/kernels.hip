@@ -601,13 +601,37 @@ constexpr int kPcgParts = 7;  // per tile: p.ap, r.zr, r.t, ap.t, r.ap, ap.ap, r
 enum : int { kPpAp = 0, kPrZ, kPrT, kPapT, kPrAp, kPapAp, kPrR };
 }  // namespace
 
+// A double moved between lanes by a DPP pattern (both halves by v_mov_b32_dpp); lanes the pattern leaves without
+// a source (rows outside RowMask) read +0.
+template <int Ctrl, int RowMask = 0xf>
+__device__ inline double dppMove(double v) {
+  const uint64_t u = uint64_t(__double_as_longlong(v));
+  const uint32_t lo = uint32_t(__builtin_amdgcn_update_dpp(0, int(uint32_t(u)), Ctrl, RowMask, 0xf, false));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_update_dpp(0, int(uint32_t(u >> 32)), Ctrl, RowMask, 0xf, false));
+  return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+}
+
+// Sum over the 64 lanes of a wavefront in a fixed order, returned to every lane, without LDS: pairs and quads
+// (quad_perm), half rows and rows of 16 (mirrors), then the rows (row_bcast15 into rows 1 and 3, row_bcast31 into
+// rows 2 and 3); lane 63 then holds the total.
+__device__ inline double waveTotal(double v) {
+  v += dppMove<0xB1>(v);        // quad_perm [1,0,3,2]
+  v += dppMove<0x4E>(v);        // quad_perm [2,3,0,1]
+  v += dppMove<0x141>(v);       // row_half_mirror
+  v += dppMove<0x140>(v);       // row_mirror
+  v += dppMove<0x142, 0xA>(v);  // row_bcast15
+  v += dppMove<0x143, 0xC>(v);  // row_bcast31
+  const uint64_t u = uint64_t(__double_as_longlong(v));
+  const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u)), 63));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u >> 32)), 63));
+  return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+}
+
 // N sums over a workgroup of NW wavefronts (fixed order), returned to every thread. lds >= N * NW doubles.
 template <int N, int NW>
 __device__ inline void blockSumN(double (&x)[N], double* lds) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-    for (int i = 0; i < N; ++i) x[i] += __shfl_down(x[i], off, 64);
+  for (int i = 0; i < N; ++i) x[i] = waveTotal(x[i]);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   __syncthreads();
   if (l == 0)

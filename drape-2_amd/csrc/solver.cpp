@@ -50,11 +50,14 @@ constexpr double kContactSnap = 0.0005;  // sim_model.hpp:15
 
 }  // namespace
 
+// Work on the null stream is not ordered with a solver's stream (created non-blocking): the fill must be complete
+// before this returns, or a kernel on that stream may write the buffer first.
 void DeviceBuffer::allocate(size_t bytes) {
   release();
   const size_t b = std::max<size_t>(bytes, 16);
   check(hipMalloc(&ptr_, b), "hipMalloc");
-  check(hipMemset(ptr_, 0, b), "hipMemset");
+  check(hipMemsetAsync(ptr_, 0, b, nullptr), "hipMemset");
+  check(hipStreamSynchronize(nullptr), "hipMemset");
   bytes_ = b;
 }
 
@@ -94,11 +97,23 @@ void HipSolver::requireBuilt() const {
   if (!built_) throw std::logic_error("hip: solver not built");
 }
 
+// Copies on the solver's stream, complete when they return (the stream is non-blocking, so copies on the null
+// stream would not be ordered with its kernels).
+void HipSolver::toDevice(void* dst, const void* src, size_t bytes) const {
+  check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_), "copy");
+  check(hipStreamSynchronize(stream_), "copy");
+}
+
+void HipSolver::fromDevice(void* dst, const void* src, size_t bytes) const {
+  check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_), "copy");
+  check(hipStreamSynchronize(stream_), "copy");
+}
+
 template <class T>
-static void upload(DeviceBuffer& b, const std::vector<T>& data, size_t count = 0) {
+void HipSolver::upload(DeviceBuffer& b, const std::vector<T>& data, size_t count) {
   const size_t n = std::max(count, data.size());
   b.allocate(sizeof(T) * std::max<size_t>(n, 1));
-  if (!data.empty()) check(hipMemcpy(b.as<void>(), data.data(), sizeof(T) * data.size(), hipMemcpyHostToDevice), "upload");
+  if (!data.empty()) toDevice(b.as<void>(), data.data(), sizeof(T) * data.size());
 }
 
 void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
@@ -430,15 +445,15 @@ void HipSolver::uploadFabricsAndMass() {
   std::vector<dk::Fabric32> f(model_.fabrics.size());
   for (size_t k = 0; k < f.size(); ++k) f[k] = fabric32(model_.fabrics[k]);
   if (!f.empty())
-    check(hipMemcpy(fabrics_.as<void>(), f.data(), sizeof(dk::Fabric32) * f.size(), hipMemcpyHostToDevice), "copy");
+    toDevice(fabrics_.as<void>(), f.data(), sizeof(dk::Fabric32) * f.size());
   std::vector<float> mass(nRows_, 1.0f);
   std::vector<double> massF64(nRows_, 0.0);
   for (uint32_t i = 0; i < model_.x.size(); ++i) {
     mass[rowOf_[i]] = static_cast<float>(model_.mass[i]);
     massF64[rowOf_[i]] = model_.mass[i];
   }
-  check(hipMemcpy(mass_.as<void>(), mass.data(), sizeof(float) * nRows_, hipMemcpyHostToDevice), "copy");
-  check(hipMemcpy(massF64_.as<void>(), massF64.data(), sizeof(double) * nRows_, hipMemcpyHostToDevice), "copy");
+  toDevice(mass_.as<void>(), mass.data(), sizeof(float) * nRows_);
+  toDevice(massF64_.as<void>(), massF64.data(), sizeof(double) * nRows_);
 }
 
 void HipSolver::uploadHinges() {
@@ -452,7 +467,7 @@ void HipSolver::uploadHinges() {
     hg[k] = g;
   }
   if (!hg.empty())
-    check(hipMemcpy(hinges_.as<void>(), hg.data(), sizeof(dk::Hinge32) * hg.size(), hipMemcpyHostToDevice), "copy");
+    toDevice(hinges_.as<void>(), hg.data(), sizeof(dk::Hinge32) * hg.size());
 }
 
 void HipSolver::uploadSeams() {
@@ -461,7 +476,7 @@ void HipSolver::uploadSeams() {
     sm[k] = dk::Seam32{rowOf_[model_.seams[k].a], rowOf_[model_.seams[k].b], model_.seams[k].garment,
                        model_.seams[k].startGap};
   if (!sm.empty())
-    check(hipMemcpy(seams_.as<void>(), sm.data(), sizeof(dk::Seam32) * sm.size(), hipMemcpyHostToDevice), "copy");
+    toDevice(seams_.as<void>(), sm.data(), sizeof(dk::Seam32) * sm.size());
 }
 
 void HipSolver::syncStateFromDevice() const {
@@ -902,7 +917,8 @@ void HipSolver::snapshotRestore(const void* in, size_t bytes) {
   uploadGarments();
   uploadSeams();
   // The realised dv is not part of the reference's snapshot: the learned warm start restarts from dv = 0.
-  check(hipMemset(dvLast_.as<void>(), 0, dvLast_.bytes()), "memset");
+  check(hipMemsetAsync(dvLast_.as<void>(), 0, dvLast_.bytes(), stream_), "memset");
+  check(hipStreamSynchronize(stream_), "sync");
 }
 
 void HipSolver::updateFabric(const std::string& garment, const std::vector<Fabric>& fabrics) {
@@ -998,8 +1014,8 @@ drape_structure_sizes HipSolver::structureSizes() const {
 void HipSolver::globalStructure(std::vector<uint32_t>& rowStart, std::vector<uint32_t>& colIndex,
                                 std::vector<uint32_t>& diagSlot, std::vector<uint32_t>& internalToGlobalSlot) const {
   std::vector<uint32_t> rs(nRows_ + 1), ci(nBlocks_);
-  check(hipMemcpy(rs.data(), rowStart_.as<void>(), sizeof(uint32_t) * (nRows_ + 1), hipMemcpyDeviceToHost), "copy");
-  check(hipMemcpy(ci.data(), colIndex_.as<void>(), sizeof(uint32_t) * nBlocks_, hipMemcpyDeviceToHost), "copy");
+  fromDevice(rs.data(), rowStart_.as<void>(), sizeof(uint32_t) * (nRows_ + 1));
+  fromDevice(ci.data(), colIndex_.as<void>(), sizeof(uint32_t) * nBlocks_);
   const size_t n = model_.x.size();
   std::vector<uint32_t> particleOfRow(nRows_, UINT32_MAX);
   for (uint32_t i = 0; i < n; ++i) particleOfRow[rowOf_[i]] = i;
@@ -1031,7 +1047,7 @@ void HipSolver::readStructure(uint32_t* rowStart, uint32_t* colIndex, uint32_t* 
   auto slots = [&](const DeviceBuffer& b, size_t count, uint32_t* out) {
     if (!out || count == 0) return;
     std::vector<uint32_t> s(count);
-    check(hipMemcpy(s.data(), b.as<void>(), sizeof(uint32_t) * count, hipMemcpyDeviceToHost), "copy");
+    fromDevice(s.data(), b.as<void>(), sizeof(uint32_t) * count);
     for (size_t k = 0; k < count; ++k) out[k] = map[s[k]];
   };
   slots(stretchSlots_, 9 * model_.stretch.size(), stretchSlots);
@@ -1061,7 +1077,7 @@ void HipSolver::debugAssemble(double h, const float* u, double* energy, float* g
   uploadSegParams(h);
   // Keep the contact force the next step's friction reads.
   std::vector<float> savedForce(nRows_);
-  check(hipMemcpy(savedForce.data(), cforce_.as<void>(), sizeof(float) * nRows_, hipMemcpyDeviceToHost), "copy");
+  fromDevice(savedForce.data(), cforce_.as<void>(), sizeof(float) * nRows_);
   dk::launchPredictContact(view_, stream_);
   std::vector<dk::F3> us(nRows_ + 1, dk::F3{0, 0, 0});
   for (size_t i = 0; i < model_.x.size(); ++i) us[rowOf_[i]] = dk::F3{u[3 * i], u[3 * i + 1], u[3 * i + 2]};
@@ -1075,7 +1091,7 @@ void HipSolver::debugAssemble(double h, const float* u, double* energy, float* g
   for (size_t s = 1; s < model_.segments.size() && energy; ++s) *energy += hostStates_[s].e0;
   if (grad) {
     std::vector<dk::F3> g(nRows_ + 1);
-    check(hipMemcpy(g.data(), grad_.as<void>(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyDeviceToHost), "copy");
+    fromDevice(g.data(), grad_.as<void>(), sizeof(dk::F3) * (nRows_ + 1));
     for (size_t i = 0; i < model_.x.size(); ++i) {
       grad[3 * i] = g[rowOf_[i]].x;
       grad[3 * i + 1] = g[rowOf_[i]].y;
@@ -1095,7 +1111,7 @@ void HipSolver::debugAssemble(double h, const float* u, double* energy, float* g
     for (uint32_t s = 0; s < nBlocks_; ++s)
       if (map[s] != UINT32_MAX) std::copy(&internal[9 * size_t(s)], &internal[9 * size_t(s)] + 9, blocks + 9 * size_t(map[s]));
   }
-  check(hipMemcpy(cforce_.as<void>(), savedForce.data(), sizeof(float) * nRows_, hipMemcpyHostToDevice), "copy");
+  toDevice(cforce_.as<void>(), savedForce.data(), sizeof(float) * nRows_);
 }
 
 void HipSolver::debugSystem(float* blocks, float* grad, float* precond, uint32_t* contactFlags,
@@ -1118,7 +1134,7 @@ void HipSolver::debugSystem(float* blocks, float* grad, float* precond, uint32_t
   }
   if (grad) {
     std::vector<dk::F3> g(nRows_ + 1);
-    check(hipMemcpy(g.data(), grad_.as<void>(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyDeviceToHost), "copy");
+    fromDevice(g.data(), grad_.as<void>(), sizeof(dk::F3) * (nRows_ + 1));
     for (size_t i = 0; i < n; ++i) {
       grad[3 * i] = g[rowOf_[i]].x;
       grad[3 * i + 1] = g[rowOf_[i]].y;
@@ -1127,17 +1143,17 @@ void HipSolver::debugSystem(float* blocks, float* grad, float* precond, uint32_t
   }
   if (precond) {
     std::vector<float> m(9 * size_t(nRows_ + 1));
-    check(hipMemcpy(m.data(), precond_.as<void>(), sizeof(float) * m.size(), hipMemcpyDeviceToHost), "copy");
+    fromDevice(m.data(), precond_.as<void>(), sizeof(float) * m.size());
     for (size_t i = 0; i < n; ++i) std::copy(&m[9 * size_t(rowOf_[i])], &m[9 * size_t(rowOf_[i])] + 9, precond + 9 * i);
   }
   if (contactFlags) {
     std::vector<uint32_t> f(nRows_ + 1);
-    check(hipMemcpy(f.data(), cflags_.as<void>(), sizeof(uint32_t) * (nRows_ + 1), hipMemcpyDeviceToHost), "copy");
+    fromDevice(f.data(), cflags_.as<void>(), sizeof(uint32_t) * (nRows_ + 1));
     for (size_t i = 0; i < n; ++i) contactFlags[i] = f[rowOf_[i]];
   }
   if (contactNormals) {
     std::vector<float> c(4 * size_t(nRows_ + 1));
-    check(hipMemcpy(c.data(), cnormal_.as<void>(), sizeof(float) * c.size(), hipMemcpyDeviceToHost), "copy");
+    fromDevice(c.data(), cnormal_.as<void>(), sizeof(float) * c.size());
     for (size_t i = 0; i < n; ++i) std::copy(&c[4 * size_t(rowOf_[i])], &c[4 * size_t(rowOf_[i])] + 4, contactNormals + 4 * i);
   }
 }
@@ -1146,7 +1162,10 @@ void HipSolver::setWarmStart(ws::Net* net) {
   requireBuilt();
   check(hipStreamSynchronize(stream_), "sync");
   net_ = net;
-  if (!net_) check(hipMemset(warm_.as<void>(), 0, warm_.bytes()), "memset");
+  if (!net_) {
+    check(hipMemsetAsync(warm_.as<void>(), 0, warm_.bytes(), stream_), "memset");
+    check(hipStreamSynchronize(stream_), "sync");
+  }
 }
 
 void HipSolver::frameInputs(float* x, float* v, float* dvPrev, float* phiNormal) {
@@ -1157,7 +1176,7 @@ void HipSolver::frameInputs(float* x, float* v, float* dvPrev, float* phiNormal)
   auto copyF3 = [&](const DeviceBuffer& b, float* out) {
     if (!out) return;
     std::vector<dk::F3> h(nRows_ + 1);
-    check(hipMemcpy(h.data(), b.as<void>(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyDeviceToHost), "copy");
+    fromDevice(h.data(), b.as<void>(), sizeof(dk::F3) * (nRows_ + 1));
     for (size_t i = 0; i < n; ++i) {
       const dk::F3 q = h[rowOf_[i]];
       out[3 * i] = q.x;
@@ -1170,7 +1189,7 @@ void HipSolver::frameInputs(float* x, float* v, float* dvPrev, float* phiNormal)
   copyF3(dvLast_, dvPrev);
   if (phiNormal) {
     std::vector<float4> h(nRows_ + 1);
-    check(hipMemcpy(h.data(), phiN_.as<void>(), sizeof(float4) * (nRows_ + 1), hipMemcpyDeviceToHost), "copy");
+    fromDevice(h.data(), phiN_.as<void>(), sizeof(float4) * (nRows_ + 1));
     for (size_t i = 0; i < n; ++i) {
       const float4 q = h[rowOf_[i]];
       phiNormal[4 * i] = q.x;

@@ -77,6 +77,10 @@ class HipSolver {
   void syncStateToDevice();           // host model -> x, v, contact force
   void pollStates() const;            // device SegState -> hostStates_
   std::vector<double> segmentPartials() const;  // launchStats -> host, kStatsPerSegment per segment
+  void toDevice(void* dst, const void* src, size_t bytes) const;
+  void fromDevice(void* dst, const void* src, size_t bytes) const;
+  template <class T>
+  void upload(DeviceBuffer& b, const std::vector<T>& data, size_t count = 0);
   bool allSegments(bool (*pred)(const dk::SegState&)) const;
   void substep(double h, int iterations);
   void runPcg();

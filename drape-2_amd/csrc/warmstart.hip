@@ -445,6 +445,7 @@ Graph::Graph(uint32_t n, const uint32_t* rowStart, const uint32_t* nbr, const do
   if (ne) check(hipMemcpy(nbr_.as<void>(), nbr, sizeof(uint32_t) * ne, hipMemcpyHostToDevice), "copy");
   check(hipMemcpy(rest2_.as<void>(), r2.data(), sizeof(float) * r2.size(), hipMemcpyHostToDevice), "copy");
   check(hipMemcpy(mask_.as<void>(), mask.data(), sizeof(uint32_t) * mask.size(), hipMemcpyHostToDevice), "copy");
+  check(hipStreamSynchronize(nullptr), "sync");  // null-stream work done before any other stream reads it
 }
 
 GraphView Graph::view() const {
@@ -478,6 +479,7 @@ Net::Net(uint64_t seed) {
   adamV_.allocate(sizeof(float) * n);
   wbf_.allocate(sizeof(uint16_t) * kPackLayout.total);
   check(hipMemcpy(params_.as<void>(), p.data(), sizeof(float) * n, hipMemcpyHostToDevice), "copy");
+  check(hipStreamSynchronize(nullptr), "sync");  // the copy is on the null stream, the packing on its own
   packWeights(packStream_);
   check(hipStreamSynchronize(packStream_), "sync");
 }
@@ -495,6 +497,7 @@ void Net::getParams(float* out) const {
 
 void Net::setParams(const float* in) {
   check(hipMemcpy(params_.as<void>(), in, sizeof(float) * layout_.total, hipMemcpyHostToDevice), "copy");
+  check(hipStreamSynchronize(nullptr), "sync");  // null-stream work done before any other stream reads it
   packWeights(packStream_);
   check(hipStreamSynchronize(packStream_), "sync");
 }
@@ -502,6 +505,7 @@ void Net::setParams(const float* in) {
 void Net::resetOptimizer() {
   check(hipMemset(adamM_.as<void>(), 0, sizeof(float) * layout_.total), "memset");
   check(hipMemset(adamV_.as<void>(), 0, sizeof(float) * layout_.total), "memset");
+  check(hipStreamSynchronize(nullptr), "sync");  // null-stream work done before any other stream reads it
   adamStep_ = 0;
 }
 
