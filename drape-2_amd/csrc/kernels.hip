@@ -668,13 +668,14 @@ __global__ void __launch_bounds__(256) kPcgStepInit(View v) {
   if (tile >= v.nTiles) return;
   const uint32_t s = v.tileSeg[tile];
   SegState* S = v.state + s;
-  if (S->newtonDone) return;
+  const uint32_t done = S->newtonDone;
+  if (threadIdx.x == 0) v.tileDone[0][tile] = done;  // read by the first kPcgStep of this solve
+  if (done) return;
   const SegParams& P = v.seg[s];
   if (tile == P.tileBegin && threadIdx.x == 0) {
     S->pcgConverged = 0u;
     S->pcgIter = 0u;
     S->kIter[0] = 0u;
-    S->convPar[0] = 0u;
   }
   const uint32_t row = tile * kTile + threadIdx.x;
   const F3 r = filterContact(v.cflags[row], v.cnormal[row], -v.grad[row]);
@@ -715,16 +716,22 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
   const uint32_t KL = ldUniform(v.lowK + slice);
   const uint32_t lbase = ldUniform(v.lowBase + slice);
   const uint32_t s = ldUniform(v.tileSeg + tile);
+  // A tile whose solve has ended (the segment's PCG converged, or its Newton loop finished) passes the flag on and
+  // loads nothing. The flag is per tile, so it arrives with the slice headers: no extra round trip for the tiles
+  // that iterate, and none of the slice's loads for a tile of a batch whose segment converged early.
+  if (ldUniform(v.tileDone[q] + tile)) {
+    if (threadIdx.x == 0) v.tileDone[o][tile] = 1u;
+    return;
+  }
   const F3* __restrict__ Zr = v.cgZr[o];
   const F3* __restrict__ T = v.cgT[o];
   const F3* __restrict__ Pp = v.cgP[o];
 
-  // Round trip 1, issued before the segment's flags are known (everything here depends only on the slice and the
-  // parity): the first batch's columns (lower and upper), lower block indices and upper block values, then the
-  // owner row's data. A launch whose solve has already ended drops them unused; one that iterates has them in
-  // flight while the flags arrive. Index loads first (unconditional, see sellColumn), block loads after, and
-  // nothing that uses an index before both are issued: the wait before the gathers is then one for the indices
-  // only (the memory counter retires in issue order), not for the 36 B per slot of block values still in flight.
+  // Round trip 1, issued before the segment's scalars are known (everything here depends only on the slice and
+  // the parity): the first batch's columns (lower and upper), lower block indices and upper block values, then the
+  // owner row's data. Index loads first (unconditional, see sellColumn), block loads after, and nothing that uses
+  // an index before both are issued: the wait before the gathers is then one for the indices only (the memory
+  // counter retires in issue order), not for the 36 B per slot of block values still in flight.
   const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
   const float* __restrict__ val = v.sellVal + base + 9u * lane;
   const uint32_t* __restrict__ lcol = v.lowCol + lbase + lane;
@@ -768,22 +775,13 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
   __builtin_amdgcn_sched_barrier(0);
 
   // Per-segment scalars through the scalar cache: none of them is written by this kernel (it writes only the
-  // other parity slot of kIter / convPar, and r0 when k == 0, where it does not read it).
+  // other parity slot of kIter, and r0 when k == 0, where it does not read it).
   SegState* S = v.state + s;
   const SegParams* P = v.seg + s;
   const uint32_t tileBegin = ldUniform(&P->tileBegin), tileEnd = ldUniform(&P->tileEnd);
   const uint32_t maxCg = ldUniform(&P->maxCg);
   const double cgTolerance = ldUniform(&P->cgTolerance);
   const uint32_t k = ldUniform(&S->kIter[q]);
-  // A solve that an earlier kernel ended: pass the flag on to the next parity slot.
-  if (ldUniform(&S->newtonDone)) return;
-  if (ldUniform(&S->convPar[q])) {
-    if (tile == tileBegin && threadIdx.x == 0) {
-      S->convPar[o] = 1u;
-      S->kIter[o] = k + 1u;
-    }
-    return;
-  }
   const double r0Prev = ldUniform(&S->r0);
   // The previous kernel's partial sums.
   const double* __restrict__ prev = v.cgPart[o];
@@ -842,10 +840,10 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
     else
       beta = rzNext / sum[kPrZ];
   }
+  if (threadIdx.x == 0) v.tileDone[o][tile] = conv;  // every workgroup of the segment decides the same
   if (tile == tileBegin && threadIdx.x == 0) {
     if (k == 0) S->r0 = r0;
     S->kIter[o] = k + 1u;
-    S->convPar[o] = conv;
     if (update) {
       S->alpha = alpha;
       S->rr = rrNext;

@@ -46,6 +46,7 @@ struct View {
   F3* cgZr[2];
   F3* cgT[2];
   double* cgPart[2];
+  uint32_t* tileDone[2];  // per tile, by kernel parity: 1 once the tile's PCG solve has ended (kPcgStep)
   float* precond;  // 9 per row, row-major: S M, the block-Jacobi inverse with the contact filter applied
   float4* cnormal; // xyz normal, w = offset r - phi(xStart)
   uint32_t* cflags;

@@ -130,18 +130,17 @@ struct SegState {
   double pAp, alpha, beta, r0, rr;      // PCG (CgScalars, gpu_layout.h:54-57, in FP64)
   double e0, eTrial, slope, lsAlpha;    // Newton energy, trial energy, grad . step, line-search step
   float maxMove;
-  uint32_t pcgConverged;  // 1 once |r| <= tol |r0| or pAp <= 0 (read by the host; the one-kernel form reads convPar)
+  uint32_t pcgConverged;  // 1 once |r| <= tol |r0| or pAp <= 0 (read by the host; kPcgStep reads per-tile flags)
   uint32_t pcgIter;
   uint32_t newtonDone;    // 1 once slope >= 0 or max move < 1e-9 (no more Newton iterations this substep)
   uint32_t lsDone;        // 1 once the line search accepted a step this Newton iteration
   uint32_t lsTrial;
   uint32_t active;        // 1 while this segment runs Newton iterations this substep
   uint32_t warmRejected;  // 1 when the learned warm start was rejected this substep
-  // One-kernel PCG: iteration index and converged flag by kernel parity. A kernel of parity q reads kIter[q] and
-  // convPar[q] (written by the previous kernel) and writes slot 1-q, so no workgroup ever reads a value another
-  // workgroup of the same launch writes.
+  // One-kernel PCG: iteration index by kernel parity. A kernel of parity q reads kIter[q] (written by the previous
+  // kernel) and writes slot 1-q, so no workgroup ever reads a value another workgroup of the same launch writes
+  // (the per-tile solve-ended flags, View::tileDone, follow the same rule).
   uint32_t kIter[2];
-  uint32_t convPar[2];
   uint64_t cntNewton, cntPcg, cntLs, cntSubsteps, cntWarmRejected, cntWarmAccepted;
 };
 

@@ -251,6 +251,7 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   partMax_.allocate(sizeof(float) * nTiles_);
   statsTile_.allocate(sizeof(double) * dk::kStatsPerSegment * nTiles_);
   statsSeg_.allocate(sizeof(double) * dk::kStatsPerSegment * std::max<size_t>(M.segments.size(), 1));
+  tileDone_.allocate(sizeof(uint32_t) * 2 * nTiles_);
   cgPart0_.allocate(sizeof(double) * 7 * nTiles_);
   cgPart1_.allocate(sizeof(double) * 7 * nTiles_);
 
@@ -370,6 +371,8 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
     v.cgZr[b] = cgBuf_[6 + b].as<dk::F3>();
     v.cgT[b] = cgBuf_[8 + b].as<dk::F3>();
   }
+  v.tileDone[0] = tileDone_.as<uint32_t>();
+  v.tileDone[1] = tileDone_.as<uint32_t>() + nTiles_;
   v.cgPart[0] = cgPart0_.as<double>();
   v.cgPart[1] = cgPart1_.as<double>();
   v.precond = precond_.as<float>();

@@ -118,7 +118,7 @@ class HipSolver {
   DeviceBuffer stretchSlots_, hingeSlots_, seamSlots_, sellK_, sellBase_, sellCol_, sellVal_, buildError_;
   DeviceBuffer lowK_, lowBase_, lowCol_, lowBlk_;
   DeviceBuffer fieldValues_, scratchF3_, scratchF3b_;
-  DeviceBuffer cgBuf_[10], cgPart0_, cgPart1_;
+  DeviceBuffer cgBuf_[10], cgPart0_, cgPart1_, tileDone_;
   int pcgVariant_ = 1;
   int pcgWaves_ = 0;      // requested (0 = tune)
   int tunedWaves_ = 0;    // chosen by the tuner (0 until it has measured every option)
