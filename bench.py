@@ -126,8 +126,9 @@ def main() -> int:
     ap.add_argument("--no-warm-start-ab", action="store_true",
                     help="skip the learned-vs-plain warm start comparison")
     ap.add_argument("--warm-start-weights", default=WEIGHTS)
-    ap.add_argument("--pcg-waves", type=int, default=0,
-                    help="wavefronts per slice in the one-kernel PCG iteration (1, 2, 4; 0 = tune during warm-up)")
+    ap.add_argument("--pcg-shape", default="0x0",
+                    help="launch shape of the one-kernel PCG iteration, wavefronts x slots: 1x8, 1x4, 2x8, 4x8 "
+                         "(0x0 = tune during warm-up)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
@@ -140,7 +141,8 @@ def main() -> int:
         return scenes.c2_sheet(n=args.n)
 
     sc = make_scene()
-    s = HipSolver(graph=not args.no_graph, pcg_variant=args.pcg_variant, pcg_waves=args.pcg_waves)
+    shape = tuple(int(t) for t in args.pcg_shape.lower().split("x"))
+    s = HipSolver(graph=not args.no_graph, pcg_variant=args.pcg_variant, pcg_shape=shape)
     s.build(sc)
     s.set_spmv_sampling(args.sample_every)
     for _ in range(args.warmup):
@@ -176,7 +178,7 @@ def main() -> int:
             "dt": 1 / 60, "substeps": 1, "newton_max": 5, "pcg_tolerance": 1e-6,
             "solver": "plain warm start (u = h v)",
             "pcg_form": "one-kernel" if args.pcg_variant == 1 else "two-kernel",
-            "pcg_waves_per_slice": s.pcg_waves() if args.pcg_variant == 1 else None,
+            "pcg_launch_shape": "%dx%d" % s.pcg_shape() if args.pcg_variant == 1 else None,
             "graph": not args.no_graph,
             "newton_per_step": c["newton_iterations"] / args.steps,
             "pcg_per_step": c["pcg_iterations"] / args.steps,
@@ -193,7 +195,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": (PCG_STEP_TRAFFIC_BYTES if args.pcg_variant == 1 and args.n == 256 and s.pcg_waves() == 1
+            "traffic": (PCG_STEP_TRAFFIC_BYTES if args.pcg_variant == 1 and args.n == 256 and s.pcg_shape() == (1, 8)
                         else None),
             "bytes_per_launch": timing["bytes_per_launch"],
             "mean_launch_us": timing["mean_us"],

@@ -329,18 +329,22 @@ int drape_solver_set_pcg_variant(drape_solver* s, int32_t variant) {
   });
 }
 
-int drape_solver_set_pcg_waves(drape_solver* s, int32_t waves) {
+int drape_solver_set_pcg_shape(drape_solver* s, int32_t waves, int32_t slots) {
   return guard([&] {
     need(s, "solver");
-    s->impl.setPcgWaves(waves);
+    s->impl.setPcgShape(waves, slots);
   });
 }
 
-int drape_solver_pcg_waves(const drape_solver* s, int32_t* out) {
+int drape_solver_pcg_shape(const drape_solver* s, int32_t* waves, int32_t* slots) {
   return guard([&] {
     need(s, "solver");
-    need(out, "output");
-    *out = s->impl.pcgWaves();
+    need(waves, "output");
+    need(slots, "output");
+    int w = 0, k = 0;
+    s->impl.pcgShape(w, k);
+    *waves = w;
+    *slots = k;
   });
 }
 

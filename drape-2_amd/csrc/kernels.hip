@@ -697,10 +697,15 @@ __global__ void __launch_bounds__(256) kPcgStepInit(View v) {
     for (int i = 0; i < kPcgParts; ++i) v.cgPart[1][kPcgParts * tile + i] = d[i];
 }
 
-template <int W>
-__global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
+// Launch shapes: W wavefronts share each 64-row slice (each takes every W-th slot), and a batch covers NB slots of
+// the lower part and NB of the upper part (B = NB / W per wavefront). NB = 8 keeps a whole sheet row in one batch
+// (about 390 registers: one workgroup per CU); NB = 4 fits two workgroups per CU, which pays when a launch has
+// many more tiles than CUs.
+template <int W, int NB>
+__global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(NB == 4 ? 2 : 1, 8)))
+kPcgStep(View v, uint32_t q) {
   constexpr int NW = 4 * W;
-  constexpr uint32_t B = 8u / W;  // lower slots, and upper slots, per wavefront in one batch of eight
+  constexpr uint32_t B = NB / W;  // lower slots, and upper slots, per wavefront in one batch
   __shared__ double red[kPcgParts * NW];
   __shared__ float comb[(W > 1 ? 4 : 1) * kSlicesPerTile * 3 * kSlice];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
@@ -857,9 +862,9 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
   if (part == 0 && update) v.step[row] = oStep + a * oP;
   if (conv) return;
 
-  // Row sums of A p_k. Slots are summed in four groups (slot index mod 4); each group takes, batch by batch of
-  // eight slots, its lower slots then its upper slots in increasing slot order, and the groups are added as
-  // ((g0 + g1) + g2) + g3 whatever W is: every W gives bit-identical results.
+  // Row sums of A p_k. Slots are summed in four groups (slot index mod 4); each group takes its lower slots, then
+  // its upper slots, in increasing slot order, and the groups are added as ((g0 + g1) + g2) + g3: every launch shape
+  // gives bit-identical results.
   constexpr uint32_t G = 4u / W;  // groups held by one wavefront: group part + W lg
   float acc[G][3];
 #pragma unroll
@@ -867,11 +872,7 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
 #pragma unroll
   for (uint32_t j = 0; j < B; ++j)
     blockTMulAdd(bt[j], pcgDirection(lz[j], lt[j], lp[j], a, b), acc[j % G][0], acc[j % G][1], acc[j % G][2]);
-#pragma unroll
-  for (uint32_t j = 0; j < B; ++j)
-    blockMulAdd(bv[j], pcgDirection(gz[j], gt[j], gp[j], a, b), acc[j % G][0], acc[j % G][1], acc[j % G][2]);
-  const uint32_t Kmax = K > KL ? K : KL;
-  for (uint32_t j0 = B; part + W * j0 < Kmax; j0 += B) {  // rows with more than eight lower or upper slots
+  for (uint32_t j0 = B; part + W * j0 < KL; j0 += B) {  // rows with more than NB lower slots
 #pragma unroll
     for (uint32_t j = 0; j < B; ++j) {
       const uint32_t t = part + W * (j0 + j);
@@ -884,6 +885,11 @@ __global__ void __launch_bounds__(256 * W) kPcgStep(View v, uint32_t q) {
       const F3 x = pcgDirection(Zr[cl[j]], T[cl[j]], Pp[cl[j]], a, b);
       if (t < KL) blockTMulAdd(v.sellVal + size_t(bl[j]) * 9u, x, acc[j % G][0], acc[j % G][1], acc[j % G][2]);
     }
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < B; ++j)
+    blockMulAdd(bv[j], pcgDirection(gz[j], gt[j], gp[j], a, b), acc[j % G][0], acc[j % G][1], acc[j % G][2]);
+  for (uint32_t j0 = B; part + W * j0 < K; j0 += B) {  // rows with more than NB upper slots
 #pragma unroll
     for (uint32_t j = 0; j < B; ++j) c[j] = sellColumn(col, part + W * (j0 + j), K, v.nRows);
 #pragma unroll
@@ -1704,15 +1710,19 @@ void launchPcgIteration(const View& v, uint32_t k, hipStream_t st) {
   launchPcgUpdate(v, k, st);
 }
 void launchPcgStepInit(const View& v, hipStream_t st) { kPcgStepInit<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
-void launchPcgStep(const View& v, uint32_t parity, uint32_t wavesPerSlice, hipStream_t st, hipEvent_t start,
-                   hipEvent_t stop) {
+void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t st, hipEvent_t start, hipEvent_t stop) {
   const dim3 grid(xcdGrid(v.nTiles));
-  switch (wavesPerSlice) {
-    case 1: hipExtLaunchKernelGGL(kPcgStep<1>, grid, dim3(256), 0, st, start, stop, 0, v, parity & 1u); break;
-    case 2: hipExtLaunchKernelGGL(kPcgStep<2>, grid, dim3(512), 0, st, start, stop, 0, v, parity & 1u); break;
-    case 4: hipExtLaunchKernelGGL(kPcgStep<4>, grid, dim3(1024), 0, st, start, stop, 0, v, parity & 1u); break;
-    default: throw std::invalid_argument("PCG wavefronts per slice must be 1, 2 or 4");
-  }
+  const uint32_t q = parity & 1u;
+  if (shape.waves == 1 && shape.slots == 8)
+    hipExtLaunchKernelGGL((kPcgStep<1, 8>), grid, dim3(256), 0, st, start, stop, 0, v, q);
+  else if (shape.waves == 1 && shape.slots == 4)
+    hipExtLaunchKernelGGL((kPcgStep<1, 4>), grid, dim3(256), 0, st, start, stop, 0, v, q);
+  else if (shape.waves == 2 && shape.slots == 8)
+    hipExtLaunchKernelGGL((kPcgStep<2, 8>), grid, dim3(512), 0, st, start, stop, 0, v, q);
+  else if (shape.waves == 4 && shape.slots == 8)
+    hipExtLaunchKernelGGL((kPcgStep<4, 8>), grid, dim3(1024), 0, st, start, stop, 0, v, q);
+  else
+    throw std::invalid_argument("PCG launch shape must be 1x8, 1x4, 2x8 or 4x8 (wavefronts per slice x slots)");
 }
 void launchContactForceSlope(const View& v, hipStream_t st) {
   kContactForceSlope<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);

@@ -132,8 +132,15 @@ void launchPcgStepInit(const View& v, hipStream_t st);
 // the non-finite flag (1.0), the maximum body penetration and the maximum seam gap. tilePart: 6 per tile.
 constexpr uint32_t kStatsPerSegment = 6;
 void launchStats(const View& v, const double* massF64, double* tilePart, double* segOut, hipStream_t st);
-// One PCG iteration of the one-reduction form; wavesPerSlice (1, 2 or 4) wavefronts share each 64-row slice.
-void launchPcgStep(const View& v, uint32_t parity, uint32_t wavesPerSlice, hipStream_t st, hipEvent_t start = nullptr,
+// Launch shape of the one-kernel PCG iteration: `waves` wavefronts share each 64-row slice, and each batch covers
+// `slots` lower and `slots` upper slots. Supported: 1x8, 1x4, 2x8, 4x8. Every shape gives bit-identical results.
+struct PcgShape {
+  uint32_t waves = 1, slots = 8;
+  bool operator==(const PcgShape& o) const { return waves == o.waves && slots == o.slots; }
+};
+constexpr PcgShape kPcgShapes[4] = {{1, 8}, {1, 4}, {2, 8}, {4, 8}};
+// One PCG iteration of the one-reduction form.
+void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t st, hipEvent_t start = nullptr,
                    hipEvent_t stop = nullptr);
 void launchContactForceSlope(const View& v, hipStream_t st);
 void launchNewtonSlope(const View& v, hipStream_t st);

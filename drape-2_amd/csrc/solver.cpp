@@ -575,10 +575,23 @@ void HipSolver::setPcgVariant(int variant) {
   pcgVariant_ = variant;
 }
 
-void HipSolver::setPcgWaves(int waves) {
-  if (waves != 0 && waves != 1 && waves != 2 && waves != 4)
-    throw std::invalid_argument("PCG wavefronts per slice must be 0 (tune), 1, 2 or 4");
-  pcgWaves_ = waves;
+void HipSolver::setPcgShape(int waves, int slots) {
+  if (waves == 0 && slots == 0) {
+    pcgShape_ = -1;
+    return;
+  }
+  for (int i = 0; i < kShapeCount; ++i)
+    if (int(dk::kPcgShapes[i].waves) == waves && int(dk::kPcgShapes[i].slots) == slots) {
+      pcgShape_ = i;
+      return;
+    }
+  throw std::invalid_argument("PCG launch shape must be 0x0 (tune), 1x8, 1x4, 2x8 or 4x8 (wavefronts x slots)");
+}
+
+void HipSolver::pcgShape(int& waves, int& slots) const {
+  const int i = shapeIndex();
+  waves = i < 0 ? 0 : int(dk::kPcgShapes[i].waves);
+  slots = i < 0 ? 0 : int(dk::kPcgShapes[i].slots);
 }
 
 void HipSolver::runPcg() {
@@ -633,41 +646,40 @@ std::pair<hipEvent_t, hipEvent_t> HipSolver::takeEvents() {
   return pr;
 }
 
-void HipSolver::timedPcgStep(uint32_t parity, uint32_t waves) {
+void HipSolver::timedPcgStep(uint32_t parity, dk::PcgShape shape) {
   const auto pr = takeEvents();
-  dk::launchPcgStep(view_, parity, waves, stream_, pr.first, pr.second);
+  dk::launchPcgStep(view_, parity, shape, stream_, pr.first, pr.second);
   eventPending_.push_back(pr);
 }
 
-// Solves run before a wave count is chosen: chunks of kGraphChunk launches rotate over W = 1, 2, 4 (the option
-// with the fewest samples goes next), each bracketed by HIP events and followed by a poll. A chunk counts only if
-// the solve was still running at its end, so every timed launch did a full iteration. Once every option has four
-// such chunks the fastest mean is kept; W only changes the launch shape, never the results.
+// Solves run before a launch shape is chosen: chunks of kGraphChunk launches rotate over the shapes of
+// dk::kPcgShapes (the one with the fewest samples goes next), each bracketed by HIP events and followed by a poll.
+// A chunk counts only if the solve was still running at its end, so every timed launch did a full iteration. Once
+// every shape has four such chunks the fastest mean is kept; the shape never changes the results.
 void HipSolver::runPcgTuning(uint32_t maxCg) {
-  static const uint32_t kOptions[3] = {1u, 2u, 4u};
   uint32_t launched = 0;
   while (launched < maxCg + 1) {
-    uint32_t o = 0;
-    for (uint32_t i = 1; i < 3; ++i)
+    int o = 0;
+    for (int i = 1; i < kShapeCount; ++i)
       if (tuneLaunches_[i] < tuneLaunches_[o]) o = i;
-    const uint32_t waves = tunedWaves_ ? uint32_t(tunedWaves_) : kOptions[o];
+    const dk::PcgShape shape = dk::kPcgShapes[tunedShape_ >= 0 ? tunedShape_ : o];
     const auto pr = takeEvents();
     check(hipEventRecord(pr.first, stream_), "event");
-    for (uint32_t i = 0; i < kGraphChunk; ++i) dk::launchPcgStep(view_, i & 1u, waves, stream_);
+    for (uint32_t i = 0; i < kGraphChunk; ++i) dk::launchPcgStep(view_, i & 1u, shape, stream_);
     check(hipEventRecord(pr.second, stream_), "event");
     launched += kGraphChunk;
     pollStates();
     const bool done = allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; });
-    if (!done && !tunedWaves_) {
+    if (!done && tunedShape_ < 0) {
       float ms = 0.0f;
       check(hipEventElapsedTime(&ms, pr.first, pr.second), "event time");
       tuneMs_[o] += ms;
       tuneLaunches_[o] += kGraphChunk;
-      if (*std::min_element(tuneLaunches_, tuneLaunches_ + 3) >= 4u * kGraphChunk) {
-        uint32_t best = 0;
-        for (uint32_t i = 1; i < 3; ++i)
+      if (*std::min_element(tuneLaunches_, tuneLaunches_ + kShapeCount) >= 4u * kGraphChunk) {
+        int best = 0;
+        for (int i = 1; i < kShapeCount; ++i)
           if (tuneMs_[i] / tuneLaunches_[i] < tuneMs_[best] / tuneLaunches_[best]) best = i;
-        tunedWaves_ = int(kOptions[best]);
+        tunedShape_ = best;
       }
     }
     eventPool_.push_back(pr);
@@ -685,19 +697,20 @@ void HipSolver::runPcgTuning(uint32_t maxCg) {
 void HipSolver::runPcgSingleKernel() {
   dk::launchPcgStepInit(view_, stream_);
   const uint32_t maxCg = maxCgOf(model_);
-  const uint32_t waves = uint32_t(pcgWaves());
-  if (waves == 0) {
+  const int shapeIdx = shapeIndex();
+  if (shapeIdx < 0) {
     runPcgTuning(maxCg);
     return;
   }
+  const dk::PcgShape shape = dk::kPcgShapes[shapeIdx];
   const bool sampling = spmvEvery_ > 0;
-  if (useGraph_ && (!graphExec_ || graphWaves_ != waves)) {
+  if (useGraph_ && (!graphExec_ || graphShape_ != shapeIdx)) {
     releaseGraph();
     check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
-    for (uint32_t i = 0; i < kGraphChunk; ++i) dk::launchPcgStep(view_, i & 1u, waves, stream_);
+    for (uint32_t i = 0; i < kGraphChunk; ++i) dk::launchPcgStep(view_, i & 1u, shape, stream_);
     check(hipStreamEndCapture(stream_, &graph_), "capture");
     check(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0), "graph instantiate");
-    graphWaves_ = waves;
+    graphShape_ = shapeIdx;
   }
   uint32_t launched = 0;
   uint32_t chunks = std::max<uint32_t>(1, (lastPcgIterations_ * 7 / 8) / kGraphChunk);
@@ -705,8 +718,8 @@ void HipSolver::runPcgSingleKernel() {
     if (useGraph_ && sampling) {
       // Two eager iterations (keeping the graph's parity pattern) open each batch; the second, which starts
       // right behind the first as graph nodes do, is event-timed.
-      dk::launchPcgStep(view_, 0u, waves, stream_);
-      timedPcgStep(1u, waves);
+      dk::launchPcgStep(view_, 0u, shape, stream_);
+      timedPcgStep(1u, shape);
       launched += 2;
     }
     for (uint32_t c = 0; c < chunks; ++c) {
@@ -715,9 +728,9 @@ void HipSolver::runPcgSingleKernel() {
       } else {
         for (uint32_t i = 0; i < kGraphChunk; ++i) {
           if (sampling && (launched + i) % uint32_t(spmvEvery_) == 1)
-            timedPcgStep(i & 1u, waves);
+            timedPcgStep(i & 1u, shape);
           else
-            dk::launchPcgStep(view_, i & 1u, waves, stream_);
+            dk::launchPcgStep(view_, i & 1u, shape, stream_);
         }
       }
       launched += kGraphChunk;

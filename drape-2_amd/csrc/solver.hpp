@@ -43,8 +43,9 @@ class HipSolver {
   void setPcgVariant(int variant);
   // Wavefronts per 64-row slice in the one-kernel PCG iteration: 1, 2 or 4, or 0 to time all three during the
   // first solves and keep the fastest (results are bit-identical for every choice).
-  void setPcgWaves(int waves);
-  int pcgWaves() const { return pcgWaves_ ? pcgWaves_ : tunedWaves_; }
+  // Launch shape of the one-kernel PCG iteration (wavefronts per slice x slots per batch; 0x0 = tune).
+  void setPcgShape(int waves, int slots);
+  void pcgShape(int& waves, int& slots) const;  // 0x0 while still tuning
   // Every `every`-th PCG SpMV launch of the step is bracketed by HIP events (0 = off).
   void setSpmvSampling(int every);
   void spmvTiming(uint64_t& launches, double& meanUs, double& bytesPerLaunch) const;
@@ -87,7 +88,8 @@ class HipSolver {
   void runPcgStandard();
   void runPcgSingleKernel();
   void releaseGraph();
-  void timedPcgStep(uint32_t parity, uint32_t waves);
+  void timedPcgStep(uint32_t parity, dk::PcgShape shape);
+  int shapeIndex() const { return pcgShape_ >= 0 ? pcgShape_ : tunedShape_; }
   std::pair<hipEvent_t, hipEvent_t> takeEvents();
   void runPcgTuning(uint32_t maxCg);
   void globalStructure(std::vector<uint32_t>& rowStart, std::vector<uint32_t>& colIndex,
@@ -120,14 +122,15 @@ class HipSolver {
   DeviceBuffer fieldValues_, scratchF3_, scratchF3b_;
   DeviceBuffer cgBuf_[10], cgPart0_, cgPart1_, tileDone_;
   int pcgVariant_ = 1;
-  int pcgWaves_ = 0;      // requested (0 = tune)
-  int tunedWaves_ = 0;    // chosen by the tuner (0 until it has measured every option)
-  double tuneMs_[3] = {0.0, 0.0, 0.0};
-  uint32_t tuneLaunches_[3] = {0u, 0u, 0u};
+  static constexpr int kShapeCount = 4;  // dk::kPcgShapes
+  int pcgShape_ = -1;     // requested index into dk::kPcgShapes (-1 = tune)
+  int tunedShape_ = -1;   // chosen by the tuner (-1 until it has measured every shape)
+  double tuneMs_[kShapeCount] = {0.0, 0.0, 0.0, 0.0};
+  uint32_t tuneLaunches_[kShapeCount] = {0u, 0u, 0u, 0u};
   static constexpr uint32_t kGraphChunk = 16;  // PCG iterations per captured graph (even: parity pattern repeats)
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graphExec_ = nullptr;
-  uint32_t graphWaves_ = 0;  // W the captured graph launches
+  int graphShape_ = -1;  // shape index the captured graph launches
   dk::SegState* hostStates_ = nullptr;  // pinned host copy
   int spmvEvery_ = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> eventPool_, eventPending_;

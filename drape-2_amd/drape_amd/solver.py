@@ -46,7 +46,7 @@ class HipSolver:
 
     backend = "hip"
 
-    def __init__(self, graph: bool = True, pcg_variant: int = 1, pcg_waves: int = 0):
+    def __init__(self, graph: bool = True, pcg_variant: int = 1, pcg_shape: tuple = (0, 0)):
         self._h = C.c_void_p()
         _check(_lib().drape_solver_create(C.byref(self._h)))
         self._n = 0
@@ -54,8 +54,8 @@ class HipSolver:
             self.set_graph(False)
         if pcg_variant != 1:
             self.set_pcg_variant(pcg_variant)
-        if pcg_waves:
-            self.set_pcg_waves(pcg_waves)
+        if tuple(pcg_shape) != (0, 0):
+            self.set_pcg_shape(*pcg_shape)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -158,14 +158,15 @@ class HipSolver:
         """1: one kernel per PCG iteration (default); 0: the two-kernel standard form."""
         _check(_lib().drape_solver_set_pcg_variant(self._h, int(variant)))
 
-    def set_pcg_waves(self, waves: int) -> None:
-        """Wavefronts per 64-row slice in the one-kernel PCG iteration: 1, 2, 4, or 0 to tune (default)."""
-        _check(_lib().drape_solver_set_pcg_waves(self._h, int(waves)))
+    def set_pcg_shape(self, waves: int, slots: int) -> None:
+        """Launch shape of the one-kernel PCG iteration: (wavefronts per 64-row slice, slots per batch) in
+        (1, 8), (1, 4), (2, 8), (4, 8), or (0, 0) to tune (default)."""
+        _check(_lib().drape_solver_set_pcg_shape(self._h, int(waves), int(slots)))
 
-    def pcg_waves(self) -> int:
-        out = C.c_int32()
-        _check(_lib().drape_solver_pcg_waves(self._h, C.byref(out)))
-        return out.value
+    def pcg_shape(self) -> tuple:
+        w, k = C.c_int32(), C.c_int32()
+        _check(_lib().drape_solver_pcg_shape(self._h, C.byref(w), C.byref(k)))
+        return (w.value, k.value)
 
     def set_spmv_sampling(self, every: int) -> None:
         _check(_lib().drape_solver_set_spmv_sampling(self._h, int(every)))

@@ -207,12 +207,13 @@ int drape_solver_set_graph(drape_solver* s, int32_t enabled);
  * p.Ap reduced directly and r.z, r.r expanded exactly over the previous iteration's vectors: the same iterates and
  * stopping rules); 0 the two-kernel standard form (cpu_solver.cpp:169-213 step by step). */
 int drape_solver_set_pcg_variant(drape_solver* s, int32_t variant);
-/* Launch shape of the one-kernel PCG iteration: 1, 2 or 4 wavefronts per 64-row slice, or 0 (default) to time
- * all three during the first solves and keep the fastest. Results are bit-identical for every value.
- * drape_solver_pcg_waves reports the value in use (0 while still tuning). No reference counterpart: the
- * reference's GPU path has one thread per row (kernels.h:272-289). */
-int drape_solver_set_pcg_waves(drape_solver* s, int32_t waves);
-int drape_solver_pcg_waves(const drape_solver* s, int32_t* out);
+/* Launch shape of the one-kernel PCG iteration: `waves` wavefronts per 64-row slice and `slots` lower and upper
+ * matrix slots per batch; 1x8, 1x4 (smaller register footprint: two workgroups per CU, for launches with many more
+ * tiles than CUs), 2x8 and 4x8, or 0x0 (default) to time every shape during the first solves and keep the fastest.
+ * Results are bit-identical for every shape. drape_solver_pcg_shape reports the shape in use (0x0 while still
+ * tuning). No reference counterpart: the reference's GPU path has one thread per row (kernels.h:272-289). */
+int drape_solver_set_pcg_shape(drape_solver* s, int32_t waves, int32_t slots);
+int drape_solver_pcg_shape(const drape_solver* s, int32_t* waves, int32_t* slots);
 
 /* Kernel timing inside the step: every `every`-th PCG SpMV launch (0 = off) is bracketed by HIP events on the
  * solver's stream. drape_solver_spmv_timing reports how many launches were timed since sampling was set, their

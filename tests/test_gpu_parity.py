@@ -285,19 +285,20 @@ def test_two_runs_bit_identical_and_snapshot_replay(hip):
 
 def test_pcg_launch_shapes_bit_identical(hip):
     """The one-kernel PCG iteration sums each row in the same fixed order whatever its launch shape (1, 2 or 4
-    wavefronts per slice), so every shape, and the tuner's choice, gives the same bits."""
+    wavefronts per slice; batches of 8 or 4 slots), so every shape, and the tuner's choice, gives the same bits."""
     sc = scenes.c1_sheet()
     out = {}
-    for waves in (1, 2, 4, 0):
-        g = hip.HipSolver(pcg_waves=waves)
+    shapes = ((1, 8), (1, 4), (2, 8), (4, 8), (0, 0))
+    for shape in shapes:
+        g = hip.HipSolver(pcg_shape=shape)
         g.build(sc)
         for _ in range(12):
             g.step(1 / 60, 1, 5)
-        out[waves] = (g.read_positions(), g.counters(), g.pcg_waves())
-        print(f"waves {waves}: in use {out[waves][2]}, counters {out[waves][1]}", flush=True)
-    for waves in (2, 4, 0):
-        np.testing.assert_array_equal(out[waves][0], out[1][0])
-        assert out[waves][1] == out[1][1]
+        out[shape] = (g.read_positions(), g.counters(), g.pcg_shape())
+        print(f"shape {shape}: in use {out[shape][2]}, counters {out[shape][1]}", flush=True)
+    for shape in shapes[1:]:
+        np.testing.assert_array_equal(out[shape][0], out[(1, 8)][0])
+        assert out[shape][1] == out[(1, 8)][1]
 
 
 def test_sheet_drop_contact_phase_stays_bounded(hip):
