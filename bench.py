@@ -111,6 +111,20 @@ def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 5
             "weights": os.path.relpath(net_path, REPO), "runs": runs}
 
 
+def algorithmic_step_bytes(s, counters: dict, steps: int, pcg_launch_bytes: float) -> float:
+    """Algorithmic bytes of one implicit step of the timed run: per Newton iteration the assembly (read x start,
+    u, each term's constants and the contact state; write the stored blocks and the gradient) and the
+    preconditioner; per PCG iteration one kPcgStep launch; per line-search trial an energy-only evaluation."""
+    z = s.structure_sizes()
+    n, S, H, K = z["n_particles"], z["n_stretch"], z["n_hinges"], z["n_seams"]
+    upper = 0.5 * (z["n_blocks"] + n)
+    terms = 24.0 * n + 36.0 * S + 24.0 * H + 16.0 * K + 32.0 * n
+    assembly = terms + 12.0 * n + 36.0 * upper
+    per_step = lambda key: counters[key] / steps  # noqa: E731
+    return (per_step("newton_iterations") * (assembly + 72.0 * n) + per_step("pcg_iterations") * pcg_launch_bytes
+            + per_step("line_search_trials") * terms)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +171,7 @@ def main() -> int:
     timing = s.spmv_timing()
     st = s.stats()
     value = args.steps / elapsed
+    step_bytes = algorithmic_step_bytes(s, c, args.steps, timing["bytes_per_launch"])
     achieved = timing["bytes_per_launch"] / (timing["mean_us"] * 1e-6) / 1e9 if timing["mean_us"] > 0 else None
     line = {
         "metric": "implicit steps/sec @65k particles",
@@ -201,6 +216,16 @@ def main() -> int:
             "mean_launch_us": timing["mean_us"],
             "timed_launches": timing["launches"],
         },
+    }
+    line["step_roofline"] = {
+        "bytes_per_step": step_bytes,
+        "achieved": step_bytes * value / 1e9,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": step_bytes * value / 1e9 / HBM_PEAK_GBS,
+        "terms": "assembly 24n + 36 S + 24 H + 16 K + 44n + 36 U per Newton iteration, preconditioner 72n per Newton "
+                 "iteration, one kPcgStep launch per PCG iteration, 24n + 36 S + 24 H + 16 K + 32n per line-search "
+                 "trial (n particles, S stretch, H hinge, K seam terms, U stored blocks)",
     }
     if not args.no_warm_start_ab and os.path.exists(args.warm_start_weights):
         line["warm_start_ab"] = warm_start_ab(make_scene, args.warmup, args.steps, args.warm_start_weights)

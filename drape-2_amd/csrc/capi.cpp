@@ -11,6 +11,7 @@
 
 #include "drape_hip.h"
 #include "kernels.hpp"
+#include "normals.hpp"
 #include "solver.hpp"
 #include "warmstart.hpp"
 
@@ -268,6 +269,14 @@ int drape_solver_stats(const drape_solver* s, drape_stats* out) {
   });
 }
 
+int drape_solver_read_positions_normals(const drape_solver* s, float* positions, float* normals, size_t n) {
+  return guard([&] {
+    need(s, "solver");
+    if (n) need(normals, "normals");
+    s->impl.readPositionsNormals(positions, normals, n);
+  });
+}
+
 int drape_solver_segment_stats(const drape_solver* s, uint32_t first, uint32_t count, drape_stats* out) {
   return guard([&] {
     need(s, "solver");
@@ -415,6 +424,25 @@ int drape_solver_debug_system(drape_solver* s, float* blocks, float* grad, float
   return guard([&] {
     need(s, "solver");
     s->impl.debugSystem(blocks, grad, precond, contact_flags, contact_normals);
+  });
+}
+
+int drape_vertex_normals(const float* positions, uint32_t n, const uint32_t* triangles, uint32_t n_triangles,
+                         const uint32_t* weld_pairs, uint32_t n_welds, float* out) {
+  return guard([&] {
+    if (n) {
+      need(positions, "positions");
+      need(out, "output");
+    }
+    if (n_triangles) need(triangles, "triangles");
+    if (n_welds) need(weld_pairs, "weld pairs");
+    Stream st;
+    const nrm::Mesh mesh(n, triangles, n_triangles, weld_pairs, n_welds);
+    Dev dp(sizeof(dk::F3) * std::max<uint32_t>(n, 1)), dn(sizeof(dk::F3) * std::max<uint32_t>(n, 1));
+    if (n) check(hipMemcpyAsync(dp.p, positions, sizeof(float) * 3 * size_t(n), hipMemcpyHostToDevice, st.s), "copy");
+    mesh.compute(dp.as<dk::F3>(), dn.as<dk::F3>(), st.s);
+    if (n) check(hipMemcpyAsync(out, dn.p, sizeof(float) * 3 * size_t(n), hipMemcpyDeviceToHost, st.s), "copy");
+    check(hipStreamSynchronize(st.s), "normals");
   });
 }
 

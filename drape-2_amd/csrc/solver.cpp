@@ -122,6 +122,7 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   releaseGraph();
   model_ = std::move(m);
   built_ = false;
+  normalMesh_.reset();
   lastH_ = -1.0;
   const Model& M = model_;
   nRows_ = M.rowCount();
@@ -816,6 +817,43 @@ void HipSolver::readVelocities(float* out, size_t n) const {
     out[3 * i] = a.x;
     out[3 * i + 1] = a.y;
     out[3 * i + 2] = a.z;
+  }
+}
+
+// vertexNormals (libs/render/src/normals.cpp:5-24) of the current positions: every garment's triangles (one
+// stretch term per mesh triangle, in triangle order), every seam pair welded, in internal rows; copied out with
+// the positions in one pass.
+void HipSolver::readPositionsNormals(float* positions, float* normals, size_t n) const {
+  requireBuilt();
+  if (n != model_.x.size()) throw std::invalid_argument("span size must equal particleCount()");
+  if (!normalMesh_) {
+    std::vector<uint32_t> tri(3 * model_.stretch.size()), weld(2 * model_.seams.size());
+    for (size_t t = 0; t < model_.stretch.size(); ++t)
+      for (int k = 0; k < 3; ++k) tri[3 * t + k] = rowOf_[model_.stretch[t].v[k]];
+    for (size_t k = 0; k < model_.seams.size(); ++k) {
+      weld[2 * k] = rowOf_[model_.seams[k].a];
+      weld[2 * k + 1] = rowOf_[model_.seams[k].b];
+    }
+    normalMesh_ = std::make_unique<nrm::Mesh>(nRows_, tri.data(), uint32_t(model_.stretch.size()), weld.data(),
+                                              uint32_t(model_.seams.size()));
+    normalsOut_.allocate(sizeof(dk::F3) * (nRows_ + 1));
+  }
+  normalMesh_->compute(x_.as<dk::F3>(), normalsOut_.as<dk::F3>(), stream_);
+  std::vector<dk::F3> xs(nRows_), ns(nRows_);
+  check(hipMemcpyAsync(xs.data(), x_.as<void>(), sizeof(dk::F3) * nRows_, hipMemcpyDeviceToHost, stream_), "copy");
+  check(hipMemcpyAsync(ns.data(), normalsOut_.as<void>(), sizeof(dk::F3) * nRows_, hipMemcpyDeviceToHost, stream_),
+        "copy");
+  check(hipStreamSynchronize(stream_), "sync");
+  for (size_t i = 0; i < n; ++i) {
+    const dk::F3 a = xs[rowOf_[i]], b = ns[rowOf_[i]];
+    if (positions) {
+      positions[3 * i] = a.x;
+      positions[3 * i + 1] = a.y;
+      positions[3 * i + 2] = a.z;
+    }
+    normals[3 * i] = b.x;
+    normals[3 * i + 1] = b.y;
+    normals[3 * i + 2] = b.z;
   }
 }
 

@@ -11,6 +11,7 @@
 #include "device_buffer.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
+#include "normals.hpp"
 #include "warmstart.hpp"
 
 namespace dh {
@@ -27,6 +28,8 @@ class HipSolver {
   void readPositions(float* out, size_t n) const;
   void readVelocities(float* out, size_t n) const;
   drape_stats stats() const;
+  // Positions and unit vertex normals (every garment's triangles, seam pairs welded) in particle order.
+  void readPositionsNormals(float* positions, float* normals, size_t n) const;
   // Stats of segments [first, first + count) (one scene of a batched build each), from one device reduction.
   void segmentStats(uint32_t first, uint32_t count, drape_stats* out) const;
   size_t snapshotSize() const;
@@ -113,6 +116,8 @@ class HipSolver {
   DeviceBuffer dvLast_, phiN_;
   DeviceBuffer massF64_, statsTile_, statsSeg_;  // device stats: FP64 masses, per-tile and per-segment partials
   std::unique_ptr<ws::Graph> meshGraph_;
+  mutable std::unique_ptr<nrm::Mesh> normalMesh_;  // built on the first normals request after a build
+  mutable DeviceBuffer normalsOut_;
   ws::Net* net_ = nullptr;
   DeviceBuffer tileSeg_, segParams_, segState_, partA_, partB_, partC_, partE_, partMax_;
   DeviceBuffer stretch_, hinges_, seams_, fabrics_, garments_;

@@ -276,6 +276,9 @@ def _declare(lib):
         "drape_solver_read_velocities": (C.c_int, [S, f32p, C.c_size_t]),
         "drape_solver_stats": (C.c_int, [S, C.POINTER(Stats)]),
         "drape_solver_segment_stats": (C.c_int, [S, C.c_uint32, C.c_uint32, C.POINTER(Stats)]),
+        "drape_solver_read_positions_normals": (C.c_int, [S, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t]),
+        "drape_vertex_normals": (C.c_int, [C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32,
+                                           C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(C.c_float)]),
         "drape_solver_snapshot_size": (C.c_int, [S, C.POINTER(C.c_size_t)]),
         "drape_solver_snapshot_save": (C.c_int, [S, C.c_void_p, C.c_size_t]),
         "drape_solver_snapshot_restore": (C.c_int, [S, C.c_void_p, C.c_size_t]),

@@ -124,6 +124,14 @@ class HipSolver:
         _check(_lib().drape_solver_stats(self._h, C.byref(st)))
         return st.as_dict()
 
+    def read_positions_normals(self):
+        """(positions, unit vertex normals) of the current state in particle order: every garment's triangles,
+        seam pairs welded."""
+        pos = np.empty((self._n, 3), np.float32)
+        nrm = np.empty((self._n, 3), np.float32)
+        _check(_lib().drape_solver_read_positions_normals(self._h, ptr(pos, C.c_float), ptr(nrm, C.c_float), self._n))
+        return pos, nrm
+
     def segment_stats(self, first: int = 0, count: int | None = None) -> list:
         """Stats of each scene of a batched build (segments [first, first + count)), one device reduction."""
         if count is None:
@@ -276,6 +284,18 @@ def make_solver(backend: str = "hip", **kw):
 
 
 # ---- standalone kernels ----
+
+def vertex_normals(positions, triangles, weld_pairs=None) -> np.ndarray:
+    """Unit vertex normals on the device (vertexNormals, libs/render/src/normals.cpp:5-24): area-weighted sums of
+    the triangles' cross products, weld pairs summed in order, normalised; (0, 1, 0) without area."""
+    x = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+    w = np.ascontiguousarray(weld_pairs if weld_pairs is not None else np.zeros((0, 2)), np.uint32).reshape(-1, 2)
+    out = np.empty_like(x)
+    _check(_lib().drape_vertex_normals(ptr(x, C.c_float), len(x), ptr(t, C.c_uint32), len(t), ptr(w, C.c_uint32),
+                                       len(w), ptr(out, C.c_float)))
+    return out
+
 
 def bsr_spmv(row_start, col_index, blocks, pinned, x, reps: int = 1):
     """y = A x on the device with the solver's SELL-64 SpMV kernel; returns (y, mean kernel microseconds)."""

@@ -253,6 +253,21 @@ int drape_solver_debug_spmv(drape_solver* s, const float* x, float* y);
 int drape_solver_debug_system(drape_solver* s, float* blocks, float* grad, float* precond, uint32_t* contact_flags,
                               float* contact_normals);
 
+/* vertexNormals (libs/render/include/drape/render/normals.hpp:13-18; normals.cpp:5-24) on the device, on caller
+ * arrays (host memory): positions 3 floats per vertex, triangles 3 indices each, weld_pairs 2 indices each (a
+ * seam's two coincident particles). Each vertex sums the cross products (x1 - x0) x (x2 - x0) of its triangles
+ * in triangle order, each weld pair in order sets both of its vertices to their sum, and every normal is
+ * normalised; a vertex without area gets (0, 1, 0). FP32, products and sums not fused. out: 3 floats per vertex.
+ * std::out_of_range for an index >= n. */
+int drape_vertex_normals(const float* positions, uint32_t n, const uint32_t* triangles, uint32_t n_triangles,
+                         const uint32_t* weld_pairs, uint32_t n_welds, float* out);
+
+/* The solver's current positions and their vertex normals (as drape_vertex_normals over every garment's mesh
+ * triangles, every seam pair welded), in particle order, in one call: the position exchange of a live viewer
+ * (the reference's viewport reads positions, then Renderer::updateGarment computes normals on the CPU,
+ * renderer.cpp:285-289). positions may be NULL; n must equal particleCount() (invalid_argument). */
+int drape_solver_read_positions_normals(const drape_solver* s, float* positions, float* normals, size_t n);
+
 /* Standalone block-CSR SpMV on caller arrays (host memory): copies in, runs the solver's SpMV kernel `reps`
  * times, copies y out, and reports the mean kernel time in microseconds (HIP events). pinned may be NULL. */
 int drape_bsr_spmv(uint32_t n, uint32_t n_blocks, const uint32_t* row_start, const uint32_t* col_index,

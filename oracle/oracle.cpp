@@ -1674,4 +1674,39 @@ int oracle_bsr_pcg(uint32_t n, const uint32_t* row_start, const uint32_t* col_in
                                 iterations, nullptr);
 }
 
+// vertexNormals (libs/render/src/normals.cpp:5-24). FP32 as in the reference's render path; each product and
+// difference rounded separately (this build uses no fused multiply-add on the host).
+int oracle_vertex_normals(const float* positions, uint32_t n, const uint32_t* triangles, uint32_t n_triangles,
+                          const uint32_t* weld_pairs, uint32_t n_welds, float* out) {
+  return guard([&] {
+    std::vector<float> acc(3 * size_t(n), 0.0f);
+    auto P = [&](uint32_t v, int k) {
+      if (v >= n) throw std::out_of_range("normals: vertex out of range");
+      return positions[3 * size_t(v) + k];
+    };
+    for (uint32_t t = 0; t < n_triangles; ++t) {
+      const uint32_t* tri = triangles + 3 * size_t(t);
+      const float ux = P(tri[1], 0) - P(tri[0], 0), uy = P(tri[1], 1) - P(tri[0], 1), uz = P(tri[1], 2) - P(tri[0], 2);
+      const float vx = P(tri[2], 0) - P(tri[0], 0), vy = P(tri[2], 1) - P(tri[0], 1), vz = P(tri[2], 2) - P(tri[0], 2);
+      const float c[3] = {uy * vz - uz * vy, uz * vx - ux * vz, ux * vy - uy * vx};
+      for (int k = 0; k < 3; ++k)
+        for (int a = 0; a < 3; ++a) acc[3 * size_t(tri[k]) + a] += c[a];
+    }
+    for (uint32_t w = 0; w < n_welds; ++w) {
+      const uint32_t a = weld_pairs[2 * size_t(w)], b = weld_pairs[2 * size_t(w) + 1];
+      if (a >= n || b >= n) throw std::out_of_range("normals: weld vertex out of range");
+      for (int k = 0; k < 3; ++k) {
+        const float s = acc[3 * size_t(a) + k] + acc[3 * size_t(b) + k];
+        acc[3 * size_t(a) + k] = s;
+        acc[3 * size_t(b) + k] = s;
+      }
+    }
+    for (uint32_t v = 0; v < n; ++v) {
+      const float* a = &acc[3 * size_t(v)];
+      const float len = std::sqrt((a[0] * a[0] + a[1] * a[1]) + a[2] * a[2]);
+      for (int k = 0; k < 3; ++k) out[3 * size_t(v) + k] = len > 0.0f ? a[k] / len : (k == 1 ? 1.0f : 0.0f);
+    }
+  });
+}
+
 }  // extern "C"

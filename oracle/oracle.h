@@ -90,6 +90,12 @@ int oracle_bsr_pcg_contact(uint32_t n, const uint32_t* row_start, const uint32_t
                            const uint8_t* pinned, const uint8_t* active, const double* normals, const double* rhs,
                            double tol, int32_t max_iter, double* x, int32_t* iterations, double* contact_force);
 
+/* vertexNormals (libs/render/src/normals.cpp:5-24), sequentially in FP32: per triangle in order add
+ * c = (x1 - x0) x (x2 - x0) to its three vertices; per weld pair in order set both vertices to their sum; then
+ * normalise each, (0, 1, 0) for a zero vector. */
+int oracle_vertex_normals(const float* positions, uint32_t n, const uint32_t* triangles, uint32_t n_triangles,
+                          const uint32_t* weld_pairs, uint32_t n_welds, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,7 @@ def lib():
         "oracle_solver_read_velocities": (C.c_int, [S, f32p, C.c_size_t]),
         "oracle_solver_write_state": (C.c_int, [S, f32p, f32p, C.c_size_t]),
         "oracle_solver_stats": (C.c_int, [S, C.POINTER(Stats)]),
+        "oracle_vertex_normals": (C.c_int, [f32p, C.c_uint32, u32p, C.c_uint32, u32p, C.c_uint32, f32p]),
         "oracle_solver_snapshot_size": (C.c_int, [S, C.POINTER(C.c_size_t)]),
         "oracle_solver_snapshot_save": (C.c_int, [S, C.c_void_p, C.c_size_t]),
         "oracle_solver_snapshot_restore": (C.c_int, [S, C.c_void_p, C.c_size_t]),
@@ -369,6 +370,17 @@ def bsr_pcg(row_start, col_index, blocks, pinned, rhs, tol, max_iter):
                                ptr(pn, C.c_uint8), ptr(b, C.c_double), float(tol), int(max_iter),
                                ptr(x, C.c_double), C.byref(it)))
     return x, it.value
+
+
+def vertex_normals(positions, triangles, weld_pairs=None) -> np.ndarray:
+    """vertexNormals (normals.cpp:5-24) in FP32, sequential: unit normal per vertex."""
+    x = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+    w = np.ascontiguousarray(weld_pairs if weld_pairs is not None else np.zeros((0, 2)), np.uint32).reshape(-1, 2)
+    out = np.empty_like(x)
+    check(lib().oracle_vertex_normals(ptr(x, C.c_float), len(x), ptr(t, C.c_uint32), len(t), ptr(w, C.c_uint32),
+                                      len(w), ptr(out, C.c_float)))
+    return out
 
 
 def bsr_pcg_contact(row_start, col_index, blocks, pinned, active, normals, rhs, tol, max_iter):
