@@ -14,9 +14,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 BUILD = os.path.join(HERE, "build")
-SOURCES = ["csrc/kernels.hip", "csrc/warmstart.hip", "csrc/normals.hip", "csrc/solver.cpp", "csrc/model.cpp", "csrc/capi.cpp"]
+SOURCES = ["csrc/kernels.hip", "csrc/warmstart.hip", "csrc/normals.hip", "csrc/bake.hip", "csrc/solver.cpp", "csrc/model.cpp", "csrc/capi.cpp"]
 HEADERS = ["csrc/layout.hpp", "csrc/kernels.hpp", "csrc/term_math.hpp", "csrc/model.hpp", "csrc/solver.hpp",
-           "csrc/device_buffer.hpp", "csrc/warmstart.hpp", "csrc/normals.hpp",
+           "csrc/device_buffer.hpp", "csrc/warmstart.hpp", "csrc/normals.hpp", "csrc/bake.hpp",
            os.path.join(REPO, "include", "drape_hip.h")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-x", "hip", "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",

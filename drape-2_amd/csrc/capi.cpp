@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "drape_hip.h"
+#include "bake.hpp"
 #include "kernels.hpp"
 #include "normals.hpp"
 #include "solver.hpp"
@@ -424,6 +425,56 @@ int drape_solver_debug_system(drape_solver* s, float* blocks, float* grad, float
   return guard([&] {
     need(s, "solver");
     s->impl.debugSystem(blocks, grad, precond, contact_flags, contact_normals);
+  });
+}
+
+namespace {
+std::vector<bake::Component> bakeComponents(const drape_body_mesh* components, uint32_t n) {
+  need(components, "components");
+  std::vector<bake::Component> c(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    need(components[i].vertices, "vertices");
+    need(components[i].triangles, "triangles");
+    for (size_t k = 0; k < 3 * size_t(components[i].n_triangles); ++k)
+      if (components[i].triangles[k] >= components[i].n_vertices)
+        throw std::out_of_range("bake: triangle vertex out of range");
+    c[i] = bake::Component{components[i].vertices, components[i].n_vertices, components[i].triangles,
+                           components[i].n_triangles};
+  }
+  return c;
+}
+bake::Options bakeOptions(const drape_bake_options* o) {
+  bake::Options b;
+  if (o) b = bake::Options{o->cell_size, o->padding, o->band};
+  return b;
+}
+}  // namespace
+
+int drape_bake_field_grid(const drape_body_mesh* components, uint32_t n, const drape_bake_options* options,
+                          drape_field_desc* out) {
+  return guard([&] {
+    need(out, "output");
+    const auto c = bakeComponents(components, n);
+    const bake::Options o = bakeOptions(options);
+    const bake::Grid g = bake::gridOf(c.data(), n, o);
+    *out = drape_field_desc{};
+    for (int a = 0; a < 3; ++a) {
+      out->origin[a] = g.origin[a];
+      out->dims[a] = g.dims[a];
+    }
+    out->cell_size = o.cellSize;
+    out->band = o.band;
+    out->values = nullptr;
+  });
+}
+
+int drape_bake_collision_field(const drape_body_mesh* components, uint32_t n, const drape_bake_options* options,
+                               float* values) {
+  return guard([&] {
+    need(values, "values");
+    const auto c = bakeComponents(components, n);
+    Stream st;
+    bake::bakeField(c.data(), n, bakeOptions(options), values, st.s);
   });
 }
 

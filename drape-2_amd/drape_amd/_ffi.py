@@ -79,6 +79,15 @@ class GarmentDesc(C.Structure):
     ]
 
 
+class BakeOptions(C.Structure):
+    _fields_ = [("cell_size", C.c_double), ("padding", C.c_double), ("band", C.c_double)]
+
+
+class BodyMesh(C.Structure):
+    _fields_ = [("vertices", C.POINTER(C.c_double)), ("n_vertices", C.c_uint32),
+                ("triangles", C.POINTER(C.c_uint32)), ("n_triangles", C.c_uint32)]
+
+
 class FieldDesc(C.Structure):
     _fields_ = [
         ("origin", C.c_double * 3),
@@ -277,6 +286,10 @@ def _declare(lib):
         "drape_solver_stats": (C.c_int, [S, C.POINTER(Stats)]),
         "drape_solver_segment_stats": (C.c_int, [S, C.c_uint32, C.c_uint32, C.POINTER(Stats)]),
         "drape_solver_read_positions_normals": (C.c_int, [S, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t]),
+        "drape_bake_field_grid": (C.c_int, [C.POINTER(BodyMesh), C.c_uint32, C.POINTER(BakeOptions),
+                                            C.POINTER(FieldDesc)]),
+        "drape_bake_collision_field": (C.c_int, [C.POINTER(BodyMesh), C.c_uint32, C.POINTER(BakeOptions),
+                                                 C.POINTER(C.c_float)]),
         "drape_vertex_normals": (C.c_int, [C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32,
                                            C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(C.c_float)]),
         "drape_solver_snapshot_size": (C.c_int, [S, C.POINTER(C.c_size_t)]),

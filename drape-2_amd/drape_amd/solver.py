@@ -285,6 +285,34 @@ def make_solver(backend: str = "hip", **kw):
 
 # ---- standalone kernels ----
 
+def body_meshes(components) -> tuple:
+    """ctypes BodyMesh array for [(vertices [n, 3], triangles [t, 3]), ...] and the arrays it points into."""
+    keep = []
+    arr = (_ffi.BodyMesh * max(len(components), 1))()
+    for i, (v, t) in enumerate(components):
+        v = np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(t, np.uint32).reshape(-1, 3)
+        keep += [v, t]
+        arr[i] = _ffi.BodyMesh(ptr(v, C.c_double), len(v), ptr(t, C.c_uint32), len(t))
+    return arr, keep
+
+
+def bake_collision_field(components, cell_size=0.004, padding=0.05, band=0.02):
+    """bakeCollisionField on the device (collision_bake.cpp:98-208): the signed distance field of the union of
+    closed triangle meshes [(vertices, triangles), ...] as a scenes.Field."""
+    from .scenes import Field
+    arr, keep = body_meshes(components)
+    opt = _ffi.BakeOptions(cell_size, padding, band)
+    desc = _ffi.FieldDesc()
+    _check(_lib().drape_bake_field_grid(arr, len(components), C.byref(opt), C.byref(desc)))
+    nx, ny, nz = desc.dims
+    values = np.empty((nz, ny, nx), np.float32)
+    _check(_lib().drape_bake_collision_field(arr, len(components), C.byref(opt), ptr(values, C.c_float)))
+    del keep
+    return Field(origin=tuple(desc.origin), cell_size=desc.cell_size, dims=(nx, ny, nz), band=desc.band,
+                 values=values)
+
+
 def vertex_normals(positions, triangles, weld_pairs=None) -> np.ndarray:
     """Unit vertex normals on the device (vertexNormals, libs/render/src/normals.cpp:5-24): area-weighted sums of
     the triangles' cross products, weld pairs summed in order, normalised; (0, 1, 0) without area."""

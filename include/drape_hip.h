@@ -253,6 +253,35 @@ int drape_solver_debug_spmv(drape_solver* s, const float* x, float* y);
 int drape_solver_debug_system(drape_solver* s, float* blocks, float* grad, float* precond, uint32_t* contact_flags,
                               float* contact_normals);
 
+/* BakeOptions (libs/avatar/include/drape/avatar/collision_bake.hpp:10-14). */
+typedef struct drape_bake_options {
+  double cell_size; /* 0.004 m */
+  double padding;   /* 0.05 m */
+  double band;      /* 0.02 m */
+} drape_bake_options;
+
+/* TriangleMesh (libs/core/include/drape/core/triangle_mesh.hpp:11-14): a closed surface, triangles
+ * counter-clockwise seen from outside. */
+typedef struct drape_body_mesh {
+  const double* vertices; /* 3 per vertex */
+  uint32_t n_vertices;
+  const uint32_t* triangles; /* 3 per triangle */
+  uint32_t n_triangles;
+} drape_body_mesh;
+
+/* bakeCollisionField (collision_bake.hpp:16-18, collision_bake.cpp:98-208): the grid of the field of the union
+ * of `n` closed components — origin, cell_size, dims and band of `out` (values left NULL). options NULL = the
+ * defaults. invalid_argument when a component is not a closed manifold (every undirected edge used by exactly two
+ * triangles, in opposite directions). */
+int drape_bake_field_grid(const drape_body_mesh* components, uint32_t n, const drape_bake_options* options,
+                          drape_field_desc* out);
+/* The field itself, baked on the device into `values` (dims[0] * dims[1] * dims[2] floats, x fastest): per
+ * component the signed distance within the band (closest triangle, sign from the angle-weighted pseudonormal of
+ * the closest feature), +-band beyond it (outside = connected to the grid boundary), clamped to +-band; the
+ * minimum over components. FP64 arithmetic, stored as float. */
+int drape_bake_collision_field(const drape_body_mesh* components, uint32_t n, const drape_bake_options* options,
+                               float* values);
+
 /* vertexNormals (libs/render/include/drape/render/normals.hpp:13-18; normals.cpp:5-24) on the device, on caller
  * arrays (host memory): positions 3 floats per vertex, triangles 3 indices each, weld_pairs 2 indices each (a
  * seam's two coincident particles). Each vertex sums the cross products (x1 - x0) x (x2 - x0) of its triangles

@@ -27,6 +27,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <limits>
 #include <map>
 #include <stdexcept>
@@ -1149,6 +1150,246 @@ void readPoints(const double* x, V3* out) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------------------------------------------
+// bakeCollisionField (collision_bake.cpp:98-208), restated sequentially in FP64.
+
+namespace bk {
+
+struct P3 {
+  double x, y, z;
+};
+inline P3 sub(P3 a, P3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline P3 add(P3 a, P3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline P3 scale(double s, P3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline double dotp(P3 a, P3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+inline P3 crossp(P3 a, P3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline P3 normalizedp(P3 a) {  // Eigen's normalized(): unchanged when the norm is zero
+  const double z = dotp(a, a);
+  if (!(z > 0.0)) return a;
+  const double nn = std::sqrt(z);
+  return {a.x / nn, a.y / nn, a.z / nn};
+}
+
+enum Feature { kFace = 0, kVa, kVb, kVc, kEab, kEbc, kEca };
+
+// Ericson RTCD 5.1.5 (collision_bake.cpp:19-55).
+P3 closestPointOnTriangle(P3 p, P3 a, P3 b, P3 c, int& f) {
+  const P3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+  const double d1 = dotp(ab, ap), d2 = dotp(ac, ap);
+  if (d1 <= 0 && d2 <= 0) {
+    f = kVa;
+    return a;
+  }
+  const P3 bp = sub(p, b);
+  const double d3 = dotp(ab, bp), d4 = dotp(ac, bp);
+  if (d3 >= 0 && d4 <= d3) {
+    f = kVb;
+    return b;
+  }
+  const double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+    f = kEab;
+    return add(a, scale(d1 / (d1 - d3), ab));
+  }
+  const P3 cp = sub(p, c);
+  const double d5 = dotp(ab, cp), d6 = dotp(ac, cp);
+  if (d6 >= 0 && d5 <= d6) {
+    f = kVc;
+    return c;
+  }
+  const double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+    f = kEca;
+    return add(a, scale(d2 / (d2 - d6), ac));
+  }
+  const double va = d3 * d6 - d5 * d4;
+  if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+    f = kEbc;
+    return add(b, scale((d4 - d3) / ((d4 - d3) + (d5 - d6)), sub(c, b)));
+  }
+  const double denom = 1.0 / (va + vb + vc);
+  f = kFace;
+  return add(add(a, scale(vb * denom, ab)), scale(vc * denom, ac));
+}
+
+struct Mesh {
+  std::vector<P3> v;
+  std::vector<std::array<uint32_t, 3>> t;
+};
+
+bool closedManifold(const Mesh& m) {
+  std::map<std::pair<uint32_t, uint32_t>, int> directed;
+  for (const auto& tri : m.t)
+    for (int k = 0; k < 3; ++k)
+      if (++directed[{tri[k], tri[(k + 1) % 3]}] > 1) return false;
+  for (const auto& [e, c] : directed)
+    if (!directed.count({e.second, e.first})) return false;
+  return !m.t.empty();
+}
+
+// Pseudonormals (collision_bake.cpp:57-95).
+struct Normals {
+  std::vector<P3> face, vertex;
+  std::map<std::pair<uint32_t, uint32_t>, P3> edge;
+  explicit Normals(const Mesh& m) : face(m.t.size()), vertex(m.v.size(), P3{0, 0, 0}) {
+    for (size_t t = 0; t < m.t.size(); ++t) {
+      const auto& tri = m.t[t];
+      face[t] = normalizedp(crossp(sub(m.v[tri[1]], m.v[tri[0]]), sub(m.v[tri[2]], m.v[tri[0]])));
+      for (int k = 0; k < 3; ++k) {
+        const P3 p = m.v[tri[k]];
+        const P3 e1 = normalizedp(sub(m.v[tri[(k + 1) % 3]], p));
+        const P3 e2 = normalizedp(sub(m.v[tri[(k + 2) % 3]], p));
+        vertex[tri[k]] = add(vertex[tri[k]], scale(std::acos(std::clamp(dotp(e1, e2), -1.0, 1.0)), face[t]));
+        auto key = std::minmax(tri[k], tri[(k + 1) % 3]);
+        auto it = edge.try_emplace(key, P3{0, 0, 0}).first;
+        it->second = add(it->second, face[t]);
+      }
+    }
+  }
+  P3 of(const Mesh& m, size_t t, int f) const {
+    const auto& tri = m.t[t];
+    switch (f) {
+      case kVa: return vertex[tri[0]];
+      case kVb: return vertex[tri[1]];
+      case kVc: return vertex[tri[2]];
+      case kEab: return edge.at(std::minmax(tri[0], tri[1]));
+      case kEbc: return edge.at(std::minmax(tri[1], tri[2]));
+      case kEca: return edge.at(std::minmax(tri[2], tri[0]));
+      default: return face[t];
+    }
+  }
+};
+
+std::vector<Mesh> meshes(const drape_body_mesh* c, uint32_t n) {
+  if (!c || n == 0) throw std::invalid_argument("bake: no components");
+  std::vector<Mesh> out(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    for (uint32_t k = 0; k < c[i].n_vertices; ++k)
+      out[i].v.push_back({c[i].vertices[3 * k], c[i].vertices[3 * k + 1], c[i].vertices[3 * k + 2]});
+    for (uint32_t k = 0; k < c[i].n_triangles; ++k) {
+      std::array<uint32_t, 3> tri{c[i].triangles[3 * k], c[i].triangles[3 * k + 1], c[i].triangles[3 * k + 2]};
+      for (uint32_t x : tri)
+        if (x >= c[i].n_vertices) throw std::out_of_range("bake: triangle vertex out of range");
+      out[i].t.push_back(tri);
+    }
+    if (!closedManifold(out[i]))
+      throw std::invalid_argument("collision component " + std::to_string(i) + " is not a closed manifold");
+  }
+  return out;
+}
+
+struct Grid {
+  P3 origin;
+  int dims[3];
+  double cell, band, padding;
+};
+
+Grid gridOf(const std::vector<Mesh>& ms, const drape_bake_options* o) {
+  Grid g{};
+  g.cell = o ? o->cell_size : 0.004;
+  g.padding = o ? o->padding : 0.05;
+  g.band = o ? o->band : 0.02;
+  const double inf = std::numeric_limits<double>::infinity();
+  double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+  for (const auto& m : ms)
+    for (const auto& p : m.v) {
+      const double c[3] = {p.x, p.y, p.z};
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], c[a]);
+        hi[a] = std::max(hi[a], c[a]);
+      }
+    }
+  double org[3];
+  for (int a = 0; a < 3; ++a) {
+    org[a] = lo[a] - g.padding;
+    g.dims[a] = static_cast<int>(std::ceil(((hi[a] - lo[a]) + 2 * g.padding) / g.cell)) + 1;
+  }
+  g.origin = {org[0], org[1], org[2]};
+  return g;
+}
+
+void bake(const std::vector<Mesh>& ms, const Grid& g, float* values) {
+  const size_t nx = g.dims[0], ny = g.dims[1], nz = g.dims[2], count = nx * ny * nz;
+  for (size_t i = 0; i < count; ++i) values[i] = static_cast<float>(g.band);
+  auto index = [&](size_t x, size_t y, size_t z) { return (z * ny + y) * nx + x; };
+  auto position = [&](size_t x, size_t y, size_t z) {
+    return P3{g.origin.x + g.cell * double(x), g.origin.y + g.cell * double(y), g.origin.z + g.cell * double(z)};
+  };
+  const double o[3] = {g.origin.x, g.origin.y, g.origin.z};
+  auto cellRange = [&](double minV, double maxV, int axis, size_t& i0, size_t& i1) {
+    i0 = static_cast<size_t>(std::max(0.0, std::ceil((minV - o[axis]) / g.cell)));
+    i1 = static_cast<size_t>(std::min<double>(g.dims[axis] - 1, std::floor((maxV - o[axis]) / g.cell)));
+  };
+  const double inf = std::numeric_limits<double>::infinity();
+  std::vector<double> dist2(count);
+  std::vector<int8_t> sign(count);
+  std::vector<uint8_t> outside(count);
+  for (const auto& m : ms) {
+    std::fill(dist2.begin(), dist2.end(), inf);
+    std::fill(sign.begin(), sign.end(), 0);
+    const Normals nrm(m);
+    for (size_t t = 0; t < m.t.size(); ++t) {
+      const P3 a = m.v[m.t[t][0]], b = m.v[m.t[t][1]], c = m.v[m.t[t][2]];
+      const P3 tmin{std::min(std::min(a.x, b.x), c.x), std::min(std::min(a.y, b.y), c.y), std::min(std::min(a.z, b.z), c.z)};
+      const P3 tmax{std::max(std::max(a.x, b.x), c.x), std::max(std::max(a.y, b.y), c.y), std::max(std::max(a.z, b.z), c.z)};
+      const double mn[3] = {tmin.x, tmin.y, tmin.z}, mx[3] = {tmax.x, tmax.y, tmax.z};
+      size_t r[3][2];
+      for (int axis = 0; axis < 3; ++axis) cellRange(mn[axis] - g.band, mx[axis] + g.band, axis, r[axis][0], r[axis][1]);
+      for (size_t z = r[2][0]; z <= r[2][1]; ++z)
+        for (size_t y = r[1][0]; y <= r[1][1]; ++y)
+          for (size_t x = r[0][0]; x <= r[0][1]; ++x) {
+            const size_t i = index(x, y, z);
+            const P3 p = position(x, y, z);
+            const P3 gap{std::max(std::max(tmin.x - p.x, p.x - tmax.x), 0.0), std::max(std::max(tmin.y - p.y, p.y - tmax.y), 0.0),
+                         std::max(std::max(tmin.z - p.z, p.z - tmax.z), 0.0)};
+            if (dotp(gap, gap) >= dist2[i]) continue;
+            int f;
+            const P3 q = closestPointOnTriangle(p, a, b, c, f);
+            const P3 pq = sub(p, q);
+            const double d2 = dotp(pq, pq);
+            if (d2 < dist2[i]) {
+              dist2[i] = d2;
+              sign[i] = dotp(pq, nrm.of(m, t, f)) >= 0 ? 1 : -1;
+            }
+          }
+    }
+    std::fill(outside.begin(), outside.end(), 0);
+    std::deque<size_t> queue;
+    auto seed = [&](size_t x, size_t y, size_t z) {
+      const size_t i = index(x, y, z);
+      if (!outside[i] && dist2[i] > g.band * g.band) {
+        outside[i] = 1;
+        queue.push_back(i);
+      }
+    };
+    for (size_t z = 0; z < nz; ++z)
+      for (size_t y = 0; y < ny; ++y)
+        for (size_t x = 0; x < nx; ++x)
+          if (x == 0 || y == 0 || z == 0 || x == nx - 1 || y == ny - 1 || z == nz - 1) seed(x, y, z);
+    while (!queue.empty()) {
+      const size_t i = queue.front();
+      queue.pop_front();
+      const size_t x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+      if (x > 0) seed(x - 1, y, z);
+      if (x + 1 < nx) seed(x + 1, y, z);
+      if (y > 0) seed(x, y - 1, z);
+      if (y + 1 < ny) seed(x, y + 1, z);
+      if (z > 0) seed(x, y, z - 1);
+      if (z + 1 < nz) seed(x, y, z + 1);
+    }
+    for (size_t i = 0; i < count; ++i) {
+      double v;
+      if (dist2[i] <= g.band * g.band)
+        v = sign[i] * std::sqrt(dist2[i]);
+      else
+        v = outside[i] ? g.band : -g.band;
+      values[i] = std::min(values[i], static_cast<float>(std::clamp(v, -g.band, g.band)));
+    }
+  }
+}
+
+}  // namespace bk
+
 extern "C" {
 
 const char* oracle_last_error(void) { return gError.c_str(); }
@@ -1672,6 +1913,29 @@ int oracle_bsr_pcg(uint32_t n, const uint32_t* row_start, const uint32_t* col_in
                    int32_t* iterations) {
   return oracle_bsr_pcg_contact(n, row_start, col_index, blocks, pinned, nullptr, nullptr, rhs, tol, max_iter, x,
                                 iterations, nullptr);
+}
+
+
+int oracle_bake_field_grid(const drape_body_mesh* components, uint32_t n, const drape_bake_options* options,
+                           drape_field_desc* out) {
+  return guard([&] {
+    const bk::Grid g = bk::gridOf(bk::meshes(components, n), options);
+    *out = drape_field_desc{};
+    out->origin[0] = g.origin.x;
+    out->origin[1] = g.origin.y;
+    out->origin[2] = g.origin.z;
+    for (int a = 0; a < 3; ++a) out->dims[a] = g.dims[a];
+    out->cell_size = g.cell;
+    out->band = g.band;
+  });
+}
+
+int oracle_bake_collision_field(const drape_body_mesh* components, uint32_t n, const drape_bake_options* options,
+                                float* values) {
+  return guard([&] {
+    const auto ms = bk::meshes(components, n);
+    bk::bake(ms, bk::gridOf(ms, options), values);
+  });
 }
 
 // vertexNormals (libs/render/src/normals.cpp:5-24). FP32 as in the reference's render path; each product and

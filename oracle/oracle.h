@@ -90,6 +90,15 @@ int oracle_bsr_pcg_contact(uint32_t n, const uint32_t* row_start, const uint32_t
                            const uint8_t* pinned, const uint8_t* active, const double* normals, const double* rhs,
                            double tol, int32_t max_iter, double* x, int32_t* iterations, double* contact_force);
 
+/* bakeCollisionField (libs/avatar/src/collision_bake.cpp:98-208), sequentially in FP64 as the reference: triangles
+ * in order with the bounding-box reject and strict-less updates, pseudonormal signs, breadth-first flood fill from
+ * the grid boundary, minimum over components. options NULL = defaults. oracle_bake_field_grid fills the grid of
+ * `out` (values NULL); oracle_bake_collision_field fills dims product floats. */
+int oracle_bake_field_grid(const drape_body_mesh* components, uint32_t n, const drape_bake_options* options,
+                           drape_field_desc* out);
+int oracle_bake_collision_field(const drape_body_mesh* components, uint32_t n, const drape_bake_options* options,
+                                float* values);
+
 /* vertexNormals (libs/render/src/normals.cpp:5-24), sequentially in FP32: per triangle in order add
  * c = (x1 - x0) x (x2 - x0) to its three vertices; per weld pair in order set both vertices to their sum; then
  * normalise each, (0, 1, 0) for a zero vector. */

@@ -19,7 +19,7 @@ REPO = os.path.dirname(HERE)
 sys.path.insert(0, os.path.join(REPO, "drape-2_amd"))
 
 from drape_amd import _ffi  # noqa: E402
-from drape_amd._ffi import (Counters, FieldDesc, SceneDesc, SceneHolder, Stats, StructureSizes, f32p, f64p,  # noqa: E402
+from drape_amd._ffi import (BakeOptions, BodyMesh, Counters, FieldDesc, SceneDesc, SceneHolder, Stats, StructureSizes, f32p, f64p,  # noqa: E402
                             ptr, u8p, u32p)
 
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
@@ -57,6 +57,9 @@ def lib():
         "oracle_solver_write_state": (C.c_int, [S, f32p, f32p, C.c_size_t]),
         "oracle_solver_stats": (C.c_int, [S, C.POINTER(Stats)]),
         "oracle_vertex_normals": (C.c_int, [f32p, C.c_uint32, u32p, C.c_uint32, u32p, C.c_uint32, f32p]),
+        "oracle_bake_field_grid": (C.c_int, [C.POINTER(BodyMesh), C.c_uint32, C.POINTER(BakeOptions),
+                                             C.POINTER(FieldDesc)]),
+        "oracle_bake_collision_field": (C.c_int, [C.POINTER(BodyMesh), C.c_uint32, C.POINTER(BakeOptions), f32p]),
         "oracle_solver_snapshot_size": (C.c_int, [S, C.POINTER(C.c_size_t)]),
         "oracle_solver_snapshot_save": (C.c_int, [S, C.c_void_p, C.c_size_t]),
         "oracle_solver_snapshot_restore": (C.c_int, [S, C.c_void_p, C.c_size_t]),
@@ -370,6 +373,22 @@ def bsr_pcg(row_start, col_index, blocks, pinned, rhs, tol, max_iter):
                                ptr(pn, C.c_uint8), ptr(b, C.c_double), float(tol), int(max_iter),
                                ptr(x, C.c_double), C.byref(it)))
     return x, it.value
+
+
+def bake_collision_field(components, cell_size=0.004, padding=0.05, band=0.02):
+    """bakeCollisionField (collision_bake.cpp:98-208), sequential FP64: a scenes.Field."""
+    from drape_amd.scenes import Field
+    from drape_amd.solver import body_meshes
+    arr, keep = body_meshes(components)
+    opt = BakeOptions(cell_size, padding, band)
+    desc = FieldDesc()
+    check(lib().oracle_bake_field_grid(arr, len(components), C.byref(opt), C.byref(desc)))
+    nx, ny, nz = desc.dims
+    values = np.empty((nz, ny, nx), np.float32)
+    check(lib().oracle_bake_collision_field(arr, len(components), C.byref(opt), ptr(values, C.c_float)))
+    del keep
+    return Field(origin=tuple(desc.origin), cell_size=desc.cell_size, dims=(nx, ny, nz), band=desc.band,
+                 values=values)
 
 
 def vertex_normals(positions, triangles, weld_pairs=None) -> np.ndarray:
