@@ -721,6 +721,7 @@ kPcgStep(View v, uint32_t q) {
   const uint32_t KL = ldUniform(v.lowK + slice);
   const uint32_t lbase = ldUniform(v.lowBase + slice);
   const uint32_t s = ldUniform(v.tileSeg + tile);
+  const uint32_t tileBegin = ldUniform(v.tileRange + 2u * tile), tileEnd = ldUniform(v.tileRange + 2u * tile + 1u);
   // A tile whose solve has ended (the segment's PCG converged, or its Newton loop finished) passes the flag on and
   // loads nothing. The flag is per tile, so it arrives with the slice headers: no extra round trip for the tiles
   // that iterate, and none of the slice's loads for a tile of a batch whose segment converged early.
@@ -728,6 +729,14 @@ kPcgStep(View v, uint32_t q) {
     if (threadIdx.x == 0) v.tileDone[o][tile] = 1u;
     return;
   }
+  // The previous kernel's partial sums first (the segment's tile range comes per tile, with the slice headers):
+  // the reduction over them then runs while this kernel's block values are still streaming in.
+  const double* __restrict__ prev = v.cgPart[o];
+  uint32_t ti = tileBegin + threadIdx.x;
+  double sum[kPcgParts];
+#pragma unroll
+  for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < tileEnd ? prev[kPcgParts * ti + n] : 0.0;
+  __builtin_amdgcn_sched_barrier(0);
   const F3* __restrict__ Zr = v.cgZr[o];
   const F3* __restrict__ T = v.cgT[o];
   const F3* __restrict__ Pp = v.cgP[o];
@@ -783,18 +792,10 @@ kPcgStep(View v, uint32_t q) {
   // other parity slot of kIter, and r0 when k == 0, where it does not read it).
   SegState* S = v.state + s;
   const SegParams* P = v.seg + s;
-  const uint32_t tileBegin = ldUniform(&P->tileBegin), tileEnd = ldUniform(&P->tileEnd);
   const uint32_t maxCg = ldUniform(&P->maxCg);
   const double cgTolerance = ldUniform(&P->cgTolerance);
   const uint32_t k = ldUniform(&S->kIter[q]);
   const double r0Prev = ldUniform(&S->r0);
-  // The previous kernel's partial sums.
-  const double* __restrict__ prev = v.cgPart[o];
-  uint32_t ti = tileBegin + threadIdx.x;
-  double sum[kPcgParts];
-#pragma unroll
-  for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < tileEnd ? prev[kPcgParts * ti + n] : 0.0;
-  // Round trip 2: gathered (zr, t, p) of the first batch.
   // Round trip 2: the transposed lower blocks, and the gathered (zr, t, p) of the lower and upper columns.
 #pragma unroll
   for (uint32_t j = 0; j < B; ++j) {

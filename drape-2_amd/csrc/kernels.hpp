@@ -54,6 +54,7 @@ struct View {
   float* clambda;
   // per tile / slice / segment
   const uint32_t* tileSeg;
+  const uint32_t* tileRange;  // per tile: its segment's first tile and end tile
   const SegParams* seg;
   SegState* state;
   double* partA;  // per tile

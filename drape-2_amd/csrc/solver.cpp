@@ -236,6 +236,12 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
     P.fieldBand = sg.body.band;
   }
   upload(tileSeg_, tileSeg);
+  std::vector<uint32_t> tileRange(2 * size_t(nTiles_), 0u);
+  for (uint32_t t = 0; t < nTiles_; ++t) {
+    tileRange[2 * t] = params[tileSeg[t]].tileBegin;
+    tileRange[2 * t + 1] = params[tileSeg[t]].tileEnd;
+  }
+  upload(tileRange_, tileRange);
   upload(segParams_, params);
   segState_.allocate(sizeof(dk::SegState) * std::max<size_t>(M.segments.size(), 1));
   if (hostStates_) (void)hipHostFree(hostStates_);
@@ -382,6 +388,7 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   v.cforce = cforce_.as<float>();
   v.clambda = clambda_.as<float>();
   v.tileSeg = tileSeg_.as<uint32_t>();
+  v.tileRange = tileRange_.as<uint32_t>();
   v.seg = segParams_.as<dk::SegParams>();
   v.state = segState_.as<dk::SegState>();
   v.partA = partA_.as<double>();

@@ -119,6 +119,7 @@ class HipSolver {
   mutable std::unique_ptr<nrm::Mesh> normalMesh_;  // built on the first normals request after a build
   mutable DeviceBuffer normalsOut_;
   ws::Net* net_ = nullptr;
+  DeviceBuffer tileRange_;
   DeviceBuffer tileSeg_, segParams_, segState_, partA_, partB_, partC_, partE_, partMax_;
   DeviceBuffer stretch_, hinges_, seams_, fabrics_, garments_;
   DeviceBuffer incCount_, incStart_, inc_, rowLen_, rowStart_, colIndex_, diagSlot_;
