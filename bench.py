@@ -141,8 +141,8 @@ def main() -> int:
                     help="skip the learned-vs-plain warm start comparison")
     ap.add_argument("--warm-start-weights", default=WEIGHTS)
     ap.add_argument("--pcg-shape", default="0x0",
-                    help="launch shape of the one-kernel PCG iteration, wavefronts x slots: 1x8, 1x4, 2x8, 4x8 "
-                         "(0x0 = tune during warm-up)")
+                    help="launch shape of the one-kernel PCG iteration, wavefronts x slots: 1x8, 1x4, 2x8, 4x8, "
+                         "or 1x8h / 1x4h (column window staged in LDS); 0x0 = tune during warm-up")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
@@ -155,7 +155,9 @@ def main() -> int:
         return scenes.c2_sheet(n=args.n)
 
     sc = make_scene()
-    shape = tuple(int(t) for t in args.pcg_shape.lower().split("x"))
+    spec = args.pcg_shape.lower()
+    halo = spec.endswith("h")
+    shape = tuple(int(t) for t in spec.rstrip("h").split("x")) + (halo,)
     s = HipSolver(graph=not args.no_graph, pcg_variant=args.pcg_variant, pcg_shape=shape)
     s.build(sc)
     s.set_spmv_sampling(args.sample_every)
@@ -193,7 +195,8 @@ def main() -> int:
             "dt": 1 / 60, "substeps": 1, "newton_max": 5, "pcg_tolerance": 1e-6,
             "solver": "plain warm start (u = h v)",
             "pcg_form": "one-kernel" if args.pcg_variant == 1 else "two-kernel",
-            "pcg_launch_shape": "%dx%d" % s.pcg_shape() if args.pcg_variant == 1 else None,
+            "pcg_launch_shape": "%dx%d%s" % (s.pcg_shape()[0], s.pcg_shape()[1], "h" if s.pcg_shape()[2] else "")
+                                if args.pcg_variant == 1 else None,
             "graph": not args.no_graph,
             "newton_per_step": c["newton_iterations"] / args.steps,
             "pcg_per_step": c["pcg_iterations"] / args.steps,
@@ -210,7 +213,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": (PCG_STEP_TRAFFIC_BYTES if args.pcg_variant == 1 and args.n == 256 and s.pcg_shape() == (1, 8)
+            "traffic": (PCG_STEP_TRAFFIC_BYTES if args.pcg_variant == 1 and args.n == 256 and s.pcg_shape() == (1, 8, False)
                         else None),
             "bytes_per_launch": timing["bytes_per_launch"],
             "mean_launch_us": timing["mean_us"],

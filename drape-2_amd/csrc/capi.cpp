@@ -339,22 +339,25 @@ int drape_solver_set_pcg_variant(drape_solver* s, int32_t variant) {
   });
 }
 
-int drape_solver_set_pcg_shape(drape_solver* s, int32_t waves, int32_t slots) {
+int drape_solver_set_pcg_shape(drape_solver* s, int32_t waves, int32_t slots, int32_t halo) {
   return guard([&] {
     need(s, "solver");
-    s->impl.setPcgShape(waves, slots);
+    s->impl.setPcgShape(waves, slots, halo != 0);
   });
 }
 
-int drape_solver_pcg_shape(const drape_solver* s, int32_t* waves, int32_t* slots) {
+int drape_solver_pcg_shape(const drape_solver* s, int32_t* waves, int32_t* slots, int32_t* halo) {
   return guard([&] {
     need(s, "solver");
     need(waves, "output");
     need(slots, "output");
+    need(halo, "output");
     int w = 0, k = 0;
-    s->impl.pcgShape(w, k);
+    bool h = false;
+    s->impl.pcgShape(w, k, h);
     *waves = w;
     *slots = k;
+    *halo = h ? 1 : 0;
   });
 }
 

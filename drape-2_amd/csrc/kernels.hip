@@ -701,13 +701,20 @@ __global__ void __launch_bounds__(256) kPcgStepInit(View v) {
 // the lower part and NB of the upper part (B = NB / W per wavefront). NB = 8 keeps a whole sheet row in one batch
 // (about 390 registers: one workgroup per CU); NB = 4 fits two workgroups per CU, which pays when a launch has
 // many more tiles than CUs.
-template <int W, int NB>
+//
+// H (halo, W = 1 only): when every tile's columns lie in one window of at most kHaloRows rows (a banded matrix such
+// as a grid sheet's), the tile loads that window's (zr, t, p) with coalesced loads in its first round trip, keeps
+// it in LDS, and gathers from there: the second round trip then carries only the transposed lower blocks.
+template <int W, int NB, bool H>
 __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(NB == 4 ? 2 : 1, 8)))
 kPcgStep(View v, uint32_t q) {
+  static_assert(!H || W == 1, "the halo form has one wavefront per slice");
   constexpr int NW = 4 * W;
   constexpr uint32_t B = NB / W;  // lower slots, and upper slots, per wavefront in one batch
+  constexpr uint32_t HR = H ? kHaloRows / 256u : 1u;  // halo rows staged per thread
   __shared__ double red[kPcgParts * NW];
   __shared__ float comb[(W > 1 ? 4 : 1) * kSlicesPerTile * 3 * kSlice];
+  __shared__ F3 hz[H ? kHaloRows : 1], ht[H ? kHaloRows : 1], hp[H ? kHaloRows : 1];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
   const uint32_t o = 1u - q;
@@ -722,6 +729,7 @@ kPcgStep(View v, uint32_t q) {
   const uint32_t lbase = ldUniform(v.lowBase + slice);
   const uint32_t s = ldUniform(v.tileSeg + tile);
   const uint32_t tileBegin = ldUniform(v.tileRange + 2u * tile), tileEnd = ldUniform(v.tileRange + 2u * tile + 1u);
+  const uint32_t hb = H ? ldUniform(v.tileHalo + 2u * tile) : 0u, hc = H ? ldUniform(v.tileHalo + 2u * tile + 1u) : 0u;
   // A tile whose solve has ended (the segment's PCG converged, or its Newton loop finished) passes the flag on and
   // loads nothing. The flag is per tile, so it arrives with the slice headers: no extra round trip for the tiles
   // that iterate, and none of the slice's loads for a tile of a batch whose segment converged early.
@@ -758,6 +766,21 @@ kPcgStep(View v, uint32_t q) {
   }
 #pragma unroll
   for (uint32_t j = 0; j < B; ++j) c[j] = col[(part + W * j) * kSlice];
+  // The halo window, coalesced, before the block values: the wait before it goes to LDS then covers the indices
+  // and the window only.
+  // (Threads past the window's end re-read its last row and store it past the end, where no gather looks: loads
+  // and stores stay unconditional, so the staging arrays live in registers.)
+  float sv[HR][9];
+  if constexpr (H) {
+#pragma unroll
+    for (uint32_t r = 0; r < HR; ++r) {
+      const uint32_t i = hb + min(r * 256u + threadIdx.x, hc - 1u);
+      const F3 z = Zr[i], tt = T[i], pp = Pp[i];
+      sv[r][0] = z.x, sv[r][1] = z.y, sv[r][2] = z.z;
+      sv[r][3] = tt.x, sv[r][4] = tt.y, sv[r][5] = tt.z;
+      sv[r][6] = pp.x, sv[r][7] = pp.y, sv[r][8] = pp.z;
+    }
+  }
   __builtin_amdgcn_sched_barrier(0);
   // The first batch's upper block values. Slots past K are read unconditionally — the value buffer carries
   // kSellSlack zero floats past its end — because their column is the dummy one, whose gathered vectors are zero,
@@ -776,9 +799,11 @@ kPcgStep(View v, uint32_t q) {
   if (part == 0) {
     oR = v.cgR[o][row];
     oAp = v.cgAp[o][row];
-    oZ = Zr[row];
-    oT = T[row];
-    oP = Pp[row];
+    if constexpr (!H) {
+      oZ = Zr[row];
+      oT = T[row];
+      oP = Pp[row];
+    }
     oStep = v.step[row];
 #pragma unroll
     for (int e = 0; e < 9; ++e) m[e] = v.precond[9u * row + e];
@@ -796,7 +821,22 @@ kPcgStep(View v, uint32_t q) {
   const double cgTolerance = ldUniform(&P->cgTolerance);
   const uint32_t k = ldUniform(&S->kIter[q]);
   const double r0Prev = ldUniform(&S->r0);
-  // Round trip 2: the transposed lower blocks, and the gathered (zr, t, p) of the lower and upper columns.
+  // (zr, t, p) of a column: from the halo window in LDS, or gathered from memory; the dummy column gives zero.
+  auto gather = [&](uint32_t cc, F3& z, F3& tt, F3& pp) {
+    if constexpr (H) {
+      const bool dummy = cc == v.nRows;
+      const uint32_t i = dummy ? 0u : cc - hb;
+      z = dummy ? f3(0, 0, 0) : hz[i];
+      tt = dummy ? f3(0, 0, 0) : ht[i];
+      pp = dummy ? f3(0, 0, 0) : hp[i];
+    } else {
+      z = Zr[cc];
+      tt = T[cc];
+      pp = Pp[cc];
+    }
+  };
+  // Round trip 2: the transposed lower blocks, and (without the halo) the gathered (zr, t, p) of the lower and
+  // upper columns.
 #pragma unroll
   for (uint32_t j = 0; j < B; ++j) {
     const uint32_t t = part + W * j;
@@ -809,18 +849,27 @@ kPcgStep(View v, uint32_t q) {
   for (uint32_t j = 0; j < B; ++j)
 #pragma unroll
     for (int e = 0; e < 9; ++e) bt[j][e] = v.sellVal[size_t(bl[j]) * 9u + e];  // 64-bit index: one wide load
-  F3 lz[B], lt[B], lp[B], gz[B], gt[B], gp[B];
+  // With the halo, the window goes to LDS only once the transposed blocks are in flight: its wait then overlaps
+  // their round trip.
+  if constexpr (H) {
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (uint32_t j = 0; j < B; ++j) {
-    lz[j] = Zr[cl[j]];
-    lt[j] = T[cl[j]];
-    lp[j] = Pp[cl[j]];
+    for (uint32_t r = 0; r < HR; ++r) {
+      const uint32_t i = r * 256u + threadIdx.x;
+      hz[i] = f3(sv[r][0], sv[r][1], sv[r][2]);
+      ht[i] = f3(sv[r][3], sv[r][4], sv[r][5]);
+      hp[i] = f3(sv[r][6], sv[r][7], sv[r][8]);
+    }
+    __syncthreads();
+    gather(row, oZ, oT, oP);
   }
+  constexpr uint32_t GB = H ? 1u : B;  // gathered values held in registers (none with the halo)
+  F3 lz[GB], lt[GB], lp[GB], gz[GB], gt[GB], gp[GB];
+  if constexpr (!H) {
 #pragma unroll
-  for (uint32_t j = 0; j < B; ++j) {
-    gz[j] = Zr[c[j]];
-    gt[j] = T[c[j]];
-    gp[j] = Pp[c[j]];
+    for (uint32_t j = 0; j < B; ++j) gather(cl[j], lz[j], lt[j], lp[j]);
+#pragma unroll
+    for (uint32_t j = 0; j < B; ++j) gather(c[j], gz[j], gt[j], gp[j]);
   }
 
   // Scalars of this kernel (identical in every workgroup of the segment), while the gathers are in flight.
@@ -871,8 +920,12 @@ kPcgStep(View v, uint32_t q) {
 #pragma unroll
   for (uint32_t lg = 0; lg < G; ++lg) acc[lg][0] = acc[lg][1] = acc[lg][2] = 0.0f;
 #pragma unroll
-  for (uint32_t j = 0; j < B; ++j)
-    blockTMulAdd(bt[j], pcgDirection(lz[j], lt[j], lp[j], a, b), acc[j % G][0], acc[j % G][1], acc[j % G][2]);
+  for (uint32_t j = 0; j < B; ++j) {
+    F3 z, tt, pp;
+    if constexpr (H) gather(cl[j], z, tt, pp);
+    else { z = lz[j]; tt = lt[j]; pp = lp[j]; }
+    blockTMulAdd(bt[j], pcgDirection(z, tt, pp, a, b), acc[j % G][0], acc[j % G][1], acc[j % G][2]);
+  }
   for (uint32_t j0 = B; part + W * j0 < KL; j0 += B) {  // rows with more than NB lower slots
 #pragma unroll
     for (uint32_t j = 0; j < B; ++j) {
@@ -883,20 +936,28 @@ kPcgStep(View v, uint32_t q) {
 #pragma unroll
     for (uint32_t j = 0; j < B; ++j) {
       const uint32_t t = part + W * (j0 + j);
-      const F3 x = pcgDirection(Zr[cl[j]], T[cl[j]], Pp[cl[j]], a, b);
+      F3 z, tt, pp;
+      gather(cl[j], z, tt, pp);
+      const F3 x = pcgDirection(z, tt, pp, a, b);
       if (t < KL) blockTMulAdd(v.sellVal + size_t(bl[j]) * 9u, x, acc[j % G][0], acc[j % G][1], acc[j % G][2]);
     }
   }
 #pragma unroll
-  for (uint32_t j = 0; j < B; ++j)
-    blockMulAdd(bv[j], pcgDirection(gz[j], gt[j], gp[j], a, b), acc[j % G][0], acc[j % G][1], acc[j % G][2]);
+  for (uint32_t j = 0; j < B; ++j) {
+    F3 z, tt, pp;
+    if constexpr (H) gather(c[j], z, tt, pp);
+    else { z = gz[j]; tt = gt[j]; pp = gp[j]; }
+    blockMulAdd(bv[j], pcgDirection(z, tt, pp, a, b), acc[j % G][0], acc[j % G][1], acc[j % G][2]);
+  }
   for (uint32_t j0 = B; part + W * j0 < K; j0 += B) {  // rows with more than NB upper slots
 #pragma unroll
     for (uint32_t j = 0; j < B; ++j) c[j] = sellColumn(col, part + W * (j0 + j), K, v.nRows);
 #pragma unroll
     for (uint32_t j = 0; j < B; ++j) {
       const uint32_t t = part + W * (j0 + j);
-      const F3 x = pcgDirection(Zr[c[j]], T[c[j]], Pp[c[j]], a, b);
+      F3 z, tt, pp;
+      gather(c[j], z, tt, pp);
+      const F3 x = pcgDirection(z, tt, pp, a, b);
       if (t < K) blockMulAdd(val + t * 9u * kSlice, x, acc[j % G][0], acc[j % G][1], acc[j % G][2]);
     }
   }
@@ -1065,14 +1126,16 @@ __global__ void __launch_bounds__(256) kLineSearchDecide(View v) {
 }
 
 // u += alpha step; max |move|; friction for the next Newton iteration uses this solve's normal forces
-// (kernels.h:407-414; cpu_solver.cpp:275-282).
+// (kernels.h:407-414; cpu_solver.cpp:275-282). Runs once per Newton iteration, for a segment whose line search
+// has decided (lsDone == 1) and not yet been applied (kNewtonEnd then sets lsDone = 2): a captured sequence can
+// launch it behind a fixed number of trials and launch it again after further trials.
 __global__ void __launch_bounds__(256) kApply(View v) {
   __shared__ float lds[8];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
   const uint32_t s = v.tileSeg[tile];
   const SegState* S = v.state + s;
-  if (S->newtonDone) return;
+  if (S->newtonDone || S->lsDone != 1u) return;
   const float alpha = float(S->lsAlpha);
   const uint32_t row = tile * kTile + threadIdx.x;
   const F3 d = alpha * v.step[row];
@@ -1083,16 +1146,17 @@ __global__ void __launch_bounds__(256) kApply(View v) {
   if (threadIdx.x == 0) v.partMax[tile] = m;
 }
 
-// Stops the Newton loop once the largest move is below 1e-9 m (cpu_solver.cpp:283).
+// Stops the Newton loop once the largest move is below 1e-9 m (cpu_solver.cpp:283); marks the step applied.
 __global__ void __launch_bounds__(256) kNewtonEnd(View v) {
   __shared__ float lds[8];
   const uint32_t s = blockIdx.x;
   SegState* S = v.state + s;
-  if (S->newtonDone) return;
+  if (S->newtonDone || S->lsDone != 1u) return;
   const SegParams& P = v.seg[s];
   const float m = rangeMax256(v.partMax, P.tileBegin, P.tileEnd, lds);
   if (threadIdx.x == 0) {
     S->maxMove = m;
+    S->lsDone = 2u;
     if (double(m) < 1e-9) S->newtonDone = 1u;
   }
 }
@@ -1714,16 +1778,21 @@ void launchPcgStepInit(const View& v, hipStream_t st) { kPcgStepInit<<<xcdGrid(v
 void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t st, hipEvent_t start, hipEvent_t stop) {
   const dim3 grid(xcdGrid(v.nTiles));
   const uint32_t q = parity & 1u;
-  if (shape.waves == 1 && shape.slots == 8)
-    hipExtLaunchKernelGGL((kPcgStep<1, 8>), grid, dim3(256), 0, st, start, stop, 0, v, q);
-  else if (shape.waves == 1 && shape.slots == 4)
-    hipExtLaunchKernelGGL((kPcgStep<1, 4>), grid, dim3(256), 0, st, start, stop, 0, v, q);
-  else if (shape.waves == 2 && shape.slots == 8)
-    hipExtLaunchKernelGGL((kPcgStep<2, 8>), grid, dim3(512), 0, st, start, stop, 0, v, q);
-  else if (shape.waves == 4 && shape.slots == 8)
-    hipExtLaunchKernelGGL((kPcgStep<4, 8>), grid, dim3(1024), 0, st, start, stop, 0, v, q);
+  if (shape.halo && !v.tileHalo) throw std::invalid_argument("PCG halo shape: a tile's columns exceed the window");
+  if (shape.waves == 1 && shape.slots == 8 && !shape.halo)
+    hipExtLaunchKernelGGL((kPcgStep<1, 8, false>), grid, dim3(256), 0, st, start, stop, 0, v, q);
+  else if (shape.waves == 1 && shape.slots == 4 && !shape.halo)
+    hipExtLaunchKernelGGL((kPcgStep<1, 4, false>), grid, dim3(256), 0, st, start, stop, 0, v, q);
+  else if (shape.waves == 2 && shape.slots == 8 && !shape.halo)
+    hipExtLaunchKernelGGL((kPcgStep<2, 8, false>), grid, dim3(512), 0, st, start, stop, 0, v, q);
+  else if (shape.waves == 4 && shape.slots == 8 && !shape.halo)
+    hipExtLaunchKernelGGL((kPcgStep<4, 8, false>), grid, dim3(1024), 0, st, start, stop, 0, v, q);
+  else if (shape.waves == 1 && shape.slots == 8 && shape.halo)
+    hipExtLaunchKernelGGL((kPcgStep<1, 8, true>), grid, dim3(256), 0, st, start, stop, 0, v, q);
+  else if (shape.waves == 1 && shape.slots == 4 && shape.halo)
+    hipExtLaunchKernelGGL((kPcgStep<1, 4, true>), grid, dim3(256), 0, st, start, stop, 0, v, q);
   else
-    throw std::invalid_argument("PCG launch shape must be 1x8, 1x4, 2x8 or 4x8 (wavefronts per slice x slots)");
+    throw std::invalid_argument("PCG launch shape must be 1x8, 1x4, 2x8 or 4x8, or 1x8 / 1x4 with the halo");
 }
 void launchContactForceSlope(const View& v, hipStream_t st) {
   kContactForceSlope<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);

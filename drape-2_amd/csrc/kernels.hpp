@@ -55,6 +55,7 @@ struct View {
   // per tile / slice / segment
   const uint32_t* tileSeg;
   const uint32_t* tileRange;  // per tile: its segment's first tile and end tile
+  const uint32_t* tileHalo;   // per tile: first row and row count of the window holding all its columns
   const SegParams* seg;
   SegState* state;
   double* partA;  // per tile
@@ -133,13 +134,16 @@ void launchPcgStepInit(const View& v, hipStream_t st);
 // the non-finite flag (1.0), the maximum body penetration and the maximum seam gap. tilePart: 6 per tile.
 constexpr uint32_t kStatsPerSegment = 6;
 void launchStats(const View& v, const double* massF64, double* tilePart, double* segOut, hipStream_t st);
-// Launch shape of the one-kernel PCG iteration: `waves` wavefronts share each 64-row slice, and each batch covers
-// `slots` lower and `slots` upper slots. Supported: 1x8, 1x4, 2x8, 4x8. Every shape gives bit-identical results.
+// Launch shape of the one-kernel PCG iteration: `waves` wavefronts share each 64-row slice, each batch covers
+// `slots` lower and `slots` upper slots, and `halo` stages the tile's column window in LDS. Supported: 1x8, 1x4,
+// 2x8, 4x8, and 1x8 / 1x4 with the halo. Every shape gives bit-identical results.
 struct PcgShape {
   uint32_t waves = 1, slots = 8;
-  bool operator==(const PcgShape& o) const { return waves == o.waves && slots == o.slots; }
+  bool halo = false;  // gathers from an LDS window of the tile's columns (every tile's window <= kHaloRows rows)
+  bool operator==(const PcgShape& o) const { return waves == o.waves && slots == o.slots && halo == o.halo; }
 };
-constexpr PcgShape kPcgShapes[4] = {{1, 8}, {1, 4}, {2, 8}, {4, 8}};
+constexpr PcgShape kPcgShapes[6] = {{1, 8, false}, {1, 4, false}, {2, 8, false},
+                                    {4, 8, false}, {1, 8, true},  {1, 4, true}};
 // One PCG iteration of the one-reduction form.
 void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t st, hipEvent_t start = nullptr,
                    hipEvent_t stop = nullptr);

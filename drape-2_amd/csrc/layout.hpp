@@ -42,6 +42,9 @@ constexpr uint32_t kSellSlack = 16u * 9u * 64u;
 // Entries after the last slice's upper columns, lower columns and lower block indices: a kernel may load the
 // indices of 16 slots past any slice's K (and discards them).
 constexpr uint32_t kSellColSlack = 16u * 64u;
+// Largest window of rows whose (zr, t, p) a tile stages in LDS for the halo form of the PCG iteration (36 B a row:
+// 54 KB; 6 rows per thread of a 256-thread workgroup).
+constexpr uint32_t kHaloRows = 1536u;
 
 // Row flags.
 constexpr uint32_t kRowPinned = 1u;
@@ -133,7 +136,7 @@ struct SegState {
   uint32_t pcgConverged;  // 1 once |r| <= tol |r0| or pAp <= 0 (read by the host; kPcgStep reads per-tile flags)
   uint32_t pcgIter;
   uint32_t newtonDone;    // 1 once slope >= 0 or max move < 1e-9 (no more Newton iterations this substep)
-  uint32_t lsDone;        // 1 once the line search accepted a step this Newton iteration
+  uint32_t lsDone;        // 1 once the line search decided this Newton iteration, 2 once the step is applied
   uint32_t lsTrial;
   uint32_t active;        // 1 while this segment runs Newton iterations this substep
   uint32_t warmRejected;  // 1 when the learned warm start was rejected this substep

@@ -338,6 +338,30 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   maxK_ = bs.maxK;
   if (maxK_ * 9ull * dk::kSlice * sizeof(float) > 64ull * 1024)
     throw std::invalid_argument("hip: a 64-row slice has more than 28 blocks on or above the diagonal in one row");
+  // Per tile, the window of rows [first, first + count) that holds every column of its rows and the rows
+  // themselves; the halo launch shapes run only when every window fits kHaloRows (a banded matrix).
+  bool haloFits = true;
+  {
+    std::vector<uint32_t> rs(nRows_ + 1), ci(std::max<uint32_t>(nBlocks_, 1));
+    fromDevice(rs.data(), rowStart_.as<void>(), sizeof(uint32_t) * (nRows_ + 1));
+    fromDevice(ci.data(), colIndex_.as<void>(), sizeof(uint32_t) * nBlocks_);
+    std::vector<uint32_t> halo(2 * size_t(nTiles_), 0u);
+    for (uint32_t t = 0; t < nTiles_; ++t) {
+      uint32_t lo = t * dk::kTile, hi = (t + 1) * dk::kTile - 1;
+      for (uint32_t r = t * dk::kTile; r < (t + 1) * dk::kTile; ++r)
+        for (uint32_t k = rs[r]; k < rs[r + 1]; ++k) {
+          lo = std::min(lo, ci[k]);
+          hi = std::max(hi, ci[k]);
+        }
+      halo[2 * t] = lo;
+      halo[2 * t + 1] = hi - lo + 1;
+      haloFits = haloFits && hi - lo + 1 <= dk::kHaloRows;
+    }
+    upload(tileHalo_, halo);
+  }
+  tunedShape_ = -1;
+  std::fill(tuneMs_, tuneMs_ + kShapeCount, 0.0);
+  std::fill(tuneLaunches_, tuneLaunches_ + kShapeCount, 0u);
   sellVal_.allocate(sizeof(float) * (sellFloats_ + dk::kSellSlack));  // zero slack: see kSellSlack
 
   dk::View& v = view_;
@@ -389,6 +413,7 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   v.clambda = clambda_.as<float>();
   v.tileSeg = tileSeg_.as<uint32_t>();
   v.tileRange = tileRange_.as<uint32_t>();
+  v.tileHalo = haloFits ? tileHalo_.as<uint32_t>() : nullptr;
   v.seg = segParams_.as<dk::SegParams>();
   v.state = segState_.as<dk::SegState>();
   v.partA = partA_.as<double>();
@@ -571,11 +596,31 @@ bool HipSolver::allSegments(bool (*pred)(const dk::SegState&)) const {
   return true;
 }
 
+void HipSolver::Captured::release() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+}
+
+template <class F>
+void HipSolver::launchCaptured(Captured& c, F&& launches) {
+  if (!c.exec) {
+    check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
+    launches();
+    check(hipStreamEndCapture(stream_, &c.graph), "capture");
+    check(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0), "graph instantiate");
+  }
+  check(hipGraphLaunch(c.exec, stream_), "graph launch");
+}
+
 void HipSolver::releaseGraph() {
   if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
   if (graph_) (void)hipGraphDestroy(graph_);
   graphExec_ = nullptr;
   graph_ = nullptr;
+  newtonPre_.release();
+  newtonPost_.release();
 }
 
 void HipSolver::setPcgVariant(int variant) {
@@ -583,23 +628,26 @@ void HipSolver::setPcgVariant(int variant) {
   pcgVariant_ = variant;
 }
 
-void HipSolver::setPcgShape(int waves, int slots) {
-  if (waves == 0 && slots == 0) {
+void HipSolver::setPcgShape(int waves, int slots, bool halo) {
+  if (waves == 0 && slots == 0 && !halo) {
     pcgShape_ = -1;
     return;
   }
   for (int i = 0; i < kShapeCount; ++i)
-    if (int(dk::kPcgShapes[i].waves) == waves && int(dk::kPcgShapes[i].slots) == slots) {
+    if (int(dk::kPcgShapes[i].waves) == waves && int(dk::kPcgShapes[i].slots) == slots &&
+        dk::kPcgShapes[i].halo == halo) {
       pcgShape_ = i;
       return;
     }
-  throw std::invalid_argument("PCG launch shape must be 0x0 (tune), 1x8, 1x4, 2x8 or 4x8 (wavefronts x slots)");
+  throw std::invalid_argument(
+      "PCG launch shape must be 0x0 (tune), 1x8, 1x4, 2x8 or 4x8 (wavefronts x slots), or 1x8 / 1x4 with the halo");
 }
 
-void HipSolver::pcgShape(int& waves, int& slots) const {
+void HipSolver::pcgShape(int& waves, int& slots, bool& halo) const {
   const int i = shapeIndex();
   waves = i < 0 ? 0 : int(dk::kPcgShapes[i].waves);
   slots = i < 0 ? 0 : int(dk::kPcgShapes[i].slots);
+  halo = i >= 0 && dk::kPcgShapes[i].halo;
 }
 
 void HipSolver::runPcg() {
@@ -661,7 +709,8 @@ void HipSolver::timedPcgStep(uint32_t parity, dk::PcgShape shape) {
 }
 
 // Solves run before a launch shape is chosen: chunks of kGraphChunk launches rotate over the shapes of
-// dk::kPcgShapes (the one with the fewest samples goes next), each bracketed by HIP events and followed by a poll.
+// dk::kPcgShapes that the matrix admits (the halo shapes need every tile's window to fit; the shape with the fewest
+// samples goes next), each bracketed by HIP events and followed by a poll.
 // A chunk counts only if the solve was still running at its end, so every timed launch did a full iteration. Once
 // every shape has four such chunks the fastest mean is kept; the shape never changes the results.
 void HipSolver::runPcgTuning(uint32_t maxCg) {
@@ -669,7 +718,7 @@ void HipSolver::runPcgTuning(uint32_t maxCg) {
   while (launched < maxCg + 1) {
     int o = 0;
     for (int i = 1; i < kShapeCount; ++i)
-      if (tuneLaunches_[i] < tuneLaunches_[o]) o = i;
+      if (shapeAvailable(i) && tuneLaunches_[i] < tuneLaunches_[o]) o = i;
     const dk::PcgShape shape = dk::kPcgShapes[tunedShape_ >= 0 ? tunedShape_ : o];
     const auto pr = takeEvents();
     check(hipEventRecord(pr.first, stream_), "event");
@@ -683,10 +732,12 @@ void HipSolver::runPcgTuning(uint32_t maxCg) {
       check(hipEventElapsedTime(&ms, pr.first, pr.second), "event time");
       tuneMs_[o] += ms;
       tuneLaunches_[o] += kGraphChunk;
-      if (*std::min_element(tuneLaunches_, tuneLaunches_ + kShapeCount) >= 4u * kGraphChunk) {
+      bool sampled = true;
+      for (int i = 0; i < kShapeCount; ++i) sampled = sampled && (!shapeAvailable(i) || tuneLaunches_[i] >= 4u * kGraphChunk);
+      if (sampled) {
         int best = 0;
         for (int i = 1; i < kShapeCount; ++i)
-          if (tuneMs_[i] / tuneLaunches_[i] < tuneMs_[best] / tuneLaunches_[best]) best = i;
+          if (shapeAvailable(i) && tuneMs_[i] / tuneLaunches_[i] < tuneMs_[best] / tuneLaunches_[best]) best = i;
         tunedShape_ = best;
       }
     }
@@ -703,9 +754,11 @@ void HipSolver::runPcgTuning(uint32_t maxCg) {
 // once. The host reads the converged flags after the first batch (sized from the previous solve) and then after
 // every chunk.
 void HipSolver::runPcgSingleKernel() {
+  const int shapeIdx = shapeIndex();
+  if (shapeIdx >= 0 && !shapeAvailable(shapeIdx))
+    throw std::invalid_argument("PCG halo shape: a 256-row tile of this matrix reaches more than 1536 rows");
   dk::launchPcgStepInit(view_, stream_);
   const uint32_t maxCg = maxCgOf(model_);
-  const int shapeIdx = shapeIndex();
   if (shapeIdx < 0) {
     runPcgTuning(maxCg);
     return;
@@ -761,22 +814,51 @@ void HipSolver::substep(double h, int iterations) {
   }
   dk::launchPredictContact(view_, stream_);
   if (net_) dk::launchWarmCheck(view_, stream_);
+  // Each Newton iteration: the pre-solve launches, the PCG solve (chunks, polled), then the post-solve launches.
+  // With graphs, the post-solve part runs the first kGraphTrials line-search trials, the apply and the end test
+  // in one replay and the host polls once; a line search that is still undecided then continues eagerly, one poll
+  // per trial, and its apply and end test run after it (they act only on a decided, not yet applied segment).
+  const auto lsDecided = [](const dk::SegState& s) { return s.newtonDone != 0u || s.lsDone != 0u; };
   for (int it = 0; it < iterations; ++it) {
-    dk::launchActivate(view_, stream_);
-    dk::launchAssemble(view_, maxK_, true, stream_);
-    dk::launchNewtonBegin(view_, stream_);
+    const auto pre = [&] {
+      dk::launchActivate(view_, stream_);
+      dk::launchAssemble(view_, maxK_, true, stream_);
+      dk::launchNewtonBegin(view_, stream_);
+    };
+    if (useGraph_)
+      launchCaptured(newtonPre_, pre);
+    else
+      pre();
     runPcg();
-    dk::launchContactForceSlope(view_, stream_);
-    dk::launchNewtonSlope(view_, stream_);
-    for (int ls = 0; ls < 20; ++ls) {
-      dk::launchEnergyTrial(view_, stream_);
-      dk::launchLineSearchDecide(view_, stream_);
+    int trials = 0;
+    if (useGraph_) {
+      launchCaptured(newtonPost_, [&] {
+        dk::launchContactForceSlope(view_, stream_);
+        dk::launchNewtonSlope(view_, stream_);
+        for (int ls = 0; ls < kGraphTrials; ++ls) {
+          dk::launchEnergyTrial(view_, stream_);
+          dk::launchLineSearchDecide(view_, stream_);
+        }
+        dk::launchApply(view_, stream_);
+        dk::launchNewtonEnd(view_, stream_);
+      });
+      trials = kGraphTrials;
       pollStates();
-      if (allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.lsDone != 0u; })) break;
+    } else {
+      dk::launchContactForceSlope(view_, stream_);
+      dk::launchNewtonSlope(view_, stream_);
     }
-    dk::launchApply(view_, stream_);
-    dk::launchNewtonEnd(view_, stream_);
-    pollStates();
+    if (!useGraph_ || !allSegments(lsDecided)) {
+      for (; trials < 20; ++trials) {
+        dk::launchEnergyTrial(view_, stream_);
+        dk::launchLineSearchDecide(view_, stream_);
+        pollStates();
+        if (allSegments(lsDecided)) break;
+      }
+      dk::launchApply(view_, stream_);
+      dk::launchNewtonEnd(view_, stream_);
+      pollStates();
+    }
     if (allSegments([](const dk::SegState& s) { return s.newtonDone != 0u; })) break;
   }
   dk::launchFinish(view_, stream_);

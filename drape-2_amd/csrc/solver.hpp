@@ -47,8 +47,8 @@ class HipSolver {
   // Wavefronts per 64-row slice in the one-kernel PCG iteration: 1, 2 or 4, or 0 to time all three during the
   // first solves and keep the fastest (results are bit-identical for every choice).
   // Launch shape of the one-kernel PCG iteration (wavefronts per slice x slots per batch; 0x0 = tune).
-  void setPcgShape(int waves, int slots);
-  void pcgShape(int& waves, int& slots) const;  // 0x0 while still tuning
+  void setPcgShape(int waves, int slots, bool halo);
+  void pcgShape(int& waves, int& slots, bool& halo) const;  // 0x0 while still tuning
   // Every `every`-th PCG SpMV launch of the step is bracketed by HIP events (0 = off).
   void setSpmvSampling(int every);
   void spmvTiming(uint64_t& launches, double& meanUs, double& bytesPerLaunch) const;
@@ -119,7 +119,7 @@ class HipSolver {
   mutable std::unique_ptr<nrm::Mesh> normalMesh_;  // built on the first normals request after a build
   mutable DeviceBuffer normalsOut_;
   ws::Net* net_ = nullptr;
-  DeviceBuffer tileRange_;
+  DeviceBuffer tileRange_, tileHalo_;
   DeviceBuffer tileSeg_, segParams_, segState_, partA_, partB_, partC_, partE_, partMax_;
   DeviceBuffer stretch_, hinges_, seams_, fabrics_, garments_;
   DeviceBuffer incCount_, incStart_, inc_, rowLen_, rowStart_, colIndex_, diagSlot_;
@@ -128,14 +128,27 @@ class HipSolver {
   DeviceBuffer fieldValues_, scratchF3_, scratchF3b_;
   DeviceBuffer cgBuf_[10], cgPart0_, cgPart1_, tileDone_;
   int pcgVariant_ = 1;
-  static constexpr int kShapeCount = 4;  // dk::kPcgShapes
+  static constexpr int kShapeCount = 6;  // dk::kPcgShapes
   int pcgShape_ = -1;     // requested index into dk::kPcgShapes (-1 = tune)
   int tunedShape_ = -1;   // chosen by the tuner (-1 until it has measured every shape)
-  double tuneMs_[kShapeCount] = {0.0, 0.0, 0.0, 0.0};
-  uint32_t tuneLaunches_[kShapeCount] = {0u, 0u, 0u, 0u};
+  double tuneMs_[kShapeCount] = {};
+  uint32_t tuneLaunches_[kShapeCount] = {};
+  bool shapeAvailable(int i) const { return !dk::kPcgShapes[i].halo || view_.tileHalo != nullptr; }
   static constexpr uint32_t kGraphChunk = 16;  // PCG iterations per captured graph (even: parity pattern repeats)
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graphExec_ = nullptr;
+  // One Newton iteration's launches before the PCG solve (activate, assemble, e0) and after it (contact force,
+  // slope, kGraphTrials line-search trials, apply, end), each captured once and replayed; the device-side flags
+  // make every launch a no-op for a segment that has no work left in it.
+  struct Captured {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void release();
+  };
+  Captured newtonPre_, newtonPost_;
+  static constexpr int kGraphTrials = 2;
+  template <class F>
+  void launchCaptured(Captured& c, F&& launches);
   int graphShape_ = -1;  // shape index the captured graph launches
   dk::SegState* hostStates_ = nullptr;  // pinned host copy
   int spmvEvery_ = 0;

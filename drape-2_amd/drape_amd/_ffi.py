@@ -300,8 +300,9 @@ def _declare(lib):
         "drape_solver_set_graph": (C.c_int, [S, C.c_int32]),
         "drape_solver_set_spmv_sampling": (C.c_int, [S, C.c_int32]),
         "drape_solver_set_pcg_variant": (C.c_int, [S, C.c_int32]),
-        "drape_solver_set_pcg_shape": (C.c_int, [S, C.c_int32, C.c_int32]),
-        "drape_solver_pcg_shape": (C.c_int, [S, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "drape_solver_set_pcg_shape": (C.c_int, [S, C.c_int32, C.c_int32, C.c_int32]),
+        "drape_solver_pcg_shape": (C.c_int, [S, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                             C.POINTER(C.c_int32)]),
         "drape_solver_spmv_timing": (C.c_int, [S, C.POINTER(C.c_uint64), f64p, f64p]),
         "drape_solver_structure_sizes": (C.c_int, [S, C.POINTER(StructureSizes)]),
         "drape_solver_read_structure": (C.c_int, [S, u32p, u32p, u32p, u32p, u32p, u32p]),

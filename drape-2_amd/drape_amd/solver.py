@@ -54,7 +54,7 @@ class HipSolver:
             self.set_graph(False)
         if pcg_variant != 1:
             self.set_pcg_variant(pcg_variant)
-        if tuple(pcg_shape) != (0, 0):
+        if tuple(pcg_shape)[:2] != (0, 0):
             self.set_pcg_shape(*pcg_shape)
 
     def __del__(self):
@@ -166,15 +166,17 @@ class HipSolver:
         """1: one kernel per PCG iteration (default); 0: the two-kernel standard form."""
         _check(_lib().drape_solver_set_pcg_variant(self._h, int(variant)))
 
-    def set_pcg_shape(self, waves: int, slots: int) -> None:
+    def set_pcg_shape(self, waves: int, slots: int, halo: bool = False) -> None:
         """Launch shape of the one-kernel PCG iteration: (wavefronts per 64-row slice, slots per batch) in
-        (1, 8), (1, 4), (2, 8), (4, 8), or (0, 0) to tune (default)."""
-        _check(_lib().drape_solver_set_pcg_shape(self._h, int(waves), int(slots)))
+        (1, 8), (1, 4), (2, 8), (4, 8), or (0, 0) to tune (default); halo=True (with (1, 8) or (1, 4)) gathers from
+        an LDS copy of each tile's column window, for matrices whose every tile spans at most 1,536 rows."""
+        _check(_lib().drape_solver_set_pcg_shape(self._h, int(waves), int(slots), int(bool(halo))))
 
     def pcg_shape(self) -> tuple:
-        w, k = C.c_int32(), C.c_int32()
-        _check(_lib().drape_solver_pcg_shape(self._h, C.byref(w), C.byref(k)))
-        return (w.value, k.value)
+        """(waves, slots, halo) of the shape in use; (0, 0, False) while still tuning."""
+        w, k, h = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(_lib().drape_solver_pcg_shape(self._h, C.byref(w), C.byref(k), C.byref(h)))
+        return (w.value, k.value, bool(h.value))
 
     def set_spmv_sampling(self, every: int) -> None:
         _check(_lib().drape_solver_set_spmv_sampling(self._h, int(every)))

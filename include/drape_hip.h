@@ -210,10 +210,13 @@ int drape_solver_set_pcg_variant(drape_solver* s, int32_t variant);
 /* Launch shape of the one-kernel PCG iteration: `waves` wavefronts per 64-row slice and `slots` lower and upper
  * matrix slots per batch; 1x8, 1x4 (smaller register footprint: two workgroups per CU, for launches with many more
  * tiles than CUs), 2x8 and 4x8, or 0x0 (default) to time every shape during the first solves and keep the fastest.
- * Results are bit-identical for every shape. drape_solver_pcg_shape reports the shape in use (0x0 while still
- * tuning). No reference counterpart: the reference's GPU path has one thread per row (kernels.h:272-289). */
-int drape_solver_set_pcg_shape(drape_solver* s, int32_t waves, int32_t slots);
-int drape_solver_pcg_shape(const drape_solver* s, int32_t* waves, int32_t* slots);
+ * halo = 1 (with 1x8 or 1x4) stages each 256-row tile's column window in LDS and gathers from there; it runs only
+ * on matrices whose every tile reaches at most 1,536 consecutive rows (a step with it fails otherwise), and the
+ * tuner tries it only there. Results are bit-identical for every shape. drape_solver_pcg_shape reports the shape
+ * in use (0x0 while still tuning). No reference counterpart: the reference's GPU path has one thread per row
+ * (kernels.h:272-289). */
+int drape_solver_set_pcg_shape(drape_solver* s, int32_t waves, int32_t slots, int32_t halo);
+int drape_solver_pcg_shape(const drape_solver* s, int32_t* waves, int32_t* slots, int32_t* halo);
 
 /* Kernel timing inside the step: every `every`-th PCG SpMV launch (0 = off) is bracketed by HIP events on the
  * solver's stream. drape_solver_spmv_timing reports how many launches were timed since sampling was set, their

@@ -246,7 +246,8 @@ def test_c1_sheet_60_steps_within_0p1mm_of_oracle(hip, oracle, variant, graph):
 
 
 def test_graph_replay_matches_eager_launches_bit_for_bit(hip):
-    """The captured PCG chunks run the same kernels in the same order as eager launches."""
+    """The captured Newton parts (pre-solve launches, PCG chunks, post-solve launches with two line-search trials)
+    run the same kernels in the same order as eager launches."""
     sc = scenes.c1_sheet()
     out = []
     for graph in (True, False):
@@ -255,6 +256,27 @@ def test_graph_replay_matches_eager_launches_bit_for_bit(hip):
         for _ in range(8):
             g.step(1 / 60, 1, 5)
         out.append((g.read_positions(), g.counters()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
+def test_graph_replay_matches_eager_through_the_contact_phase(hip):
+    """Steps 1-30 of the 65,536-particle drop, into first contact with the sphere, where line searches take more
+    trials than the captured post-solve part holds: the eager continuation after the replay, and the apply that
+    runs once per segment whichever launch reaches it, give the eager run's bits."""
+    sc = scenes.c2_sheet()
+    out = []
+    for graph in (True, False):
+        g = hip.HipSolver(graph=graph, pcg_shape=(1, 8))
+        g.build(sc)
+        trials = []
+        for _ in range(30):
+            g.reset_counters()
+            g.step(1 / 60, 1, 5)
+            c = g.counters()
+            trials.append((c["line_search_trials"], c["newton_iterations"]))
+        out.append((g.read_positions(), trials))
+    print(f"line-search trials / Newton iterations per step: {out[0][1]}", flush=True)
     np.testing.assert_array_equal(out[0][0], out[1][0])
     assert out[0][1] == out[1][1]
 
@@ -285,10 +307,11 @@ def test_two_runs_bit_identical_and_snapshot_replay(hip):
 
 def test_pcg_launch_shapes_bit_identical(hip):
     """The one-kernel PCG iteration sums each row in the same fixed order whatever its launch shape (1, 2 or 4
-    wavefronts per slice; batches of 8 or 4 slots), so every shape, and the tuner's choice, gives the same bits."""
+    wavefronts per slice; batches of 8 or 4 slots; gathers from memory or from the tile's column window in LDS),
+    so every shape, and the tuner's choice, gives the same bits."""
     sc = scenes.c1_sheet()
     out = {}
-    shapes = ((1, 8), (1, 4), (2, 8), (4, 8), (0, 0))
+    shapes = ((1, 8), (1, 4), (2, 8), (4, 8), (1, 8, True), (1, 4, True), (0, 0))
     for shape in shapes:
         g = hip.HipSolver(pcg_shape=shape)
         g.build(sc)
@@ -299,6 +322,23 @@ def test_pcg_launch_shapes_bit_identical(hip):
     for shape in shapes[1:]:
         np.testing.assert_array_equal(out[shape][0], out[(1, 8)][0])
         assert out[shape][1] == out[(1, 8)][1]
+
+
+def test_pcg_halo_shape_needs_a_banded_matrix(hip):
+    """The tube's seams tie each row of the front panel to a row of the back panel 2,400 rows away, so its tiles'
+    column windows exceed the LDS window: the halo shape is refused before anything is launched, and a tuned
+    solver never tries it there."""
+    sc = scenes.c3_tube(nx=40, ny=60)
+    g = hip.HipSolver(pcg_shape=(1, 8, True))
+    g.build(sc)
+    with pytest.raises(ValueError, match="halo"):
+        g.step(1 / 60, 1, 5)
+    t = hip.HipSolver()
+    t.build(sc)
+    for _ in range(20):
+        t.step(1 / 60, 1, 5)
+    print(f"tuned shape on the tube: {t.pcg_shape()}", flush=True)
+    assert not t.pcg_shape()[2]
 
 
 def test_sheet_drop_contact_phase_stays_bounded(hip):
