@@ -125,6 +125,65 @@ def algorithmic_step_bytes(s, counters: dict, steps: int, pcg_launch_bytes: floa
             + per_step("line_search_trials") * terms)
 
 
+# The reference's published tee drapes to rest (men's tee, size M, Apple M1): particles, CPU wall s and frames,
+# Metal wall s and frames (docs/superpowers/specs/2026-10-25-drape-phase-b-mac-app-design.md:186-194).
+TEE_PUBLISHED = {"draft": (2647, 10.00, 418, 13.13, 507), "standard": (10342, 38.59, 397, 19.21, 375),
+                 "fine": (40046, 397.50, 480, 107.02, 351)}
+TEE_CPU_FRAMES = {"draft": 120, "standard": 30, "fine": 6}
+
+
+def tee_bench(quality: str, no_cpu: bool) -> dict:
+    """The men's tee (size M) on the male mannequin draped to rest on the GPU: wall time from the first frame to
+    the settled one, after the scene is built (mesh, placement, GPU bake of the body field, solver build)."""
+    from drape_amd import HipSolver, bake_collision_field, tee
+    m = tee.load_mannequin("male")
+    t0 = time.perf_counter()
+    field = bake_collision_field([(m.vertices, m.triangles)])
+    bake_s = time.perf_counter() - t0
+    ts = tee.tee_scene("male", quality, field=field)
+    s = HipSolver()
+    s.build(ts.scene)
+    opts = tee.options_for(quality)
+    t0 = time.perf_counter()
+    r = tee.drape_to_rest(s, ["tee"], opts)
+    wall = time.perf_counter() - t0
+    st = s.stats()
+    c = s.counters()
+    pub = TEE_PUBLISHED[quality]
+    line = {
+        "metric": f"tee drape to rest ({quality})", "value": wall, "unit": "s", "higher_is_better": False,
+        "n_gpus": 1, "dtype": "f32 (FP64 reductions)", "data": "synthetic pattern, reference mannequin asset",
+        "vs_baseline": None,
+        "config": {"workload": f"men's test tee size M on the male mannequin, {quality} ({ts.scene.n} particles)",
+                   "particles": ts.scene.n, "substeps": opts.substeps, "newton": opts.iterations,
+                   "pcg": "<= 50 iterations to 1e-2", "settled": r.settled, "failed": r.failed,
+                   "frames": r.frames_run, "simulated_s": r.simulated_seconds, "retries": r.retries,
+                   "assembly_s": r.assembly_seconds, "assembly_end_seam_gap_mm": r.assembly_end_seam_gap * 1e3,
+                   "max_penetration_mm": st["max_body_penetration"] * 1e3,
+                   "hem_error_mm": (tee.hem_height(ts, s.read_positions()) - tee.hem_target(ts)) * 1e3,
+                   "frames_per_s": r.frames_run / wall, "body_field_bake_s": bake_s,
+                   "newton_iterations": c["newton_iterations"], "pcg_iterations": c["pcg_iterations"]},
+        "published_m1": {"particles": pub[0], "cpu_s": pub[1], "cpu_frames": pub[2], "metal_s": pub[3],
+                         "metal_frames": pub[4]},
+    }
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import pyoracle
+        pyoracle.set_threads(1)
+        o = pyoracle.OracleSolver()
+        o.build(ts.scene)
+        o.set_phase("tee", tee.ASSEMBLING)
+        n = TEE_CPU_FRAMES[quality]
+        t0 = time.perf_counter()
+        for _ in range(n):
+            o.step(opts.frame_dt, opts.substeps, opts.iterations)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": n / dt, "unit": "frames/s", "cores": 1, "kind": "port",
+                                "sample": f"the first {n} frames of the same drape (assembling), FP64 oracle, "
+                                          f"1 thread, {cpu_model()}"}
+    return line
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,7 +202,12 @@ def main() -> int:
     ap.add_argument("--pcg-shape", default="0x0",
                     help="launch shape of the one-kernel PCG iteration, wavefronts x slots: 1x8, 1x4, 2x8, 4x8, "
                          "or 1x8h / 1x4h (column window staged in LDS); 0x0 = tune during warm-up")
+    ap.add_argument("--tee", choices=("draft", "standard", "fine"),
+                    help="instead of the sheet benchmark: drape the reference's tee to rest at this quality")
     args = ap.parse_args()
+    if args.tee:
+        print(json.dumps(tee_bench(args.tee, args.no_cpu_baseline)), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
               f"(the batched path shards by scene, see DESIGN.md)", file=sys.stderr)

@@ -1,0 +1,79 @@
+"""The reference's tee scene end to end on the MI355X solver: the tee on the mannequin drapes to rest under the
+reference's integration criteria (tee_drape_helpers.hpp:38-88: settled, assembly closed the seams to < 1 mm,
+penetration <= 2 mm, hem within 3 cm of the body-aware target, <= 15 s simulated), and continuing from the
+settled state the GPU stays within the reference's parity bound of the FP64 oracle (test_gpu_parity.cpp:29-82:
+< 1 mm after 60 frames), with the oracle's own spread under a 1 micrometre nudge printed beside it."""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import max_distance
+from drape_amd import tee
+
+pytestmark = pytest.mark.gpu
+
+_fields = {}
+
+
+def _field(hip, body):
+    if body not in _fields:
+        m = tee.load_mannequin(body)
+        t0 = time.perf_counter()
+        _fields[body] = hip.bake_collision_field([(m.vertices, m.triangles)])
+        print(f"{body} mannequin field {_fields[body].dims} baked in {time.perf_counter() - t0:.3f} s", flush=True)
+    return _fields[body]
+
+
+def _drape(hip, body, quality):
+    ts = tee.tee_scene(body, quality, field=_field(hip, body))
+    s = hip.HipSolver()
+    s.build(ts.scene)
+    t0 = time.perf_counter()
+    r = tee.drape_to_rest(s, ["tee"], tee.options_for(quality))
+    wall = time.perf_counter() - t0
+    st = s.stats()
+    hem, target = tee.hem_height(ts, s.read_positions()), tee.hem_target(ts)
+    print(f"tee {body} {quality}: {ts.scene.n} particles, settled={r.settled} failed={r.failed} "
+          f"frames={r.frames_run} simulated={r.simulated_seconds:.2f}s assembly={r.assembly_seconds:.2f}s "
+          f"retries={r.retries} seam gap={r.assembly_end_seam_gap * 1e3:.2f}mm "
+          f"penetration={st['max_body_penetration'] * 1e3:.2f}mm hem={hem:.3f}m target={target:.3f}m "
+          f"wall={wall:.2f}s ({r.frames_run / wall:.1f} frames/s)", flush=True)
+    return ts, s, r, st, hem, target
+
+
+@pytest.mark.parametrize("body,quality", [("male", "draft"), ("male", "standard"), ("female", "draft")])
+def test_tee_drapes_to_rest(hip, body, quality):
+    ts, s, r, st, hem, target = _drape(hip, body, quality)
+    assert r.settled and not r.failed
+    assert r.assembly_end_seam_gap < 1e-3
+    assert st["max_body_penetration"] <= 2e-3
+    assert abs(hem - target) <= 0.03
+    assert r.simulated_seconds <= 15.0
+    assert not st["has_non_finite"]
+
+
+def test_tee_continued_from_rest_matches_the_oracle(hip, oracle):
+    """Draft male tee settled on the GPU, then 60 more frames on the GPU and on the FP64 oracle from that state."""
+    ts, s, r, st, _, _ = _drape(hip, "male", "draft")
+    assert r.settled
+    snap = s.snapshot()
+    o = oracle.OracleSolver()
+    o.build(ts.scene)
+    o.restore(snap)
+    nudged = oracle.OracleSolver()
+    nudged.build(ts.scene)
+    nudged.restore(snap)
+    rng = np.random.default_rng(5)
+    x0 = nudged.read_positions().astype(np.float64)
+    nudged.write_state(x=x0 + rng.uniform(-1e-6, 1e-6, x0.shape))
+    opts = tee.options_for("draft")
+    for _ in range(60):
+        s.step(opts.frame_dt, opts.substeps, opts.iterations)
+        o.step(opts.frame_dt, opts.substeps, opts.iterations)
+        nudged.step(opts.frame_dt, opts.substeps, opts.iterations)
+    d = max_distance(s.read_positions(), o.read_positions())
+    spread = max_distance(nudged.read_positions(), o.read_positions())
+    print(f"tee draft, 60 frames from rest: GPU vs FP64 oracle max distance {d * 1e3:.4f} mm; oracle self-spread "
+          f"(1 um nudge) {spread * 1e3:.4f} mm", flush=True)
+    assert d < 1e-3
