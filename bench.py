@@ -184,6 +184,48 @@ def tee_bench(quality: str, no_cpu: bool) -> dict:
     return line
 
 
+def batch_bench(steps: int, warmup: int, sample_every: int, shape: tuple, graph: bool = True) -> dict:
+    """The 64-sheet batch (log-uniform 2k-40k particles per sheet, each on its own sphere or capsule plus ground,
+    PCG to 1e-6 with the cap at 2,000): one solver, one launch per kernel for all 64 segments."""
+    from drape_amd import HipSolver, scenes
+    batch = scenes.c4_batch()
+    n = sum(sc.n for sc in batch)
+    s = HipSolver(graph=graph, pcg_shape=shape)
+    s.build_batch(batch)
+    s.set_spmv_sampling(sample_every)
+    for _ in range(warmup):
+        s.step(1 / 60, 1, 5)
+    s.reset_counters()
+    s.set_spmv_sampling(sample_every)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step(1 / 60, 1, 5)
+    elapsed = time.perf_counter() - t0
+    c = s.counters()
+    timing = s.spmv_timing()
+    stats = s.segment_stats()
+    achieved = timing["bytes_per_launch"] / (timing["mean_us"] * 1e-6) / 1e9 if timing["mean_us"] > 0 else None
+    w, k, h = s.pcg_shape()
+    return {
+        "metric": "batched implicit steps/sec, 64 uneven sheets", "value": steps / elapsed,
+        "unit": "implicit steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1000.0 * elapsed / steps, "higher_is_better": True, "scaling": "weak",
+        "dtype": "f32 (FP64 reductions)", "data": "synthetic", "vs_baseline": None,
+        "config": {"workload": f"64 sheets, {n} particles (2k-40k each), dropped on spheres/capsules + ground",
+                   "particles": n, "particle_steps_per_s": n * steps / elapsed,
+                   "pcg_launch_shape": f"{w}x{k}{'h' if h else ''}",
+                   "newton_per_step": c["newton_iterations"] / steps, "pcg_per_step": c["pcg_iterations"] / steps,
+                   "non_finite_segments": sum(bool(x["has_non_finite"]) for x in stats),
+                   "max_penetration_mm": 1e3 * max(x["max_body_penetration"] for x in stats)},
+        "roofline": {"kernel": "kPcgStep (one launch covers all 64 segments; bytes count the segments still "
+                               "solving at each timed launch)", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "bytes_per_launch": timing["bytes_per_launch"], "mean_launch_us": timing["mean_us"],
+                     "timed_launches": timing["launches"]},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,7 +246,15 @@ def main() -> int:
                          "or 1x8h / 1x4h (column window staged in LDS); 0x0 = tune during warm-up")
     ap.add_argument("--tee", choices=("draft", "standard", "fine"),
                     help="instead of the sheet benchmark: drape the reference's tee to rest at this quality")
+    ap.add_argument("--batch", action="store_true",
+                    help="instead of the sheet benchmark: the 64-sheet batch (one solver, one launch per kernel)")
     args = ap.parse_args()
+    if args.batch:
+        spec = args.pcg_shape.lower()
+        shape = tuple(int(t) for t in spec.rstrip("h").split("x")) + (spec.endswith("h"),)
+        print(json.dumps(batch_bench(args.steps, args.warmup, args.sample_every, shape, not args.no_graph)),
+              flush=True)
+        return 0
     if args.tee:
         print(json.dumps(tee_bench(args.tee, args.no_cpu_baseline)), flush=True)
         return 0

@@ -345,6 +345,16 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
     std::vector<uint32_t> rs(nRows_ + 1), ci(std::max<uint32_t>(nBlocks_, 1));
     fromDevice(rs.data(), rowStart_.as<void>(), sizeof(uint32_t) * (nRows_ + 1));
     fromDevice(ci.data(), colIndex_.as<void>(), sizeof(uint32_t) * nBlocks_);
+    segBlocks_.assign(M.segments.size(), 0.0);
+    segParticles_.assign(M.segments.size(), 0.0);
+    for (size_t s = 0; s < M.segments.size(); ++s) {
+      const Segment& sg = M.segments[s];
+      for (uint32_t r = sg.rowBegin; r < sg.rowBegin + sg.rowCount; ++r) {
+        if (flags[r] & dk::kRowPad) continue;
+        segBlocks_[s] += double(rs[r + 1] - rs[r]);
+        segParticles_[s] += 1.0;
+      }
+    }
     std::vector<uint32_t> halo(2 * size_t(nTiles_), 0u);
     for (uint32_t t = 0; t < nTiles_; ++t) {
       uint32_t lo = t * dk::kTile, hi = (t + 1) * dk::kTile - 1;
@@ -552,6 +562,7 @@ void HipSolver::pollStates() const {
                        hipMemcpyDeviceToHost, stream_),
         "copy");
   check(hipStreamSynchronize(stream_), "step");
+  mirrorFresh_ = true;
   harvestEvents();
 }
 
@@ -559,35 +570,64 @@ void HipSolver::setSpmvSampling(int every) {
   spmvEvery_ = std::max(0, every);
   spmvTimed_ = 0;
   spmvTotalMs_ = 0.0;
+  spmvTotalBytes_ = 0.0;
+}
+
+// Algorithmic bytes of one PCG matrix launch over `blocks` stored blocks of `particles` rows. The matrix is stored
+// symmetric: every block on or above the diagonal once with its column (40 B; (blocks + n) / 2 of them), every
+// slot below it as a column and the index of its transposed block (8 B; (blocks - n) / 2). Per particle
+// - two-kernel form (kPcgSpmv): row and contact flags (8 B), z and p_old gathered once (24 B), p_new and ap
+//   written (24 B): 56 B;
+// - one-kernel form (kPcgStep): zr, t, p gathered once (36 B), own r and ap (24 B), step (12 B), the
+//   preconditioner (36 B), contact flags and normal (20 B), row flags (4 B) read; r, ap, p, zr, t and step
+//   written (72 B): 204 B.
+double HipSolver::launchBytes(double blocks, double n) const {
+  return 40.0 * 0.5 * (blocks + n) + 8.0 * 0.5 * (blocks - n) + (pcgVariant_ == 1 ? 204.0 : 56.0) * n;
+}
+
+double HipSolver::activeLaunchBytes(bool solveStarted) const {
+  double b = 0.0;
+  for (size_t s = 0; s < model_.segments.size(); ++s) {
+    const dk::SegState& st = hostStates_[s];
+    if (mirrorFresh_ && (st.newtonDone || (solveStarted && st.pcgConverged))) continue;
+    b += launchBytes(segBlocks_[s], segParticles_[s]);
+  }
+  return b;
+}
+
+void HipSolver::pushTimed(std::pair<hipEvent_t, hipEvent_t> pr, bool solveStarted) {
+  eventPending_.push_back(pr);
+  eventBytes_.push_back(activeLaunchBytes(solveStarted));
 }
 
 // Reads the elapsed times of completed event pairs (the stream was synchronised) and recycles them.
 void HipSolver::harvestEvents() const {
   auto* self = const_cast<HipSolver*>(this);
-  for (auto& pr : self->eventPending_) {
+  for (size_t i = 0; i < self->eventPending_.size(); ++i) {
+    const auto pr = self->eventPending_[i];
     float ms = 0.0f;
     check(hipEventElapsedTime(&ms, pr.first, pr.second), "event");
     spmvTotalMs_ += ms;
+    spmvTotalBytes_ += self->eventBytes_[i];
     ++spmvTimed_;
     self->eventPool_.push_back(pr);
   }
   self->eventPending_.clear();
+  self->eventBytes_.clear();
 }
 
+// Mean duration and mean algorithmic bytes of the timed launches: each counts the segments that were still solving
+// at the host's last look (a segment that converges between that poll and the launch is counted in full). With
+// no timed launch, the bytes of a launch over every segment.
 void HipSolver::spmvTiming(uint64_t& launches, double& meanUs, double& bytesPerLaunch) const {
   launches = spmvTimed_;
   meanUs = spmvTimed_ ? 1000.0 * spmvTotalMs_ / double(spmvTimed_) : 0.0;
+  if (spmvTimed_) {
+    bytesPerLaunch = spmvTotalBytes_ / double(spmvTimed_);
+    return;
+  }
   const double n = double(model_.x.size());
-  const double blocks = double(nBlocks_) - (double(nRows_) - n);
-  // Algorithmic bytes of one PCG matrix launch. The matrix is stored symmetric: every block on or above the
-  // diagonal once with its column (40 B; (blocks + n) / 2 of them), every slot below it as a column and the index
-  // of its transposed block (8 B; (blocks - n) / 2). Per particle
-  // - two-kernel form (kPcgSpmv): row and contact flags (8 B), z and p_old gathered once (24 B), p_new and ap
-  //   written (24 B): 56 B;
-  // - one-kernel form (kPcgStep): zr, t, p gathered once (36 B), own r and ap (24 B), step (12 B), the
-  //   preconditioner (36 B), contact flags and normal (20 B), row flags (4 B) read; r, ap, p, zr, t and step
-  //   written (72 B): 204 B.
-  bytesPerLaunch = 40.0 * 0.5 * (blocks + n) + 8.0 * 0.5 * (blocks - n) + (pcgVariant_ == 1 ? 204.0 : 56.0) * n;
+  bytesPerLaunch = launchBytes(double(nBlocks_) - (double(nRows_) - n), n);
 }
 
 bool HipSolver::allSegments(bool (*pred)(const dk::SegState&)) const {
@@ -669,19 +709,21 @@ void HipSolver::runPcgStandard() {
   // First chunk sized from the previous solve; later chunks short. Iterations past convergence are no-ops.
   uint32_t k = 0;
   uint32_t chunk = std::max<uint32_t>(8, lastPcgIterations_ * 3 / 4);
+  bool polled = false;  // the host mirror holds this solve's convergence flags
   while (k < maxCg) {
     const uint32_t end = std::min(maxCg, k + chunk);
     for (; k < end; ++k) {
       if (spmvEvery_ > 0 && k % uint32_t(spmvEvery_) == 0) {
         const auto pr = takeEvents();
         dk::launchPcgSpmv(view_, k, stream_, pr.first, pr.second);
-        eventPending_.push_back(pr);
+        pushTimed(pr, polled);
         dk::launchPcgUpdate(view_, k, stream_);
       } else {
         dk::launchPcgIteration(view_, k, stream_);
       }
     }
     pollStates();
+    polled = true;
     if (allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; })) break;
     chunk = 16;
   }
@@ -702,10 +744,10 @@ std::pair<hipEvent_t, hipEvent_t> HipSolver::takeEvents() {
   return pr;
 }
 
-void HipSolver::timedPcgStep(uint32_t parity, dk::PcgShape shape) {
+void HipSolver::timedPcgStep(uint32_t parity, dk::PcgShape shape, bool solveStarted) {
   const auto pr = takeEvents();
   dk::launchPcgStep(view_, parity, shape, stream_, pr.first, pr.second);
-  eventPending_.push_back(pr);
+  pushTimed(pr, solveStarted);
 }
 
 // Solves run before a launch shape is chosen: chunks of kGraphChunk launches rotate over the shapes of
@@ -775,12 +817,13 @@ void HipSolver::runPcgSingleKernel() {
   }
   uint32_t launched = 0;
   uint32_t chunks = std::max<uint32_t>(1, (lastPcgIterations_ * 7 / 8) / kGraphChunk);
+  bool polled = false;  // the host mirror holds this solve's convergence flags
   while (launched < maxCg + 1) {
     if (useGraph_ && sampling) {
       // Two eager iterations (keeping the graph's parity pattern) open each batch; the second, which starts
       // right behind the first as graph nodes do, is event-timed.
       dk::launchPcgStep(view_, 0u, shape, stream_);
-      timedPcgStep(1u, shape);
+      timedPcgStep(1u, shape, polled);
       launched += 2;
     }
     for (uint32_t c = 0; c < chunks; ++c) {
@@ -789,7 +832,7 @@ void HipSolver::runPcgSingleKernel() {
       } else {
         for (uint32_t i = 0; i < kGraphChunk; ++i) {
           if (sampling && (launched + i) % uint32_t(spmvEvery_) == 1)
-            timedPcgStep(i & 1u, shape);
+            timedPcgStep(i & 1u, shape, polled);
           else
             dk::launchPcgStep(view_, i & 1u, shape, stream_);
         }
@@ -797,6 +840,7 @@ void HipSolver::runPcgSingleKernel() {
       launched += kGraphChunk;
     }
     pollStates();
+    polled = true;
     if (allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; })) break;
     chunks = 1;
   }
@@ -806,6 +850,7 @@ void HipSolver::runPcgSingleKernel() {
 }
 
 void HipSolver::substep(double h, int iterations) {
+  mirrorFresh_ = false;  // every segment starts the substep's Newton loop
   uploadSegParams(h);
   if (net_) {
     dk::launchWarmFeatures(view_, stream_);

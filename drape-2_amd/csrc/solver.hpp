@@ -91,7 +91,7 @@ class HipSolver {
   void runPcgStandard();
   void runPcgSingleKernel();
   void releaseGraph();
-  void timedPcgStep(uint32_t parity, dk::PcgShape shape);
+  void timedPcgStep(uint32_t parity, dk::PcgShape shape, bool solveStarted);
   int shapeIndex() const { return pcgShape_ >= 0 ? pcgShape_ : tunedShape_; }
   std::pair<hipEvent_t, hipEvent_t> takeEvents();
   void runPcgTuning(uint32_t maxCg);
@@ -154,7 +154,14 @@ class HipSolver {
   int spmvEvery_ = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> eventPool_, eventPending_;
   mutable uint64_t spmvTimed_ = 0;
-  mutable double spmvTotalMs_ = 0.0;
+  mutable double spmvTotalMs_ = 0.0, spmvTotalBytes_ = 0.0;
+  std::vector<double> eventBytes_;        // algorithmic bytes of each pending timed launch
+  mutable bool mirrorFresh_ = false;      // the host mirror was read during the current substep
+  std::vector<double> segBlocks_, segParticles_;  // per segment: stored blocks of its particle rows, particles
+  double launchBytes(double blocks, double particles) const;
+  // Bytes of a PCG launch over the segments still solving at the last poll (at a solve's start: not finished).
+  double activeLaunchBytes(bool solveStarted) const;
+  void pushTimed(std::pair<hipEvent_t, hipEvent_t> pr, bool solveStarted);
   void harvestEvents() const;
 };
 
