@@ -77,3 +77,53 @@ def test_tee_continued_from_rest_matches_the_oracle(hip, oracle):
     print(f"tee draft, 60 frames from rest: GPU vs FP64 oracle max distance {d * 1e3:.4f} mm; oracle self-spread "
           f"(1 um nudge) {spread * 1e3:.4f} mm", flush=True)
     assert d < 1e-3
+
+
+def _oracle_drape(job):
+    """One FP64 oracle drape of the draft tee to rest (run in a worker process, which never touches the GPU).
+    `seed` > 0 moves every initial position by up to 1 nm (test_gpu_parity_slow.cpp:37-47)."""
+    body, seed = job
+    import pyoracle
+    pyoracle.set_threads(1)
+    m = tee.load_mannequin(body)
+    field = pyoracle.bake_collision_field([(m.vertices, m.triangles)])
+    ts = tee.tee_scene(body, "draft", field=field)
+    if seed:
+        rng = np.random.default_rng(seed)
+        g = ts.scene.garments[0]
+        g.initial_positions = g.initial_positions + 1e-9 * rng.uniform(-1, 1, g.initial_positions.shape)
+    o = pyoracle.OracleSolver()
+    o.build(ts.scene)
+    r = tee.drape_to_rest(o, ["tee"], tee.options_for("draft"))
+    x = o.read_positions().astype(np.float64)
+    return r.settled, r.simulated_seconds, tee.hem_height(ts, x), x.mean(axis=0)
+
+
+@pytest.mark.parametrize("body", ["male", "female"])
+def test_tee_settles_like_the_oracle(hip, body):
+    """The reference's long-run parity protocol (test_gpu_parity_slow.cpp:19-72): the draft tee draped to rest on
+    the GPU and on the FP64 oracle; hem height and centre of mass agree within 2 mm, or within 1.5 times the
+    oracle's own spread when that is larger: the largest difference from the unperturbed oracle over ten oracle
+    runs whose start moves by up to 1 nm (the assembly's contact decisions make the settled hem vary by
+    millimetres on its own). The eleven oracle drapes run in worker processes on the host's cores. The GPU field
+    is the device bake, the oracle's the oracle bake: the two are equal sample for sample (tests/test_bake.py)."""
+    import multiprocessing
+    from concurrent.futures import ProcessPoolExecutor
+    jobs = [(body, seed) for seed in range(11)]
+    with ProcessPoolExecutor(max_workers=11, mp_context=multiprocessing.get_context("spawn")) as ex:
+        futures = [ex.submit(_oracle_drape, j) for j in jobs]
+        ts, s, r, st, hem_gpu, _ = _drape(hip, body, "draft")
+        com_gpu = s.read_positions().astype(np.float64).mean(axis=0)
+        results = [f.result(timeout=600) for f in futures]
+    cpu = results[0]
+    spread_hem = max(abs(n[2] - cpu[2]) for n in results[1:])
+    spread_com = max(float(np.linalg.norm(n[3] - cpu[3])) for n in results[1:])
+    tol_hem, tol_com = max(2e-3, 1.5 * spread_hem), max(2e-3, 1.5 * spread_com)
+    d_hem, d_com = abs(hem_gpu - cpu[2]), float(np.linalg.norm(com_gpu - cpu[3]))
+    print(f"parity {body}: oracle t={cpu[1]:.2f}s gpu t={r.simulated_seconds:.2f}s | hem diff {d_hem * 1e3:.2f} mm "
+          f"(tol {tol_hem * 1e3:.2f}) | com diff {d_com * 1e3:.2f} mm (tol {tol_com * 1e3:.2f}); nudged oracle "
+          f"runs settled {sum(n[0] for n in results[1:])}/10", flush=True)
+    assert cpu[0] and r.settled
+    assert all(n[0] for n in results[1:])
+    assert d_hem <= tol_hem
+    assert d_com <= tol_com
