@@ -723,6 +723,37 @@ kPcgStep(View v, uint32_t q) {
   // Slice-uniform values through the scalar cache, issued with the segment index: neither depends on the other.
   const uint32_t slice = __builtin_amdgcn_readfirstlane(tile * kSlicesPerTile + sl);
   const uint32_t row = slice * kSlice + lane;
+  const F3* __restrict__ Zr = v.cgZr[o];
+  const F3* __restrict__ T = v.cgT[o];
+  const F3* __restrict__ Pp = v.cgP[o];
+  // The owner row's data depends on nothing loaded. The 8-slot shapes (one workgroup per CU, for launches of at
+  // most about one tile per CU, whose tiles rarely find their solve ended) issue it first, so that it streams in
+  // while the slice headers make their scalar round trip; the 4-slot shapes, for launches with many tiles per CU,
+  // issue it after the solve-ended test, so that the tiles of converged segments exit without loading it.
+  constexpr bool kOwnFirst = NB == 8;
+  F3 oR, oAp, oZ, oT, oP, oStep;
+  float m[9];
+  uint32_t oFlags = 0u, oCf = 0u;
+  float4 oCn = make_float4(0.f, 0.f, 0.f, 0.f);
+  auto loadOwner = [&] {
+    if (part == 0) {
+      oR = v.cgR[o][row];
+      oAp = v.cgAp[o][row];
+      if constexpr (!H) {
+        oZ = Zr[row];
+        oT = T[row];
+        oP = Pp[row];
+      }
+      oStep = v.step[row];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) m[e] = v.precond[9u * row + e];
+      oFlags = v.rowFlags[row];
+      oCf = v.cflags[row];
+      oCn = v.cnormal[row];
+    }
+  };
+  if constexpr (kOwnFirst) loadOwner();
+  __builtin_amdgcn_sched_barrier(0);
   const uint32_t K = ldUniform(v.sellK + slice);
   const uint32_t base = ldUniform(v.sellBase + slice);
   const uint32_t KL = ldUniform(v.lowK + slice);
@@ -731,8 +762,8 @@ kPcgStep(View v, uint32_t q) {
   const uint32_t tileBegin = ldUniform(v.tileRange + 2u * tile), tileEnd = ldUniform(v.tileRange + 2u * tile + 1u);
   const uint32_t hb = H ? ldUniform(v.tileHalo + 2u * tile) : 0u, hc = H ? ldUniform(v.tileHalo + 2u * tile + 1u) : 0u;
   // A tile whose solve has ended (the segment's PCG converged, or its Newton loop finished) passes the flag on and
-  // loads nothing. The flag is per tile, so it arrives with the slice headers: no extra round trip for the tiles
-  // that iterate, and none of the slice's loads for a tile of a batch whose segment converged early.
+  // loads nothing more. The flag is per tile, so it arrives with the slice headers: no extra round trip for the
+  // tiles that iterate, and none of the slice's matrix loads for a tile of a batch whose segment converged early.
   if (ldUniform(v.tileDone[q] + tile)) {
     if (threadIdx.x == 0) v.tileDone[o][tile] = 1u;
     return;
@@ -745,13 +776,10 @@ kPcgStep(View v, uint32_t q) {
 #pragma unroll
   for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < tileEnd ? prev[kPcgParts * ti + n] : 0.0;
   __builtin_amdgcn_sched_barrier(0);
-  const F3* __restrict__ Zr = v.cgZr[o];
-  const F3* __restrict__ T = v.cgT[o];
-  const F3* __restrict__ Pp = v.cgP[o];
 
   // Round trip 1, issued before the segment's scalars are known (everything here depends only on the slice and
-  // the parity): the first batch's columns (lower and upper), lower block indices and upper block values, then the
-  // owner row's data. Index loads first (unconditional, see sellColumn), block loads after, and nothing that uses
+  // the parity): the first batch's columns (lower and upper), lower block indices and upper block values. Index
+  // loads first (unconditional, see sellColumn), block loads after, and nothing that uses
   // an index before both are issued: the wait before the gathers is then one for the indices only (the memory
   // counter retires in issue order), not for the 36 B per slot of block values still in flight.
   const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
@@ -792,25 +820,7 @@ kPcgStep(View v, uint32_t q) {
 #pragma unroll
     for (int e = 0; e < 9; ++e) bv[j][e] = val[t * 9u * kSlice + e];
   }
-  F3 oR, oAp, oZ, oT, oP, oStep;
-  float m[9];
-  uint32_t oFlags = 0u, oCf = 0u;
-  float4 oCn = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (part == 0) {
-    oR = v.cgR[o][row];
-    oAp = v.cgAp[o][row];
-    if constexpr (!H) {
-      oZ = Zr[row];
-      oT = T[row];
-      oP = Pp[row];
-    }
-    oStep = v.step[row];
-#pragma unroll
-    for (int e = 0; e < 9; ++e) m[e] = v.precond[9u * row + e];
-    oFlags = v.rowFlags[row];
-    oCf = v.cflags[row];
-    oCn = v.cnormal[row];
-  }
+  if constexpr (!kOwnFirst) loadOwner();
   __builtin_amdgcn_sched_barrier(0);
 
   // Per-segment scalars through the scalar cache: none of them is written by this kernel (it writes only the
