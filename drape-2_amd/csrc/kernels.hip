@@ -697,6 +697,13 @@ __global__ void __launch_bounds__(256) kPcgStepInit(View v) {
     for (int i = 0; i < kPcgParts; ++i) v.cgPart[1][kPcgParts * tile + i] = d[i];
 }
 
+// One of a View's parity-buffered pointers by a select between the two, which are kernel arguments: indexing the
+// pair with the parity would turn into a second, dependent load from the argument segment.
+template <class Ptr>
+__device__ inline Ptr par(Ptr const (&pair)[2], uint32_t parity) {
+  return parity ? pair[1] : pair[0];
+}
+
 // Launch shapes: W wavefronts share each 64-row slice (each takes every W-th slot), and a batch covers NB slots of
 // the lower part and NB of the upper part (B = NB / W per wavefront). NB = 8 keeps a whole sheet row in one batch
 // (about 390 registers: one workgroup per CU); NB = 4 fits two workgroups per CU, which pays when a launch has
@@ -705,10 +712,17 @@ __global__ void __launch_bounds__(256) kPcgStepInit(View v) {
 // H (halo, W = 1 only): when every tile's columns lie in one window of at most kHaloRows rows (a banded matrix such
 // as a grid sheet's), the tile loads that window's (zr, t, p) with coalesced loads in its first round trip, keeps
 // it in LDS, and gathers from there: the second round trip then carries only the transposed lower blocks.
-template <int W, int NB, bool H>
+//
+// U (1x8 only, for a launch of one segment in the uniform layout): every slice has the same upper and lower slot
+// counts (at most 8: one batch), so its matrix sits at slice times those counts, and the segment's tiles are all
+// tiles: the partial sums and the first round trip (indices and block values) are issued at once, without
+// waiting for the slice headers.
+template <int W, int NB, bool H, bool U = false>
 __global__ void __launch_bounds__(256 * W) __attribute__((amdgpu_waves_per_eu(NB == 4 ? 2 : 1, 8)))
 kPcgStep(View v, uint32_t q) {
   static_assert(!H || W == 1, "the halo form has one wavefront per slice");
+  static_assert(!U || (W == 1 && NB == int(kUniformSlots) && !H), "the uniform form is the 1x8 shape");
+  // (In the uniform form K and KL are at most NB: the first batch is the only one.)
   constexpr int NW = 4 * W;
   constexpr uint32_t B = NB / W;  // lower slots, and upper slots, per wavefront in one batch
   constexpr uint32_t HR = H ? kHaloRows / 256u : 1u;  // halo rows staged per thread
@@ -723,9 +737,9 @@ kPcgStep(View v, uint32_t q) {
   // Slice-uniform values through the scalar cache, issued with the segment index: neither depends on the other.
   const uint32_t slice = __builtin_amdgcn_readfirstlane(tile * kSlicesPerTile + sl);
   const uint32_t row = slice * kSlice + lane;
-  const F3* __restrict__ Zr = v.cgZr[o];
-  const F3* __restrict__ T = v.cgT[o];
-  const F3* __restrict__ Pp = v.cgP[o];
+  const F3* __restrict__ Zr = par(v.cgZr, o);
+  const F3* __restrict__ T = par(v.cgT, o);
+  const F3* __restrict__ Pp = par(v.cgP, o);
   // The owner row's data depends on nothing loaded. The 8-slot shapes (one workgroup per CU, for launches of at
   // most about one tile per CU, whose tiles rarely find their solve ended) issue it first, so that it streams in
   // while the slice headers make their scalar round trip; the 4-slot shapes, for launches with many tiles per CU,
@@ -737,8 +751,8 @@ kPcgStep(View v, uint32_t q) {
   float4 oCn = make_float4(0.f, 0.f, 0.f, 0.f);
   auto loadOwner = [&] {
     if (part == 0) {
-      oR = v.cgR[o][row];
-      oAp = v.cgAp[o][row];
+      oR = par(v.cgR, o)[row];
+      oAp = par(v.cgAp, o)[row];
       if constexpr (!H) {
         oZ = Zr[row];
         oT = T[row];
@@ -754,74 +768,96 @@ kPcgStep(View v, uint32_t q) {
   };
   if constexpr (kOwnFirst) loadOwner();
   __builtin_amdgcn_sched_barrier(0);
-  const uint32_t K = ldUniform(v.sellK + slice);
-  const uint32_t base = ldUniform(v.sellBase + slice);
-  const uint32_t KL = ldUniform(v.lowK + slice);
-  const uint32_t lbase = ldUniform(v.lowBase + slice);
-  const uint32_t s = ldUniform(v.tileSeg + tile);
-  const uint32_t tileBegin = ldUniform(v.tileRange + 2u * tile), tileEnd = ldUniform(v.tileRange + 2u * tile + 1u);
-  const uint32_t hb = H ? ldUniform(v.tileHalo + 2u * tile) : 0u, hc = H ? ldUniform(v.tileHalo + 2u * tile + 1u) : 0u;
-  // A tile whose solve has ended (the segment's PCG converged, or its Newton loop finished) passes the flag on and
-  // loads nothing more. The flag is per tile, so it arrives with the slice headers: no extra round trip for the
-  // tiles that iterate, and none of the slice's matrix loads for a tile of a batch whose segment converged early.
-  if (ldUniform(v.tileDone[q] + tile)) {
-    if (threadIdx.x == 0) v.tileDone[o][tile] = 1u;
-    return;
-  }
-  // The previous kernel's partial sums first (the segment's tile range comes per tile, with the slice headers):
-  // the reduction over them then runs while this kernel's block values are still streaming in.
-  const double* __restrict__ prev = v.cgPart[o];
-  uint32_t ti = tileBegin + threadIdx.x;
-  double sum[kPcgParts];
-#pragma unroll
-  for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < tileEnd ? prev[kPcgParts * ti + n] : 0.0;
-  __builtin_amdgcn_sched_barrier(0);
+  const uint32_t K = U ? v.uniformK : ldUniform(v.sellK + slice);
+  const uint32_t base = U ? slice * v.uniformK * 9u * kSlice : ldUniform(v.sellBase + slice);
+  const uint32_t KL = U ? v.uniformKL : ldUniform(v.lowK + slice);
+  const uint32_t lbase = U ? slice * v.uniformKL * kSlice : ldUniform(v.lowBase + slice);
 
-  // Round trip 1, issued before the segment's scalars are known (everything here depends only on the slice and
-  // the parity): the first batch's columns (lower and upper), lower block indices and upper block values. Index
-  // loads first (unconditional, see sellColumn), block loads after, and nothing that uses
-  // an index before both are issued: the wait before the gathers is then one for the indices only (the memory
-  // counter retires in issue order), not for the 36 B per slot of block values still in flight.
+  // Round trip 1 (everything here depends only on the slice and the parity): the first batch's columns (lower and
+  // upper), lower block indices and upper block values. Index loads first (unconditional, see sellColumn), block
+  // loads after, and nothing that uses an index before both are issued: the wait before the gathers is then one
+  // for the indices only (the memory counter retires in issue order), not for the 36 B per slot of block values
+  // still in flight. Issued at once in the uniform layout, else after the slice headers and the solve-ended test.
   const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
   const float* __restrict__ val = v.sellVal + base + 9u * lane;
   const uint32_t* __restrict__ lcol = v.lowCol + lbase + lane;
   const uint32_t* __restrict__ lblk = v.lowBlk + lbase + lane;
+  constexpr uint32_t HR0 = HR;  // (named for the lambda below)
   uint32_t c[B], cl[B], bl[B];
-#pragma unroll
-  for (uint32_t j = 0; j < B; ++j) {
-    cl[j] = lcol[(part + W * j) * kSlice];
-    bl[j] = lblk[(part + W * j) * kSlice];
-  }
-#pragma unroll
-  for (uint32_t j = 0; j < B; ++j) c[j] = col[(part + W * j) * kSlice];
-  // The halo window, coalesced, before the block values: the wait before it goes to LDS then covers the indices
-  // and the window only.
-  // (Threads past the window's end re-read its last row and store it past the end, where no gather looks: loads
-  // and stores stay unconditional, so the staging arrays live in registers.)
-  float sv[HR][9];
-  if constexpr (H) {
-#pragma unroll
-    for (uint32_t r = 0; r < HR; ++r) {
-      const uint32_t i = hb + min(r * 256u + threadIdx.x, hc - 1u);
-      const F3 z = Zr[i], tt = T[i], pp = Pp[i];
-      sv[r][0] = z.x, sv[r][1] = z.y, sv[r][2] = z.z;
-      sv[r][3] = tt.x, sv[r][4] = tt.y, sv[r][5] = tt.z;
-      sv[r][6] = pp.x, sv[r][7] = pp.y, sv[r][8] = pp.z;
-    }
-  }
-  __builtin_amdgcn_sched_barrier(0);
-  // The first batch's upper block values. Slots past K are read unconditionally — the value buffer carries
-  // kSellSlack zero floats past its end — because their column is the dummy one, whose gathered vectors are zero,
-  // so they add exactly nothing.
+  float sv[HR0][9];
   float bv[B][9];
+  uint32_t hb = 0u, hc = 0u;
+  auto issueMatrix = [&] {
 #pragma unroll
-  for (uint32_t j = 0; j < B; ++j) {
-    const uint32_t t = part + W * j;
+    for (uint32_t j = 0; j < B; ++j) {
+      cl[j] = lcol[(part + W * j) * kSlice];
+      bl[j] = lblk[(part + W * j) * kSlice];
+    }
 #pragma unroll
-    for (int e = 0; e < 9; ++e) bv[j][e] = val[t * 9u * kSlice + e];
+    for (uint32_t j = 0; j < B; ++j) c[j] = col[(part + W * j) * kSlice];
+    // The halo window, coalesced, before the block values: the wait before it goes to LDS then covers the indices
+    // and the window only. (Threads past the window's end re-read its last row and store it past the end, where
+    // no gather looks: loads and stores stay unconditional, so the staging arrays live in registers.)
+    if constexpr (H) {
+#pragma unroll
+      for (uint32_t r = 0; r < HR0; ++r) {
+        const uint32_t i = hb + min(r * 256u + threadIdx.x, hc - 1u);
+        const F3 z = Zr[i], tt = T[i], pp = Pp[i];
+        sv[r][0] = z.x, sv[r][1] = z.y, sv[r][2] = z.z;
+        sv[r][3] = tt.x, sv[r][4] = tt.y, sv[r][5] = tt.z;
+        sv[r][6] = pp.x, sv[r][7] = pp.y, sv[r][8] = pp.z;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // The first batch's upper block values. Slots past K are read unconditionally — the value buffer carries
+    // kSellSlack zero floats past its end — because their column is the dummy one, whose gathered vectors are
+    // zero, so they add exactly nothing.
+#pragma unroll
+    for (uint32_t j = 0; j < B; ++j) {
+      const uint32_t t = part + W * j;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) bv[j][e] = val[t * 9u * kSlice + e];
+    }
+    if constexpr (!kOwnFirst) loadOwner();
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  // The previous kernel's partial sums go before the matrix: the reduction over them then runs while this
+  // kernel's block values are still streaming in. The segment's tile range comes per tile, with the slice headers
+  // (in the uniform form the launch is one segment: all tiles).
+  const double* __restrict__ prev = par(v.cgPart, o);
+  double sum[kPcgParts];
+  uint32_t ti = 0u, tileBegin = 0u, tileEnd = v.nTiles;
+  auto loadPartials = [&] {
+    ti = tileBegin + threadIdx.x;
+#pragma unroll
+    for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < tileEnd ? prev[kPcgParts * ti + n] : 0.0;
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  if constexpr (U) {
+    loadPartials();
+    issueMatrix();
   }
-  if constexpr (!kOwnFirst) loadOwner();
-  __builtin_amdgcn_sched_barrier(0);
+
+  const uint32_t s = U ? 0u : ldUniform(v.tileSeg + tile);
+  if constexpr (!U) {
+    tileBegin = ldUniform(v.tileRange + 2u * tile);
+    tileEnd = ldUniform(v.tileRange + 2u * tile + 1u);
+  }
+  if constexpr (H) {
+    hb = ldUniform(v.tileHalo + 2u * tile);
+    hc = ldUniform(v.tileHalo + 2u * tile + 1u);
+  }
+  // A tile whose solve has ended (the segment's PCG converged, or its Newton loop finished) passes the flag on and
+  // loads nothing more. The flag is per tile, so it arrives with the slice headers: no extra round trip for the
+  // tiles that iterate, and none of the slice's matrix loads for a tile of a batch whose segment converged early.
+  if (ldUniform(par(v.tileDone, q) + tile)) {
+    if (threadIdx.x == 0) par(v.tileDone, o)[tile] = 1u;
+    return;
+  }
+  if constexpr (!U) {
+    loadPartials();
+    issueMatrix();
+  }
 
   // Per-segment scalars through the scalar cache: none of them is written by this kernel (it writes only the
   // other parity slot of kIter, and r0 when k == 0, where it does not read it).
@@ -905,7 +941,7 @@ kPcgStep(View v, uint32_t q) {
     else
       beta = rzNext / sum[kPrZ];
   }
-  if (threadIdx.x == 0) v.tileDone[o][tile] = conv;  // every workgroup of the segment decides the same
+  if (threadIdx.x == 0) par(v.tileDone, o)[tile] = conv;  // every workgroup of the segment decides the same
   if (tile == tileBegin && threadIdx.x == 0) {
     if (k == 0) S->r0 = r0;
     S->kIter[o] = k + 1u;
@@ -936,7 +972,7 @@ kPcgStep(View v, uint32_t q) {
     else { z = lz[j]; tt = lt[j]; pp = lp[j]; }
     blockTMulAdd(bt[j], pcgDirection(z, tt, pp, a, b), acc[j % G][0], acc[j % G][1], acc[j % G][2]);
   }
-  for (uint32_t j0 = B; part + W * j0 < KL; j0 += B) {  // rows with more than NB lower slots
+  for (uint32_t j0 = B; !U && part + W * j0 < KL; j0 += B) {  // rows with more than NB lower slots
 #pragma unroll
     for (uint32_t j = 0; j < B; ++j) {
       const uint32_t t = part + W * (j0 + j);
@@ -959,7 +995,7 @@ kPcgStep(View v, uint32_t q) {
     else { z = gz[j]; tt = gt[j]; pp = gp[j]; }
     blockMulAdd(bv[j], pcgDirection(z, tt, pp, a, b), acc[j % G][0], acc[j % G][1], acc[j % G][2]);
   }
-  for (uint32_t j0 = B; part + W * j0 < K; j0 += B) {  // rows with more than NB upper slots
+  for (uint32_t j0 = B; !U && part + W * j0 < K; j0 += B) {  // rows with more than NB upper slots
 #pragma unroll
     for (uint32_t j = 0; j < B; ++j) c[j] = sellColumn(col, part + W * (j0 + j), K, v.nRows);
 #pragma unroll
@@ -1012,11 +1048,11 @@ kPcgStep(View v, uint32_t q) {
     const F3 apk = (oFlags & kRowPinned) ? f3(0, 0, 0) : filterContact(oCf, oCn, f3(a0, a1, a2));
     const F3 zr = mulRow9(m, rk);
     const F3 tk = mulRow9(m, apk);
-    v.cgR[q][row] = rk;
-    v.cgAp[q][row] = apk;
-    v.cgP[q][row] = pk;
-    v.cgZr[q][row] = zr;
-    v.cgT[q][row] = tk;
+    par(v.cgR, q)[row] = rk;
+    par(v.cgAp, q)[row] = apk;
+    par(v.cgP, q)[row] = pk;
+    par(v.cgZr, q)[row] = zr;
+    par(v.cgT, q)[row] = tk;
     d[kPpAp] = dot64(pk, apk);
     d[kPrZ] = dot64(rk, zr);
     d[kPrT] = dot64(rk, tk);
@@ -1028,7 +1064,7 @@ kPcgStep(View v, uint32_t q) {
   blockSumN<kPcgParts, NW>(d, red);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int i = 0; i < kPcgParts; ++i) v.cgPart[q][kPcgParts * tile + i] = d[i];
+    for (int i = 0; i < kPcgParts; ++i) par(v.cgPart, q)[kPcgParts * tile + i] = d[i];
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -1728,6 +1764,16 @@ __global__ void kSellK(BuildView b) {
   b.lowSize[slice] = KL * kSlice;
 }
 
+// Every slice to the largest slice's upper and lower slot counts (when neither exceeds kUniformSlots).
+__global__ void kSellUniform(BuildView b, uint32_t K, uint32_t KL) {
+  const uint32_t slice = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slice >= b.nSlices) return;
+  b.sellK[slice] = K;
+  b.sellSize[slice] = K * 9u * kSlice;
+  b.lowK[slice] = KL;
+  b.lowSize[slice] = KL * kSlice;
+}
+
 // Upper slot k of a row: column colIndex[diag + k]; lower slot k: column colIndex[rowStart + k] and the block of
 // (that column, this row) in the column's upper slots. Pinned columns point at the dummy column nRows; padding
 // slots at the dummy column and the zero block.
@@ -1789,7 +1835,9 @@ void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t s
   const dim3 grid(xcdGrid(v.nTiles));
   const uint32_t q = parity & 1u;
   if (shape.halo && !v.tileHalo) throw std::invalid_argument("PCG halo shape: a tile's columns exceed the window");
-  if (shape.waves == 1 && shape.slots == 8 && !shape.halo)
+  if (shape.waves == 1 && shape.slots == 8 && !shape.halo && v.uniformK && v.nSegments == 1)
+    hipExtLaunchKernelGGL((kPcgStep<1, 8, false, true>), grid, dim3(256), 0, st, start, stop, 0, v, q);
+  else if (shape.waves == 1 && shape.slots == 8 && !shape.halo)
     hipExtLaunchKernelGGL((kPcgStep<1, 8, false>), grid, dim3(256), 0, st, start, stop, 0, v, q);
   else if (shape.waves == 1 && shape.slots == 4 && !shape.halo)
     hipExtLaunchKernelGGL((kPcgStep<1, 4, false>), grid, dim3(256), 0, st, start, stop, 0, v, q);
@@ -1876,6 +1924,21 @@ void launchBuildStructure(const BuildView& bIn, BuiltStructure& out, hipStream_t
   const uint32_t terms = b.nStretch + b.nHinge + b.nSeam;
   if (terms) kSlots<<<(terms + 255) / 256, 256, 0, st>>>(b);
   kSellK<<<(b.nSlices + 255) / 256, 256, 0, st>>>(b);
+  // The largest slot counts: copied back (small) to size the assembly kernel's LDS and pick the layout.
+  {
+    std::vector<uint32_t> hk(b.nSlices), hl(b.nSlices);
+    hipCheck(hipMemcpyAsync(hk.data(), b.sellK, sizeof(uint32_t) * b.nSlices, hipMemcpyDeviceToHost, st), "copy");
+    hipCheck(hipMemcpyAsync(hl.data(), b.lowK, sizeof(uint32_t) * b.nSlices, hipMemcpyDeviceToHost, st), "copy");
+    hipCheck(hipStreamSynchronize(st), "sync");
+    out.maxK = 0;
+    out.maxLowK = 0;
+    for (uint32_t i = 0; i < b.nSlices; ++i) {
+      out.maxK = std::max(out.maxK, hk[i]);
+      out.maxLowK = std::max(out.maxLowK, hl[i]);
+    }
+  }
+  out.uniform = out.maxK <= kUniformSlots && out.maxLowK <= kUniformSlots && out.maxK > 0;
+  if (out.uniform) kSellUniform<<<(b.nSlices + 255) / 256, 256, 0, st>>>(b, out.maxK, out.maxLowK);
   kScan<<<1, 1024, 0, st>>>(b.sellSize, b.sellSize, b.nSlices);
   kScan<<<1, 1024, 0, st>>>(b.lowSize, b.lowSize, b.nSlices);
   uint32_t hostSell = 0, hostLow = 0;
@@ -1892,19 +1955,6 @@ void launchBuildStructure(const BuildView& bIn, BuiltStructure& out, hipStream_t
   launchFill(b.sellCol + hostSell / 9u, nRows, kSellColSlack, st);
   launchFill(b.lowCol + hostLow, nRows, kSellColSlack, st);
   launchFill(b.lowBlk + hostLow, zeroBlock, kSellColSlack, st);
-  // The largest slot counts: copied back (small) to size the assembly kernel's LDS and pick kernels.
-  {
-    std::vector<uint32_t> hk(b.nSlices), hl(b.nSlices);
-    hipCheck(hipMemcpyAsync(hk.data(), b.sellK, sizeof(uint32_t) * b.nSlices, hipMemcpyDeviceToHost, st), "copy");
-    hipCheck(hipMemcpyAsync(hl.data(), b.lowK, sizeof(uint32_t) * b.nSlices, hipMemcpyDeviceToHost, st), "copy");
-    hipCheck(hipStreamSynchronize(st), "sync");
-    out.maxK = 0;
-    out.maxLowK = 0;
-    for (uint32_t i = 0; i < b.nSlices; ++i) {
-      out.maxK = std::max(out.maxK, hk[i]);
-      out.maxLowK = std::max(out.maxLowK, hl[i]);
-    }
-  }
   out.colIndex = b.colIndex;
   out.sellCol = b.sellCol;
   out.lowCol = b.lowCol;

@@ -15,6 +15,7 @@ struct View {
   uint32_t nRows, nTiles, nSlices, nSegments;
   uint32_t nStretch, nHinge, nSeam;
   uint32_t zeroBlock;  // index (in blocks) of an all-zero block after the last slice's values
+  uint32_t uniformK, uniformKL;  // > 0: every slice has this many upper / lower slots (0: see sellK, lowK)
   // per row
   const uint32_t* rowFlags;
   const float* mass;
@@ -176,6 +177,7 @@ struct BuiltStructure {
   uint32_t nBlocks = 0;
   uint32_t maxK = 0;          // upper slots of the largest slice
   uint32_t maxLowK = 0;       // lower slots of the largest slice
+  bool uniform = false;       // every slice has maxK upper and maxLowK lower slots (see kSellUniform)
   uint64_t sellFloats = 0;    // upper block values (the value buffer adds kSellSlack zero floats)
   uint64_t lowEntries = 0;    // lower slots times 64
   uint32_t* colIndex = nullptr;  // nBlocks

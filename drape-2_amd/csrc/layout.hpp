@@ -42,6 +42,11 @@ constexpr uint32_t kSellSlack = 16u * 9u * 64u;
 // Entries after the last slice's upper columns, lower columns and lower block indices: a kernel may load the
 // indices of 16 slots past any slice's K (and discards them).
 constexpr uint32_t kSellColSlack = 16u * 64u;
+// When no slice needs more than this many upper or lower slots, every slice gets the largest slice's counts (the
+// extra slots of a smaller slice are padding: the dummy column and the zero block, as past a shorter row's last
+// slot). A slice's first slot then sits at slice times that count, so the iteration kernel can address its
+// matrix without reading the slice headers first.
+constexpr uint32_t kUniformSlots = 8u;
 // Largest window of rows whose (zr, t, p) a tile stages in LDS for the halo form of the PCG iteration (36 B a row:
 // 54 KB; 6 rows per thread of a 256-thread workgroup).
 constexpr uint32_t kHaloRows = 1536u;

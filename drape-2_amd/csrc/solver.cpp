@@ -336,6 +336,8 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   sellFloats_ = bs.sellFloats;
   lowEntries_ = bs.lowEntries;
   maxK_ = bs.maxK;
+  maxLowK_ = bs.maxLowK;
+  uniformLayout_ = bs.uniform;
   if (maxK_ * 9ull * dk::kSlice * sizeof(float) > 64ull * 1024)
     throw std::invalid_argument("hip: a 64-row slice has more than 28 blocks on or above the diagonal in one row");
   // Per tile, the window of rows [first, first + count) that holds every column of its rows and the rows
@@ -384,6 +386,8 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   v.nHinge = H;
   v.nSeam = K;
   v.zeroBlock = static_cast<uint32_t>(sellFloats_ / 9);
+  v.uniformK = uniformLayout_ ? maxK_ : 0u;
+  v.uniformKL = uniformLayout_ ? maxLowK_ : 0u;
   v.rowFlags = rowFlags_.as<uint32_t>();
   v.mass = mass_.as<float>();
   v.rowFabric = rowFabric_.as<uint32_t>();

@@ -104,6 +104,8 @@ class HipSolver {
   bool useGraph_ = true;
   double lastH_ = -1.0;
   uint32_t nRows_ = 0, nTiles_ = 0, nSlices_ = 0, nBlocks_ = 0, maxK_ = 0;
+  uint32_t maxLowK_ = 0;
+  bool uniformLayout_ = false;  // every slice has maxK_ upper and maxLowK_ lower slots
   uint64_t sellFloats_ = 0, lowEntries_ = 0;
   uint32_t lastPcgIterations_ = 0;
   std::vector<uint32_t> rowOf_;  // global particle -> internal row
