@@ -828,7 +828,9 @@ kPcgStep(View v, uint32_t q) {
   double sum[kPcgParts];
   uint32_t ti = 0u, tileBegin = 0u, tileEnd = v.nTiles;
   auto loadPartials = [&] {
-    ti = tileBegin + threadIdx.x;
+    // The first 256 threads take tiles t, t + 256, ... whatever the shape, and the wavefronts past the fourth add
+    // zeros: the sums round the same way in every shape.
+    ti = threadIdx.x < 256u ? tileBegin + threadIdx.x : tileEnd;
 #pragma unroll
     for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < tileEnd ? prev[kPcgParts * ti + n] : 0.0;
     __builtin_amdgcn_sched_barrier(0);
@@ -919,7 +921,7 @@ kPcgStep(View v, uint32_t q) {
   }
 
   // Scalars of this kernel (identical in every workgroup of the segment), while the gathers are in flight.
-  for (ti += 256u * W; ti < tileEnd; ti += 256u * W)
+  for (ti += 256u; ti < tileEnd; ti += 256u)
 #pragma unroll
     for (int n = 0; n < kPcgParts; ++n) sum[n] += prev[kPcgParts * ti + n];
   blockSumN<kPcgParts, NW>(sum, red);

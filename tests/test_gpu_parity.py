@@ -324,6 +324,23 @@ def test_pcg_launch_shapes_bit_identical(hip):
         assert out[shape][1] == out[(1, 8)][1]
 
 
+def test_pcg_launch_shapes_bit_identical_beyond_256_tiles(hip):
+    """A 288 x 288 sheet (82,944 particles, 324 tiles): more tiles than the 256 threads that sum the previous
+    partials, so each thread adds two tiles' partials; every wavefront count must still give the same bits."""
+    sc = scenes.c2_sheet(n=288)
+    out = {}
+    for shape in ((1, 8), (2, 8), (4, 8), (1, 4)):
+        g = hip.HipSolver(pcg_shape=shape)
+        g.build(sc)
+        for _ in range(3):
+            g.step(1 / 60, 1, 5)
+        out[shape] = (g.read_positions(), g.counters())
+        print(f"shape {shape}: counters {out[shape][1]}", flush=True)
+    for shape in list(out)[1:]:
+        np.testing.assert_array_equal(out[shape][0], out[(1, 8)][0])
+        assert out[shape][1] == out[(1, 8)][1]
+
+
 def test_pcg_halo_shape_needs_a_banded_matrix(hip):
     """The tube's seams tie each row of the front panel to a row of the back panel 2,400 rows away, so its tiles'
     column windows exceed the LDS window: the halo shape is refused before anything is launched, and a tuned
