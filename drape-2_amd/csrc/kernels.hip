@@ -628,12 +628,14 @@ __device__ inline double waveTotal(double v) {
 }
 
 // N sums over a workgroup of NW wavefronts (fixed order), returned to every thread. lds >= N * NW doubles.
-template <int N, int NW>
+// Fresh: `lds` has not been read earlier in this kernel, so the barrier that protects its previous readers is
+// left out.
+template <int N, int NW, bool Fresh = false>
 __device__ inline void blockSumN(double (&x)[N], double* lds) {
 #pragma unroll
   for (int i = 0; i < N; ++i) x[i] = waveTotal(x[i]);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  __syncthreads();
+  if constexpr (!Fresh) __syncthreads();
   if (l == 0)
 #pragma unroll
     for (int i = 0; i < N; ++i) lds[i * NW + w] = x[i];
@@ -726,7 +728,7 @@ kPcgStep(View v, uint32_t q) {
   constexpr int NW = 4 * W;
   constexpr uint32_t B = NB / W;  // lower slots, and upper slots, per wavefront in one batch
   constexpr uint32_t HR = H ? kHaloRows / 256u : 1u;  // halo rows staged per thread
-  __shared__ double red[kPcgParts * NW];
+  __shared__ double red[kPcgParts * NW], red2[kPcgParts * NW];  // one buffer per reduction: no barrier before either
   __shared__ float comb[(W > 1 ? 4 : 1) * kSlicesPerTile * 3 * kSlice];
   __shared__ F3 hz[H ? kHaloRows : 1], ht[H ? kHaloRows : 1], hp[H ? kHaloRows : 1];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
@@ -924,7 +926,7 @@ kPcgStep(View v, uint32_t q) {
   for (ti += 256u; ti < tileEnd; ti += 256u)
 #pragma unroll
     for (int n = 0; n < kPcgParts; ++n) sum[n] += prev[kPcgParts * ti + n];
-  blockSumN<kPcgParts, NW>(sum, red);
+  blockSumN<kPcgParts, NW, true>(sum, red);
   uint32_t conv = 0u;
   bool update = false;
   double alpha = 0.0, beta = 0.0, r0 = 0.0, rrNext = sum[kPrR];
@@ -1063,7 +1065,7 @@ kPcgStep(View v, uint32_t q) {
     d[kPapAp] = dot64(apk, apk);
     d[kPrR] = dot64(rk, rk);
   }
-  blockSumN<kPcgParts, NW>(d, red);
+  blockSumN<kPcgParts, NW, true>(d, red2);
   if (threadIdx.x == 0)
 #pragma unroll
     for (int i = 0; i < kPcgParts; ++i) par(v.cgPart, q)[kPcgParts * tile + i] = d[i];
