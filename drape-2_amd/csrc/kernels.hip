@@ -699,6 +699,13 @@ __global__ void __launch_bounds__(256) kPcgStepInit(View v) {
     for (int i = 0; i < kPcgParts; ++i) v.cgPart[1][kPcgParts * tile + i] = d[i];
 }
 
+// A vector store the kernel never reads back (the next launch reads it from beyond the L2): marked nontemporal.
+__device__ inline void storeStreaming(F3* p, F3 x) {
+  __builtin_nontemporal_store(x.x, &p->x);
+  __builtin_nontemporal_store(x.y, &p->y);
+  __builtin_nontemporal_store(x.z, &p->z);
+}
+
 // One of a View's parity-buffered pointers by a select between the two, which are kernel arguments: indexing the
 // pair with the parity would turn into a second, dependent load from the argument segment.
 template <class Ptr>
@@ -959,7 +966,7 @@ kPcgStep(View v, uint32_t q) {
   }
   if (!update && conv) return;
   const float a = float(alpha), b = float(beta);
-  if (part == 0 && update) v.step[row] = oStep + a * oP;
+  if (part == 0 && update) storeStreaming(v.step + row, oStep + a * oP);
   if (conv) return;
 
   // Row sums of A p_k. Slots are summed in four groups (slot index mod 4); each group takes its lower slots, then
@@ -1052,11 +1059,11 @@ kPcgStep(View v, uint32_t q) {
     const F3 apk = (oFlags & kRowPinned) ? f3(0, 0, 0) : filterContact(oCf, oCn, f3(a0, a1, a2));
     const F3 zr = mulRow9(m, rk);
     const F3 tk = mulRow9(m, apk);
-    par(v.cgR, q)[row] = rk;
-    par(v.cgAp, q)[row] = apk;
-    par(v.cgP, q)[row] = pk;
-    par(v.cgZr, q)[row] = zr;
-    par(v.cgT, q)[row] = tk;
+    storeStreaming(par(v.cgR, q) + row, rk);
+    storeStreaming(par(v.cgAp, q) + row, apk);
+    storeStreaming(par(v.cgP, q) + row, pk);
+    storeStreaming(par(v.cgZr, q) + row, zr);
+    storeStreaming(par(v.cgT, q) + row, tk);
     d[kPpAp] = dot64(pk, apk);
     d[kPrZ] = dot64(rk, zr);
     d[kPrT] = dot64(rk, tk);
