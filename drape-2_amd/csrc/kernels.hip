@@ -706,6 +706,12 @@ __device__ inline void storeStreaming(F3* p, F3 x) {
   __builtin_nontemporal_store(x.z, &p->z);
 }
 
+// A vector load of data no other lane of the launch reads: marked nontemporal, so that it does not push out of
+// the L2 the lines the gathers and the transposed-block loads reuse.
+__device__ inline F3 loadStreaming(const F3* p) {
+  return f3(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y), __builtin_nontemporal_load(&p->z));
+}
+
 // One of a View's parity-buffered pointers by a select between the two, which are kernel arguments: indexing the
 // pair with the parity would turn into a second, dependent load from the argument segment.
 template <class Ptr>
@@ -758,21 +764,24 @@ kPcgStep(View v, uint32_t q) {
   float m[9];
   uint32_t oFlags = 0u, oCf = 0u;
   float4 oCn = make_float4(0.f, 0.f, 0.f, 0.f);
+  // (zr, t and p are gathered by the neighbours as well; the rest is read by this row only: nontemporal.)
   auto loadOwner = [&] {
     if (part == 0) {
-      oR = par(v.cgR, o)[row];
-      oAp = par(v.cgAp, o)[row];
+      oR = loadStreaming(par(v.cgR, o) + row);
+      oAp = loadStreaming(par(v.cgAp, o) + row);
       if constexpr (!H) {
         oZ = Zr[row];
         oT = T[row];
         oP = Pp[row];
       }
-      oStep = v.step[row];
+      oStep = loadStreaming(v.step + row);
 #pragma unroll
-      for (int e = 0; e < 9; ++e) m[e] = v.precond[9u * row + e];
-      oFlags = v.rowFlags[row];
-      oCf = v.cflags[row];
-      oCn = v.cnormal[row];
+      for (int e = 0; e < 9; ++e) m[e] = __builtin_nontemporal_load(v.precond + 9u * row + e);
+      oFlags = __builtin_nontemporal_load(v.rowFlags + row);
+      oCf = __builtin_nontemporal_load(v.cflags + row);
+      const float* cn = reinterpret_cast<const float*>(v.cnormal + row);
+      oCn = make_float4(__builtin_nontemporal_load(cn), __builtin_nontemporal_load(cn + 1),
+                        __builtin_nontemporal_load(cn + 2), __builtin_nontemporal_load(cn + 3));
     }
   };
   if constexpr (kOwnFirst) loadOwner();
@@ -799,11 +808,11 @@ kPcgStep(View v, uint32_t q) {
   auto issueMatrix = [&] {
 #pragma unroll
     for (uint32_t j = 0; j < B; ++j) {
-      cl[j] = lcol[(part + W * j) * kSlice];
-      bl[j] = lblk[(part + W * j) * kSlice];
+      cl[j] = __builtin_nontemporal_load(lcol + (part + W * j) * kSlice);
+      bl[j] = __builtin_nontemporal_load(lblk + (part + W * j) * kSlice);
     }
 #pragma unroll
-    for (uint32_t j = 0; j < B; ++j) c[j] = col[(part + W * j) * kSlice];
+    for (uint32_t j = 0; j < B; ++j) c[j] = __builtin_nontemporal_load(col + (part + W * j) * kSlice);
     // The halo window, coalesced, before the block values: the wait before it goes to LDS then covers the indices
     // and the window only. (Threads past the window's end re-read its last row and store it past the end, where
     // no gather looks: loads and stores stay unconditional, so the staging arrays live in registers.)
