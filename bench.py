@@ -226,6 +226,35 @@ def batch_bench(steps: int, warmup: int, sample_every: int, shape: tuple, graph:
     }
 
 
+def tube_bench(steps: int) -> dict:
+    """The seamed tube (two 200 x 300 panels sewn into a tube, 120,000 particles) falling onto the capsule
+    mannequin and settling: mean time per implicit step over `steps` steps from the start, and the last 100."""
+    from drape_amd import HipSolver, scenes
+    sc = scenes.c3_tube()
+    s = HipSolver()
+    s.build(sc)
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        s.step(1 / 60, 1, 5)
+        times.append(time.perf_counter() - t0)
+    st = s.stats()
+    c = s.counters()
+    w, k, h = s.pcg_shape()
+    tail = times[-100:] if len(times) >= 100 else times
+    return {
+        "metric": "seamed tube, ms per implicit step", "value": 1000.0 * sum(times) / len(times), "unit": "ms",
+        "higher_is_better": False, "n_gpus": 1, "steps": steps, "dtype": "f32 (FP64 reductions)",
+        "data": "synthetic", "vs_baseline": None,
+        "config": {"workload": f"tube of two 200x300 panels ({sc.n} particles), seams k = 1e3 N/m, on the capsule "
+                               "mannequin, PCG to 1e-6 (cap 2,000), Newton <= 5",
+                   "last_100_steps_ms": 1000.0 * sum(tail) / len(tail),
+                   "pcg_launch_shape": f"{w}x{k}{'h' if h else ''}",
+                   "newton_per_step": c["newton_iterations"] / steps, "pcg_per_step": c["pcg_iterations"] / steps,
+                   "final_stats": st},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,9 +275,14 @@ def main() -> int:
                          "or 1x8h / 1x4h (column window staged in LDS); 0x0 = tune during warm-up")
     ap.add_argument("--tee", choices=("draft", "standard", "fine"),
                     help="instead of the sheet benchmark: drape the reference's tee to rest at this quality")
+    ap.add_argument("--tube", action="store_true",
+                    help="instead of the sheet benchmark: the seamed tube settling on the mannequin (--steps steps)")
     ap.add_argument("--batch", action="store_true",
                     help="instead of the sheet benchmark: the 64-sheet batch (one solver, one launch per kernel)")
     args = ap.parse_args()
+    if args.tube:
+        print(json.dumps(tube_bench(args.steps)), flush=True)
+        return 0
     if args.batch:
         spec = args.pcg_shape.lower()
         shape = tuple(int(t) for t in spec.rstrip("h").split("x")) + (spec.endswith("h"),)
