@@ -261,9 +261,8 @@ def test_graph_replay_matches_eager_launches_bit_for_bit(hip):
 
 
 def test_graph_replay_matches_eager_through_the_contact_phase(hip):
-    """Steps 1-30 of the 65,536-particle drop, into first contact with the sphere, where line searches take more
-    trials than the captured post-solve part holds: the eager continuation after the replay, and the apply that
-    runs once per segment whichever launch reaches it, give the eager run's bits."""
+    """Steps 1-30 of the 65,536-particle drop, into first contact with the sphere: the captured Newton
+    iterations, and the apply that runs once per segment whichever launch reaches it, give the eager run's bits."""
     sc = scenes.c2_sheet()
     out = []
     for graph in (True, False):
@@ -277,6 +276,29 @@ def test_graph_replay_matches_eager_through_the_contact_phase(hip):
             trials.append((c["line_search_trials"], c["newton_iterations"]))
         out.append((g.read_positions(), trials))
     print(f"line-search trials / Newton iterations per step: {out[0][1]}", flush=True)
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
+def test_graph_replay_matches_eager_past_the_captured_line_search_trials(hip):
+    """The 1,024-particle sheet at 0.2 s steps with one Newton iteration each: some line searches need three or
+    more trials, more than the captured post-solve part holds, so the eager continuation after the replay runs.
+    Positions and counters equal the eager run's bit for bit."""
+    sc = scenes.c1_sheet()
+    out = []
+    for graph in (True, False):
+        g = hip.HipSolver(graph=graph)
+        g.build(sc)
+        trials = []
+        for _ in range(12):
+            g.reset_counters()
+            g.step(0.2, 1, 1)
+            c = g.counters()
+            trials.append(c["line_search_trials"])
+        out.append((g.read_positions(), trials, g.stats()))
+    print(f"line-search trials per step (one Newton iteration each): {out[0][1]}", flush=True)
+    assert max(out[0][1]) > 2
+    assert not out[0][2]["has_non_finite"]
     np.testing.assert_array_equal(out[0][0], out[1][0])
     assert out[0][1] == out[1][1]
 
