@@ -201,6 +201,17 @@ class SceneHolder:
                 "seam_pairs": np.ascontiguousarray(m.seam_pairs, dtype=np.uint32).reshape(-1, 4),
                 "init": np.ascontiguousarray(g.initial_positions, dtype=np.float64),
             }
+            # The descriptor carries one count per array kind, so every per-vertex and per-triangle array must
+            # have it: a short array would be read past its end (sim_model.cpp:82-85 raises for the positions).
+            nv, nt = len(arrays["rest"]), len(arrays["triangles"])
+            if arrays["init"].shape != (nv, 3):
+                raise ValueError(f"garment '{g.id}': one initial position per mesh vertex required")
+            for k, count, per in (("vertex_piece", nv, "vertex"), ("vertex_area", nv, "vertex"),
+                                  ("boundary", nv, "vertex"), ("triangle_piece", nt, "triangle")):
+                if len(arrays[k]) != count:
+                    raise ValueError(f"garment '{g.id}': mesh {k} needs one entry per {per}")
+            if arrays["triangles"].ndim != 2 or arrays["triangles"].shape[1] != 3:
+                raise ValueError(f"garment '{g.id}': triangles must be (n, 3)")
             keep.append(arrays)
             fabs = (Fabric * max(1, len(g.piece_fabric)))(*[fabric_struct(f) for f in g.piece_fabric])
             keep.append(fabs)

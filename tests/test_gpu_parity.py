@@ -327,6 +327,41 @@ def test_two_runs_bit_identical_and_snapshot_replay(hip):
     np.testing.assert_array_equal(first, g.read_positions())
 
 
+def test_solver_errors_follow_the_reference_and_leave_it_usable(hip):
+    """The Solver surface's errors (sim_model.cpp:15, 82-85, 184, 258; test_cpu_solver.cpp:186): calls before
+    build are a logic error, an unknown garment id and a pin past the particle count are out of range, a fabric
+    list that does not match the pieces and a snapshot of another scene are invalid arguments. After each the
+    solver steps exactly as a fresh one."""
+    import copy
+    g = hip.HipSolver()
+    with pytest.raises(AssertionError, match="not built"):
+        g.step(1 / 60, 1, 5)
+    sc = scenes.c1_sheet()
+    g.build(sc)
+    with pytest.raises(IndexError, match="zzz"):
+        g.range("zzz")
+    with pytest.raises(IndexError, match="zzz"):
+        g.set_phase("zzz", 0)
+    other = hip.HipSolver()
+    other.build(scenes.c1_sheet(n=16))
+    with pytest.raises(ValueError, match="snapshot does not match"):
+        g.restore(other.snapshot())
+    bad = copy.deepcopy(sc)
+    bad.pins = [(sc.n + 5, (0.0, 0.0, 0.0))]
+    with pytest.raises(IndexError, match="pin particle out of range"):
+        hip.HipSolver().build(bad)
+    bad = copy.deepcopy(sc)
+    bad.garments[0].piece_fabric = bad.garments[0].piece_fabric * 2
+    with pytest.raises(ValueError, match="one fabric per pattern piece"):
+        hip.HipSolver().build(bad)
+    fresh = hip.HipSolver()
+    fresh.build(sc)
+    for s in (g, fresh):
+        for _ in range(3):
+            s.step(1 / 60, 1, 5)
+    np.testing.assert_array_equal(g.read_positions(), fresh.read_positions())
+
+
 def test_pcg_launch_shapes_bit_identical(hip):
     """The one-kernel PCG iteration sums each row in the same fixed order whatever its launch shape (1, 2 or 4
     wavefronts per slice; batches of 8 or 4 slots; gathers from memory or from the tile's column window in LDS),

@@ -130,6 +130,27 @@ def test_errors_map_to_reference_exception_types():
         drape_amd.HipSolver()
 
 
+def test_scene_descriptor_rejects_arrays_shorter_than_the_mesh():
+    """The C descriptor carries one vertex count and one triangle count per garment: a garment whose positions or
+    per-vertex / per-triangle arrays disagree with them is refused before any pointer is handed over
+    (sim_model.cpp:82-85: "one initial position per mesh vertex required")."""
+    import copy
+    sc = scenes.c1_sheet(n=8)
+    _ffi.SceneHolder(sc)  # the scene as built packs
+    cases = [("initial_positions", lambda g: g.initial_positions[:-1], "one initial position per mesh vertex"),
+             ("vertex_area", lambda g: g.mesh.vertex_area[:-1], "vertex_area needs one entry per vertex"),
+             ("triangle_piece", lambda g: g.mesh.triangle_piece[:-1], "triangle_piece needs one entry per triangle")]
+    for field, cut, message in cases:
+        bad = copy.deepcopy(sc)
+        g = bad.garments[0]
+        if field == "initial_positions":
+            g.initial_positions = cut(g)
+        else:
+            setattr(g.mesh, field, cut(g))
+        with pytest.raises(ValueError, match=message):
+            _ffi.SceneHolder(bad)
+
+
 def test_cpp_wrapper_header_compiles():
     """include/drape/sim/hip_solver.hpp (the C++ Solver-surface wrapper) compiles with the host compiler."""
     src = '#include "drape/sim/hip_solver.hpp"\nint main() { return drape::sim::hip::isAvailable() ? 0 : 1; }\n'
