@@ -122,7 +122,7 @@ def algorithmic_step_bytes(s, counters: dict, steps: int, pcg_launch_bytes: floa
     terms = 24.0 * n + 36.0 * S + 24.0 * H + 16.0 * K + 32.0 * n
     assembly = terms + 12.0 * n + 36.0 * upper
     per_step = lambda key: counters[key] / steps  # noqa: E731
-    return (per_step("newton_iterations") * (assembly + 72.0 * n) + per_step("pcg_iterations") * pcg_launch_bytes
+    return (per_step("newton_iterations") * (assembly + 48.0 * n) + per_step("pcg_iterations") * pcg_launch_bytes
             + per_step("line_search_trials") * terms)
 
 
@@ -375,7 +375,7 @@ def main() -> int:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": step_bytes * value / 1e9 / HBM_PEAK_GBS,
-        "terms": "assembly 24n + 36 S + 24 H + 16 K + 44n + 36 U per Newton iteration, preconditioner 72n per Newton "
+        "terms": "assembly 24n + 36 S + 24 H + 16 K + 44n + 36 U per Newton iteration, preconditioner 48n per Newton "
                  "iteration, one kPcgStep launch per PCG iteration, 24n + 36 S + 24 H + 16 K + 32n per line-search "
                  "trial (n particles, S stretch, H hinge, K seam terms, U stored blocks)",
     }

@@ -386,8 +386,8 @@ __global__ void __launch_bounds__(64) kAssemble(View v) {
   if (lane == 0) v.partE[slice] = rowE;
   __syncthreads();
   // Block-Jacobi preconditioner from the finished diagonal block: its inverse, the identity when it is not
-  // positive definite, zero for pinned rows (cpu_solver.cpp:171-178; kernels.h:237-248). Stored with the contact
-  // filter applied, S M, since the solve only ever uses filter(M x) (cpu_solver.cpp:179-181).
+  // positive definite, zero for pinned rows (cpu_solver.cpp:171-178; kernels.h:237-248). Stored as the upper
+  // triangle; the solve applies the contact filter after it (precondApply).
   {
     float M[9];
     if (flags & kRowPinned) {
@@ -402,18 +402,9 @@ __global__ void __launch_bounds__(64) kAssemble(View v) {
         for (int c = 0; c < 9; ++c) M[c] = (c % 4 == 0) ? 1.0f : 0.0f;
       }
     }
-    if (cf & kContactActive) {
-      const F3 n = f3(cn.x, cn.y, cn.z);
+    const float up[kPrecondFloats] = {M[0], M[1], M[2], M[4], M[5], M[8]};
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const float nm = n.x * M[c] + n.y * M[3 + c] + n.z * M[6 + c];
-        M[c] -= n.x * nm;
-        M[3 + c] -= n.y * nm;
-        M[6 + c] -= n.z * nm;
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 9; ++c) v.precond[9u * row + c] = M[c];
+    for (uint32_t c = 0; c < kPrecondFloats; ++c) v.precond[kPrecondFloats * row + c] = up[c];
   }
   float* out = v.sellVal + v.sellBase[slice];
   for (uint32_t i = lane; i < K * 9u * kSlice; i += kSlice) out[i] = acc[i];
@@ -454,10 +445,7 @@ __global__ void __launch_bounds__(256) kPcgInit(View v) {
   const uint32_t cf = v.cflags[row];
   const float4 cn = v.cnormal[row];
   const F3 ri = filterContact(cf, cn, -v.grad[row]);
-  M3 Mm;
-#pragma unroll
-  for (int c = 0; c < 9; ++c) Mm.m[c] = v.precond[9u * row + c];
-  const F3 zi = mul(Mm, ri);  // the stored preconditioner already carries the contact filter
+  const F3 zi = precondApply(v.precond + kPrecondFloats * row, cf, cn, ri);
   v.step[row] = f3(0, 0, 0);
   v.r[row] = ri;
   v.z[row] = zi;
@@ -562,10 +550,7 @@ __global__ void __launch_bounds__(256) kPcgUpdate(View v, uint32_t k, const F3* 
   v.step[row] = v.step[row] + a * pi;
   const F3 ri = v.r[row] - a * v.ap[row];
   v.r[row] = ri;
-  M3 M;
-#pragma unroll
-  for (int c = 0; c < 9; ++c) M.m[c] = v.precond[9u * row + c];
-  const F3 zi = mul(M, ri);
+  const F3 zi = precondApply(v.precond + kPrecondFloats * row, v.cflags[row], v.cnormal[row], ri);
   v.z[row] = zi;
   const double rr = double(ri.x) * ri.x + double(ri.y) * ri.y + double(ri.z) * ri.z;
   const double rz = double(ri.x) * zi.x + double(ri.y) * zi.y + double(ri.z) * zi.z;
@@ -585,8 +570,8 @@ __global__ void __launch_bounds__(256) kPcgUpdate(View v, uint32_t k, const F3* 
 // partial dot products the next kernel needs. Standard PCG needs a second global reduction after its update, for
 // r_k.z_k (beta) and r_k.r_k (the stopping test); here both are expanded over the previous iteration's vectors,
 //   r_k.z_k = r.zr - 2 alpha r.t + alpha^2 ap.t,    r_k.r_k = r.r - 2 alpha r.ap + alpha^2 ap.ap,
-// with zr = Mf r and t = Mf ap computed directly by each row's owner (Mf = S M, the contact-filtered block-Jacobi
-// inverse). Every term is a dot product of stored vectors, so no scalar is carried by recurrence and no
+// with zr = Mf r and t = Mf ap computed directly by each row's owner (Mf x = S M x: the block-Jacobi inverse, then
+// the contact filter). Every term is a dot product of stored vectors, so no scalar is carried by recurrence and no
 // orthogonality is assumed; p.ap is reduced directly, as in the standard form. A gathered column's z_k is rebuilt
 // from its owner's zr and t (z_k = zr - alpha t = Mf r_k). Stopping rules of cpu_solver.cpp:188-213: r0 = 0,
 // p.Ap <= 0 before an update, |r| <= tol |r0| after one, at most maxCg updates.
@@ -655,11 +640,6 @@ __device__ inline F3 pcgDirection(F3 zr, F3 t, F3 p, float alpha, float beta) {
   return (zr - alpha * t) + beta * p;
 }
 
-__device__ inline F3 mulRow9(const float* m, F3 x) {
-  return f3(m[0] * x.x + m[1] * x.y + m[2] * x.z, m[3] * x.x + m[4] * x.y + m[5] * x.z,
-            m[6] * x.x + m[7] * x.y + m[8] * x.z);
-}
-
 __device__ inline double dot64(F3 a, F3 b) { return double(a.x) * b.x + double(a.y) * b.y + double(a.z) * b.z; }
 
 // Init: r_0 = S(-grad), zr_0 = Mf r_0, written as the outputs of a kernel "-1" (slot 1) with ap = p = t = 0;
@@ -680,11 +660,10 @@ __global__ void __launch_bounds__(256) kPcgStepInit(View v) {
     S->kIter[0] = 0u;
   }
   const uint32_t row = tile * kTile + threadIdx.x;
-  const F3 r = filterContact(v.cflags[row], v.cnormal[row], -v.grad[row]);
-  float m[9];
-#pragma unroll
-  for (int e = 0; e < 9; ++e) m[e] = v.precond[9u * row + e];
-  const F3 zr = mulRow9(m, r);
+  const uint32_t cf = v.cflags[row];
+  const float4 cn = v.cnormal[row];
+  const F3 r = filterContact(cf, cn, -v.grad[row]);
+  const F3 zr = precondApply(v.precond + kPrecondFloats * row, cf, cn, r);
   const F3 zero = f3(0, 0, 0);
   v.cgR[1][row] = r;
   v.cgAp[1][row] = zero;
@@ -761,7 +740,7 @@ kPcgStep(View v, uint32_t q) {
   // issue it after the solve-ended test, so that the tiles of converged segments exit without loading it.
   constexpr bool kOwnFirst = NB == 8;
   F3 oR, oAp, oZ, oT, oP, oStep;
-  float m[9];
+  float m[kPrecondFloats];
   uint32_t oFlags = 0u, oCf = 0u;
   float4 oCn = make_float4(0.f, 0.f, 0.f, 0.f);
   // (zr, t and p are gathered by the neighbours as well; the rest is read by this row only: nontemporal.)
@@ -776,7 +755,8 @@ kPcgStep(View v, uint32_t q) {
       }
       oStep = loadStreaming(v.step + row);
 #pragma unroll
-      for (int e = 0; e < 9; ++e) m[e] = __builtin_nontemporal_load(v.precond + 9u * row + e);
+      for (uint32_t e = 0; e < kPrecondFloats; ++e)
+        m[e] = __builtin_nontemporal_load(v.precond + kPrecondFloats * row + e);
       oFlags = __builtin_nontemporal_load(v.rowFlags + row);
       oCf = __builtin_nontemporal_load(v.cflags + row);
       const float* cn = reinterpret_cast<const float*>(v.cnormal + row);
@@ -1066,8 +1046,8 @@ kPcgStep(View v, uint32_t q) {
     const F3 rk = oR - a * oAp;
     const F3 pk = pcgDirection(oZ, oT, oP, a, b);
     const F3 apk = (oFlags & kRowPinned) ? f3(0, 0, 0) : filterContact(oCf, oCn, f3(a0, a1, a2));
-    const F3 zr = mulRow9(m, rk);
-    const F3 tk = mulRow9(m, apk);
+    const F3 zr = precondApply(m, oCf, oCn, rk);
+    const F3 tk = precondApply(m, oCf, oCn, apk);
     storeStreaming(par(v.cgR, q) + row, rk);
     storeStreaming(par(v.cgAp, q) + row, apk);
     storeStreaming(par(v.cgP, q) + row, pk);

@@ -48,7 +48,7 @@ struct View {
   F3* cgT[2];
   double* cgPart[2];
   uint32_t* tileDone[2];  // per tile, by kernel parity: 1 once the tile's PCG solve has ended (kPcgStep)
-  float* precond;  // 9 per row, row-major: S M, the block-Jacobi inverse with the contact filter applied
+  float* precond;  // kPrecondFloats per row: the symmetric block-Jacobi inverse's upper triangle (00 01 02 11 12 22)
   float4* cnormal; // xyz normal, w = offset r - phi(xStart)
   uint32_t* cflags;
   float* cforce;

@@ -50,6 +50,8 @@ constexpr uint32_t kUniformSlots = 8u;
 // Largest window of rows whose (zr, t, p) a tile stages in LDS for the halo form of the PCG iteration (36 B a row:
 // 54 KB; 6 rows per thread of a 256-thread workgroup).
 constexpr uint32_t kHaloRows = 1536u;
+// Floats per row of the block-Jacobi preconditioner: the upper triangle of the symmetric inverse diagonal block.
+constexpr uint32_t kPrecondFloats = 6u;
 
 // Row flags.
 constexpr uint32_t kRowPinned = 1u;

@@ -185,7 +185,7 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   }
   // Row arrays a gathered column may index hold the zero dummy entry nRows as well.
   for (DeviceBuffer& b : cgBuf_) b.allocate(vecBytes);
-  precond_.allocate(sizeof(float) * 9 * (nRows_ + 1));
+  precond_.allocate(sizeof(float) * dk::kPrecondFloats * (nRows_ + 1));
   cnormal_.allocate(sizeof(float4) * (nRows_ + 1));
   cflags_.allocate(sizeof(uint32_t) * (nRows_ + 1));
   cforce_.allocate(sizeof(float) * nRows_);
@@ -583,10 +583,10 @@ void HipSolver::setSpmvSampling(int every) {
 // - two-kernel form (kPcgSpmv): row and contact flags (8 B), z and p_old gathered once (24 B), p_new and ap
 //   written (24 B): 56 B;
 // - one-kernel form (kPcgStep): zr, t, p gathered once (36 B), own r and ap (24 B), step (12 B), the
-//   preconditioner (36 B), contact flags and normal (20 B), row flags (4 B) read; r, ap, p, zr, t and step
-//   written (72 B): 204 B.
+//   preconditioner (24 B: the symmetric inverse's upper triangle), contact flags and normal (20 B), row flags
+//   (4 B) read; r, ap, p, zr, t and step written (72 B): 192 B.
 double HipSolver::launchBytes(double blocks, double n) const {
-  return 40.0 * 0.5 * (blocks + n) + 8.0 * 0.5 * (blocks - n) + (pcgVariant_ == 1 ? 204.0 : 56.0) * n;
+  return 40.0 * 0.5 * (blocks + n) + 8.0 * 0.5 * (blocks - n) + (pcgVariant_ == 1 ? 192.0 : 56.0) * n;
 }
 
 double HipSolver::activeLaunchBytes(bool solveStarted) const {
@@ -1333,10 +1333,12 @@ void HipSolver::debugSystem(float* blocks, float* grad, float* precond, uint32_t
       grad[3 * i + 2] = g[rowOf_[i]].z;
     }
   }
-  if (precond) {
-    std::vector<float> m(9 * size_t(nRows_ + 1));
+  if (precond) {  // stored as the upper triangle (00, 01, 02, 11, 12, 22), exported as the full block
+    std::vector<float> m(dk::kPrecondFloats * size_t(nRows_ + 1));
     fromDevice(m.data(), precond_.as<void>(), sizeof(float) * m.size());
-    for (size_t i = 0; i < n; ++i) std::copy(&m[9 * size_t(rowOf_[i])], &m[9 * size_t(rowOf_[i])] + 9, precond + 9 * i);
+    static const int kFull[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+    for (size_t i = 0; i < n; ++i)
+      for (int c = 0; c < 9; ++c) precond[9 * i + c] = m[dk::kPrecondFloats * size_t(rowOf_[i]) + kFull[c]];
   }
   if (contactFlags) {
     std::vector<uint32_t> f(nRows_ + 1);

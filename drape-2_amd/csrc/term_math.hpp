@@ -80,6 +80,15 @@ __device__ inline F3 filterContact(uint32_t flags, float4 cn, F3 v) {
   return v - n * dot(n, v);
 }
 
+// A row's block-Jacobi preconditioner is stored as the upper triangle of the inverse diagonal block (00, 01, 02,
+// 11, 12, 22: the inverse of a symmetric block is symmetric) and applied with the contact filter after it,
+// filter(M x) (cpu_solver.cpp:179-181).
+__device__ inline F3 precondApply(const float* m, uint32_t flags, float4 cn, F3 x) {
+  const F3 y = F3{m[0] * x.x + m[1] * x.y + m[2] * x.z, m[1] * x.x + m[3] * x.y + m[4] * x.z,
+                  m[2] * x.x + m[4] * x.y + m[5] * x.z};
+  return filterContact(flags, cn, y);
+}
+
 // ---- Stretch (stretch.hpp:15-49, energies.cpp:10-62) ----
 
 // Coefficient of vertex `local` in f (column 0: f_u, 1: f_v) = sum_k d_k x_k.

@@ -3,7 +3,8 @@ order of operations, against the oracle's solveLinear (cpu_solver.cpp:169-213).
 
 The kernel takes two scalars of standard PCG from exact expansions over the previous iteration's vectors
 (r_k.z_k = r.zr - 2 alpha r.t + alpha^2 ap.t and r_k.r_k = r.r - 2 alpha r.ap + alpha^2 ap.ap), rebuilds each
-gathered column's z as zr - alpha t, and stores the preconditioner with the contact filter applied (Mf = S M).
+gathered column's z as zr - alpha t, and applies the block-Jacobi inverse (stored as its upper triangle) with the
+contact filter after it, filter(M x) (cpu_solver.cpp:179-181).
 These tests pin that algebra and the stopping rules (r0 = 0, p.Ap <= 0 before an update, tolerance and iteration
 cap after one) against the oracle's PCG, with and without held contacts: same number of updates, same step.
 """
@@ -33,21 +34,19 @@ def one_kernel_pcg(oracle, rs, ci, bl, pin, active, normals, rhs, tol, max_cg):
         x = x.reshape(n, 3)
         return (x - nrm * np.sum(nrm * x, axis=1, keepdims=True)).reshape(-1)
 
-    # Filtered block-Jacobi inverse Mf = S M (identity when the block is not positive definite, zero if pinned).
-    Mf = np.zeros((n, 3, 3))
+    # Block-Jacobi inverse M (identity when the block is not positive definite, zero if pinned), symmetrised as
+    # the kernel stores it (upper triangle); applied as filter(M x).
+    M = np.zeros((n, 3, 3))
     for i in range(n):
         if pin[i]:
             continue
         d = np.where(ci[rs[i]:rs[i + 1]] == i)[0][0] + rs[i]
         ok, inv = oracle.invert_spd3(bl[d])
         Mi = inv if ok else np.eye(3)
-        if active[i]:
-            ni = normals[i]
-            Mi = Mi - np.outer(ni, ni @ Mi)
-        Mf[i] = Mi
+        M[i] = np.triu(Mi) + np.triu(Mi, 1).T
 
     def mul(x):
-        return np.einsum("nij,nj->ni", Mf, x.reshape(n, 3)).reshape(-1)
+        return S(np.einsum("nij,nj->ni", M, x.reshape(n, 3)).reshape(-1))
 
     def SA(p):
         return S((A @ (p * free)) * free)
