@@ -28,7 +28,7 @@ sys.path.insert(0, os.path.join(REPO, "drape-2_amd"))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # HBM bytes per kPcgStep launch on the 256x256 sheet (one wavefront per slice), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE scaled by a plain copy's readings (x2.0, x1.0): profiles/kPcgStep_sheet65k_kernel_trace.md.
-PCG_STEP_TRAFFIC_BYTES = 35.89e6
+PCG_STEP_TRAFFIC_BYTES = 35.10e6
 
 
 def cpu_model() -> str:
