@@ -223,9 +223,9 @@ int drape_solver_pcg_shape(const drape_solver* s, int32_t* waves, int32_t* slots
  * mean duration in microseconds and their mean algorithmic bytes: each timed launch counts the scenes still solving
  * at the host's last look at the convergence flags (all of them when nothing was timed). The matrix is stored
  * symmetric: 40 B per block on or above the diagonal (values and column index) and 8 B per slot below it (column
- * and the index of the transposed block); plus per particle 204 B for the one-kernel form (zr, t, p gathered; r,
- * ap, step, preconditioner, contact and row flags read; r, ap, p, zr, t, step written) or 56 B for the two-kernel
- * SpMV (row and contact flags, z and p_old gathered, p_new and ap written). */
+ * and the index of the transposed block); plus per particle 192 B for the one-kernel form (zr, t, p gathered; r,
+ * ap, step, preconditioner (6 floats), contact and row flags read; r, ap, p, zr, t, step written) or 56 B for the
+ * two-kernel SpMV (row and contact flags, z and p_old gathered, p_new and ap written). */
 int drape_solver_set_spmv_sampling(drape_solver* s, int32_t every);
 int drape_solver_spmv_timing(const drape_solver* s, uint64_t* launches, double* mean_us, double* bytes_per_launch);
 
