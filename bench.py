@@ -64,18 +64,18 @@ def cpu_baseline(make_scene, steps: int) -> dict:
 WEIGHTS = os.path.join(REPO, "drape-2_amd", "drape_amd", "weights", "warmstart.npy")
 
 
-def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 5), judge_iterations: int = 15,
+def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 5), ref_iterations: int = 15,
                   sample_every: int = 4) -> dict:
     """Learned warm start against the previous-velocity guess. Each method steps the drop with Newton capped at
-    each of `caps`; every `sample_every`-th timed step a tightly converged plain solve (Newton <= judge_iterations,
-    same PCG) from that method's own state judges its result (max particle distance, mm). Reports steps/s over the
-    timed steps (the judge excluded), iteration counts and how often the energy check kept the learned guess."""
+    each of `caps`; every `sample_every`-th timed step a tightly converged plain solve (Newton <= ref_iterations,
+    same PCG) from that method's own state measures its error (max particle distance, mm). Reports steps/s over the
+    timed steps (the reference solves excluded), iteration counts and how often the energy check kept the learned guess."""
     from drape_amd import HipSolver, learn
     net = learn.WarmStartNet()
     net.load(net_path)
     sc = make_scene()
-    judge = HipSolver()
-    judge.build(sc)
+    ref = HipSolver()
+    ref.build(sc)
     runs = []
     for learned in (False, True):
         for k in caps:
@@ -87,9 +87,9 @@ def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 5
             for i in range(warmup + steps):
                 check = i >= warmup and (i - warmup) % sample_every == 0
                 if check:
-                    judge.restore(s.snapshot())
-                    judge.step(1 / 60, 1, judge_iterations)
-                    xj = judge.read_positions()
+                    ref.restore(s.snapshot())
+                    ref.step(1 / 60, 1, ref_iterations)
+                    xj = ref.read_positions()
                 if i == warmup:
                     s.reset_counters()
                 t0 = time.perf_counter()
@@ -104,10 +104,10 @@ def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 5
                          "steps_per_s": steps / t, "newton_per_step": c["newton_iterations"] / steps,
                          "pcg_per_step": c["pcg_iterations"] / steps,
                          "max_step_error_mm": max(errs), "median_step_error_mm": float(np.median(errs)),
-                         "steps_within_0p1mm": int(sum(e <= 0.1 for e in errs)), "judged_steps": len(errs),
+                         "steps_within_0p1mm": int(sum(e <= 0.1 for e in errs)), "checked_steps": len(errs),
                          "learned_guess_kept": c["warm_start_accepted"],
                          "learned_guess_rejected": c["warm_start_fallbacks"]})
-    return {"judge": f"plain solve with Newton <= {judge_iterations} from the method's own state, "
+    return {"error_reference": f"plain solve with Newton <= {ref_iterations} from the method's own state, "
                      f"every {sample_every} timed steps",
             "weights": os.path.relpath(net_path, REPO), "runs": runs}
 

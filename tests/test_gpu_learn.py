@@ -135,9 +135,9 @@ def test_trained_network_guess_beats_previous_velocity_guess(hip):
     assert ratio < 0.5
 
 
-def _judged_errors(hip, sc, net, cap, steps=60, every=4, judge_iterations=15):
-    judge = hip.HipSolver()
-    judge.build(sc)
+def _step_errors(hip, sc, net, cap, steps=60, every=4, ref_iterations=15):
+    ref = hip.HipSolver()
+    ref.build(sc)
     s = hip.HipSolver()
     s.build(sc)
     if net is not None:
@@ -145,9 +145,9 @@ def _judged_errors(hip, sc, net, cap, steps=60, every=4, judge_iterations=15):
     errs = []
     for i in range(steps):
         if i % every == 2:
-            judge.restore(s.snapshot())
-            judge.step(1 / 60, 1, judge_iterations)
-            xj = judge.read_positions()
+            ref.restore(s.snapshot())
+            ref.step(1 / 60, 1, ref_iterations)
+            xj = ref.read_positions()
         s.step(1 / 60, 1, cap)
         if i % every == 2:
             errs.append(float(np.max(np.linalg.norm(s.read_positions().astype(np.float64) - xj, axis=1))))
@@ -155,7 +155,7 @@ def _judged_errors(hip, sc, net, cap, steps=60, every=4, judge_iterations=15):
 
 
 def test_shipped_weights_keep_the_sheet_step_within_0p1mm(hip):
-    """The shipped network (drape_amd/weights) on the 32x32 sheet drop: with Newton <= 5 every judged step lies
+    """The shipped network (drape_amd/weights) on the 32x32 sheet drop: with Newton <= 5 every checked step lies
     within 0.1 mm of a tightly converged solve (Newton <= 15) from the same state, where the previous-velocity
     guess does not (one step of its drop lands 0.87 mm off)."""
     import os
@@ -164,9 +164,9 @@ def test_shipped_weights_keep_the_sheet_step_within_0p1mm(hip):
     net = learn.WarmStartNet()
     net.load(path)
     sc = scenes.c1_sheet()
-    plain5, _ = _judged_errors(hip, sc, None, 5)
-    learned5, c5 = _judged_errors(hip, sc, net, 5)
-    print(f"max judged error (mm): plain Newton 5 {plain5.max() * 1e3:.4f}, learned Newton 5 "
+    plain5, _ = _step_errors(hip, sc, None, 5)
+    learned5, c5 = _step_errors(hip, sc, net, 5)
+    print(f"max step error (mm): plain Newton 5 {plain5.max() * 1e3:.4f}, learned Newton 5 "
           f"{learned5.max() * 1e3:.4f}; learned guess kept {c5['warm_start_accepted']} / {c5['substeps']}",
           flush=True)
     assert learned5.max() <= 1e-4
