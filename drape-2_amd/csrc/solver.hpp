@@ -136,7 +136,10 @@ class HipSolver {
   double tuneMs_[kShapeCount] = {};
   uint32_t tuneLaunches_[kShapeCount] = {};
   bool shapeAvailable(int i) const { return !dk::kPcgShapes[i].halo || view_.tileHalo != nullptr; }
-  static constexpr uint32_t kGraphChunk = 16;  // PCG iterations per captured graph (even: parity pattern repeats)
+  // PCG iterations per captured graph, i.e. between two looks at the convergence flags after a solve's first
+  // batch (even: the parity pattern repeats). Measured on the 65k sheet over 8 ... 64: a look at the flags costs
+  // more than the launches that find the solve already ended.
+  static constexpr uint32_t kGraphChunk = 64;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graphExec_ = nullptr;
   // One Newton iteration's launches before the PCG solve (activate, assemble, e0) and after it (contact force,
