@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 BUILD = os.path.join(HERE, "build")
 SOURCES = ["csrc/kernels.hip", "csrc/warmstart.hip", "csrc/normals.hip", "csrc/bake.hip", "csrc/solver.cpp", "csrc/model.cpp", "csrc/capi.cpp"]
-HEADERS = ["csrc/layout.hpp", "csrc/kernels.hpp", "csrc/term_math.hpp", "csrc/model.hpp", "csrc/solver.hpp",
+HEADERS = ["csrc/layout.hpp", "csrc/kernels.hpp", "csrc/terms.hpp", "csrc/field.hpp", "csrc/model.hpp", "csrc/solver.hpp",
            "csrc/device_buffer.hpp", "csrc/warmstart.hpp", "csrc/normals.hpp", "csrc/bake.hpp",
            os.path.join(REPO, "include", "drape_hip.h")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -20,7 +20,8 @@
 #include <vector>
 
 #include "kernels.hpp"
-#include "term_math.hpp"
+#include "field.hpp"
+#include "terms.hpp"
 
 namespace dk {
 
@@ -177,169 +178,274 @@ __device__ inline F3 sellRowProduct(const View& v, uint32_t slice, uint32_t lane
   return f3(a0, a1, a2);
 }
 
-__device__ inline float contactRadius(const SegParams& P, const Fabric32& f) {
+// ---------------------------------------------------------------------------------------------------------------
+// Contact planes and the per-row substep kernels.
+
+// Contact radius of a particle: the segment's collision offset plus half its fabric's thickness (sim_model.hpp:71).
+__device__ inline float fabricRadius(const SegParams& P, const Fabric32& f) {
   return P.collisionOffset + 0.5f * f.thickness;
 }
 
-// Term vertex position relative to the term's first vertex at substep start: (xStart_k - xStart_v0) + u_k
-// (kernels.h:131-134). `alpha` > 0 evaluates at the line-search trial u + alpha step.
-__device__ inline F3 termPos(const View& v, uint32_t v0, uint32_t vk, float alpha) {
-  F3 uk = v.u[vk];
-  if (alpha != 0.0f) uk = uk + alpha * v.step[vk];
-  return (v.xStart[vk] - v.xStart[v0]) + uk;
+// A held particle's vector with its component along the contact normal removed (the constraint filter of the
+// solve, cpu_solver.cpp:69-73). The plane is stored as (normal, offset) in one float4.
+__device__ inline F3 onPlane(uint32_t cflags, float4 plane, F3 w) {
+  if (!(cflags & kContactActive)) return w;
+  const F3 n = f3(plane.x, plane.y, plane.z);
+  return w - dot(n, w) * n;
 }
 
-// ---------------------------------------------------------------------------------------------------------------
-// Substep setup: predict + contact setup (kernels.h:34-82; cpu_solver.cpp:234-263).
+// filter(M w) with the block-Jacobi inverse M stored as its upper triangle (00 01 02 11 12 22)
+// (cpu_solver.cpp:179-181).
+__device__ inline F3 applyPrecond(const float* m, uint32_t cflags, float4 plane, F3 w) {
+  const F3 y = f3(m[0] * w.x + m[1] * w.y + m[2] * w.z, m[1] * w.x + m[3] * w.y + m[4] * w.z,
+                  m[2] * w.x + m[4] * w.y + m[5] * w.z);
+  return onPlane(cflags, plane, y);
+}
 
-__global__ void __launch_bounds__(256) kPredictContact(View v) {
+// Substep start (cpu_solver.cpp:234-263). Per row: the start position; the inertial target relative to it,
+// h v + h^2 (g + f/m), with gravity only while the garment settles; the Newton guess u = h (v + dv), where dv = 0
+// is the reference's previous-velocity guess and a learned warm start supplies dv otherwise; and, for a particle
+// within reach of the body, its contact plane: the field's tangent plane at the start position, offset by the
+// contact radius, with friction weighted by the normal force of the previous solve. Pinned and padding rows have
+// no target, no guess and no contact.
+__global__ void __launch_bounds__(256) kSubstepBegin(View v) {
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
   const uint32_t s = v.tileSeg[tile];
   const SegParams& P = v.seg[s];
-  SegState* S = v.state + s;
   if (tile == P.tileBegin && threadIdx.x == 0) {
+    SegState* S = v.state + s;
     S->newtonDone = 0u;
     S->pcgConverged = 0u;
     S->lsDone = 0u;
     S->cntSubsteps += 1u;
   }
   const uint32_t row = tile * kTile + threadIdx.x;
-  const uint32_t flags = v.rowFlags[row];
-  const F3 xs = v.x[row];
-  v.xStart[row] = xs;
-  const float lastForce = v.cforce[row];
-  uint32_t cf = 0u;
-  float lambda = 0.0f;
+  const F3 here = v.x[row];
+  const float priorForce = v.cforce[row];
+  v.xStart[row] = here;
   v.cforce[row] = 0.0f;
-  if (flags & (kRowPinned | kRowPad)) {
-    v.yRel[row] = f3(0, 0, 0);
-    v.u[row] = f3(0, 0, 0);
-  } else {
+  F3 target = f3(0, 0, 0), guess = f3(0, 0, 0);
+  uint32_t contact = 0u;
+  float frictionWeight = 0.0f;
+  if (!(v.rowFlags[row] & (kRowPinned | kRowPad))) {
     const float h = P.h;
     const F3 vel = v.v[row];
     const F3 g = v.garments[v.rowGarment[row]].phase == 1u ? f3(P.gx, P.gy, P.gz) : f3(0, 0, 0);
-    v.yRel[row] = h * vel + (h * h) * (g + v.fExt[row] / v.mass[row]);
-    // Warm start u = h (v + dv): dv = 0 is the reference's previous-velocity guess (cpu_solver.cpp:244).
-    v.u[row] = h * (vel + v.warm[row]);
+    const F3 f = v.fExt[row];
+    const float invMass = 1.0f / v.mass[row];
+    target = h * vel + (h * h) * (g + invMass * f);
+    guess = h * (vel + v.warm[row]);
     if (P.field.present) {
-      const Fabric32 fab = v.fabrics[v.rowFabric[row]];
-      const float radius = contactRadius(P, fab);
-      const float phi = fieldSample(v.fieldValues, P.field, xs);
-      if (phi < radius + P.contactReach) {
-        const F3 n = fieldGradient(v.fieldValues, P.field, xs);
-        if (!isZero(n)) {
-          cf = kContactNear;
-          v.cnormal[row] = make_float4(n.x, n.y, n.z, radius - phi);
-          lambda = lastForce;
-        }
+      const float radius = fabricRadius(P, v.fabrics[v.rowFabric[row]]);
+      const float dist = bodyDistance(v.fieldValues, P.field, here);
+      const F3 n = dist < radius + P.contactReach ? bodyNormal(v.fieldValues, P.field, here) : f3(0, 0, 0);
+      if (!allZero(n)) {
+        contact = kContactNear;
+        v.cnormal[row] = make_float4(n.x, n.y, n.z, radius - dist);
+        frictionWeight = priorForce;
       }
     }
   }
-  v.cflags[row] = cf;
-  v.clambda[row] = lambda;
+  v.yRel[row] = target;
+  v.u[row] = guess;
+  v.cflags[row] = contact;
+  v.clambda[row] = frictionWeight;
 }
 
-// Holds near particles on their contact plane, projecting any below it (kernels.h:86-99; cpu_solver.cpp:218-223).
-__global__ void __launch_bounds__(256) kActivate(View v) {
+// Newton iteration start (cpu_solver.cpp:217-223): a particle near the body is held on its contact plane once its
+// guess is within the snap tolerance of it, and stays held until the assembly releases it; a held guess below the
+// plane is moved onto it.
+__global__ void __launch_bounds__(256) kHoldContacts(View v) {
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
   const uint32_t s = v.tileSeg[tile];
   if (v.state[s].newtonDone) return;
-  const SegParams& P = v.seg[s];
   const uint32_t row = tile * kTile + threadIdx.x;
-  uint32_t cf = v.cflags[row];
+  const uint32_t cf = v.cflags[row];
   if (!(cf & kContactNear)) return;
-  const float4 cn = v.cnormal[row];
-  const F3 n = f3(cn.x, cn.y, cn.z);
-  F3 ui = v.u[row];
-  const float gap = cn.w - dot(n, ui);
-  if (gap > -P.contactSnap) cf |= kContactActive;
-  if ((cf & kContactActive) && gap > 0.0f) {
-    ui += gap * n;
-    v.u[row] = ui;
-  }
-  v.cflags[row] = cf;
+  const float4 plane = v.cnormal[row];
+  const F3 n = f3(plane.x, plane.y, plane.z);
+  const F3 guess = v.u[row];
+  const float depth = plane.w - dot(n, guess);  // how far the guess lies below its plane
+  const bool held = (cf & kContactActive) || depth > -v.seg[s].contactSnap;
+  if (held && depth > 0.0f) v.u[row] = guess + depth * n;
+  if (held) v.cflags[row] = cf | kContactActive;
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Energy of a row's own term (inertia + friction, kernels.h:104-129) and the terms it owns (local vertex 0).
+// Energy, gradient and Hessian of the incremental potential (the terms of cpu_solver.cpp:99-138 and
+// energies.cpp), assembled atomic-free by row owners. One lane per row evaluates its own term (inertia and
+// friction) and then every term of its incidence list, in the list's order (kind, term, local vertex — the
+// reference's gather order, gpu_solver_core.cpp:184-224), and keeps only its own row of the gradient and of the
+// Hessian; a term's energy is counted by its local vertex 0.
+//
+// A term's vertices are placed relative to the term's first vertex at the substep start, x_k - xStart_0 =
+// (xStart_k - xStart_0) + u_k (+ alpha step_k in a line-search trial): the energies are translation invariant,
+// so no absolute coordinate enters the FP32 term arithmetic, and every row of a term evaluates it from identical
+// inputs — the rows' shares of a term's force then cancel as they do in exact arithmetic (a per-row origin lets
+// FP32 rounding differ between the rows and leaves a net force: a free-falling sheet drifts sideways).
+//
+// Entries are taken kIncBatch at a time: the batch's list words, then every record, slot list, fabric and vertex
+// position it needs are issued before any term is evaluated, so a lane has a batch of independent loads in flight
+// instead of one dependent chain per term. Each table is read through a selected base with a kind-dependent
+// stride (no branch on the kind before the loads), and every per-term array is indexed by compile-time indices.
 
+constexpr uint32_t kIncBatch = 4;
+constexpr uint32_t kKindNone = 3;  // an empty batch position
+
+struct IncTerm {
+  uint32_t kind, a, t;
+  uint4 w0, w1, w2;   // record words: stretch 48 B, hinge 32 B (w2 = w0), seam 16 B (w1 = w2 = w0)
+  uint32_t slot[4];   // BSR slots of blocks (a, b), b < vertices of the term
+  Fabric32 fab;       // stretch: its fabric
+  Garment32 gar;      // seam: its garment
+  F3 x[4];            // vertices relative to the row's start position
+};
+
+__device__ inline uint32_t incVertex(const IncTerm& it, int k) {
+  // stretch: w0 = (v0, v1, v2, fabric); hinge: w0 = (v0, v1, v2, v3); seam: w0 = (a, b, garment, startGap)
+  if (k == 0) return it.w0.x;
+  if (k == 1) return it.w0.y;
+  if (k == 2) return it.kind == kKindSeam ? it.w0.x : it.w0.z;
+  return it.kind == kKindHinge ? it.w0.w : it.w0.x;
+}
+
+template <bool Trial>
+__device__ inline void loadBatch(const View& v, uint32_t i, uint32_t end, float alpha, bool ownedOnly,
+                                 IncTerm (&b)[kIncBatch]) {
+#pragma unroll
+  for (uint32_t j = 0; j < kIncBatch; ++j) {
+    const uint32_t e = i + j < end ? v.inc[i + j] : 0u;
+    b[j].kind = i + j < end ? incKind(e) : kKindNone;
+    b[j].a = incLocal(e);
+    b[j].t = incTerm(e);
+    if (ownedOnly && b[j].a != 0u) b[j].kind = kKindNone;
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kIncBatch; ++j) {
+    const uint32_t k = b[j].kind, t = b[j].kind == kKindNone ? 0u : b[j].t;
+    const uint4* head = k == kKindStretch ? reinterpret_cast<const uint4*>(v.stretch) + 3u * t
+                        : k == kKindHinge ? reinterpret_cast<const uint4*>(v.hinges) + 2u * t
+                                          : reinterpret_cast<const uint4*>(v.seams) + (k == kKindSeam ? t : 0u);
+    b[j].w0 = head[0];
+    b[j].w1 = head[k == kKindSeam || k == kKindNone ? 0 : 1];
+    b[j].w2 = head[k == kKindStretch ? 2 : 0];
+    const uint32_t m = k == kKindStretch ? 3u : k == kKindHinge ? 4u : 2u;
+    const uint32_t* sl = k == kKindStretch ? v.stretchSlots + 9u * t + 3u * b[j].a
+                         : k == kKindHinge ? v.hingeSlots + 16u * t + 4u * b[j].a
+                                           : v.seamSlots + (k == kKindSeam ? 4u * t + 2u * b[j].a : 0u);
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) b[j].slot[q] = sl[q < m ? q : 0u];
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kIncBatch; ++j) {
+    const uint32_t k = b[j].kind;
+    b[j].fab = v.fabrics[k == kKindStretch ? b[j].w0.w : 0u];
+    b[j].gar = v.garments[k == kKindSeam ? b[j].w0.z : 0u];
+    F3 start[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t vk = k == kKindNone ? 0u : incVertex(b[j], q);
+      F3 d = v.u[vk];
+      if (Trial) d += alpha * v.step[vk];
+      start[q] = v.xStart[vk];
+      b[j].x[q] = d;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) b[j].x[q] = (start[q] - start[0]) + b[j].x[q];
+  }
+}
+
+// Seam target gap: closing from the start gap at the closing speed while the garment assembles, zero once it
+// settles (sim_model.cpp:29-33).
+__device__ inline float seamGap(const SegParams& P, const Garment32& g, float startGap) {
+  return g.phase == 0u ? fmaxf(0.0f, startGap - P.closingSpeed * g.assemblyTime) : 0.0f;
+}
+
+// Adds block B to the row's upper slot (slot - kd) in the slice's LDS accumulator; blocks left of the diagonal are
+// dropped (their transposes are assembled by the column's row).
+__device__ inline void addUpper(float* acc, uint32_t slot, uint32_t kd, uint32_t lane, const Blk& B) {
+  if (slot < kd) return;
+  float* p = acc + ((slot - kd) * kSlice + lane) * 9u;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) p[c] += B.e[c];
+}
+
+// Evaluates one loaded entry for its row: returns the term's energy when the row is its local vertex 0; with Deriv
+// adds the row's gradient to g and its upper blocks to the accumulator.
 template <bool Deriv>
-__device__ inline float ownTerm(const View& v, const SegParams& P, uint32_t row, uint32_t flags, float alpha, F3& g,
-                                M3& H) {
+__device__ inline float evalEntry(const View& v, const SegParams& P, const IncTerm& it, F3& g, float* acc,
+                                  uint32_t kd, uint32_t lane) {
+  float energy = 0.0f;
+  if (it.kind == kKindStretch) {
+    const float dmInv[4] = {__uint_as_float(it.w1.x), __uint_as_float(it.w1.y), __uint_as_float(it.w1.z),
+                            __uint_as_float(it.w1.w)};
+    const Shape3 sh = shapeOf(dmInv);
+    const F3 xs[3] = {it.x[0], it.x[1], it.x[2]};
+    const StretchForm form = stretchForm<Deriv>(xs, sh, __uint_as_float(it.w2.x), it.fab, P.limitScale);
+    energy = form.energy;
+    if (Deriv) {
+      const StretchRow w = stretchRow(form, sh, it.a);
+      g += w.g;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) addUpper(acc, it.slot[b], kd, lane, stretchBlock(w, sh, b));
+    }
+  } else if (it.kind == kKindHinge) {
+    const HingeForm form = hingeForm(it.x);
+    const float K = __uint_as_float(it.w1.x);
+    const float bend = form.theta - __uint_as_float(it.w1.y);
+    energy = 0.5f * K * bend * bend;
+    if (Deriv) {
+      const F3 ga = pick(it.a, form.g[0], form.g[1], form.g[2], form.g[3]);
+      g += (K * bend) * ga;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) addUpper(acc, it.slot[b], kd, lane, blkOuter(ga, form.g[b], K));
+    }
+  } else if (it.kind == kKindSeam) {
+    const float target = seamGap(P, it.gar, __uint_as_float(it.w0.w));
+    const SeamForm form = seamForm<Deriv>(it.x[0], it.x[1], target, P.seamStiffness);
+    energy = form.energy;
+    if (Deriv && form.active) {
+      g += it.a == 0u ? form.g0 : -form.g0;
+      Blk neg = blkZero();
+      blkAxpy(neg, -1.0f, form.S);
+      addUpper(acc, it.slot[0], kd, lane, it.a == 0u ? form.S : neg);
+      addUpper(acc, it.slot[1], kd, lane, it.a == 0u ? neg : form.S);
+    }
+  }
+  return it.a == 0u ? energy : 0.0f;
+}
+
+// The row's own term: inertia 1/2 (m / h^2) |u - y|^2 for a free particle and smoothed Coulomb friction for one
+// with a friction weight (kernels of cpu_solver.cpp:99-108, :127-138).
+template <bool Deriv>
+__device__ inline float ownEnergy(const View& v, const SegParams& P, uint32_t row, uint32_t flags, F3 disp, F3& g,
+                                  Blk& H) {
   float e = 0.0f;
-  F3 ui = v.u[row];
-  if (alpha != 0.0f) ui = ui + alpha * v.step[row];
   if (!(flags & kRowPinned)) {
     const float k = v.mass[row] * P.invH2;
-    const F3 d = ui - v.yRel[row];
-    e = 0.5f * k * len2(d);
+    const F3 off = disp - v.yRel[row];
+    e = 0.5f * k * norm2(off);
     if (Deriv) {
-      g = k * d;
-      H = identityScaled(k);
+      g = k * off;
+      H = blkIsoPlusRank1(k, 0.0f, f3(0, 0, 0));
     }
   }
   const float lambda = v.clambda[row];
   if (lambda > 0.0f) {
-    const float4 cn = v.cnormal[row];
-    const Fabric32 fab = v.fabrics[v.rowFabric[row]];
-    e += frictionTerm<Deriv>(ui, f3(cn.x, cn.y, cn.z), fab.friction * lambda, P.frictionSlip * P.h, g, H);
+    const float4 plane = v.cnormal[row];
+    const float mu = v.fabrics[v.rowFabric[row]].friction;
+    e += frictionAdd<Deriv>(disp, f3(plane.x, plane.y, plane.z), mu * lambda, P.frictionSlip * P.h, g, H);
   }
   return e;
 }
 
-__device__ inline float seamTarget(const View& v, const SegParams& P, const Seam32& sm) {
-  const Garment32 g = v.garments[sm.garment];
-  return g.phase == 0u ? fmaxf(0.0f, sm.startGap - P.closingSpeed * g.assemblyTime) : 0.0f;
-}
-
-// Evaluates incidence `e` of `row`: returns the term energy; with Deriv adds the row's gradient to gt and its
-// blocks (row, b) to H[0..m-1], and reports m and the slot list of row a.
-template <bool Deriv>
-__device__ inline float incidence(const View& v, const SegParams& P, uint32_t e, float alpha, F3& gt, M3 H[4],
-                                  uint32_t& m, const uint32_t*& slots) {
-  const uint32_t kind = incKind(e), t = incTerm(e), a = incLocal(e);
-  if (kind == kKindStretch) {
-    const Stretch32 st = v.stretch[t];
-    F3 x[3];
-#pragma unroll
-    for (int b = 0; b < 3; ++b) x[b] = termPos(v, st.v[0], st.v[b], alpha);
-    m = 3;
-    slots = v.stretchSlots + 9u * t + 3u * a;
-    return stretchRow<Deriv>(st, x, v.fabrics[st.fabric], P.limitScale, int(a), gt, H);
-  }
-  if (kind == kKindHinge) {
-    const Hinge32 hg = v.hinges[t];
-    F3 x[4];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) x[b] = termPos(v, hg.v[0], hg.v[b], alpha);
-    m = 4;
-    slots = v.hingeSlots + 16u * t + 4u * a;
-    return hingeRow<Deriv>(hg, x, int(a), gt, H);
-  }
-  const Seam32 sm = v.seams[t];
-  F3 x[2];
-  x[0] = termPos(v, sm.a, sm.a, alpha);
-  x[1] = termPos(v, sm.a, sm.b, alpha);
-  m = 2;
-  slots = v.seamSlots + 4u * t + 2u * a;
-  return seamRow<Deriv>(x, seamTarget(v, P, sm), P.seamStiffness, int(a), gt, H);
-}
-
-// ---------------------------------------------------------------------------------------------------------------
-// Assembly: energy, gradient and the block-CSR Hessian, atomic-free. One wavefront per 64-row slice; each lane
-// owns a row, evaluates every term incident to it (kernels.h:104-224 semantics) and accumulates its upper blocks
-// (the diagonal and the columns above it) in LDS in the slice's SELL layout; the slice is then written out with
-// coalesced stores. A block below the diagonal is the transpose of its column's upper block and is not stored.
-// Contact release follows (kernels.h:227-231).
-
-__global__ void __launch_bounds__(64) kAssemble(View v) {
-  extern __shared__ float acc[];  // K * 9 * 64
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) kAssemble(View v) {
+  extern __shared__ float acc[];  // K * 9 * 64: the slice's upper blocks in SELL order
   const uint32_t slice = xcdMap(blockIdx.x, v.nSlices);
   if (slice >= v.nSlices) return;
-  const uint32_t tile = slice / kSlicesPerTile;
-  const uint32_t s = v.tileSeg[tile];
+  const uint32_t s = v.tileSeg[slice / kSlicesPerTile];
   if (v.state[s].newtonDone) return;
   const SegParams& P = v.seg[s];
   const uint32_t lane = threadIdx.x;
@@ -348,64 +454,47 @@ __global__ void __launch_bounds__(64) kAssemble(View v) {
   for (uint32_t i = lane; i < K * 9u * kSlice; i += kSlice) acc[i] = 0.0f;
   __syncthreads();
   const uint32_t flags = v.rowFlags[row];
-  double rowE = 0.0;
+  double rowEnergy = 0.0;
   F3 g = f3(0, 0, 0);
   if (!(flags & kRowPad)) {
-    const uint32_t kd = v.diagSlot[row];  // upper slot k of the row is BSR slot kd + k
-    M3 Hown = zeroM();
-    rowE += double(ownTerm<true>(v, P, row, flags, 0.0f, g, Hown));
+    const uint32_t kd = v.diagSlot[row];  // upper slot q of the row is BSR slot kd + q
+    Blk own = blkZero();
+    rowEnergy += double(ownEnergy<true>(v, P, row, flags, v.u[row], g, own));
+    addUpper(acc, kd, kd, lane, own);
+    const uint32_t end = v.incStart[row + 1];
+    for (uint32_t i = v.incStart[row]; i < end; i += kIncBatch) {
+      IncTerm batch[kIncBatch];
+      loadBatch<false>(v, i, end, 0.0f, false, batch);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) acc[lane * 9u + c] += Hown.m[c];
-    const uint32_t i1 = v.incStart[row + 1];
-    for (uint32_t i = v.incStart[row]; i < i1; ++i) {
-      const uint32_t e = v.inc[i];
-      F3 gt = f3(0, 0, 0);
-      M3 H[4] = {zeroM(), zeroM(), zeroM(), zeroM()};
-      uint32_t m;
-      const uint32_t* slots;
-      const float E = incidence<true>(v, P, e, 0.0f, gt, H, m, slots);
-      if (incLocal(e) == 0u) rowE += double(E);
-      g += gt;
-      for (uint32_t b = 0; b < m; ++b) {
-        if (slots[b] < kd) continue;
-        const uint32_t kk = slots[b] - kd;
-#pragma unroll
-        for (int c = 0; c < 9; ++c) acc[(kk * kSlice + lane) * 9u + c] += H[b].m[c];
-      }
+      for (uint32_t j = 0; j < kIncBatch; ++j) rowEnergy += double(evalEntry<true>(v, P, batch[j], g, acc, kd, lane));
     }
   }
   if (flags & kRowPinned) g = f3(0, 0, 0);
   v.grad[row] = g;
-  uint32_t cf = v.cflags[row];
-  const float4 cn = v.cnormal[row];
-  if ((cf & kContactActive) && dot(g, f3(cn.x, cn.y, cn.z)) < 0.0f) {
-    cf &= ~kContactActive;
-    v.cflags[row] = cf;
-  }
-  rowE = waveSum(rowE);
-  if (lane == 0) v.partE[slice] = rowE;
+  // Contact release (cpu_solver.cpp:225-226): a held particle whose other forces pull it off the body is freed.
+  const uint32_t cf = v.cflags[row];
+  const float4 plane = v.cnormal[row];
+  if ((cf & kContactActive) && dot(g, f3(plane.x, plane.y, plane.z)) < 0.0f) v.cflags[row] = cf & ~kContactActive;
+  rowEnergy = waveSum(rowEnergy);
+  if (lane == 0) v.partE[slice] = rowEnergy;
   __syncthreads();
-  // Block-Jacobi preconditioner from the finished diagonal block: its inverse, the identity when it is not
-  // positive definite, zero for pinned rows (cpu_solver.cpp:171-178; kernels.h:237-248). Stored as the upper
-  // triangle; the solve applies the contact filter after it (precondApply).
-  {
-    float M[9];
-    if (flags & kRowPinned) {
+  // Block-Jacobi preconditioner from the finished diagonal block: the inverse when it is positive definite, the
+  // identity otherwise, zero for a pinned row (cpu_solver.cpp:171-178); stored as its upper triangle.
+  float inv[kPrecondFloats] = {1.0f, 0.0f, 0.0f, 1.0f, 0.0f, 1.0f};
+  if (flags & kRowPinned) {
 #pragma unroll
-      for (int c = 0; c < 9; ++c) M[c] = 0.0f;
-    } else {
-      float A[9];
+    for (uint32_t c = 0; c < kPrecondFloats; ++c) inv[c] = 0.0f;
+  } else {
+    float diag[9];
 #pragma unroll
-      for (int c = 0; c < 9; ++c) A[c] = acc[lane * 9u + c];
-      if (!invertSpd3(A, M)) {
+    for (int c = 0; c < 9; ++c) diag[c] = acc[lane * 9u + c];
+    float m[kPrecondFloats];
+    if (spdInverseUpper(diag, m))
 #pragma unroll
-        for (int c = 0; c < 9; ++c) M[c] = (c % 4 == 0) ? 1.0f : 0.0f;
-      }
-    }
-    const float up[kPrecondFloats] = {M[0], M[1], M[2], M[4], M[5], M[8]};
-#pragma unroll
-    for (uint32_t c = 0; c < kPrecondFloats; ++c) v.precond[kPrecondFloats * row + c] = up[c];
+      for (uint32_t c = 0; c < kPrecondFloats; ++c) inv[c] = m[c];
   }
+#pragma unroll
+  for (uint32_t c = 0; c < kPrecondFloats; ++c) v.precond[kPrecondFloats * row + c] = inv[c];
   float* out = v.sellVal + v.sellBase[slice];
   for (uint32_t i = lane; i < K * 9u * kSlice; i += kSlice) out[i] = acc[i];
 }
@@ -444,8 +533,8 @@ __global__ void __launch_bounds__(256) kPcgInit(View v) {
   const uint32_t row = tile * kTile + threadIdx.x;
   const uint32_t cf = v.cflags[row];
   const float4 cn = v.cnormal[row];
-  const F3 ri = filterContact(cf, cn, -v.grad[row]);
-  const F3 zi = precondApply(v.precond + kPrecondFloats * row, cf, cn, ri);
+  const F3 ri = onPlane(cf, cn, -v.grad[row]);
+  const F3 zi = applyPrecond(v.precond + kPrecondFloats * row, cf, cn, ri);
   v.step[row] = f3(0, 0, 0);
   v.r[row] = ri;
   v.z[row] = zi;
@@ -515,7 +604,7 @@ __global__ void __launch_bounds__(256) kPcgSpmv(View v, uint32_t k, const F3* __
   F3 acc = (flags & kRowPinned) ? f3(0, 0, 0) : prod;
   const F3 pi = z[row] + b * pOld[row];
   pNew[row] = pi;
-  const F3 api = filterContact(v.cflags[row], v.cnormal[row], acc);
+  const F3 api = onPlane(v.cflags[row], v.cnormal[row], acc);
   v.ap[row] = api;
   const double pap = double(pi.x) * api.x + double(pi.y) * api.y + double(pi.z) * api.z;
   const double sp = blockSum256(pap, lds);
@@ -550,7 +639,7 @@ __global__ void __launch_bounds__(256) kPcgUpdate(View v, uint32_t k, const F3* 
   v.step[row] = v.step[row] + a * pi;
   const F3 ri = v.r[row] - a * v.ap[row];
   v.r[row] = ri;
-  const F3 zi = precondApply(v.precond + kPrecondFloats * row, v.cflags[row], v.cnormal[row], ri);
+  const F3 zi = applyPrecond(v.precond + kPrecondFloats * row, v.cflags[row], v.cnormal[row], ri);
   v.z[row] = zi;
   const double rr = double(ri.x) * ri.x + double(ri.y) * ri.y + double(ri.z) * ri.z;
   const double rz = double(ri.x) * zi.x + double(ri.y) * zi.y + double(ri.z) * zi.z;
@@ -662,8 +751,8 @@ __global__ void __launch_bounds__(256) kPcgStepInit(View v) {
   const uint32_t row = tile * kTile + threadIdx.x;
   const uint32_t cf = v.cflags[row];
   const float4 cn = v.cnormal[row];
-  const F3 r = filterContact(cf, cn, -v.grad[row]);
-  const F3 zr = precondApply(v.precond + kPrecondFloats * row, cf, cn, r);
+  const F3 r = onPlane(cf, cn, -v.grad[row]);
+  const F3 zr = applyPrecond(v.precond + kPrecondFloats * row, cf, cn, r);
   const F3 zero = f3(0, 0, 0);
   v.cgR[1][row] = r;
   v.cgAp[1][row] = zero;
@@ -1045,9 +1134,9 @@ kPcgStep(View v, uint32_t q) {
   if (part == 0) {
     const F3 rk = oR - a * oAp;
     const F3 pk = pcgDirection(oZ, oT, oP, a, b);
-    const F3 apk = (oFlags & kRowPinned) ? f3(0, 0, 0) : filterContact(oCf, oCn, f3(a0, a1, a2));
-    const F3 zr = precondApply(m, oCf, oCn, rk);
-    const F3 tk = precondApply(m, oCf, oCn, apk);
+    const F3 apk = (oFlags & kRowPinned) ? f3(0, 0, 0) : onPlane(oCf, oCn, f3(a0, a1, a2));
+    const F3 zr = applyPrecond(m, oCf, oCn, rk);
+    const F3 tk = applyPrecond(m, oCf, oCn, apk);
     storeStreaming(par(v.cgR, q) + row, rk);
     storeStreaming(par(v.cgAp, q) + row, apk);
     storeStreaming(par(v.cgP, q) + row, pk);
@@ -1117,8 +1206,9 @@ __global__ void __launch_bounds__(256) kNewtonSlope(View v) {
   }
 }
 
-// Line-search trial energy at u + alpha step (kernels.h:399-403 + energy-only term evaluation).
-__global__ void __launch_bounds__(256) kEnergyTrial(View v) {
+// Line-search trial (cpu_solver.cpp:269-274): the incremental potential at u + alpha step, energy only. Each term
+// is evaluated once, by its local vertex 0's row, at the same relative placement the assembly uses.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) kEnergyTrial(View v) {
   __shared__ double lds[8];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
@@ -1129,24 +1219,23 @@ __global__ void __launch_bounds__(256) kEnergyTrial(View v) {
   const float alpha = float(S->lsAlpha);
   const uint32_t row = tile * kTile + threadIdx.x;
   const uint32_t flags = v.rowFlags[row];
-  double rowE = 0.0;
+  double rowEnergy = 0.0;
   if (!(flags & kRowPad)) {
-    F3 g;
-    M3 H;
-    rowE += double(ownTerm<false>(v, P, row, flags, alpha, g, H));
-    const uint32_t i1 = v.incStart[row + 1];
-    for (uint32_t i = v.incStart[row]; i < i1; ++i) {
-      const uint32_t e = v.inc[i];
-      if (incLocal(e) != 0u) continue;
-      F3 gt;
-      M3 Hs[4];
-      uint32_t m;
-      const uint32_t* slots;
-      rowE += double(incidence<false>(v, P, e, alpha, gt, Hs, m, slots));
+    F3 g = f3(0, 0, 0);
+    Blk unused;
+    const F3 disp = v.u[row] + alpha * v.step[row];
+    rowEnergy += double(ownEnergy<false>(v, P, row, flags, disp, g, unused));
+    const uint32_t end = v.incStart[row + 1];
+    for (uint32_t i = v.incStart[row]; i < end; i += kIncBatch) {
+      IncTerm batch[kIncBatch];
+      loadBatch<true>(v, i, end, alpha, true, batch);
+#pragma unroll
+      for (uint32_t j = 0; j < kIncBatch; ++j)
+        rowEnergy += double(evalEntry<false>(v, P, batch[j], g, nullptr, 0u, 0u));
     }
   }
-  const double se = blockSum256(rowE, lds);
-  if (threadIdx.x == 0) v.partC[tile] = se;
+  const double total = blockSum256(rowEnergy, lds);
+  if (threadIdx.x == 0) v.partC[tile] = total;
 }
 
 // Armijo test E(trial) <= e0 + 1e-4 alpha slope; else halve, at most 20 trials (cpu_solver.cpp:269-274).
@@ -1171,10 +1260,11 @@ __global__ void __launch_bounds__(256) kLineSearchDecide(View v) {
   }
 }
 
-// u += alpha step; max |move|; friction for the next Newton iteration uses this solve's normal forces
-// (kernels.h:407-414; cpu_solver.cpp:275-282). Runs once per Newton iteration, for a segment whose line search
-// has decided (lsDone == 1) and not yet been applied (kNewtonEnd then sets lsDone = 2): a captured sequence can
-// launch it behind a fixed number of trials and launch it again after further trials.
+// Accepts the line search's step (cpu_solver.cpp:275-282): u += alpha step, the largest component of the move
+// into the tile's partial, and the friction weight of the next Newton iteration from this solve's normal force
+// (zero for a particle no longer held). Acts once per Newton iteration, on a segment whose line search has
+// decided (lsDone == 1) and whose step is not yet taken (kNewtonEnd sets lsDone = 2), so a captured sequence may
+// launch it behind a fixed number of trials and again after further trials.
 __global__ void __launch_bounds__(256) kApply(View v) {
   __shared__ float lds[8];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
@@ -1182,96 +1272,78 @@ __global__ void __launch_bounds__(256) kApply(View v) {
   const uint32_t s = v.tileSeg[tile];
   const SegState* S = v.state + s;
   if (S->newtonDone || S->lsDone != 1u) return;
-  const float alpha = float(S->lsAlpha);
   const uint32_t row = tile * kTile + threadIdx.x;
-  const F3 d = alpha * v.step[row];
-  v.u[row] = v.u[row] + d;
-  const float moved = fmaxf(fabsf(d.x), fmaxf(fabsf(d.y), fabsf(d.z)));
+  const F3 move = float(S->lsAlpha) * v.step[row];
+  v.u[row] += move;
   v.clambda[row] = (v.cflags[row] & kContactActive) ? v.cforce[row] : 0.0f;
-  const float m = blockMax256(moved, lds);
-  if (threadIdx.x == 0) v.partMax[tile] = m;
+  const float largest = blockMax256(fmaxf(fmaxf(fabsf(move.x), fabsf(move.y)), fabsf(move.z)), lds);
+  if (threadIdx.x == 0) v.partMax[tile] = largest;
 }
 
-// Stops the Newton loop once the largest move is below 1e-9 m (cpu_solver.cpp:283); marks the step applied.
+// Per segment: the Newton loop ends once no particle moved by 1e-9 m (cpu_solver.cpp:283); the step is marked
+// taken.
 __global__ void __launch_bounds__(256) kNewtonEnd(View v) {
   __shared__ float lds[8];
   const uint32_t s = blockIdx.x;
   SegState* S = v.state + s;
   if (S->newtonDone || S->lsDone != 1u) return;
   const SegParams& P = v.seg[s];
-  const float m = rangeMax256(v.partMax, P.tileBegin, P.tileEnd, lds);
+  const float largest = rangeMax256(v.partMax, P.tileBegin, P.tileEnd, lds);
   if (threadIdx.x == 0) {
-    S->maxMove = m;
+    S->maxMove = largest;
     S->lsDone = 2u;
-    if (double(m) < 1e-9) S->newtonDone = 1u;
+    S->newtonDone = double(largest) < 1e-9 ? 1u : 0u;
   }
 }
 
-// x = xStart + u with the post-solve safety push-out, pins, damped velocity (kernels.h:419-448;
-// cpu_solver.cpp:285-310).
+// Substep end (cpu_solver.cpp:285-310): x = xStart + u. A free particle that still ends deeper in the body than
+// the snap tolerance is moved out (stopped where its path met the body when the field gives no direction there);
+// pins hold; the velocity is the substep's displacement over h, damped by max(0, 1 - c h) with c the fabric's
+// damping rate (plus the assembly damping while the garment assembles). The velocity change the substep realised
+// (u / h - v) is kept as a feature of the learned warm start.
 __global__ void __launch_bounds__(256) kFinish(View v) {
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
-  const uint32_t s = v.tileSeg[tile];
-  const SegParams& P = v.seg[s];
+  const SegParams& P = v.seg[v.tileSeg[tile]];
   const uint32_t row = tile * kTile + threadIdx.x;
   const uint32_t flags = v.rowFlags[row];
   if (flags & kRowPad) return;
   const Fabric32 fab = v.fabrics[v.rowFabric[row]];
-  const F3 xs = v.xStart[row];
-  F3 xn = xs + v.u[row];
+  const F3 from = v.xStart[row];
+  const F3 disp = v.u[row];
+  F3 end = from + disp;
   if (flags & kRowPinned) {
-    xn = v.pinPos[row];
+    end = v.pinPos[row];
   } else if (P.field.present) {
-    const float radius = contactRadius(P, fab);
-    if (fieldSample(v.fieldValues, P.field, xn) < radius - P.contactSnap) {
-      if (isZero(fieldGradient(v.fieldValues, P.field, xn))) {
-        xn = landOnBody(v.fieldValues, P.field, radius, xs, xn);
-      } else {
-        collideBody(v.fieldValues, P.field, radius, fab.friction, xs, xn);
-      }
+    const float radius = fabricRadius(P, fab);
+    const float dist = bodyDistance(v.fieldValues, P.field, end);
+    if (dist < radius - P.contactSnap) {
+      const F3 n = bodyNormal(v.fieldValues, P.field, end);
+      if (allZero(n))
+        end = stopAtBody(v.fieldValues, P.field, radius, from, end);
+      else
+        pushOut(dist, n, radius, fab.friction, from, end);
     }
   }
-  float c = P.dampingScale * fab.damping;
-  if (v.garments[v.rowGarment[row]].phase == 0u) c += P.assemblyExtraDamping;
-  const F3 vOld = v.v[row];
-  v.dvLast[row] = (flags & kRowPinned) ? f3(0, 0, 0) : v.u[row] / P.h - vOld;
-  v.x[row] = xn;
-  v.v[row] = ((xn - xs) / P.h) * fmaxf(0.0f, 1.0f - c * P.h);
+  const float rate = P.dampingScale * fab.damping +
+                     (v.garments[v.rowGarment[row]].phase == 0u ? P.assemblyExtraDamping : 0.0f);
+  const float invH = 1.0f / P.h;
+  v.dvLast[row] = (flags & kRowPinned) ? f3(0, 0, 0) : invH * disp - v.v[row];
+  v.x[row] = end;
+  v.v[row] = (invH * fmaxf(0.0f, 1.0f - rate * P.h)) * (end - from);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
 // Stats on the device (SimModel::stats, sim_model.cpp:222-241): per-tile partials, then one workgroup per segment
 // sums its tiles in a fixed order. Only the per-segment results leave the device.
 
-// CollisionField::sample (collision_field.cpp:8-27) in FP64 from the body's FP64 origin and cell.
-__device__ inline double fieldSampleF64(const float* values, const SegParams& P, F3 p) {
-  const Field32& f = P.field;
-  const double l[3] = {(double(p.x) - P.fieldOrigin[0]) / P.fieldCell, (double(p.y) - P.fieldOrigin[1]) / P.fieldCell,
-                       (double(p.z) - P.fieldOrigin[2]) / P.fieldCell};
-  const uint32_t dims[3] = {f.dx, f.dy, f.dz};
-  int i[3];
-  double t[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    if (!(l[a] >= 0.0 && l[a] <= double(dims[a]) - 1.0)) return P.fieldBand;
-    const int fl = int(floor(l[a]));
-    const int top = int(dims[a]) - 2;
-    i[a] = fl < top ? fl : top;
-    t[a] = l[a] - double(i[a]);
-  }
-  const float* v = values + f.offset;
-  double r = 0.0;
-#pragma unroll
-  for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-    for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-      for (int dx = 0; dx < 2; ++dx) {
-        const double w = (dx ? t[0] : 1.0 - t[0]) * (dy ? t[1] : 1.0 - t[1]) * (dz ? t[2] : 1.0 - t[2]);
-        r += w * double(v[(size_t(uint32_t(i[2] + dz)) * f.dy + uint32_t(i[1] + dy)) * f.dx + uint32_t(i[0] + dx)]);
-      }
-  return r;
+// The body distance in FP64 (CollisionField::sample, collision_field.cpp:8-27) from the grid's FP64 origin and
+// cell, for the settle-loop stats.
+__device__ inline double bodyDistance64(const float* values, const SegParams& P, F3 p) {
+  const double inv = 1.0 / P.fieldCell;
+  return gridSample<double>(values, P.field, (double(p.x) - P.fieldOrigin[0]) * inv,
+                            (double(p.y) - P.fieldOrigin[1]) * inv, (double(p.z) - P.fieldOrigin[2]) * inv,
+                            P.fieldBand);
 }
 
 __device__ inline bool finite3(F3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
@@ -1316,7 +1388,7 @@ __global__ void __launch_bounds__(256) kStatsTile(View v, const double* massF64,
       sum[2] = 1.0;
     }
     if (P.field.present) {
-      const double pen = -fieldSampleF64(v.fieldValues, P, x);
+      const double pen = -bodyDistance64(v.fieldValues, P, x);
       if (pen > mx[1]) mx[1] = pen;
     }
     // Seams whose first vertex is this row: seam entries end the row's sorted incidence list.
@@ -1372,8 +1444,8 @@ __global__ void __launch_bounds__(256) kWarmFeatures(View v) {
   float4 pn = make_float4(0.f, 0.f, 0.f, P.field.present ? P.field.band : 0.02f);
   if (!(v.rowFlags[row] & kRowPad) && P.field.present) {
     const F3 x = v.x[row];
-    const F3 n = fieldGradient(v.fieldValues, P.field, x);
-    pn = make_float4(n.x, n.y, n.z, fieldSample(v.fieldValues, P.field, x));
+    const F3 n = bodyNormal(v.fieldValues, P.field, x);
+    pn = make_float4(n.x, n.y, n.z, bodyDistance(v.fieldValues, P.field, x));
   }
   v.phiN[row] = pn;
 }
@@ -1515,7 +1587,18 @@ __global__ void __launch_bounds__(256) kSellToSlots(View v, float* out) {
     for (int c = 0; c < 9; ++c) out[9u * sl + c] = base[(sl - kd) * 9u * kSlice + c];
 }
 
-// Batch term evaluation (full gradient and Hessian from the row kernels, a = 0..N-1).
+// Standalone term evaluation for the FP32-vs-FP64 checks: energy, every vertex's gradient and every block (a, b)
+// of one term per thread, from the same term forms the assembly uses. kind 0 stretch (params dmInv[4], area),
+// 1 hinge (K, rest angle), 2 seam (target gap, K), 3 friction (x = slip, params normal[3], mu lambda, eps).
+__device__ inline void putTerm(float* grad, float* hess, uint32_t i, int N, int a, F3 ga, const Blk* rowBlocks) {
+  float* gp = grad + (size_t(i) * N + a) * 3;
+  gp[0] = ga.x;
+  gp[1] = ga.y;
+  gp[2] = ga.z;
+  for (int b = 0; b < N; ++b)
+    for (int c = 0; c < 9; ++c) hess[(size_t(i) * N * N + a * N + b) * 9 + c] = rowBlocks[b].e[c];
+}
+
 __global__ void kEvalTerms(int kind, uint32_t n, const float* x, const float* params, const Fabric32* fabrics,
                            float limitScale, float* energy, float* grad, float* hess) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1525,42 +1608,52 @@ __global__ void kEvalTerms(int kind, uint32_t n, const float* x, const float* pa
   F3 xs[4];
   for (int a = 0; a < N; ++a) xs[a] = f3(x[(i * N + a) * 3], x[(i * N + a) * 3 + 1], x[(i * N + a) * 3 + 2]);
   const float* pr = params + size_t(i) * np;
-  float E = 0.0f;
-  for (int a = 0; a < N; ++a) {
-    F3 ga = f3(0, 0, 0);
-    M3 H[4] = {zeroM(), zeroM(), zeroM(), zeroM()};
-    float e = 0.0f;
-    if (kind == 0) {
-      Stretch32 t{};
-      for (int q = 0; q < 4; ++q) t.dmInv[q] = pr[q];
-      t.area = pr[4];
-      e = stretchRow<true>(t, xs, fabrics[i], limitScale, a, ga, H);
-    } else if (kind == 1) {
-      Hinge32 t{};
-      t.K = pr[0];
-      t.restAngle = pr[1];
-      e = hingeRow<true>(t, xs, a, ga, H);
-    } else if (kind == 2) {
-      e = seamRow<true>(xs, pr[0], pr[1], a, ga, H);
-    } else {
-      e = frictionTerm<true>(xs[0], f3(pr[0], pr[1], pr[2]), pr[3], pr[4], ga, H[0]);
+  Blk rows[4];
+  if (kind == 0) {
+    const float dmInv[4] = {pr[0], pr[1], pr[2], pr[3]};
+    const Shape3 sh = shapeOf(dmInv);
+    const StretchForm form = stretchForm<true>(xs, sh, pr[4], fabrics[i], limitScale);
+    energy[i] = form.energy;
+    for (int a = 0; a < 3; ++a) {
+      const StretchRow w = stretchRow(form, sh, uint32_t(a));
+      for (int b = 0; b < 3; ++b) rows[b] = stretchBlock(w, sh, b);
+      putTerm(grad, hess, i, 3, a, w.g, rows);
     }
-    if (a == 0) E = e;
-    grad[(size_t(i) * N + a) * 3 + 0] = ga.x;
-    grad[(size_t(i) * N + a) * 3 + 1] = ga.y;
-    grad[(size_t(i) * N + a) * 3 + 2] = ga.z;
-    for (int b = 0; b < N; ++b)
-      for (int c = 0; c < 9; ++c) hess[(size_t(i) * N * N + a * N + b) * 9 + c] = H[b].m[c];
+  } else if (kind == 1) {
+    const HingeForm form = hingeForm(xs);
+    const float K = pr[0], bend = form.theta - pr[1];
+    energy[i] = 0.5f * K * bend * bend;
+    for (int a = 0; a < 4; ++a) {
+      for (int b = 0; b < 4; ++b) rows[b] = blkOuter(form.g[a], form.g[b], K);
+      putTerm(grad, hess, i, 4, a, (K * bend) * form.g[a], rows);
+    }
+  } else if (kind == 2) {
+    const SeamForm form = seamForm<true>(xs[0], xs[1], pr[0], pr[1]);
+    energy[i] = form.energy;
+    Blk neg = blkZero();
+    if (form.active) blkAxpy(neg, -1.0f, form.S);
+    const Blk pos = form.active ? form.S : blkZero();
+    rows[0] = pos;
+    rows[1] = neg;
+    putTerm(grad, hess, i, 2, 0, form.g0, rows);
+    rows[0] = neg;
+    rows[1] = pos;
+    putTerm(grad, hess, i, 2, 1, -form.g0, rows);
+  } else {
+    F3 g = f3(0, 0, 0);
+    Blk H = blkZero();
+    energy[i] = frictionAdd<true>(xs[0], f3(pr[0], pr[1], pr[2]), pr[3], pr[4], g, H);
+    rows[0] = H;
+    putTerm(grad, hess, i, 1, 0, g, rows);
   }
-  energy[i] = E;
 }
 
 __global__ void kFieldQuery(const float* values, Field32 f, uint32_t n, const float* p, float* phi, float* normal) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const F3 q = f3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
-  phi[i] = fieldSample(values, f, q);
-  const F3 g = fieldGradient(values, f, q);
+  phi[i] = bodyDistance(values, f, q);
+  const F3 g = bodyNormal(values, f, q);
   normal[3 * i] = g.x;
   normal[3 * i + 1] = g.y;
   normal[3 * i + 2] = g.z;
@@ -1808,10 +1901,10 @@ __global__ void kSellCol(BuildView b, const uint32_t* sellBase, const uint32_t* 
 // ---------------------------------------------------------------------------------------------------------------
 // Launchers.
 
-void launchPredictContact(const View& v, hipStream_t st) {
-  kPredictContact<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);
+void launchSubstepBegin(const View& v, hipStream_t st) {
+  kSubstepBegin<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);
 }
-void launchActivate(const View& v, hipStream_t st) { kActivate<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
+void launchHoldContacts(const View& v, hipStream_t st) { kHoldContacts<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
 void launchAssemble(const View& v, uint32_t maxK, bool, hipStream_t st) {
   kAssemble<<<xcdGrid(v.nSlices), 64, maxK * 9u * kSlice * sizeof(float), st>>>(v);
 }

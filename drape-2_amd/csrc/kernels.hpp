@@ -121,8 +121,8 @@ struct BuildView {
 };
 
 // ---- launchers (kernels.hip); all asynchronous on `st` ----
-void launchPredictContact(const View& v, hipStream_t st);
-void launchActivate(const View& v, hipStream_t st);
+void launchSubstepBegin(const View& v, hipStream_t st);
+void launchHoldContacts(const View& v, hipStream_t st);
 void launchAssemble(const View& v, uint32_t maxK, bool contact, hipStream_t st);
 void launchNewtonBegin(const View& v, hipStream_t st);
 void launchPcgInit(const View& v, hipStream_t st);
