@@ -396,6 +396,29 @@ int drape_solver_read_structure(const drape_solver* s, uint32_t* row_start, uint
   });
 }
 
+int drape_solver_gather_sizes(const drape_solver* s, uint32_t* n_grad, uint32_t* n_hess) {
+  return guard([&] {
+    need(s, "solver");
+    std::vector<uint32_t> gs, gi, bs, bi;
+    s->impl.gatherLists(gs, gi, bs, bi);
+    if (n_grad) *n_grad = static_cast<uint32_t>(gi.size());
+    if (n_hess) *n_hess = static_cast<uint32_t>(bi.size());
+  });
+}
+
+int drape_solver_read_gather(const drape_solver* s, uint32_t* grad_start, uint32_t* grad_idx, uint32_t* block_start,
+                             uint32_t* block_idx) {
+  return guard([&] {
+    need(s, "solver");
+    std::vector<uint32_t> gs, gi, bs, bi;
+    s->impl.gatherLists(gs, gi, bs, bi);
+    if (grad_start) std::copy(gs.begin(), gs.end(), grad_start);
+    if (grad_idx) std::copy(gi.begin(), gi.end(), grad_idx);
+    if (block_start) std::copy(bs.begin(), bs.end(), block_start);
+    if (block_idx) std::copy(bi.begin(), bi.end(), block_idx);
+  });
+}
+
 int drape_solver_read_terms(const drape_solver* s, uint32_t* stretch_v, uint32_t* hinge_v, double* hinge_k,
                             uint32_t* seam_ab) {
   return guard([&] {

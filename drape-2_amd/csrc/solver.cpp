@@ -1250,6 +1250,63 @@ void HipSolver::readStructure(uint32_t* rowStart, uint32_t* colIndex, uint32_t* 
   slots(seamSlots_, 4 * model_.seams.size(), seamSlots);
 }
 
+// Each row adds its own term, then its incidence entries in list order (kind, term, local vertex a), each term's
+// block (a, c) going to slot(a, c): so a gradient's summands are [own, entries in order] and a slot's summands are
+// [own (diagonal), the entries' (a, c) landing there, in entry order] — the reference's term-ordered gather lists
+// when the device lists are sorted as the reference's are.
+void HipSolver::gatherLists(std::vector<uint32_t>& gradStart, std::vector<uint32_t>& gradIdx,
+                            std::vector<uint32_t>& blockStart, std::vector<uint32_t>& blockIdx) const {
+  requireBuilt();
+  check(hipStreamSynchronize(stream_), "sync");
+  std::vector<uint32_t> rs, ci, ds, map;
+  globalStructure(rs, ci, ds, map);
+  const uint32_t n = static_cast<uint32_t>(model_.x.size());
+  const uint32_t S = static_cast<uint32_t>(model_.stretch.size()), H = static_cast<uint32_t>(model_.hinges.size());
+  const uint32_t K = static_cast<uint32_t>(model_.seams.size());
+  std::vector<uint32_t> incStart(nRows_ + 1);
+  fromDevice(incStart.data(), incStart_.as<void>(), sizeof(uint32_t) * (nRows_ + 1));
+  std::vector<uint32_t> inc(incStart[nRows_]);
+  if (!inc.empty()) fromDevice(inc.data(), inc_.as<void>(), sizeof(uint32_t) * inc.size());
+  auto fetch = [&](const DeviceBuffer& b, size_t count) {
+    std::vector<uint32_t> v(count);
+    if (count) fromDevice(v.data(), b.as<void>(), sizeof(uint32_t) * count);
+    return v;
+  };
+  const std::vector<uint32_t> sSlots = fetch(stretchSlots_, 9ull * S), hSlots = fetch(hingeSlots_, 16ull * H);
+  const std::vector<uint32_t> kSlots = fetch(seamSlots_, 4ull * K);
+  const uint32_t nBlocks = static_cast<uint32_t>(ci.size());
+  std::vector<std::vector<uint32_t>> perSlot(nBlocks);
+  gradStart.assign(1, 0);
+  gradIdx.clear();
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t r = rowOf_[i];
+    gradIdx.push_back(i);
+    perSlot[ds[i]].push_back(i);
+    for (uint32_t q = incStart[r]; q < incStart[r + 1]; ++q) {
+      const uint32_t e = inc[q], kind = dk::incKind(e), t = dk::incTerm(e), a = dk::incLocal(e);
+      uint32_t m, contrib, hessBase;
+      const uint32_t* sl;
+      if (kind == dk::kKindStretch) {
+        m = 3, contrib = n + 3 * t + a, hessBase = n + 9 * t + 3 * a, sl = &sSlots[9 * size_t(t) + 3 * a];
+      } else if (kind == dk::kKindHinge) {
+        m = 4, contrib = n + 3 * S + 4 * t + a, hessBase = n + 9 * S + 16 * t + 4 * a, sl = &hSlots[16 * size_t(t) + 4 * a];
+      } else {
+        m = 2, contrib = n + 3 * S + 4 * H + 2 * t + a, hessBase = n + 9 * S + 16 * H + 4 * t + 2 * a,
+        sl = &kSlots[4 * size_t(t) + 2 * a];
+      }
+      gradIdx.push_back(contrib);
+      for (uint32_t c = 0; c < m; ++c) perSlot[map[sl[c]]].push_back(hessBase + c);
+    }
+    gradStart.push_back(static_cast<uint32_t>(gradIdx.size()));
+  }
+  blockStart.assign(1, 0);
+  blockIdx.clear();
+  for (const auto& l : perSlot) {
+    blockIdx.insert(blockIdx.end(), l.begin(), l.end());
+    blockStart.push_back(static_cast<uint32_t>(blockIdx.size()));
+  }
+}
+
 void HipSolver::readTerms(uint32_t* stretchV, uint32_t* hingeV, double* hingeK, uint32_t* seamAB) const {
   requireBuilt();
   for (size_t k = 0; k < model_.stretch.size(); ++k)

@@ -57,6 +57,9 @@ class HipSolver {
   void readStructure(uint32_t* rowStart, uint32_t* colIndex, uint32_t* diagSlot, uint32_t* stretchSlots,
                      uint32_t* hingeSlots, uint32_t* seamSlots) const;
   void readTerms(uint32_t* stretchV, uint32_t* hingeV, double* hingeK, uint32_t* seamAB) const;
+  // The assembly's summation order as gather lists in the reference's contribution numbering.
+  void gatherLists(std::vector<uint32_t>& gradStart, std::vector<uint32_t>& gradIdx, std::vector<uint32_t>& blockStart,
+                   std::vector<uint32_t>& blockIdx) const;
   void debugAssemble(double h, const float* u, double* energy, float* grad, float* blocks);
   void debugSpmv(const float* x, float* y);
   void debugSystem(float* blocks, float* grad, float* precond, uint32_t* contactFlags, float* contactNormals);

@@ -318,6 +318,8 @@ def _declare(lib):
         "drape_solver_structure_sizes": (C.c_int, [S, C.POINTER(StructureSizes)]),
         "drape_solver_read_structure": (C.c_int, [S, u32p, u32p, u32p, u32p, u32p, u32p]),
         "drape_solver_read_terms": (C.c_int, [S, u32p, u32p, f64p, u32p]),
+        "drape_solver_gather_sizes": (C.c_int, [S, u32p, u32p]),
+        "drape_solver_read_gather": (C.c_int, [S, u32p, u32p, u32p, u32p]),
         "drape_solver_debug_assemble": (C.c_int, [S, C.c_double, f32p, f64p, f32p, f32p]),
         "drape_solver_debug_spmv": (C.c_int, [S, f32p, f32p]),
         "drape_solver_debug_system": (C.c_int, [S, f32p, f32p, f32p, u32p, f32p]),

@@ -207,6 +207,21 @@ class HipSolver:
             "row_start", "col_index", "diag_slot", "stretch_slots", "hinge_slots", "seam_slots")]))
         return out
 
+    def gather_lists(self) -> dict:
+        """The assembly's summation order in the reference's gather-list numbering (gpu_solver_core.cpp:184-224)."""
+        ng, nh = np.zeros(1, np.uint32), np.zeros(1, np.uint32)
+        _check(_lib().drape_solver_gather_sizes(self._h, ptr(ng, C.c_uint32), ptr(nh, C.c_uint32)))
+        sz = self.structure_sizes()
+        out = {
+            "grad_start": np.empty(sz["n_particles"] + 1, np.uint32),
+            "grad_idx": np.empty(int(ng[0]), np.uint32),
+            "block_start": np.empty(sz["n_blocks"] + 1, np.uint32),
+            "block_idx": np.empty(int(nh[0]), np.uint32),
+        }
+        _check(_lib().drape_solver_read_gather(self._h, *[ptr(out[k], C.c_uint32) for k in (
+            "grad_start", "grad_idx", "block_start", "block_idx")]))
+        return out
+
     def terms(self) -> dict:
         sz = self.structure_sizes()
         out = {

@@ -244,6 +244,16 @@ int drape_solver_read_structure(const drape_solver* s, uint32_t* row_start, uint
 int drape_solver_read_terms(const drape_solver* s, uint32_t* stretch_v, uint32_t* hinge_v, double* hinge_k,
                             uint32_t* seam_ab);
 
+/* The assembly's summation order (replaces GpuSolverCore's gather lists, gpu_solver_core.cpp:184-224, in the
+ * contribution numbering of kernels.h:6-9): per particle the contributions summed into its gradient [own i,
+ * stretch n+3t+a, hinge n+3S+4h+a, seam n+3S+4H+2k+a], per BSR slot those summed into its block [diag i, stretch
+ * n+9t+3a+c, hinge n+9S+16h+4a+c, seam n+9S+16H+4k+2a+c], each in term order — derived from the incidence lists and
+ * slot tables built on the device, which fix the order the row-owner assembly adds its terms in. Sizes first; any
+ * output may be NULL. */
+int drape_solver_gather_sizes(const drape_solver* s, uint32_t* n_grad, uint32_t* n_hess);
+int drape_solver_read_gather(const drape_solver* s, uint32_t* grad_start, uint32_t* grad_idx, uint32_t* block_start,
+                             uint32_t* block_idx);
+
 /* Assembles at substep start with displacement u (3 floats per particle, relative to the current positions)
  * and no contact: total energy (FP64 sum), gradient (3 per particle) and the blocks in BSR slot order (9 per
  * block, row-major). Uses h for the inertia term. Any output may be NULL. */

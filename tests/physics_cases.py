@@ -195,3 +195,118 @@ def snapshot_replay(make):
     for _ in range(20):
         s.step(1 / 60, 1, 2)
     return first, s.read_positions()
+
+
+def test_aniso() -> dict:
+    """testAniso (libs/core/src/test_fabrics.cpp:8): testStiff with a softer weft bend."""
+    return scenes.fabric(20000, 20000, 5000, 0.02, 0.02, 100e-6, 40e-6, 0.450, 0.0008, 0.3, 1.0)
+
+
+def _cdt_garment(gid, pattern, h, fabs, offsets=None):
+    """A pattern meshed as the reference meshes it (buildSimMesh: boundary samples, hex lattice, constrained
+    Delaunay triangulation) and laid flat in the XZ plane, piece k shifted by offsets[k] along x
+    (flatGarment, sim_test_utils.hpp:15-25)."""
+    from drape_amd import garment
+    mesh = garment.build_sim_mesh(pattern, h)
+    shift = np.zeros(mesh.n) if offsets is None else np.asarray(offsets)[mesh.vertex_piece]
+    pos = np.stack([mesh.rest[:, 0] + shift, np.zeros(mesh.n), mesh.rest[:, 1]], axis=1)
+    return scenes.Garment(id=gid, mesh=mesh, initial_positions=pos, piece_fabric=list(fabs))
+
+
+def cantilever_ratio(make, fab, grain, B, h=0.002, width=0.025):
+    """Peirce cantilever (test_cantilever.cpp:17-62): a strip overhanging l = 2 (B / W)^(1/3) beyond a 1 cm clamp
+    (W = weight g), meshed at h, sags under gravity until still; the bending length c = overhang (cos(theta / 2) /
+    (8 tan theta))^(1/3) from the tip's droop angle theta gives the flexural rigidity G = W c^3. Returns (frames,
+    G / B, theta in degrees)."""
+    from drape_amd import garment
+    W = fab["weight"] * G
+    length = 2.0 * (B / W) ** (1.0 / 3.0) + 0.01
+    pat = garment.Pattern(pieces=[garment.rectangle_piece("strip", length, width, grain)], seams=[])
+    g = _cdt_garment("strip", pat, h, [fab])
+    m = g.mesh
+    pinned = m.rest[:, 0] <= 0.01
+    sc = scenes.Scene(garments=[g])
+    sc.pins = [(int(v), tuple(g.initial_positions[v])) for v in np.flatnonzero(pinned)]
+    # Clamp line: mean rest x of the pinned vertices that share a triangle with a free one.
+    frontier = np.zeros(m.n, bool)
+    for t in m.triangles:
+        if pinned[t].any() and not pinned[t].all():
+            frontier[t[pinned[t]]] = True
+    x_clamp = float(m.rest[frontier, 0].mean())
+    tip = np.flatnonzero(m.rest[:, 0] >= length - 1e-9)
+    s = make()
+    s.build(sc)
+    frames = run_until_still(s, 1 / 60, 1, 2, 2e-4, 60, 10.0)
+    x = s.read_positions().astype(np.float64)
+    tip_pos = x[tip].mean(axis=0)
+    theta = math.atan2(-tip_pos[1], tip_pos[0] - x_clamp)
+    overhang = float(m.rest[tip, 0].mean()) - x_clamp
+    c = overhang * (math.cos(theta / 2) / (8 * math.tan(theta))) ** (1.0 / 3.0)
+    return frames, W * c ** 3 / B, math.degrees(theta)
+
+
+def _tip_drop(make, garment_, tip_x):
+    """Settled tip drop of a cantilever pinned where its initial x <= 1 cm (test_seam_hinge.cpp:17-36)."""
+    pos = garment_.initial_positions
+    sc = scenes.Scene(garments=[garment_])
+    sc.pins = [(int(v), tuple(pos[v])) for v in np.flatnonzero(pos[:, 0] <= 0.01)]
+    tip = np.flatnonzero(pos[:, 0] >= tip_x - 1e-9)
+    s = make()
+    s.build(sc)
+    frames = run_until_still(s, 1 / 60, 1, 2, 2e-4, 60, 10.0)
+    return frames, float(-s.read_positions().astype(np.float64)[tip, 1].mean())
+
+
+def seam_hinge_drops(make, h=0.004):
+    """SeamDoesNotHinge (test_seam_hinge.cpp:38-59): a 6 x 2 cm testStiff cantilever in one piece, and the same
+    strip as two 3 cm pieces side by side, piece a's right edge sewn to piece b's left edge (zero-length seam
+    springs, and seam hinges across consecutive seam pairs). Returns (single drop, sewn drop, frames of each)."""
+    from drape_amd import garment
+    fab = scenes.test_stiff()
+    one = garment.Pattern(pieces=[garment.rectangle_piece("one", 0.06, 0.02)], seams=[])
+    pa, pb = garment.rectangle_piece("a", 0.03, 0.02), garment.rectangle_piece("b", 0.03, 0.02)
+    two = garment.Pattern(pieces=[pa, pb], seams=[garment.Seam("s", garment.segment_edge(pa, "right", "right", False),
+                                                               garment.segment_edge(pb, "left", "left", True))])
+    f1, d1 = _tip_drop(make, _cdt_garment("one", one, h, [fab]), 0.06)
+    f2, d2 = _tip_drop(make, _cdt_garment("two", two, h, [fab, fab], offsets=[0.0, 0.03]), 0.06)
+    return d1, d2, f1, f2
+
+
+def hanging_settles(make):
+    """HangingSheetSettlesSymmetric (test_drape_runner.cpp:108-125): the 0.5 m testJersey sheet hanging from its
+    top corners drapes to rest within 15 simulated seconds; returns (settled, centre-of-mass x, bottom-left y,
+    bottom-right y)."""
+    from drape_amd import tee
+    sc = scenes.hanging_sheet()
+    n = int(round(0.5 / 0.01)) + 1
+    s = make()
+    s.build(sc)
+    r = tee.drape_to_rest(s, ["sheet"], tee.DrapeOptions(max_sim_seconds=15.0))
+    x = s.read_positions().astype(np.float64)
+    return r.settled, float(x[:, 0].mean()), float(x[0, 1]), float(x[n - 1, 1])
+
+
+def two_triangle_hinge(L, a, hA, b, hB, grain_angle, bend_warp, bend_weft):
+    """One hinge: edge (0,0) -> (L,0), wings (a, hA) and (b, -hB), grain at grain_angle, as a one-garment scene."""
+    rest = np.array([[0.0, 0.0], [L, 0.0], [a, hA], [b, -hB]])
+    tris = np.array([[0, 1, 2], [1, 0, 3]], np.uint32)
+    area = np.full(4, 0.25 * L * (hA + hB) / 2)
+    grain = np.array([[math.cos(grain_angle), math.sin(grain_angle)]])
+    mesh = scenes.Mesh(rest=rest, vertex_piece=np.zeros(4, np.uint32), triangles=tris,
+                       triangle_piece=np.zeros(2, np.uint32), vertex_area=area, boundary=np.ones(4, np.uint8),
+                       piece_grain=grain, seam_pairs=np.zeros((0, 4), np.uint32))
+    fab = dict(scenes.test_jersey(), bend_warp=bend_warp, bend_weft=bend_weft)
+    pos = np.stack([rest[:, 0], np.zeros(4), rest[:, 1]], axis=1)
+    g = scenes.Garment(id="hinge", mesh=mesh, initial_positions=pos, piece_fabric=[fab])
+    return scenes.Scene(garments=[g])
+
+
+def hinge_k_closed_form(L, hA, hB, grain_angle, bend_warp, bend_weft):
+    """Hinge stiffness of two_triangle_hinge from the geometry: the directional rigidity of the edge (the grain
+    at angle g to the edge: p sin^2 g + q cos^2 g with p = max((3 B_warp - B_weft) / 2, f), q = max(B_weft, f),
+    f = B_min / 20; both sides of one piece, half each) times |e|^2 over the two faces' area L (hA + hB) / 2."""
+    floor = 0.05 * min(bend_warp, bend_weft)
+    p = max((3 * bend_warp - bend_weft) / 2, floor)
+    q = max(bend_weft, floor)
+    rigidity = p * math.sin(grain_angle) ** 2 + q * math.cos(grain_angle) ** 2
+    return rigidity * L * L / (0.5 * L * (hA + hB))

@@ -570,3 +570,100 @@ def test_exported_blocks_match_the_device_product_on_the_65k_sheet(hip):
           f"worst row {row_err.max():.3e}", flush=True)
     assert np.linalg.norm(y - y64) <= 1e-6 * np.linalg.norm(y64)
     assert row_err.max() <= 1e-5
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_device_gather_order_is_reference_order(hip, oracle, name):
+    """A5: the summation order fixed by the device-built incidence lists and slot tables equals the reference's
+    gather lists (gpu_solver_core.cpp:184-224) as the oracle builds them."""
+    sc = SCENES[name]()
+    g = hip.HipSolver()
+    g.build(sc)
+    o = oracle.OracleSolver()
+    o.build(sc)
+    gl, ol = g.gather_lists(), o.gather_lists()
+    for key in ("grad_start", "grad_idx", "block_start", "block_idx"):
+        np.testing.assert_array_equal(gl[key], ol[key], err_msg=key)
+
+
+@pytest.mark.parametrize("name", ["c2_sheet_65536", "c3_tube_120000"])
+def test_structure_and_gather_order_at_full_size(hip, oracle, name):
+    """A4 and A5 at the benchmark sizes: the device-built pattern, slot tables, terms and summation order equal the
+    oracle's bit for bit on the 65,536-particle sheet and the 120,000-particle seamed tube."""
+    sc = scenes.c2_sheet() if name.startswith("c2") else scenes.c3_tube()
+    g = hip.HipSolver()
+    g.build(sc)
+    o = oracle.OracleSolver()
+    o.build(sc)
+    print(f"case {name}: sizes {g.structure_sizes()}", flush=True)
+    assert g.structure_sizes() == o.structure_sizes()
+    gs, os_ = g.structure(), o.structure()
+    for key in gs:
+        np.testing.assert_array_equal(gs[key], os_[key], err_msg=key)
+    assert np.all(gs["col_index"][gs["diag_slot"]] == np.arange(sc.n))
+    gt, ot = g.terms(), o.terms()
+    for key in ("stretch_v", "hinge_v", "seam_ab"):
+        np.testing.assert_array_equal(gt[key], ot[key], err_msg=key)
+    np.testing.assert_allclose(gt["hinge_k"], ot["hinge_k"], rtol=1e-15, atol=0)
+    gl, ol = g.gather_lists(), o.gather_lists()
+    for key in ("grad_start", "grad_idx", "block_start", "block_idx"):
+        np.testing.assert_array_equal(gl[key], ol[key], err_msg=key)
+
+
+def test_c2_drop_within_oracle_spread_at_steps_10_and_20(hip):
+    """The 65,536-particle drop (the benchmark's step: dt 1/60, Newton <= 5, PCG 1e-6) against the FP64 oracle's
+    positions after 10 and 20 steps (tests/golden/c2_drop_oracle.npz, written by make_c2_drop.py). Bound:
+    max(0.1 mm, 1.5 x the oracle's own spread when its start moves by 1e-9 m), the reference's parity gate
+    (test_gpu_parity_slow.cpp:13-22) at the north star's 0.1 mm."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_drop_oracle.npz"))
+    steps = [int(k) for k in fx["steps"]]
+    g = hip.HipSolver()
+    g.build(scenes.c2_sheet())
+    for k in range(1, max(steps) + 1):
+        g.step(1 / 60, 1, 5)
+        if k in steps:
+            d = np.linalg.norm(g.read_positions().astype(np.float64) - fx[f"x{k}"].astype(np.float64), axis=1)
+            spread = float(fx[f"spread{k}"][0])
+            bound = max(1e-4, 1.5 * spread)
+            print(f"c2 step {k}: GPU vs FP64 oracle max {d.max() * 1e3:.4f} mm rms {np.sqrt(np.mean(d * d)) * 1e3:.4f} mm;"
+                  f" oracle self-spread {spread * 1e3:.4f} mm; bound {bound * 1e3:.4f} mm", flush=True)
+            assert not g.stats()["has_non_finite"]
+            assert d.max() <= bound
+
+
+def _default_scene_apart(hip, oracle, sc, frames=60, nudge=0.0):
+    """Max particle distance after `frames` frames of {1/60, 1, 2} between the oracle and either the HIP backend
+    (nudge 0) or an oracle whose start positions moved by up to `nudge` per component (test_gpu_parity.cpp:27-53)."""
+    a = oracle.OracleSolver()
+    a.build(sc)
+    if nudge > 0:
+        b = oracle.OracleSolver()
+        b.build(sc)
+        x = b.read_positions()
+        b.write_state(x=x + nudge * np.random.default_rng(1).uniform(-1, 1, size=x.shape))
+    else:
+        b = hip.HipSolver()
+        b.build(sc)
+    for _ in range(frames):
+        a.step(1 / 60, 1, 2)
+        b.step(1 / 60, 1, 2)
+    assert not b.stats()["has_non_finite"]
+    return max_distance(a.read_positions(), b.read_positions())
+
+
+@pytest.mark.parametrize("name", ["hanging_sheet", "patch_on_sphere"])
+def test_reference_default_settings_parity_60_frames(hip, oracle, name):
+    """HangingSheet60Frames and PatchOnSphere60Frames (test_gpu_parity.cpp:63-71): the reference's default settings
+    (PCG 1e-2, at most 50 iterations), 60 frames, GPU within 1 mm of the FP64 oracle; the oracle's self-spread under
+    a 1e-6 m start nudge is printed beside it (informational for the flat hanging sheet, whose out-of-plane nudge
+    buckles it)."""
+    import physics_cases as pc
+    if name == "hanging_sheet":
+        sc = scenes.hanging_sheet()
+    else:
+        sc = scenes.Scene(garments=[pc.centred_sheet(0.004, 0.004, 0.2)], body=pc.sphere_field())
+    self_spread = _default_scene_apart(hip, oracle, sc, nudge=1e-6)
+    apart = _default_scene_apart(hip, oracle, sc)
+    print(f"parity {name}: oracle self-spread {self_spread * 1e3:.3f} mm, hip vs oracle {apart * 1e3:.3f} mm", flush=True)
+    assert apart < 1e-3

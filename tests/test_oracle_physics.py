@@ -105,3 +105,55 @@ def test_stats_closed_forms(oracle):
     assert st["max_seam_gap"] == pytest.approx(0.005, rel=1e-5)
     assert st["max_body_penetration"] == 0.0
     assert st["kinetic_energy"] == 0.0 and st["rms_speed"] == 0.0
+
+
+# The reference's bending KATs: they pin the hinge stiffness K = sum over sides of 1/2 B(phi) |e|^2 / (A1 + A2)
+# (sim_model.cpp:20-27, bending.cpp:7-18) and the seam-hinge rule (sim_model.cpp:147-175) end to end.
+CANTILEVERS = {
+    "isotropic": (lambda: pc.scenes.test_stiff(), (1.0, 0.0), 100e-6),  # CantileverIsotropic
+    "aniso_warp": (pc.test_aniso, (1.0, 0.0), 100e-6),                  # CantileverAnisoWarp
+    "aniso_weft": (pc.test_aniso, (0.0, 1.0), 40e-6),                   # CantileverAnisoWeft
+}
+
+
+@pytest.mark.parametrize("case", list(CANTILEVERS))
+def test_cantilever_rigidity_within_10pct(make, case):
+    """test_cantilever.cpp:66-74: Peirce's flexural rigidity from the settled droop within 10 % of B."""
+    fab, grain, B = CANTILEVERS[case]
+    frames, ratio, theta = pc.cantilever_ratio(make, fab(), grain, B)
+    print(f"cantilever {case}: G/B {ratio:.3f}, theta {theta:.1f} deg, still after {frames} frames")
+    assert frames > 0
+    assert abs(ratio - 1.0) <= 0.10
+
+
+def test_seam_does_not_hinge(make):
+    """test_seam_hinge.cpp:38-59: a strip sewn across its middle droops like the uncut strip (0.8 .. 1.25)."""
+    single, sewn, f1, f2 = pc.seam_hinge_drops(make)
+    print(f"tip drop single {single:.4f} sewn {sewn:.4f} (frames {f1}, {f2})")
+    assert f1 > 0 and f2 > 0
+    assert 0.8 <= sewn / single <= 1.25
+
+
+def test_hanging_sheet_settles_symmetric(make):
+    """test_drape_runner.cpp:108-125: drapes to rest, centre of mass at x = 0.25 m, bottom corners level (2 mm)."""
+    settled, com_x, y_left, y_right = pc.hanging_settles(make)
+    print(f"hanging sheet: settled {settled}, com x {com_x:.5f}, bottom corners {y_left:.5f} / {y_right:.5f}")
+    assert settled
+    assert abs(com_x - 0.25) <= 0.002
+    assert abs(y_left - y_right) <= 0.002
+
+
+@pytest.mark.parametrize("L,a,hA,b,hB,grain,bw,bf", [
+    (0.01, 0.003, 0.008, 0.006, 0.012, 0.3, 2e-6, 1.5e-6),     # testJersey-like, grain off the edge
+    (0.004, 0.001, 0.004, 0.002, 0.002, 1.2, 100e-6, 40e-6),  # anisotropic, grain near the edge normal
+    (0.02, 0.01, 0.01, 0.01, 0.01, 0.0, 1e-5, 1e-5),          # isotropic, grain along the edge
+    (0.01, 0.002, 0.007, 0.009, 0.005, 0.9, 1e-6, 50e-6),     # weft much stiffer: the 1/20 floor on p binds
+])
+def test_hinge_stiffness_closed_form(oracle, L, a, hA, b, hB, grain, bw, bf):
+    """The oracle's hinge stiffness on one two-triangle hinge against the closed form from the heights and the
+    grain angle (not from cross products or grain dot products)."""
+    o = oracle.OracleSolver()
+    o.build(pc.two_triangle_hinge(L, a, hA, b, hB, grain, bw, bf))
+    k = o.terms()["hinge_k"]
+    assert len(k) == 1
+    assert k[0] == pytest.approx(pc.hinge_k_closed_form(L, hA, hB, grain, bw, bf), rel=1e-12)
