@@ -29,6 +29,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # HBM bytes per kPcgStep launch on the 256x256 sheet (one wavefront per slice), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE scaled by a plain copy's readings (x2.0, x1.0): profiles/kPcgStep_sheet65k_kernel_trace.md.
 PCG_STEP_TRAFFIC_BYTES = 35.10e6
+# Mean duration of the kPcgStep launches that ran an iteration on the 256x256 sheet in the rocprofv3 kernel trace of
+# the same command: profiles/kPcgStep_sheet65k_kernel_trace.md.
+KERNEL_TRACE_US = 9.62
 
 
 def cpu_model() -> str:
@@ -42,23 +45,41 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(make_scene, steps: int) -> dict:
+def cpu_baseline(make_scene, snapshot: bytes, runs: int = 3) -> tuple:
     """The FP64 oracle (this project's restatement of the reference's single-threaded CpuSolver) on one host
-    thread: `steps` implicit steps of the same scene from the same initial state."""
+    thread: one implicit step from the GPU's own state at the start of the timed window (its snapshot restored into
+    the oracle), `runs` times; the median. Returns the baseline and the oracle's positions after that step."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle
     pyoracle.set_threads(1)
     s = pyoracle.OracleSolver()
     s.build(make_scene())
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    secs = []
+    for _ in range(runs):
+        s.restore(snapshot)
+        c0 = s.counters()
+        t0 = time.perf_counter()
         s.step(1 / 60, 1, 5)
-    dt = time.perf_counter() - t0
+        secs.append(time.perf_counter() - t0)
     c = s.counters()
-    return {"value": steps / dt, "unit": "implicit steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} step(s) from rest of the same 65,536-particle drop, FP64 oracle, 1 thread "
-                      f"({c['newton_iterations']} Newton, {c['pcg_iterations']} PCG iterations), {cpu_model()}, "
-                      f"nproc {os.cpu_count()}"}
+    med = float(np.median(secs))
+    return ({"value": 1.0 / med, "unit": "implicit steps/s", "cores": 1, "kind": "port",
+             "sample": f"one implicit step from the GPU's state at the start of the timed window, restored into the "
+                       f"FP64 oracle, 1 thread, median of {runs} runs ({', '.join(f'{x:.2f}' for x in secs)} s; "
+                       f"{c['newton_iterations'] - c0['newton_iterations']} Newton, "
+                       f"{c['pcg_iterations'] - c0['pcg_iterations']} PCG iterations per run), {cpu_model()}, "
+                       f"nproc {os.cpu_count()}"},
+            s.read_positions().astype(np.float64))
+
+
+def survey_pcg_bytes(z: dict) -> float:
+    """Algorithmic bytes of one PCG iteration as the scope table defines them: the block SpMV
+    nnzb (36 + 4) + 4 (n + 1) + 12 n + 12 n, plus 14 vector passes of 12 n."""
+    n, nnzb = z["n_particles"], z["n_blocks"]
+    return nnzb * 40.0 + 4.0 * (n + 1) + 24.0 * n + 14.0 * 12.0 * n
+
+
+GOLDEN = os.path.join(REPO, "tests", "golden", "c2_drop_oracle.npz")
 
 
 WEIGHTS = os.path.join(REPO, "drape-2_amd", "drape_amd", "weights", "warmstart.npy")
@@ -262,12 +283,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256, help="sheet vertices per side (256 = the 65,536-particle sheet)")
-    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sample-every", type=int, default=8,
                     help="event-time PCG iteration launches (graph path: one per poll; eager: every k-th)")
     ap.add_argument("--pcg-variant", type=int, default=1, help="1 single-kernel PCG (default), 0 two-kernel")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-refinement", action="store_true", help="turn the FP64 true-residual refinement off")
     ap.add_argument("--no-warm-start-ab", action="store_true",
                     help="skip the learned-vs-plain warm start comparison")
     ap.add_argument("--warm-start-weights", default=WEIGHTS)
@@ -309,21 +330,36 @@ def main() -> int:
     shape = tuple(int(t) for t in spec.rstrip("h").split("x")) + (halo,)
     s = HipSolver(graph=not args.no_graph, pcg_variant=args.pcg_variant, pcg_shape=shape)
     s.build(sc)
+    if args.no_refinement:
+        s.set_refinement(0)
     s.set_spmv_sampling(args.sample_every)
     for _ in range(args.warmup):
         s.step(1 / 60, 1, 5)
+    window_start = s.snapshot()
     s.reset_counters()
     s.set_spmv_sampling(args.sample_every)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    fixture = np.load(GOLDEN) if args.n == 256 and os.path.exists(GOLDEN) else None
+    checkpoints, elapsed = [], 0.0
+    for i in range(args.steps):
+        t0 = time.perf_counter()
         s.step(1 / 60, 1, 5)  # returns after the device finished the step
-    elapsed = time.perf_counter() - t0
+        elapsed += time.perf_counter() - t0
+        k = args.warmup + i + 1  # steps from rest
+        if fixture is not None and k in [int(x) for x in fixture["steps"]]:
+            d = np.linalg.norm(s.read_positions().astype(np.float64) - fixture[f"x{k}"].astype(np.float64), axis=1)
+            checkpoints.append({"step": k, "max_mm": float(d.max()) * 1e3,
+                                "rms_mm": float(np.sqrt(np.mean(d * d))) * 1e3,
+                                "oracle_self_spread_mm": float(fixture[f"spread{k}"][0]) * 1e3})
     c = s.counters()
+    rep = s.pcg_report()
     timing = s.spmv_timing()
     st = s.stats()
     value = args.steps / elapsed
-    step_bytes = algorithmic_step_bytes(s, c, args.steps, timing["bytes_per_launch"])
-    achieved = timing["bytes_per_launch"] / (timing["mean_us"] * 1e-6) / 1e9 if timing["mean_us"] > 0 else None
+    z = s.structure_sizes()
+    pcg_bytes = survey_pcg_bytes(z)
+    step_bytes = algorithmic_step_bytes(s, c, args.steps, pcg_bytes)
+    per_iter_us = timing["mean_us"]
+    achieved = pcg_bytes / (per_iter_us * 1e-6) / 1e9 if per_iter_us > 0 else None
     line = {
         "metric": "implicit steps/sec @65k particles; learned-warm-start vs plain at 0.1 mm error",
         "value": value,
@@ -341,17 +377,22 @@ def main() -> int:
             "workload": f"sheet {args.n}x{args.n} ({sc.n} particles) dropped from y=0.4 m on sphere r=0.25 m + "
                         f"ground y=-0.3 m; steps {args.warmup + 1}..{args.warmup + args.steps} of the drop",
             "particles": sc.n,
-            "dt": 1 / 60, "substeps": 1, "newton_max": 5, "pcg_tolerance": 1e-6,
+            "dt": 1 / 60, "substeps": 1, "newton_max": 5, "pcg_tolerance": 1e-6, "pcg_cap": 2000,
             "solver": "plain warm start (u = h v)",
             "pcg_form": "one-kernel" if args.pcg_variant == 1 else "two-kernel",
             "pcg_launch_shape": "%dx%d%s" % (s.pcg_shape()[0], s.pcg_shape()[1], "h" if s.pcg_shape()[2] else "")
                                 if args.pcg_variant == 1 else None,
+            "fp64_true_residual_refinement": not args.no_refinement,
             "graph": not args.no_graph,
             "newton_per_step": c["newton_iterations"] / args.steps,
             "pcg_per_step": c["pcg_iterations"] / args.steps,
             "line_search_trials_per_step": c["line_search_trials"] / args.steps,
             "final_stats": st,
         },
+        "pcg": {"solves": rep["solves"], "exit_tolerance": rep["exit_tolerance"], "exit_cap": rep["exit_cap"],
+                "exit_curvature": rep["exit_curvature"], "refinement_restarts": rep["restarts"],
+                "true_rel_residual_mean": rep["true_rel_mean"], "true_rel_residual_max": rep["true_rel_max"],
+                "true_rel_residual_max_tolerance_exits": rep["true_rel_max_converged"]},
         "roofline": {
             "kernel": ("kPcgStep (one-kernel PCG iteration: update, convergence test, SELL-64 block SpMV with the "
                        "search direction rebuilt per gathered column, contact filter, FP64 partials)")
@@ -364,9 +405,17 @@ def main() -> int:
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": (PCG_STEP_TRAFFIC_BYTES if args.pcg_variant == 1 and args.n == 256 and s.pcg_shape() == (1, 8, False)
                         else None),
-            "bytes_per_launch": timing["bytes_per_launch"],
-            "mean_launch_us": timing["mean_us"],
+            "bytes_per_launch": pcg_bytes,
+            "bytes_definition": "scope-table algorithmic bytes of one PCG iteration: full block SpMV nnzb (36 + 4) "
+                                "+ 4 (n + 1) + 24 n, plus 14 vector passes of 12 n",
+            "layout_bytes_per_launch": timing["bytes_per_launch"],
+            "per_iteration_us": per_iter_us,
+            "timing": "HIP events around whole replayed 64-launch chunks of the timed steps, counted when the solve "
+                      "was still running after the chunk (every launch did an iteration); the gaps between launches "
+                      "are included",
             "timed_launches": timing["launches"],
+            "kernel_trace_us": KERNEL_TRACE_US if args.n == 256 else None,
+            "kernel_trace_frac": (pcg_bytes / (KERNEL_TRACE_US * 1e-6) / 1e9 / HBM_PEAK_GBS) if args.n == 256 else None,
         },
     }
     line["step_roofline"] = {
@@ -379,10 +428,19 @@ def main() -> int:
                  "iteration, one kPcgStep launch per PCG iteration, 24n + 36 S + 24 H + 16 K + 32n per line-search "
                  "trial (n particles, S stretch, H hinge, K seam terms, U stored blocks)",
     }
+    accuracy = {"reference": "FP64 oracle (this project's restatement of CpuSolver), same scene and step settings",
+                "checkpoints_from_rest": checkpoints}
+    if not args.no_cpu_baseline:
+        base, x_oracle = cpu_baseline(make_scene, window_start)
+        line["cpu_baseline"] = base
+        s.restore(window_start)
+        s.step(1 / 60, 1, 5)
+        d = np.linalg.norm(s.read_positions().astype(np.float64) - x_oracle, axis=1)
+        accuracy["one_step_from_window_start"] = {"step": args.warmup + 1, "max_mm": float(d.max()) * 1e3,
+                                                  "rms_mm": float(np.sqrt(np.mean(d * d))) * 1e3}
+    line["accuracy"] = accuracy
     if not args.no_warm_start_ab and os.path.exists(args.warm_start_weights):
         line["warm_start_ab"] = warm_start_ab(make_scene, args.warmup, args.steps, args.warm_start_weights)
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(make_scene, args.cpu_steps)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -318,6 +318,22 @@ int drape_solver_counters(const drape_solver* s, drape_counters* out) {
   });
 }
 
+int drape_solver_set_refinement(drape_solver* s, int32_t restarts) {
+  return guard([&] {
+    need(s, "solver");
+    if (restarts < 0) throw std::invalid_argument("refinement restarts must be >= 0");
+    s->impl.setRefinement(restarts);
+  });
+}
+
+int drape_solver_pcg_report(const drape_solver* s, drape_pcg_report* out) {
+  return guard([&] {
+    need(s, "solver");
+    need(out, "out");
+    *out = s->impl.pcgReport();
+  });
+}
+
 int drape_solver_reset_counters(drape_solver* s) {
   return guard([&] {
     need(s, "solver");

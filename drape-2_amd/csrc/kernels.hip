@@ -178,6 +178,40 @@ __device__ inline F3 sellRowProduct(const View& v, uint32_t slice, uint32_t lane
   return f3(a0, a1, a2);
 }
 
+// The same row product accumulated in FP64 (each FP32 block entry times an FP32 vector entry is exact in FP64, so
+// the sum carries no FP32 rounding): the true residual of the refinement and of the solve report.
+struct D3 {
+  double x, y, z;
+};
+template <class Gather>
+__device__ inline D3 sellRowProduct64(const View& v, uint32_t slice, uint32_t lane, Gather gather) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  if (v.lowK) {
+    const uint32_t KL = v.lowK[slice];
+    const uint32_t lb = v.lowBase[slice] + lane;
+    for (uint32_t k = 0; k < KL; ++k) {
+      const uint32_t c = v.lowCol[lb + k * kSlice];
+      const float* b = v.sellVal + size_t(v.lowBlk[lb + k * kSlice]) * 9u;  // transposed
+      const F3 x = gather(c);
+      a0 += double(b[0]) * x.x + double(b[3]) * x.y + double(b[6]) * x.z;
+      a1 += double(b[1]) * x.x + double(b[4]) * x.y + double(b[7]) * x.z;
+      a2 += double(b[2]) * x.x + double(b[5]) * x.y + double(b[8]) * x.z;
+    }
+  }
+  const uint32_t K = v.sellK[slice];
+  const uint32_t base = v.sellBase[slice];
+  const float* __restrict__ val = v.sellVal + base + 9u * lane;
+  const uint32_t* __restrict__ col = v.sellCol + base / 9u + lane;
+  for (uint32_t k = 0; k < K; ++k) {
+    const float* b = val + k * 9u * kSlice;
+    const F3 x = gather(col[k * kSlice]);
+    a0 += double(b[0]) * x.x + double(b[1]) * x.y + double(b[2]) * x.z;
+    a1 += double(b[3]) * x.x + double(b[4]) * x.y + double(b[5]) * x.z;
+    a2 += double(b[6]) * x.x + double(b[7]) * x.y + double(b[8]) * x.z;
+  }
+  return D3{a0, a1, a2};
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // Contact planes and the per-row substep kernels.
 
@@ -571,13 +605,17 @@ __global__ void __launch_bounds__(256) kPcgSpmv(View v, uint32_t k, const F3* __
       S->r0 = r0;
       S->rr = rr;
       S->beta = 0.0;
-      if (conv) S->pcgConverged = 1u;
+      if (conv) {
+        S->pcgConverged = 1u;
+        S->pcgExit = kPcgExitZero;
+      }
     }
   } else {
     conv = S->pcgConverged;
     if (!conv) {
       if (sqrt(rr) <= P.cgTolerance * S->r0) {
         conv = 1u;
+        if (first) S->pcgExit = kPcgExitTolerance;
       } else {
         beta = rzNew / S->rzPar[(k + 1u) & 1u];
       }
@@ -590,7 +628,10 @@ __global__ void __launch_bounds__(256) kPcgSpmv(View v, uint32_t k, const F3* __
     }
   }
   if (k >= P.maxCg) {
-    if (first) S->pcgConverged = 1u;
+    if (first && !conv) {
+      S->pcgConverged = 1u;
+      S->pcgExit = kPcgExitCap;
+    }
     conv = 1u;
   }
   if (conv) return;
@@ -623,7 +664,10 @@ __global__ void __launch_bounds__(256) kPcgUpdate(View v, uint32_t k, const F3* 
   if (S->newtonDone || S->pcgConverged || k >= P.maxCg) return;
   const bool first = tile == P.tileBegin && threadIdx.x == 0;
   if (!(pAp > 0.0)) {
-    if (first) S->pcgConverged = 1u;
+    if (first) {
+      S->pcgConverged = 1u;
+      S->pcgExit = kPcgExitCurvature;
+    }
     return;
   }
   const double alpha = S->rzPar[k & 1u] / pAp;
@@ -747,6 +791,9 @@ __global__ void __launch_bounds__(256) kPcgStepInit(View v) {
     S->pcgConverged = 0u;
     S->pcgIter = 0u;
     S->kIter[0] = 0u;
+    S->restartActive = 0u;
+    S->kBase = 0u;
+    S->restartsThisSolve = 0u;
   }
   const uint32_t row = tile * kTile + threadIdx.x;
   const uint32_t cf = v.cflags[row];
@@ -956,6 +1003,9 @@ kPcgStep(View v, uint32_t q) {
   const double cgTolerance = ldUniform(&P->cgTolerance);
   const uint32_t k = ldUniform(&S->kIter[q]);
   const double r0Prev = ldUniform(&S->r0);
+  // A continuation from the FP64 true residual keeps the solve's r0 and its iteration count.
+  const uint32_t restarted = ldUniform(&S->restartActive);
+  const uint32_t kDone = k + ldUniform(&S->kBase);
   // (zr, t, p) of a column: from the halo window in LDS, or gathered from memory; the dummy column gives zero.
   auto gather = [&](uint32_t cc, F3& z, F3& tt, F3& pp) {
     if constexpr (H) {
@@ -1016,21 +1066,26 @@ kPcgStep(View v, uint32_t q) {
   bool update = false;
   double alpha = 0.0, beta = 0.0, r0 = 0.0, rrNext = sum[kPrR];
   if (k == 0) {
-    r0 = sqrt(sum[kPrR]);
-    if (!(r0 > 0.0) || maxCg == 0u) conv = 1u;
+    r0 = restarted ? r0Prev : sqrt(sum[kPrR]);
+    if (!(sum[kPrR] > 0.0))
+      conv = restarted ? kPcgExitTolerance : kPcgExitZero;
+    else if (kDone >= maxCg)
+      conv = kPcgExitCap;
   } else if (!(sum[kPpAp] > 0.0)) {
-    conv = 1u;
+    conv = kPcgExitCurvature;
   } else {
     update = true;
     alpha = sum[kPrZ] / sum[kPpAp];
     rrNext = sum[kPrR] - 2.0 * alpha * sum[kPrAp] + alpha * alpha * sum[kPapAp];
     const double rzNext = sum[kPrZ] - 2.0 * alpha * sum[kPrT] + alpha * alpha * sum[kPapT];
-    if (sqrt(fmax(rrNext, 0.0)) <= cgTolerance * r0Prev || k >= maxCg)
-      conv = 1u;
+    if (sqrt(fmax(rrNext, 0.0)) <= cgTolerance * r0Prev)
+      conv = kPcgExitTolerance;
+    else if (kDone >= maxCg)
+      conv = kPcgExitCap;
     else
       beta = rzNext / sum[kPrZ];
   }
-  if (threadIdx.x == 0) par(v.tileDone, o)[tile] = conv;  // every workgroup of the segment decides the same
+  if (threadIdx.x == 0) par(v.tileDone, o)[tile] = conv ? 1u : 0u;  // every workgroup of the segment decides the same
   if (tile == tileBegin && threadIdx.x == 0) {
     if (k == 0) S->r0 = r0;
     S->kIter[o] = k + 1u;
@@ -1040,7 +1095,10 @@ kPcgStep(View v, uint32_t q) {
       S->pcgIter += 1u;
       S->cntPcg += 1u;
     }
-    if (conv) S->pcgConverged = 1u;
+    if (conv) {
+      S->pcgConverged = 1u;
+      S->pcgExit = conv;
+    }
   }
   if (!update && conv) return;
   const float a = float(alpha), b = float(beta);
@@ -1157,9 +1215,78 @@ kPcgStep(View v, uint32_t q) {
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// After the solve: each held particle's normal force from the unfiltered residual -grad - A step
-// (cpu_solver.cpp:210-212; kernels.h:391-396), and the Newton slope grad . step partials.
+// FP64 true-residual refinement of the FP32 PCG (mixed-precision iterative refinement). The FP32 recursion for r
+// drifts from the true residual b - A step (b = S(-grad)) as the iteration proceeds, so a solve the recursion calls
+// converged can be short of |b - A step| <= tol |r0| (the reference's FP64 solve meets its test on a residual that
+// stays close to the true one, cpu_solver.cpp:188-204). Per row: r = S(-grad - A step) from an FP64 product, and the
+// continuation's first state written as the output of a kernel "-1" (r, zr = S M r, p = ap = t = 0; step kept);
+// per tile: |r|^2 and the partial sums the next kPcgStep reads.
+__global__ void __launch_bounds__(256) kTrueResidual(View v) {
+  __shared__ double lds[kPcgParts * 4];
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t s = v.tileSeg[tile];
+  const SegState* S = v.state + s;
+  if (S->newtonDone || S->pcgExit != kPcgExitTolerance) return;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t slice = tile * kSlicesPerTile + wave;
+  const uint32_t row = slice * kSlice + lane;
+  const F3* __restrict__ stp = v.step;
+  const D3 as = sellRowProduct64(v, slice, lane, [&](uint32_t c) { return stp[c]; });
+  const F3 g = v.grad[row];
+  const F3 resid = f3(float(-double(g.x) - as.x), float(-double(g.y) - as.y), float(-double(g.z) - as.z));
+  const uint32_t cf = v.cflags[row];
+  const float4 plane = v.cnormal[row];
+  const F3 r = (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : onPlane(cf, plane, resid);
+  const F3 zr = applyPrecond(v.precond + kPrecondFloats * row, cf, plane, r);
+  const F3 zero = f3(0, 0, 0);
+  v.cgR[1][row] = r;
+  v.cgAp[1][row] = zero;
+  v.cgP[1][row] = zero;
+  v.cgZr[1][row] = zr;
+  v.cgT[1][row] = zero;
+  double d[kPcgParts] = {0.0, dot64(r, zr), 0.0, 0.0, 0.0, 0.0, dot64(r, r)};
+  blockSumN<kPcgParts, 4>(d, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < kPcgParts; ++i) v.cgPart[1][kPcgParts * tile + i] = d[i];
+    v.partB[tile] = d[kPrR];
+  }
+}
 
+// A continuation pays when the recursion has drifted well away from the true residual, not for a true residual a
+// few percent above the tolerance (FP32 recursions end there routinely): it starts above kRefineFactor tol r0.
+constexpr double kRefineFactor = 4.0;
+
+// Per segment: a solve that stopped on the recursion's tolerance test continues from its true residual when that
+// is above kRefineFactor tol r0 and iterations remain (at most `maxRestarts` continuations per solve). Every tile of the segment
+// gets its solve-ended flag for the next kPcgStep; the host looks at pcgConverged afterwards.
+__global__ void __launch_bounds__(256) kRefineDecide(View v, uint32_t maxRestarts) {
+  __shared__ double lds[8];
+  const uint32_t s = blockIdx.x;
+  SegState* S = v.state + s;
+  const SegParams& P = v.seg[s];
+  const bool candidate = !S->newtonDone && S->pcgExit == kPcgExitTolerance;
+  const double trueSq = candidate ? rangeSum256(v.partB, P.tileBegin, P.tileEnd, lds) : 0.0;
+  const bool restart = candidate && sqrt(trueSq) > kRefineFactor * P.cgTolerance * S->r0 && S->pcgIter < P.maxCg &&
+                       S->restartsThisSolve < maxRestarts;
+  for (uint32_t t = P.tileBegin + threadIdx.x; t < P.tileEnd; t += 256u) v.tileDone[0][t] = restart ? 0u : 1u;
+  if (threadIdx.x == 0) {
+    S->restartActive = restart ? 1u : 0u;
+    if (restart) {
+      S->pcgConverged = 0u;
+      S->kIter[0] = 0u;
+      S->kBase = S->pcgIter;
+      S->restartsThisSolve += 1u;
+      S->cntRestarts += 1u;
+    }
+  }
+}
+
+// After the solve, from one product A step over every row: each held particle's normal force from the unfiltered
+// residual -grad - A step (cpu_solver.cpp:210-212: the reference keeps that residual as a running vector; here it is
+// formed explicitly), the FP64 norm partials of the filtered true residual S(-grad - A step) (pinned rows excluded)
+// for the solve report, and the Newton slope grad . step partials.
 __global__ void __launch_bounds__(256) kContactForceSlope(View v) {
   __shared__ double lds[8];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
@@ -1172,21 +1299,22 @@ __global__ void __launch_bounds__(256) kContactForceSlope(View v) {
   const uint32_t cf = v.cflags[row];
   const F3 gi = v.grad[row];
   const F3 si = v.step[row];
-  float force = 0.0f;
-  if (cf & kContactActive) {
-    const F3* __restrict__ stp = v.step;
-    const F3 as = sellRowProduct(v, slice, lane, [&](uint32_t c) { return stp[c]; });
-    const F3 resid = -gi - as;
-    const float4 cn = v.cnormal[row];
-    force = fmaxf(0.0f, -dot(resid, f3(cn.x, cn.y, cn.z)));
+  const F3* __restrict__ stp = v.step;
+  const D3 as = sellRowProduct64(v, slice, lane, [&](uint32_t c) { return stp[c]; });
+  const F3 residual = f3(float(-double(gi.x) - as.x), float(-double(gi.y) - as.y), float(-double(gi.z) - as.z));
+  const float4 plane = v.cnormal[row];
+  v.cforce[row] = (cf & kContactActive) ? fmaxf(0.0f, -dot(residual, f3(plane.x, plane.y, plane.z))) : 0.0f;
+  const F3 filtered = (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : onPlane(cf, plane, residual);
+  const double slope = blockSum256(dot64(gi, si), lds);
+  const double trueSq = blockSum256(dot64(filtered, filtered), lds);
+  if (threadIdx.x == 0) {
+    v.partA[tile] = slope;
+    v.partB[tile] = trueSq;
   }
-  v.cforce[row] = force;
-  const double sl = double(gi.x) * si.x + double(gi.y) * si.y + double(gi.z) * si.z;
-  const double ss = blockSum256(sl, lds);
-  if (threadIdx.x == 0) v.partA[tile] = ss;
 }
 
-// slope = grad . step; no line search when it is not negative (cpu_solver.cpp:267-268).
+// slope = grad . step; no line search when it is not negative (cpu_solver.cpp:267-268). Also closes the solve's
+// report: its exit reason and its true relative residual.
 __global__ void __launch_bounds__(256) kNewtonSlope(View v) {
   __shared__ double lds[8];
   const uint32_t s = blockIdx.x;
@@ -1194,8 +1322,19 @@ __global__ void __launch_bounds__(256) kNewtonSlope(View v) {
   if (S->newtonDone) return;
   const SegParams& P = v.seg[s];
   const double slope = rangeSum256(v.partA, P.tileBegin, P.tileEnd, lds);
+  const double trueSq = rangeSum256(v.partB, P.tileBegin, P.tileEnd, lds);
   if (threadIdx.x == 0) {
     S->slope = slope;
+    const double rel = S->r0 > 0.0 ? sqrt(trueSq) / S->r0 : 0.0;
+    S->cntSolves += 1u;
+    S->cntExitTolerance += S->pcgExit == kPcgExitTolerance ? 1u : 0u;
+    S->cntExitCap += S->pcgExit == kPcgExitCap ? 1u : 0u;
+    S->cntExitCurvature += S->pcgExit == kPcgExitCurvature ? 1u : 0u;
+    S->cntExitZero += S->pcgExit == kPcgExitZero ? 1u : 0u;
+    S->trueRelLast = rel;
+    S->trueRelSum += rel;
+    S->trueRelMax = fmax(S->trueRelMax, rel);
+    if (S->pcgExit == kPcgExitTolerance) S->trueRelMaxConverged = fmax(S->trueRelMaxConverged, rel);
     if (!(slope < 0.0)) {
       S->newtonDone = 1u;
     } else {
@@ -1944,6 +2083,10 @@ void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t s
     hipExtLaunchKernelGGL((kPcgStep<1, 4, true>), grid, dim3(256), 0, st, start, stop, 0, v, q);
   else
     throw std::invalid_argument("PCG launch shape must be 1x8, 1x4, 2x8 or 4x8, or 1x8 / 1x4 with the halo");
+}
+void launchPcgRefine(const View& v, uint32_t maxRestarts, hipStream_t st) {
+  kTrueResidual<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);
+  kRefineDecide<<<v.nSegments, 256, 0, st>>>(v, maxRestarts);
 }
 void launchContactForceSlope(const View& v, hipStream_t st) {
   kContactForceSlope<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);

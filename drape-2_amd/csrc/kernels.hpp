@@ -148,6 +148,9 @@ constexpr PcgShape kPcgShapes[6] = {{1, 8, false}, {1, 4, false}, {2, 8, false},
 // One PCG iteration of the one-reduction form.
 void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t st, hipEvent_t start = nullptr,
                    hipEvent_t stop = nullptr);
+// FP64 true residual of the finished solve and, per segment, a continuation of the solve from it when it is above
+// tol r0 (at most maxRestarts per solve; see kTrueResidual).
+void launchPcgRefine(const View& v, uint32_t maxRestarts, hipStream_t st);
 void launchContactForceSlope(const View& v, hipStream_t st);
 void launchNewtonSlope(const View& v, hipStream_t st);
 void launchEnergyTrial(const View& v, hipStream_t st);

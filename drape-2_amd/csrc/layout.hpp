@@ -152,6 +152,20 @@ struct SegState {
   // (the per-tile solve-ended flags, View::tileDone, follow the same rule).
   uint32_t kIter[2];
   uint64_t cntNewton, cntPcg, cntLs, cntSubsteps, cntWarmRejected, cntWarmAccepted;
+  // Linear-solve report: why each PCG solve ended (kPcgExit*), counted per reason, and the FP64 true relative
+  // residual |S(-grad - A step)| / |S(-grad)| at each solve's end (S the contact filter).
+  uint32_t pcgExit;
+  // FP64 true-residual refinement: restartActive = 1 while a solve continues from its true residual (it keeps the
+  // solve's r0); kBase = the updates done before the continuation (the iteration cap counts the whole solve).
+  uint32_t restartActive, kBase, restartsThisSolve, pad3;
+  uint64_t cntSolves, cntExitTolerance, cntExitCap, cntExitCurvature, cntExitZero, cntRestarts;
+  double trueRelMax, trueRelSum, trueRelLast, trueRelMaxConverged;  // the last: over solves ended on the tolerance test
 };
+
+// Why a PCG solve ended (cpu_solver.cpp:188-204 stopping rules).
+constexpr uint32_t kPcgExitTolerance = 1;  // |r| <= tol |r0| after an update
+constexpr uint32_t kPcgExitCap = 2;        // maxCg updates done
+constexpr uint32_t kPcgExitCurvature = 3;  // p.Ap <= 0 before an update
+constexpr uint32_t kPcgExitZero = 4;       // r0 = 0: nothing to solve
 
 }  // namespace dk

@@ -154,6 +154,18 @@ Model Model::build(const drape_scene_desc* scenes, uint32_t nScenes) {
         throw std::invalid_argument("garment '" + id + "': one initial position per mesh vertex required");
       if (sg.n_piece_fabrics != mesh.n_pieces)
         throw std::invalid_argument("garment '" + id + "': one fabric per pattern piece required");
+      // Every index the mesh carries must address its own arrays before anything is derived from it (an index past
+      // them would read past the host arrays and write past the device term tables).
+      for (uint32_t vtx = 0; vtx < mesh.n_vertices; ++vtx)
+        if (mesh.vertex_piece[vtx] >= mesh.n_pieces)
+          throw std::out_of_range("garment '" + id + "': vertex piece out of range");
+      for (uint32_t t = 0; t < mesh.n_triangles; ++t) {
+        if (mesh.triangle_piece[t] >= mesh.n_pieces)
+          throw std::out_of_range("garment '" + id + "': triangle piece out of range");
+        for (int k = 0; k < 3; ++k)
+          if (mesh.triangles[3 * size_t(t) + k] >= mesh.n_vertices)
+            throw std::out_of_range("garment '" + id + "': triangle corner out of range");
+      }
       Garment g;
       g.id = id;
       g.offset = static_cast<uint32_t>(m.x.size());

@@ -372,6 +372,7 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
     upload(tileHalo_, halo);
   }
   tunedShape_ = -1;
+  tuneSolves_ = 0;
   std::fill(tuneMs_, tuneMs_ + kShapeCount, 0.0);
   std::fill(tuneLaunches_, tuneLaunches_ + kShapeCount, 0u);
   sellVal_.allocate(sizeof(float) * (sellFloats_ + dk::kSellSlack));  // zero slack: see kSellSlack
@@ -665,6 +666,8 @@ void HipSolver::releaseGraph() {
   graph_ = nullptr;
   newtonPre_.release();
   newtonPost_.release();
+  refineExec_.release();
+  for (auto& c : tuneExec_) c.release();
 }
 
 void HipSolver::setPcgVariant(int variant) {
@@ -754,45 +757,71 @@ void HipSolver::timedPcgStep(uint32_t parity, dk::PcgShape shape, bool solveStar
   pushTimed(pr, solveStarted);
 }
 
-// Solves run before a launch shape is chosen: chunks of kGraphChunk launches rotate over the shapes of
+// Solves run before a launch shape is chosen: chunks of kTuneChunk launches rotate over the shapes of
 // dk::kPcgShapes that the matrix admits (the halo shapes need every tile's window to fit; the shape with the fewest
-// samples goes next), each bracketed by HIP events and followed by a poll.
-// A chunk counts only if the solve was still running at its end, so every timed launch did a full iteration. Once
-// every shape has four such chunks the fastest mean is kept; the shape never changes the results.
+// samples goes next), each bracketed by HIP events and followed by a poll. A chunk counts only if the solve was
+// still running at its end, so every timed launch did a full iteration. Once every shape has four such chunks the
+// fastest mean is kept; after kTuneSolves solves without that (solves too short to fill a chunk), the fastest shape
+// sampled so far is kept, else 1x8. The shape never changes the results.
 void HipSolver::runPcgTuning(uint32_t maxCg) {
   uint32_t launched = 0;
   while (launched < maxCg + 1) {
     int o = 0;
     for (int i = 1; i < kShapeCount; ++i)
       if (shapeAvailable(i) && tuneLaunches_[i] < tuneLaunches_[o]) o = i;
-    const dk::PcgShape shape = dk::kPcgShapes[tunedShape_ >= 0 ? tunedShape_ : o];
+    const int si = tunedShape_ >= 0 ? tunedShape_ : o;
+    const dk::PcgShape shape = dk::kPcgShapes[si];
     const auto pr = takeEvents();
     check(hipEventRecord(pr.first, stream_), "event");
-    for (uint32_t i = 0; i < kGraphChunk; ++i) dk::launchPcgStep(view_, i & 1u, shape, stream_);
+    // Each shape's chunk is a captured graph, as the solves after tuning replay them: the events then time the
+    // launches themselves, not how fast the host can submit them.
+    if (useGraph_)
+      launchCaptured(tuneExec_[si], [&] {
+        for (uint32_t i = 0; i < kTuneChunk; ++i) dk::launchPcgStep(view_, i & 1u, shape, stream_);
+      });
+    else
+      for (uint32_t i = 0; i < kTuneChunk; ++i) dk::launchPcgStep(view_, i & 1u, shape, stream_);
     check(hipEventRecord(pr.second, stream_), "event");
-    launched += kGraphChunk;
+    launched += kTuneChunk;
     pollStates();
     const bool done = allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; });
     if (!done && tunedShape_ < 0) {
       float ms = 0.0f;
       check(hipEventElapsedTime(&ms, pr.first, pr.second), "event time");
       tuneMs_[o] += ms;
-      tuneLaunches_[o] += kGraphChunk;
+      tuneLaunches_[o] += kTuneChunk;
       bool sampled = true;
-      for (int i = 0; i < kShapeCount; ++i) sampled = sampled && (!shapeAvailable(i) || tuneLaunches_[i] >= 4u * kGraphChunk);
-      if (sampled) {
-        int best = 0;
-        for (int i = 1; i < kShapeCount; ++i)
-          if (shapeAvailable(i) && tuneMs_[i] / tuneLaunches_[i] < tuneMs_[best] / tuneLaunches_[best]) best = i;
-        tunedShape_ = best;
-      }
+      for (int i = 0; i < kShapeCount; ++i) sampled = sampled && (!shapeAvailable(i) || tuneLaunches_[i] >= 4u * kTuneChunk);
+      if (sampled) tunedShape_ = fastestSampledShape();
     }
     eventPool_.push_back(pr);
     if (done) break;
   }
+  if (tunedShape_ < 0 && ++tuneSolves_ >= kTuneSolves) tunedShape_ = fastestSampledShape();
+  const dk::PcgShape shape = dk::kPcgShapes[tunedShape_ >= 0 ? tunedShape_ : 0];
+  for (uint32_t r = 0; r < refineRestarts_; ++r) {
+    dk::launchPcgRefine(view_, refineRestarts_, stream_);
+    pollStates();
+    if (allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; })) break;
+    for (uint32_t n = 0; n < maxCg + 1; n += kTuneChunk) {
+      for (uint32_t i = 0; i < kTuneChunk; ++i) dk::launchPcgStep(view_, i & 1u, shape, stream_);
+      pollStates();
+      if (allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; })) break;
+    }
+  }
   uint32_t total = 0;
   for (size_t s = 0; s < model_.segments.size(); ++s) total = std::max(total, hostStates_[s].pcgIter);
   lastPcgIterations_ = total;
+}
+
+// The shape with the lowest mean chunk time among those sampled (1x8 when none is).
+int HipSolver::fastestSampledShape() const {
+  int best = -1;
+  for (int i = 0; i < kShapeCount; ++i)
+    if (shapeAvailable(i) && tuneLaunches_[i] > 0 &&
+        (best < 0 || tuneMs_[i] / tuneLaunches_[i] < tuneMs_[best] / tuneLaunches_[best]))
+      best = i;
+  return best < 0 ? 0 : best;
 }
 
 // Chunks of kGraphChunk one-kernel iterations, replayed from a captured hipGraph (or launched eagerly). The
@@ -819,37 +848,84 @@ void HipSolver::runPcgSingleKernel() {
     check(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0), "graph instantiate");
     graphShape_ = shapeIdx;
   }
-  uint32_t launched = 0;
+  // Chunks until every segment's solve has ended: a first batch of `chunks`, then one chunk per look at the flags.
+  const auto runChunks = [&](uint32_t chunks, uint32_t budget) {
+    uint32_t launched = 0;
+    bool polled = false;  // the host mirror holds this solve's convergence flags
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> chunkEvents;
+    while (launched < budget + 1) {
+      for (uint32_t c = 0; c < chunks; ++c) {
+        if (useGraph_) {
+          // With sampling, each replayed chunk is bracketed by events: a chunk's time over its 64 launches is the
+          // per-iteration time with the gaps between launches, counted below when every launch did an iteration.
+          std::pair<hipEvent_t, hipEvent_t> pr{};
+          if (sampling) {
+            pr = takeEvents();
+            check(hipEventRecord(pr.first, stream_), "event");
+          }
+          check(hipGraphLaunch(graphExec_, stream_), "graph launch");
+          if (sampling) {
+            check(hipEventRecord(pr.second, stream_), "event");
+            chunkEvents.push_back(pr);
+          }
+        } else {
+          for (uint32_t i = 0; i < kGraphChunk; ++i) {
+            if (sampling && (launched + i) % uint32_t(spmvEvery_) == 1)
+              timedPcgStep(i & 1u, shape, polled);
+            else
+              dk::launchPcgStep(view_, i & 1u, shape, stream_);
+          }
+        }
+        launched += kGraphChunk;
+      }
+      pollStates();
+      polled = true;
+      const bool ended = allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; });
+      // The batch's chunks count only when a solve was still running at this look (no launch found it ended).
+      for (const auto& pr : chunkEvents) {
+        if (!ended) {
+          float ms = 0.0f;
+          check(hipEventElapsedTime(&ms, pr.first, pr.second), "event");
+          spmvTotalMs_ += ms;
+          spmvTimed_ += kGraphChunk;
+          spmvTotalBytes_ += double(kGraphChunk) * activeLaunchBytes(true);
+        }
+        eventPool_.push_back(pr);
+      }
+      chunkEvents.clear();
+      if (ended) break;
+      chunks = 1;
+    }
+  };
   // First batch: five eighths of the previous solve's iterations, in whole chunks (measured on the 65k sheet with
   // 64-launch chunks against 3/4 and 7/8: a batch that overshoots the solve costs more in launches that find it
   // ended than one more look at the flags costs).
-  uint32_t chunks = std::max<uint32_t>(1, (lastPcgIterations_ * 5 / 8) / kGraphChunk);
-  bool polled = false;  // the host mirror holds this solve's convergence flags
-  while (launched < maxCg + 1) {
-    if (useGraph_ && sampling) {
-      // Two eager iterations (keeping the graph's parity pattern) open each batch; the second, which starts
-      // right behind the first as graph nodes do, is event-timed.
-      dk::launchPcgStep(view_, 0u, shape, stream_);
-      timedPcgStep(1u, shape, polled);
-      launched += 2;
-    }
-    for (uint32_t c = 0; c < chunks; ++c) {
-      if (useGraph_) {
-        check(hipGraphLaunch(graphExec_, stream_), "graph launch");
+  runChunks(std::max<uint32_t>(1, (lastPcgIterations_ * 5 / 8) / kGraphChunk), maxCg);
+  // FP64 true-residual refinement: a segment whose solve stopped on the FP32 recursion's test while its true
+  // residual is still above tol r0 continues from the true residual (kTrueResidual, kRefineDecide).
+  // The decision and a first short chunk of the continuation go out together (the chunk's launches are no-ops for
+  // a segment that does not continue), so a look at the flags follows both.
+  for (uint32_t r = 0; r < refineRestarts_; ++r) {
+    dk::launchPcgRefine(view_, refineRestarts_, stream_);
+    if (useGraph_) {
+      if (!refineExec_.exec || refineShape_ != shapeIdx) {
+        refineExec_.release();
+        launchCaptured(refineExec_, [&] {
+          for (uint32_t i = 0; i < kRefineChunk; ++i) dk::launchPcgStep(view_, i & 1u, shape, stream_);
+        });
+        refineShape_ = shapeIdx;
       } else {
-        for (uint32_t i = 0; i < kGraphChunk; ++i) {
-          if (sampling && (launched + i) % uint32_t(spmvEvery_) == 1)
-            timedPcgStep(i & 1u, shape, polled);
-          else
-            dk::launchPcgStep(view_, i & 1u, shape, stream_);
-        }
+        check(hipGraphLaunch(refineExec_.exec, stream_), "graph launch");
       }
-      launched += kGraphChunk;
+    } else {
+      for (uint32_t i = 0; i < kRefineChunk; ++i) dk::launchPcgStep(view_, i & 1u, shape, stream_);
     }
     pollStates();
-    polled = true;
-    if (allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; })) break;
-    chunks = 1;
+    bool continued = false;
+    for (size_t g = 0; g < model_.segments.size(); ++g) continued = continued || hostStates_[g].restartActive != 0u;
+    if (!continued) break;
+    if (!allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; }))
+      runChunks(1, maxCg);
   }
   uint32_t total = 0;
   for (size_t s = 0; s < model_.segments.size(); ++s) total = std::max(total, hostStates_[s].pcgIter);
@@ -1180,12 +1256,36 @@ drape_counters HipSolver::counters() const {
   return c;
 }
 
+drape_pcg_report HipSolver::pcgReport() const {
+  requireBuilt();
+  pollStates();
+  drape_pcg_report r{};
+  double sum = 0.0;
+  for (size_t s = 0; s < model_.segments.size(); ++s) {
+    const dk::SegState& st = hostStates_[s];
+    r.solves += st.cntSolves;
+    r.exit_tolerance += st.cntExitTolerance;
+    r.exit_cap += st.cntExitCap;
+    r.exit_curvature += st.cntExitCurvature;
+    r.exit_zero += st.cntExitZero;
+    r.restarts += st.cntRestarts;
+    r.true_rel_max = std::max(r.true_rel_max, st.trueRelMax);
+    r.true_rel_last = std::max(r.true_rel_last, st.trueRelLast);
+    r.true_rel_max_converged = std::max(r.true_rel_max_converged, st.trueRelMaxConverged);
+    sum += st.trueRelSum;
+  }
+  r.true_rel_mean = r.solves ? sum / double(r.solves) : 0.0;
+  return r;
+}
+
 void HipSolver::resetCounters() {
   requireBuilt();
   pollStates();
   for (size_t s = 0; s < model_.segments.size(); ++s) {
     dk::SegState& st = hostStates_[s];
     st.cntSubsteps = st.cntNewton = st.cntPcg = st.cntLs = st.cntWarmRejected = st.cntWarmAccepted = 0;
+    st.cntSolves = st.cntExitTolerance = st.cntExitCap = st.cntExitCurvature = st.cntExitZero = st.cntRestarts = 0;
+    st.trueRelMax = st.trueRelSum = st.trueRelLast = st.trueRelMaxConverged = 0.0;
   }
   check(hipMemcpyAsync(segState_.as<void>(), hostStates_, sizeof(dk::SegState) * model_.segments.size(),
                        hipMemcpyHostToDevice, stream_),

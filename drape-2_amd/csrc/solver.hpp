@@ -40,10 +40,15 @@ class HipSolver {
   uint32_t segmentCount() const { return static_cast<uint32_t>(model_.segments.size()); }
   void segmentRange(uint32_t s, uint32_t& offset, uint32_t& count) const;
   drape_counters counters() const;
+  drape_pcg_report pcgReport() const;
   void resetCounters();
   void setGraph(bool enabled) { useGraph_ = enabled; }
   // 1: one kernel and one reduction per PCG iteration (default); 0: two kernels per iteration.
   void setPcgVariant(int variant);
+  // FP64 true-residual refinement of the one-kernel PCG: at most `restarts` continuations of a solve from its true
+  // residual (0: off).
+  void setRefinement(int restarts) { refineRestarts_ = static_cast<uint32_t>(std::max(0, restarts)); }
+  int refinement() const { return static_cast<int>(refineRestarts_); }
   // Wavefronts per 64-row slice in the one-kernel PCG iteration: 1, 2 or 4, or 0 to time all three during the
   // first solves and keep the fastest (results are bit-identical for every choice).
   // Launch shape of the one-kernel PCG iteration (wavefronts per slice x slots per batch; 0x0 = tune).
@@ -133,6 +138,7 @@ class HipSolver {
   DeviceBuffer fieldValues_, scratchF3_, scratchF3b_;
   DeviceBuffer cgBuf_[10], cgPart0_, cgPart1_, tileDone_;
   int pcgVariant_ = 1;
+  uint32_t refineRestarts_ = 4;
   static constexpr int kShapeCount = 6;  // dk::kPcgShapes
   int pcgShape_ = -1;     // requested index into dk::kPcgShapes (-1 = tune)
   int tunedShape_ = -1;   // chosen by the tuner (-1 until it has measured every shape)
@@ -143,6 +149,13 @@ class HipSolver {
   // batch (even: the parity pattern repeats). Measured on the 65k sheet over 8 ... 64: a look at the flags costs
   // more than the launches that find the solve already ended.
   static constexpr uint32_t kGraphChunk = 64;
+  // The tuner times chunks of 16 launches (a solve must outlast a chunk for it to count: with the reference's
+  // default cap of 50 iterations a 64-launch chunk never would) and gives up waiting for full samples after
+  // kTuneSolves solves.
+  static constexpr uint32_t kTuneChunk = 16;
+  static constexpr uint32_t kTuneSolves = 64;
+  uint32_t tuneSolves_ = 0;
+  int fastestSampledShape() const;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graphExec_ = nullptr;
   // One Newton iteration's launches before the PCG solve (activate, assemble, e0) and after it (contact force,
@@ -154,6 +167,11 @@ class HipSolver {
     void release();
   };
   Captured newtonPre_, newtonPost_;
+  // The first launches of a continuation from the true residual, replayed behind the refinement decision.
+  static constexpr uint32_t kRefineChunk = 16;
+  Captured refineExec_;
+  Captured tuneExec_[6];  // the tuner's chunk of each launch shape (kShapeCount)
+  int refineShape_ = -1;
   static constexpr int kGraphTrials = 2;
   template <class F>
   void launchCaptured(Captured& c, F&& launches);

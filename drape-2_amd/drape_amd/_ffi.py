@@ -143,6 +143,15 @@ class Counters(C.Structure):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
 
 
+class PcgReport(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "solves", "exit_tolerance", "exit_cap", "exit_curvature", "exit_zero", "restarts")] + [
+        (n, C.c_double) for n in ("true_rel_max", "true_rel_mean", "true_rel_last", "true_rel_max_converged")]
+
+    def as_dict(self) -> dict:
+        return {n: (int(v) if isinstance(v, int) else float(v)) for n, _ in self._fields_ for v in [getattr(self, n)]}
+
+
 class StructureSizes(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("n_particles", "n_stretch", "n_hinges", "n_seams", "n_blocks")]
 
@@ -319,6 +328,8 @@ def _declare(lib):
         "drape_solver_read_structure": (C.c_int, [S, u32p, u32p, u32p, u32p, u32p, u32p]),
         "drape_solver_read_terms": (C.c_int, [S, u32p, u32p, f64p, u32p]),
         "drape_solver_gather_sizes": (C.c_int, [S, u32p, u32p]),
+        "drape_solver_pcg_report": (C.c_int, [S, C.POINTER(PcgReport)]),
+        "drape_solver_set_refinement": (C.c_int, [S, C.c_int32]),
         "drape_solver_read_gather": (C.c_int, [S, u32p, u32p, u32p, u32p]),
         "drape_solver_debug_assemble": (C.c_int, [S, C.c_double, f32p, f64p, f32p, f32p]),
         "drape_solver_debug_spmv": (C.c_int, [S, f32p, f32p]),

@@ -14,7 +14,7 @@ import ctypes as C
 import numpy as np
 
 from . import _ffi
-from ._ffi import (Counters, SceneDesc, SceneHolder, Stats, StructureSizes, load_library, ptr)
+from ._ffi import (Counters, PcgReport, SceneDesc, SceneHolder, Stats, StructureSizes, load_library, ptr)
 
 
 def _lib():
@@ -206,6 +206,16 @@ class HipSolver:
         _check(_lib().drape_solver_read_structure(self._h, *[ptr(out[k], C.c_uint32) for k in (
             "row_start", "col_index", "diag_slot", "stretch_slots", "hinge_slots", "seam_slots")]))
         return out
+
+    def set_refinement(self, restarts: int) -> None:
+        """FP64 true-residual continuations per PCG solve (0: off; default 4)."""
+        _check(_lib().drape_solver_set_refinement(self._h, int(restarts)))
+
+    def pcg_report(self) -> dict:
+        """Why the PCG solves ended and their FP64 true relative residuals (drape_pcg_report)."""
+        r = PcgReport()
+        _check(_lib().drape_solver_pcg_report(self._h, C.byref(r)))
+        return r.as_dict()
 
     def gather_lists(self) -> dict:
         """The assembly's summation order in the reference's gather-list numbering (gpu_solver_core.cpp:184-224)."""

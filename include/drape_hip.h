@@ -197,6 +197,24 @@ int drape_solver_snapshot_save(const drape_solver* s, void* out, size_t bytes);
 int drape_solver_snapshot_restore(drape_solver* s, const void* in, size_t bytes);
 
 int drape_solver_counters(const drape_solver* s, drape_counters* out);
+/* Linear-solve report (no counterpart in the reference's Solver; it makes the PCG's stopping visible): per Newton
+ * iteration one solve, counted by why it ended (cpu_solver.cpp:188-204: |r| <= tol |r0|, the iteration cap,
+ * p.Ap <= 0, r0 = 0), and its FP64 true relative residual |S(-grad - A step)| / |S(-grad)| at the end (S the
+ * contact filter; the recursive FP32 residual the stopping test uses can drift from it). Summed over segments;
+ * cumulative since build or drape_solver_reset_counters(). */
+typedef struct drape_pcg_report {
+  uint64_t solves;
+  uint64_t exit_tolerance;
+  uint64_t exit_cap;
+  uint64_t exit_curvature;
+  uint64_t exit_zero;
+  uint64_t restarts;    /* solves continued from their FP64 true residual (drape_solver_set_true_residual) */
+  double true_rel_max;  /* largest true relative residual at a solve's end */
+  double true_rel_mean;
+  double true_rel_last; /* of the last solve (largest over segments) */
+  double true_rel_max_converged; /* largest over the solves that ended on the tolerance test */
+} drape_pcg_report;
+int drape_solver_pcg_report(const drape_solver* s, drape_pcg_report* out);
 int drape_solver_reset_counters(drape_solver* s);
 
 /* Execution options. graph: capture each substep's launch sequence in a hipGraph and replay it (1, default) or
@@ -207,6 +225,11 @@ int drape_solver_set_graph(drape_solver* s, int32_t enabled);
  * p.Ap reduced directly and r.z, r.r expanded exactly over the previous iteration's vectors: the same iterates and
  * stopping rules); 0 the two-kernel standard form (cpu_solver.cpp:169-213 step by step). */
 int drape_solver_set_pcg_variant(drape_solver* s, int32_t variant);
+/* FP64 true-residual refinement of the one-kernel PCG (no counterpart in the reference, whose CG runs in FP64):
+ * after a solve stops on the FP32 recursion's test |r| <= tol |r0|, its true residual S(-grad - A step) is formed
+ * with FP64 accumulation, and a solve whose true residual is still above tol |r0| continues from it (same r0, same
+ * iteration cap), at most `restarts` times per solve. 0 turns it off; default 4. */
+int drape_solver_set_refinement(drape_solver* s, int32_t restarts);
 /* Launch shape of the one-kernel PCG iteration: `waves` wavefronts per 64-row slice and `slots` lower and upper
  * matrix slots per batch; 1x8, 1x4 (smaller register footprint: two workgroups per CU, for launches with many more
  * tiles than CUs), 2x8 and 4x8, or 0x0 (default) to time every shape during the first solves and keep the fastest.

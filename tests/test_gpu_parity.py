@@ -355,9 +355,9 @@ def test_two_runs_bit_identical_and_snapshot_replay(hip):
 
 def test_solver_errors_follow_the_reference_and_leave_it_usable(hip):
     """The Solver surface's errors (sim_model.cpp:15, 82-85, 184, 258; test_cpu_solver.cpp:186): calls before
-    build are a logic error, an unknown garment id and a pin past the particle count are out of range, a fabric
-    list that does not match the pieces and a snapshot of another scene are invalid arguments. After each the
-    solver steps exactly as a fresh one."""
+    build are a logic error, an unknown garment id, a pin past the particle count and a mesh index past its arrays
+    are out of range, a fabric list that does not match the pieces and a snapshot of another scene are invalid
+    arguments. After each the solver steps exactly as a fresh one."""
     import copy
     g = hip.HipSolver()
     with pytest.raises(AssertionError, match="not built"):
@@ -379,6 +379,19 @@ def test_solver_errors_follow_the_reference_and_leave_it_usable(hip):
     bad = copy.deepcopy(sc)
     bad.garments[0].piece_fabric = bad.garments[0].piece_fabric * 2
     with pytest.raises(ValueError, match="one fabric per pattern piece"):
+        hip.HipSolver().build(bad)
+    # Mesh indices are checked on the host before anything reaches the device.
+    bad = copy.deepcopy(sc)
+    bad.garments[0].mesh.triangles[7, 1] = sc.n + 3
+    with pytest.raises(IndexError, match="triangle corner out of range"):
+        hip.HipSolver().build(bad)
+    bad = copy.deepcopy(sc)
+    bad.garments[0].mesh.vertex_piece[5] = 2
+    with pytest.raises(IndexError, match="vertex piece out of range"):
+        hip.HipSolver().build(bad)
+    bad = copy.deepcopy(sc)
+    bad.garments[0].mesh.triangle_piece[5] = 1
+    with pytest.raises(IndexError, match="triangle piece out of range"):
         hip.HipSolver().build(bad)
     fresh = hip.HipSolver()
     fresh.build(sc)
@@ -667,3 +680,37 @@ def test_reference_default_settings_parity_60_frames(hip, oracle, name):
     apart = _default_scene_apart(hip, oracle, sc)
     print(f"parity {name}: oracle self-spread {self_spread * 1e3:.3f} mm, hip vs oracle {apart * 1e3:.3f} mm", flush=True)
     assert apart < 1e-3
+
+
+def test_true_residual_refinement_meets_the_tolerance_on_the_true_residual(hip):
+    """The FP32 recursion stops PCG on |r| <= tol |r0| while the true residual S(-grad - A step) may have drifted
+    far above it (the 65k drop's contact phase, steps 11-22: up to 9e-4 against tol 1e-6). Continuing such solves
+    from their FP64 true residual brings the worst and the mean true residual down by more than tenfold (measured:
+    9.1e-4 -> 4.0e-5, 1.1e-4 -> 5.6e-6 with at most 8 continuations per solve)."""
+    def run(restarts):
+        g = hip.HipSolver()
+        g.build(scenes.c2_sheet())
+        g.set_refinement(restarts)
+        for _ in range(22):
+            g.step(1 / 60, 1, 5)
+        return g.pcg_report(), g.read_positions()
+    off, x_off = run(0)
+    on, x_on = run(8)
+    print(f"refinement off: {off}\nrefinement on: {on}", flush=True)
+    assert off["restarts"] == 0 and on["restarts"] > 0
+    assert off["true_rel_max_converged"] > 4e-6
+    assert on["true_rel_max_converged"] < off["true_rel_max_converged"] / 10
+    assert on["true_rel_mean"] < off["true_rel_mean"] / 10
+    assert np.all(np.isfinite(x_on))
+
+
+def test_launch_shape_tuner_finishes_with_the_reference_defaults(hip):
+    """With the reference's default PCG settings (at most 50 iterations, tol 1e-2) every solve ends within one
+    64-launch chunk; the tuner still picks a launch shape after a bounded number of solves."""
+    g = hip.HipSolver()
+    g.build(scenes.hanging_sheet())
+    g.set_pcg_shape(0, 0)
+    for _ in range(60):
+        g.step(1 / 60, 1, 2)
+    waves, slots, _ = g.pcg_shape()
+    assert (waves, slots) != (0, 0)
