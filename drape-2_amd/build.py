@@ -52,6 +52,16 @@ def build(verbose: bool = False) -> str:
     if rc != 0:
         raise RuntimeError(f"link failed:\n{out}")
     os.replace(tmp, lib)
+    # The compiled C++ test of the boundary (tests/cpp/hip_solver_test.cpp): the reference-shaped C++ wrapper over
+    # this library, linked the way a reference build would link it.
+    test_src = os.path.join(REPO, "tests", "cpp", "hip_solver_test.cpp")
+    if os.path.exists(test_src):
+        exe = os.path.join(BUILD, "hip_solver_test")
+        rc, out = _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(REPO, "include"), test_src,
+                        "-L", BUILD, "-ldrape_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib",
+                        "-o", exe])
+        if rc != 0:
+            raise RuntimeError(f"C++ boundary test failed to build:\n{out}")
     return lib
 
 
