@@ -539,6 +539,21 @@ int drape_vertex_normals(const float* positions, uint32_t n, const uint32_t* tri
   });
 }
 
+int drape_vertex_normals_noalloc(const float* positions, uint32_t n, const uint32_t* triangles, uint32_t n_triangles,
+                                 const uint32_t* weld_pairs, uint32_t n_welds, float* out, void* stream) {
+  return guard([&] {
+    if (n) {
+      need(positions, "positions");
+      need(out, "output");
+    }
+    if (n_triangles) need(triangles, "triangles");
+    if (n_welds) need(weld_pairs, "weld pairs");
+    requireDevice();
+    nrm::vertexNormalsNoAlloc(positions, n, triangles, n_triangles, weld_pairs, n_welds, out,
+                              static_cast<hipStream_t>(stream));
+  });
+}
+
 int drape_bsr_spmv(uint32_t n, uint32_t n_blocks, const uint32_t* row_start, const uint32_t* col_index,
                    const float* blocks, const uint8_t* pinned, const float* x, float* y, int32_t reps,
                    double* mean_us) {

@@ -2,6 +2,7 @@
 #include "normals.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -126,6 +127,189 @@ void Mesh::compute(const dk::F3* pos, dk::F3* out, hipStream_t st) const {
                                          weldB_.as<uint32_t>(), acc_.as<dk::F3>());
   kNormalize<<<blocks, 256, 0, st>>>(n_, acc_.as<dk::F3>(), out);
   check(hipGetLastError(), "normals launch");
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Allocation-free form for callers that cannot allocate (code under graph capture, serving loops): the caller's
+// device arrays in and out, and every working array — the incident-triangle lists and the per-vertex sums — in
+// one fixed device array of this file, `scratch`, held by each GPU until the process ends. Its address on a
+// device comes from hipGetSymbolAddress (the array's name in host code is only a stand-in for it), looked up once
+// per device with that device current.
+
+namespace {
+
+// Layout of `scratch` in 32-bit words: the error flag, per-vertex counts (n + 1), list starts (n + 1), the
+// incident triangle of each corner (3 T), the per-vertex sums (3 n floats).
+constexpr size_t kFlagWords = 64;
+constexpr size_t kScratchWords =
+    kFlagWords + 2 * (size_t(kNoAllocMaxVertices) + 1) + 3 * size_t(kNoAllocMaxTriangles) + 3 * size_t(kNoAllocMaxVertices);
+
+}  // namespace
+
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex gScratchLock;
+uint32_t* gScratch[kMaxDevices] = {};
+
+// This device's copy of `scratch` (looked up on the first call on the device).
+uint32_t* deviceScratch() {
+  int dev = 0;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("hip: device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(gScratchLock);
+  if (!gScratch[dev]) {
+    void* p = nullptr;
+    check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    gScratch[dev] = static_cast<uint32_t*>(p);
+  }
+  return gScratch[dev];
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) kNaClear(uint32_t* words, uint32_t count) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < count) words[i] = 0u;
+}
+
+// Counts each vertex's corners; a corner index >= n sets the error flag and is left out.
+__global__ void __launch_bounds__(256) kNaCount(const uint32_t* __restrict__ tri, uint32_t corners, uint32_t n,
+                                                uint32_t* counts, uint32_t* flag) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= corners) return;
+  const uint32_t v = tri[i];
+  if (v >= n) {
+    atomicOr(flag, 1u);
+    return;
+  }
+  atomicAdd(counts + v, 1u);
+}
+
+// Exclusive scan of counts[0, n) into starts[0, n] (one workgroup).
+__global__ void __launch_bounds__(1024) kNaScan(const uint32_t* counts, uint32_t* starts, uint32_t n) {
+  __shared__ uint32_t buf[1024];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < n; base += 1024u) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t own = i < n ? counts[i] : 0u;
+    buf[threadIdx.x] = own;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+      const uint32_t add = threadIdx.x >= off ? buf[threadIdx.x - off] : 0u;
+      __syncthreads();
+      buf[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (i < n) starts[i] = carry + buf[threadIdx.x] - own;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += buf[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) starts[n] = carry;
+}
+
+// Lists each corner's triangle under its vertex (slot order arbitrary; sorted per vertex below).
+__global__ void __launch_bounds__(256) kNaFill(const uint32_t* __restrict__ tri, uint32_t corners, uint32_t n,
+                                               const uint32_t* __restrict__ starts, uint32_t* cursor,
+                                               uint32_t* list) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= corners) return;
+  const uint32_t v = tri[i];
+  if (v >= n) return;
+  list[starts[v] + atomicAdd(cursor + v, 1u)] = i / 3u;
+}
+
+// Sorts a vertex's triangles into increasing order, then sums their cross products in that order — the order,
+// and the arithmetic, of kAccumulate over the host-built lists.
+__global__ void __launch_bounds__(256) kNaSortAccumulate(uint32_t n, const dk::F3* __restrict__ pos,
+                                                         const uint32_t* __restrict__ tri,
+                                                         const uint32_t* __restrict__ starts, uint32_t* list,
+                                                         dk::F3* acc) {
+#pragma clang fp contract(off)
+  const uint32_t v = blockIdx.x * 256u + threadIdx.x;
+  if (v >= n) return;
+  const uint32_t b = starts[v], e = starts[v + 1];
+  for (uint32_t i = b + 1; i < e; ++i) {
+    const uint32_t key = list[i];
+    uint32_t j = i;
+    while (j > b && list[j - 1] > key) {
+      list[j] = list[j - 1];
+      --j;
+    }
+    list[j] = key;
+  }
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  for (uint32_t k = b; k < e; ++k) {
+    const uint32_t* t = tri + 3u * size_t(list[k]);
+    // Every corner of a listed triangle was checked by kNaCount: a triangle with a bad corner still lists its
+    // good corners, so its corners are checked again here and a bad one contributes nothing.
+    if (t[0] >= n || t[1] >= n || t[2] >= n) continue;
+    const dk::F3 c = triangleCross(pos[t[0]], pos[t[1]], pos[t[2]]);
+    sx = sx + c.x;
+    sy = sy + c.y;
+    sz = sz + c.z;
+  }
+  acc[v] = dk::F3{sx, sy, sz};
+}
+
+// The weld pairs in order, one lane (the reference's sequential pass, normals.cpp:13-17); a pair with an index
+// >= n sets the flag and is skipped.
+__global__ void __launch_bounds__(64) kNaWeld(const uint32_t* __restrict__ welds, uint32_t nWelds, uint32_t n,
+                                              dk::F3* acc, uint32_t* flag) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0) return;
+  for (uint32_t i = 0; i < nWelds; ++i) {
+    const uint32_t a = welds[2 * size_t(i)], b = welds[2 * size_t(i) + 1];
+    if (a >= n || b >= n) {
+      atomicOr(flag, 1u);
+      continue;
+    }
+    const dk::F3 na = acc[a], nb = acc[b];
+    const dk::F3 s{na.x + nb.x, na.y + nb.y, na.z + nb.z};
+    acc[a] = s;
+    acc[b] = s;
+  }
+}
+
+static void normalsNoAlloc(const dk::F3* pos, uint32_t n, const uint32_t* tri, uint32_t nTri, const uint32_t* welds,
+                    uint32_t nWelds, dk::F3* out, hipStream_t st) {
+  if (n > kNoAllocMaxVertices || nTri > kNoAllocMaxTriangles)
+    throw std::invalid_argument("normals without allocation: at most " + std::to_string(kNoAllocMaxVertices) +
+                                " vertices and " + std::to_string(kNoAllocMaxTriangles) + " triangles");
+  if (n == 0) return;
+  uint32_t* base = deviceScratch();
+  uint32_t* flag = base;
+  uint32_t* counts = base + kFlagWords;
+  uint32_t* starts = counts + (n + 1);
+  uint32_t* list = starts + (n + 1);
+  dk::F3* acc = reinterpret_cast<dk::F3*>(list + 3 * size_t(nTri));
+  const uint32_t corners = 3u * nTri;
+  // Previous calls leave their contents: the flag and the counts start from zero (a fill kernel, not a memset).
+  const uint32_t clear = uint32_t(kFlagWords) + n + 1;
+  kNaClear<<<(clear + 255u) / 256u, 256, 0, st>>>(base, clear);
+  if (corners) kNaCount<<<(corners + 255u) / 256u, 256, 0, st>>>(tri, corners, n, counts, flag);
+  kNaScan<<<1, 1024, 0, st>>>(counts, starts, n);
+  kNaClear<<<(n + 255u) / 256u, 256, 0, st>>>(counts, n);  // reused as the fill cursors
+  if (corners) kNaFill<<<(corners + 255u) / 256u, 256, 0, st>>>(tri, corners, n, starts, counts, list);
+  kNaSortAccumulate<<<(n + 255u) / 256u, 256, 0, st>>>(n, pos, tri, starts, list, acc);
+  if (nWelds) kNaWeld<<<1, 64, 0, st>>>(welds, nWelds, n, acc, flag);
+  kNormalize<<<(n + 255u) / 256u, 256, 0, st>>>(n, acc, out);
+  check(hipGetLastError(), "normals launch");
+  uint32_t bad = 0;
+  check(hipMemcpyAsync(&bad, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "copy");
+  check(hipStreamSynchronize(st), "normals");
+  if (bad) throw std::out_of_range("normals: triangle or weld vertex out of range");
+}
+
+void vertexNormalsNoAlloc(const float* positions, uint32_t n, const uint32_t* triangles, uint32_t nTriangles,
+                          const uint32_t* welds, uint32_t nWelds, float* out, hipStream_t st) {
+  normalsNoAlloc(reinterpret_cast<const dk::F3*>(positions), n, triangles, nTriangles, welds, nWelds,
+                 reinterpret_cast<dk::F3*>(out), st);
 }
 
 }  // namespace nrm

@@ -22,6 +22,16 @@
 
 namespace nrm {
 
+// Limits of the allocation-free form (its fixed working array: about 11.5 MB per GPU).
+constexpr uint32_t kNoAllocMaxVertices = 1u << 18;   // 262,144
+constexpr uint32_t kNoAllocMaxTriangles = 1u << 19;  // 524,288
+
+// The allocation-free form: positions (3 floats per vertex), triangles, welds and out are device arrays; no
+// memory is allocated. Calls that overlap on one GPU share its working array and race. Synchronises `st`.
+// std::invalid_argument beyond the limits, std::out_of_range for an index >= n (the result is then undefined).
+void vertexNormalsNoAlloc(const float* positions, uint32_t n, const uint32_t* triangles, uint32_t nTriangles,
+                          const uint32_t* welds, uint32_t nWelds, float* out, hipStream_t st);
+
 class Mesh {
  public:
   // n vertices; triangles: 3 vertex indices each; welds: (a, b) pairs. Indices are checked (std::out_of_range).

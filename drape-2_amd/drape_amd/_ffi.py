@@ -329,6 +329,8 @@ def _declare(lib):
         "drape_solver_read_terms": (C.c_int, [S, u32p, u32p, f64p, u32p]),
         "drape_solver_gather_sizes": (C.c_int, [S, u32p, u32p]),
         "drape_solver_pcg_report": (C.c_int, [S, C.POINTER(PcgReport)]),
+        "drape_vertex_normals_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                                  C.c_uint32, C.c_void_p, C.c_void_p]),
         "drape_solver_set_refinement": (C.c_int, [S, C.c_int32]),
         "drape_solver_read_gather": (C.c_int, [S, u32p, u32p, u32p, u32p]),
         "drape_solver_debug_assemble": (C.c_int, [S, C.c_double, f32p, f64p, f32p, f32p]),

@@ -329,6 +329,17 @@ int drape_bake_collision_field(const drape_body_mesh* components, uint32_t n, co
 int drape_vertex_normals(const float* positions, uint32_t n, const uint32_t* triangles, uint32_t n_triangles,
                          const uint32_t* weld_pairs, uint32_t n_welds, float* out);
 
+/* drape_vertex_normals without any allocation. No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). positions (3
+ * floats per vertex), triangles, weld_pairs and out are DEVICE pointers the caller owns; stream is a hipStream_t
+ * (NULL: the null stream), synchronised before the call returns. The working memory (incident-triangle lists and
+ * per-vertex sums) is a fixed device array of the library, about 11.5 MB per GPU, held until the process ends:
+ * at most 262,144 vertices and 524,288 triangles (std::invalid_argument beyond). Every call on a GPU shares that
+ * array: calls that overlap on one GPU race. Same results as drape_vertex_normals, bit for bit; std::out_of_range
+ * for an index >= n (out is then undefined). */
+int drape_vertex_normals_noalloc(const float* positions, uint32_t n, const uint32_t* triangles, uint32_t n_triangles,
+                                 const uint32_t* weld_pairs, uint32_t n_welds, float* out, void* stream);
+
 /* The solver's current positions and their vertex normals (as drape_vertex_normals over every garment's mesh
  * triangles, every seam pair welded), in particle order, in one call: the position exchange of a live viewer
  * (the reference's viewport reads positions, then Renderer::updateGarment computes normals on the CPU,
