@@ -85,12 +85,14 @@ GOLDEN = os.path.join(REPO, "tests", "golden", "c2_drop_oracle.npz")
 WEIGHTS = os.path.join(REPO, "drape-2_amd", "drape_amd", "weights", "warmstart.npy")
 
 
-def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 5), ref_iterations: int = 15,
-                  sample_every: int = 4) -> dict:
-    """Learned warm start against the previous-velocity guess. Each method steps the drop with Newton capped at
-    each of `caps`; every `sample_every`-th timed step a tightly converged plain solve (Newton <= ref_iterations,
-    same PCG) from that method's own state measures its error (max particle distance, mm). Reports steps/s over the
-    timed steps (the reference solves excluded), iteration counts and how often the energy check kept the learned guess."""
+def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 4, 5), ref_iterations: int = 40,
+                  sample_every: int = 2) -> dict:
+    """Learned warm start against the previous-velocity guess at matched error. Each method steps the drop with
+    Newton capped at each of `caps`; every `sample_every`-th timed step the converged plain solve (Newton <=
+    ref_iterations, same PCG) from that method's own state before the step judges the step's error (max and RMS
+    particle distance, mm). Reports steps/s over the timed steps (the reference solves excluded), iteration counts,
+    how often the energy check kept the learned guess, and for each learned run the plain run it is compared
+    with: the most accurate plain run whose mean RMS error is still at least the learned run's."""
     from drape_amd import HipSolver, learn
     net = learn.WarmStartNet()
     net.load(net_path)
@@ -104,13 +106,11 @@ def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 5
             s.build(sc)
             if learned:
                 s.set_warm_start(net)
-            errs, t = [], 0.0
+            errs, rms, t = [], [], 0.0
             for i in range(warmup + steps):
                 check = i >= warmup and (i - warmup) % sample_every == 0
                 if check:
-                    ref.restore(s.snapshot())
-                    ref.step(1 / 60, 1, ref_iterations)
-                    xj = ref.read_positions()
+                    snap = s.snapshot()
                 if i == warmup:
                     s.reset_counters()
                 t0 = time.perf_counter()
@@ -118,19 +118,34 @@ def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 5
                 if i >= warmup:
                     t += time.perf_counter() - t0
                 if check:
-                    d = np.linalg.norm(s.read_positions().astype(np.float64) - xj, axis=1)
+                    ref.restore(snap)
+                    ref.step(1 / 60, 1, ref_iterations)
+                    d = np.linalg.norm(s.read_positions().astype(np.float64) - ref.read_positions(), axis=1)
                     errs.append(float(d.max()) * 1e3)
+                    rms.append(float(np.sqrt(np.mean(d * d))) * 1e3)
             c = s.counters()
             runs.append({"guess": "learned" if learned else "previous velocity", "newton_cap": k,
                          "steps_per_s": steps / t, "newton_per_step": c["newton_iterations"] / steps,
                          "pcg_per_step": c["pcg_iterations"] / steps,
+                         "mean_rms_error_mm": float(np.mean(rms)), "mean_max_error_mm": float(np.mean(errs)),
                          "max_step_error_mm": max(errs), "median_step_error_mm": float(np.median(errs)),
                          "steps_within_0p1mm": int(sum(e <= 0.1 for e in errs)), "checked_steps": len(errs),
                          "learned_guess_kept": c["warm_start_accepted"],
                          "learned_guess_rejected": c["warm_start_fallbacks"]})
-    return {"error_reference": f"plain solve with Newton <= {ref_iterations} from the method's own state, "
-                     f"every {sample_every} timed steps",
-            "weights": os.path.relpath(net_path, REPO), "runs": runs}
+    plain = [r for r in runs if r["guess"] != "learned"]
+    matched = []
+    for r in runs:
+        if r["guess"] != "learned":
+            continue
+        worse = [p for p in plain if p["mean_rms_error_mm"] >= r["mean_rms_error_mm"]]
+        p = min(worse, key=lambda q: q["mean_rms_error_mm"]) if worse else None
+        matched.append({"learned_cap": r["newton_cap"], "plain_cap": p["newton_cap"] if p else None,
+                        "speedup": r["steps_per_s"] / p["steps_per_s"] if p else None,
+                        "rms_error_mm": [r["mean_rms_error_mm"], p["mean_rms_error_mm"] if p else None],
+                        "mean_max_error_mm": [r["mean_max_error_mm"], p["mean_max_error_mm"] if p else None]})
+    return {"error_reference": f"converged plain solve (Newton <= {ref_iterations}) from the method's own state "
+                               f"before the step, every {sample_every} timed steps",
+            "weights": os.path.relpath(net_path, REPO), "runs": runs, "matched_rms_error": matched}
 
 
 def algorithmic_step_bytes(s, counters: dict, steps: int, pcg_launch_bytes: float) -> float:

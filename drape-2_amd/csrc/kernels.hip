@@ -237,12 +237,14 @@ __device__ inline F3 applyPrecond(const float* m, uint32_t cflags, float4 plane,
 }
 
 // Substep start (cpu_solver.cpp:234-263). Per row: the start position; the inertial target relative to it,
-// h v + h^2 (g + f/m), with gravity only while the garment settles; the Newton guess u = h (v + dv), where dv = 0
-// is the reference's previous-velocity guess and a learned warm start supplies dv otherwise; and, for a particle
+// y = h v + h^2 (g + f/m), with gravity only while the garment settles; the Newton guess, the reference's
+// previous-velocity guess u = h v (cpu_solver.cpp:244), or with a learned warm start u = y + h d, the inertial
+// target corrected by the network's velocity residual d (what the elastic and contact forces add to the free
+// flight; near zero in free flight, so a network's error scales with a small quantity); and, for a particle
 // within reach of the body, its contact plane: the field's tangent plane at the start position, offset by the
 // contact radius, with friction weighted by the normal force of the previous solve. Pinned and padding rows have
 // no target, no guess and no contact.
-__global__ void __launch_bounds__(256) kSubstepBegin(View v) {
+__global__ void __launch_bounds__(256) kSubstepBegin(View v, uint32_t learned) {
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
   const uint32_t s = v.tileSeg[tile];
@@ -269,7 +271,7 @@ __global__ void __launch_bounds__(256) kSubstepBegin(View v) {
     const F3 f = v.fExt[row];
     const float invMass = 1.0f / v.mass[row];
     target = h * vel + (h * h) * (g + invMass * f);
-    guess = h * (vel + v.warm[row]);
+    guess = learned ? target + h * v.warm[row] : h * vel;
     if (P.field.present) {
       const float radius = fabricRadius(P, v.fabrics[v.rowFabric[row]]);
       const float dist = bodyDistance(v.fieldValues, P.field, here);
@@ -1589,8 +1591,8 @@ __global__ void __launch_bounds__(256) kWarmFeatures(View v) {
   v.phiN[row] = pn;
 }
 
-// Per row: step = the move from the learned guess to the previous-velocity guess (-h dv); each segment evaluates
-// its energy at the learned guess first (lsAlpha = 0).
+// Per row: step = the move from the learned guess (in u) to the previous-velocity guess h v; each segment
+// evaluates its energy at the learned guess first (lsAlpha = 0).
 __global__ void __launch_bounds__(256) kWarmPrepare(View v) {
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
@@ -1598,7 +1600,7 @@ __global__ void __launch_bounds__(256) kWarmPrepare(View v) {
   const SegParams& P = v.seg[s];
   if (tile == P.tileBegin && threadIdx.x == 0) v.state[s].lsAlpha = 0.0;
   const uint32_t row = tile * kTile + threadIdx.x;
-  v.step[row] = (v.rowFlags[row] & (kRowPinned | kRowPad)) ? f3(0, 0, 0) : -P.h * v.warm[row];
+  v.step[row] = (v.rowFlags[row] & (kRowPinned | kRowPad)) ? f3(0, 0, 0) : P.h * v.v[row] - v.u[row];
 }
 
 // Per segment: energy of the learned guess into eTrial, then evaluate the previous-velocity guess (lsAlpha = 1).
@@ -2040,8 +2042,8 @@ __global__ void kSellCol(BuildView b, const uint32_t* sellBase, const uint32_t* 
 // ---------------------------------------------------------------------------------------------------------------
 // Launchers.
 
-void launchSubstepBegin(const View& v, hipStream_t st) {
-  kSubstepBegin<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);
+void launchSubstepBegin(const View& v, bool learned, hipStream_t st) {
+  kSubstepBegin<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, learned ? 1u : 0u);
 }
 void launchHoldContacts(const View& v, hipStream_t st) { kHoldContacts<<<xcdGrid(v.nTiles), 256, 0, st>>>(v); }
 void launchAssemble(const View& v, uint32_t maxK, bool, hipStream_t st) {

@@ -121,7 +121,9 @@ struct BuildView {
 };
 
 // ---- launchers (kernels.hip); all asynchronous on `st` ----
-void launchSubstepBegin(const View& v, hipStream_t st);
+// learned: the Newton guess is the inertial target plus h times the network's residual in View::warm (else the
+// reference's u = h v).
+void launchSubstepBegin(const View& v, bool learned, hipStream_t st);
 void launchHoldContacts(const View& v, hipStream_t st);
 void launchAssemble(const View& v, uint32_t maxK, bool contact, hipStream_t st);
 void launchNewtonBegin(const View& v, hipStream_t st);

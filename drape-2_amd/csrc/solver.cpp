@@ -940,7 +940,7 @@ void HipSolver::substep(double h, int iterations) {
     const ws::FrameView f{view_.x, view_.v, view_.dvLast, view_.phiN};
     net_->predict(meshGraph_->view(), f, warm_.as<dk::F3>(), stream_);
   }
-  dk::launchSubstepBegin(view_, stream_);
+  dk::launchSubstepBegin(view_, net_ != nullptr, stream_);
   if (net_) dk::launchWarmCheck(view_, stream_);
   // Each Newton iteration: the pre-solve launches, the PCG solve (chunks, polled), then the post-solve launches.
   // With graphs, the post-solve part runs the first kGraphTrials line-search trials, the apply and the end test
@@ -1430,7 +1430,7 @@ void HipSolver::debugAssemble(double h, const float* u, double* energy, float* g
   // Keep the contact force the next step's friction reads.
   std::vector<float> savedForce(nRows_);
   fromDevice(savedForce.data(), cforce_.as<void>(), sizeof(float) * nRows_);
-  dk::launchSubstepBegin(view_, stream_);
+  dk::launchSubstepBegin(view_, false, stream_);
   std::vector<dk::F3> us(nRows_ + 1, dk::F3{0, 0, 0});
   for (size_t i = 0; i < model_.x.size(); ++i) us[rowOf_[i]] = dk::F3{u[3 * i], u[3 * i + 1], u[3 * i + 2]};
   check(hipMemcpyAsync(u_.as<void>(), us.data(), sizeof(dk::F3) * (nRows_ + 1), hipMemcpyHostToDevice, stream_), "copy");

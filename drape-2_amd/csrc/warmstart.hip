@@ -3,9 +3,10 @@
 // Layout: every activation is row-major [rows x columns]; rows are the graph's rows (a multiple of 256). BF16
 // values are stored as their 16-bit patterns (uint16_t) and converted with round-to-nearest-even.
 //
-// MFMA: v_mfma_f32_16x16x16_bf16 on one wavefront. Lane l holds A[l % 16][4 (l / 16) + j] and
-// B[4 (l / 16) + j][l % 16] (j = 0..3, four BF16 values in two VGPRs) and receives D[4 (l / 16) + j][l % 16]
-// (four FP32 values). The GEMM tests check this layout against a host product.
+// MFMA on one wavefront, gfx950's v_mfma_f32_16x16x32_bf16 wherever the reduction depth is a multiple of 32 (every
+// layer but the 16-wide ones, which use v_mfma_f32_16x16x16_bf16): lane l holds A[l % 16][K (l / 16) + j] and
+// B[K (l / 16) + j][l % 16] (j < K, K = 8 or 4 BF16 values) and receives D[4 (l / 16) + j][l % 16] (four FP32
+// values). The GEMM tests check this layout against a host product.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -25,6 +26,8 @@ void check(hipError_t e, const char* what) {
 }
 
 typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr uint32_t kChunk = 256;  // rows per workgroup in the reduction kernels
@@ -43,58 +46,73 @@ __device__ inline float fromBf16(uint16_t b) { return __uint_as_float(uint32_t(b
 // ---------------------------------------------------------------------------------------------------------------
 // GEMMs.
 
+// One MFMA step over a reduction slice: 32 deep (8 BF16 values per lane) or 16 deep (4 values).
+__device__ inline f32x4 mfma32(u16x8 a, u16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                 0, 0, 0);
+}
+__device__ inline f32x4 mfma16(s16x4 a, s16x4 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+
 // Y[N x Mo] (FP32, row stride ldy) = X[N x Ki] (BF16, row stride ldx) . Bt^T with Bt [Mo x Ki] BF16 row-major.
 // One wavefront per 16 rows, four per workgroup; N a multiple of 64.
 template <int Ki, int Mo>
 __global__ void __launch_bounds__(256) kGemm(const uint16_t* __restrict__ X, int ldx, const uint16_t* __restrict__ Bt,
                                              float* __restrict__ Y, int ldy, uint32_t N) {
+  constexpr bool K32 = Ki % 32 == 0;
+  constexpr int KL = K32 ? 8 : 4;  // reduction values per lane per step
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   const uint32_t row0 = (blockIdx.x * 4u + wave) * 16u;
   if (row0 >= N) return;
-  const uint32_t r = lane & 15u, kq = (lane >> 4) * 4u;
+  const uint32_t r = lane & 15u, kq = (lane >> 4) * KL, oq = (lane >> 4) * 4u;
   f32x4 acc[Mo / 16];
 #pragma unroll
   for (int t = 0; t < Mo / 16; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int k0 = 0; k0 < Ki; k0 += 16) {
-    const s16x4 a = *reinterpret_cast<const s16x4*>(X + size_t(row0 + r) * ldx + k0 + kq);
+  for (int k0 = 0; k0 < Ki; k0 += 4 * KL) {
+    if constexpr (K32) {
+      const u16x8 a = *reinterpret_cast<const u16x8*>(X + size_t(row0 + r) * ldx + k0 + kq);
 #pragma unroll
-    for (int t = 0; t < Mo / 16; ++t) {
-      const s16x4 b = *reinterpret_cast<const s16x4*>(Bt + size_t(t * 16 + r) * Ki + k0 + kq);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+      for (int t = 0; t < Mo / 16; ++t)
+        acc[t] = mfma32(a, *reinterpret_cast<const u16x8*>(Bt + size_t(t * 16 + r) * Ki + k0 + kq), acc[t]);
+    } else {
+      const s16x4 a = *reinterpret_cast<const s16x4*>(X + size_t(row0 + r) * ldx + k0 + kq);
+#pragma unroll
+      for (int t = 0; t < Mo / 16; ++t)
+        acc[t] = mfma16(a, *reinterpret_cast<const s16x4*>(Bt + size_t(t * 16 + r) * Ki + k0 + kq), acc[t]);
     }
   }
 #pragma unroll
   for (int t = 0; t < Mo / 16; ++t)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) Y[size_t(row0 + kq + j) * ldy + t * 16 + r] = acc[t][j];
+    for (int j = 0; j < 4; ++j) Y[size_t(row0 + oq + j) * ldy + t * 16 + r] = acc[t][j];
 }
 
 // partial[chunk][Ki x Mo] = X[chunk rows]^T . D[chunk rows] (X BF16 [N x Ki], D BF16 [N x Mo]); 256 rows per
-// workgroup, N a multiple of 256. The 16x16 output tiles are dealt to the four wavefronts in turn.
+// workgroup, N a multiple of 256, 32 rows per MFMA step. The 16x16 output tiles are dealt to the four wavefronts
+// in turn.
 template <int Ki, int Mo>
 __global__ void __launch_bounds__(256) kGemmTN(const uint16_t* __restrict__ X, int ldx, const uint16_t* __restrict__ D,
                                                int ldd, float* __restrict__ partial) {
   constexpr int TN = Mo / 16, T = (Ki / 16) * TN, PER = (T + 3) / 4;
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   const uint32_t rBase = blockIdx.x * kChunk;
-  const uint32_t r = lane & 15u, kq = (lane >> 4) * 4u;
+  const uint32_t r = lane & 15u, kq = (lane >> 4) * 8u, oq = (lane >> 4) * 4u;
   f32x4 acc[PER];
 #pragma unroll
   for (int p = 0; p < PER; ++p) acc[p] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (uint32_t rs = rBase; rs < rBase + kChunk; rs += 16u) {
+  for (uint32_t rs = rBase; rs < rBase + kChunk; rs += 32u) {
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
       const int t = int(wave) + 4 * p;
       if (t < T) {
         const int ti = t / TN, tn = t % TN;
-        s16x4 a, b;
+        u16x8 a, b;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          a[j] = short(X[size_t(rs + kq + j) * ldx + ti * 16 + r]);
-          b[j] = short(D[size_t(rs + kq + j) * ldd + tn * 16 + r]);
+        for (int j = 0; j < 8; ++j) {
+          a[j] = X[size_t(rs + kq + j) * ldx + ti * 16 + r];
+          b[j] = D[size_t(rs + kq + j) * ldd + tn * 16 + r];
         }
-        acc[p] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[p], 0, 0, 0);
+        acc[p] = mfma32(a, b, acc[p]);
       }
     }
   }
@@ -105,7 +123,7 @@ __global__ void __launch_bounds__(256) kGemmTN(const uint16_t* __restrict__ X, i
     if (t < T) {
       const int ti = t / TN, tn = t % TN;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) out[(ti * 16 + kq + j) * Mo + tn * 16 + r] = acc[p][j];
+      for (int j = 0; j < 4; ++j) out[(ti * 16 + oq + j) * Mo + tn * 16 + r] = acc[p][j];
     }
   }
 }

@@ -1,8 +1,9 @@
 """Learned warm start of the Newton solve: the network, its graphs, training data from the solver, training.
 
-The network (drape-2_amd/csrc/warmstart.hpp) predicts each step's velocity change dv from the state; attached to
-a `HipSolver` it seeds Newton with u = h (v + dv) instead of the reference's u = h v (cpu_solver.cpp:244), and
-the solver keeps the plain guess whenever the prediction's energy is higher. Training runs on the GPU through the
+The network (drape-2_amd/csrc/warmstart.hpp) predicts from the state each step's velocity residual beyond free
+flight, d = dv - h (g + f_ext / m) (dv the step's velocity change); attached to a `HipSolver` it seeds Newton with
+the inertial target plus h d, u = h v + h^2 (g + f_ext / m) + h d, instead of the reference's u = h v
+(cpu_solver.cpp:244), and the solver keeps the plain guess whenever the prediction's energy is higher. Training runs on the GPU through the
 network's own forward/backward kernels; this module only moves arrays and drives the loop.
 """
 from __future__ import annotations
@@ -179,11 +180,24 @@ def stack(samples: list) -> Sample:
     return Sample(g, frame, np.concatenate([s.target for s in samples]))
 
 
+def residual_target(scene, dv: np.ndarray, h: float) -> np.ndarray:
+    """The network's target for a realised velocity change dv: its residual beyond free flight, dv - h g for a
+    free particle of a settling garment (the guess is then the inertial target plus h times the residual), zero
+    for a pinned one. Scenes with external forces are not used for training."""
+    if scene.forces:
+        raise ValueError("training scenes carry no external forces")
+    out = dv.astype(np.float32) - np.float32(h) * np.asarray(scene.settings["gravity"], np.float32)
+    if scene.pins:
+        out[[p for p, _ in scene.pins]] = 0.0
+    return out
+
+
 def collect(scene, steps: int, dt: float = 1 / 60, iterations: int = 5, every: int = 1, solver=None,
             graph: MeshGraph | None = None) -> list:
-    """Solver-generated training pairs: the inputs of the state before a step and the dv the step realised.
-    The plain (previous-velocity) guess is used; every `every`-th step is kept. Pass `graph` to share one
-    graph between scenes of the same mesh (so that their frames can be stacked)."""
+    """Solver-generated training pairs: the inputs of the state before a step and the velocity residual the step
+    realised beyond free flight (residual_target). The plain (previous-velocity) guess is used; every `every`-th
+    step is kept. Pass `graph` to share one graph between scenes of the same mesh (so that their frames can be
+    stacked)."""
     from .solver import HipSolver
     s = solver or HipSolver()
     if solver is None:
@@ -194,7 +208,7 @@ def collect(scene, steps: int, dt: float = 1 / 60, iterations: int = 5, every: i
         frame = s.frame_inputs() if k % every == 0 else None
         s.step(dt, 1, iterations)
         if frame is not None:
-            out.append(Sample(graph, frame, s.frame_inputs()["dv_prev"].copy()))
+            out.append(Sample(graph, frame, residual_target(scene, s.frame_inputs()["dv_prev"], dt)))
     return out
 
 

@@ -258,7 +258,9 @@ class HipSolver:
     # ---- learned warm start ----
 
     def set_warm_start(self, net) -> None:
-        """Attach a `learn.WarmStartNet` (None restores the previous-velocity guess). Keep `net` alive."""
+        """Attach a `learn.WarmStartNet` (None restores the previous-velocity guess): each substep then seeds
+        Newton at the inertial target plus h times the predicted velocity residual when that lowers the energy.
+        Keep `net` alive."""
         self._warm_net = net
         _check(_lib().drape_solver_set_warm_start(self._h, net.handle if net is not None else None))
 

@@ -366,9 +366,11 @@ int drape_eval_terms(int32_t kind, uint32_t n, const float* x, const float* para
 int drape_field_query(const drape_field_desc* field, uint32_t n, const float* p, float* phi, float* normal);
 
 /* ---- Learned warm start. No counterpart in the reference: these entry points replace its previous-velocity
- * Newton guess u = h v (cpu_solver.cpp:244) with u = h (v + dv) where dv is predicted by a per-vertex
- * message-passing network over the mesh edge graph, and keep the plain guess whenever the prediction's energy
- * is higher. Network: 3 rounds, hidden width 64, BF16 MFMA layers with FP32 accumulation, trained on the GPU. */
+ * Newton guess u = h v (cpu_solver.cpp:244) with u = y + h d, where y = h v + h^2 (g + f_ext / m) is the inertial
+ * target of the step and d, the velocity residual of the step beyond free flight (its dv minus h (g + f_ext / m)),
+ * is predicted by a per-vertex message-passing network over the mesh edge graph; the plain guess is kept whenever
+ * the prediction's energy is higher. Network: 3 rounds, hidden width 64, BF16 MFMA layers with FP32
+ * accumulation, trained on the GPU. */
 typedef struct drape_net drape_net;     /* weights, Adam state and working buffers on the device */
 typedef struct drape_graph drape_graph; /* a mesh edge graph on the device */
 
@@ -386,7 +388,8 @@ int drape_graph_create(drape_graph** out, uint32_t n, const uint32_t* row_start,
 void drape_graph_destroy(drape_graph* g);
 
 /* Host arrays of n vertices: x, v, dv_prev, target, dv_out 3 floats each; phi_normal 4 (normal, SDF distance).
- * predict: dv (m/s) per vertex. train_step: the mean squared error of dv / 0.5 m/s into *loss and, when grad
+ * predict: the velocity residual d (m/s) per vertex, into dv_out. train_step: the mean squared error of
+ * (prediction - target) / 0.5 m/s into *loss and, when grad
  * is not NULL, the parameter gradient; lr > 0 then takes one Adam step. */
 int drape_net_predict(drape_net* net, const drape_graph* g, const float* x, const float* v, const float* dv_prev,
                       const float* phi_normal, float* dv_out);
@@ -395,7 +398,7 @@ int drape_net_train_step(drape_net* net, const drape_graph* g, const float* x, c
                          float* grad);
 
 /* Attach a network to a solver (NULL detaches; the caller keeps the network alive while attached). Each substep
- * then predicts dv, evaluates the energy at both guesses and keeps the lower (drape_counters.warm_start_*). */
+ * then predicts d, evaluates the energy at both guesses and keeps the lower (drape_counters.warm_start_*). */
 int drape_solver_set_warm_start(drape_solver* s, drape_net* net);
 /* The network inputs of the solver's current state, particle order (any output may be NULL). After a step,
  * dv_prev holds the dv that step realised: the training target for the inputs read before it. */

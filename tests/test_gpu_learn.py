@@ -87,8 +87,10 @@ def test_training_on_solver_frames_reduces_loss(hip):
 
 
 def test_energy_fallback_keeps_the_plain_trajectory(hip):
-    """A network that predicts dv = 0 gives the plain guess (accepted); one that predicts 50 m/s raises the
-    energy and is rejected every substep. Either way the trajectory is the plain one, bit for bit."""
+    """A network that predicts a residual of 50 m/s raises the energy and is rejected every substep: the
+    trajectory is the plain one, bit for bit. One that predicts d = 0 offers the inertial target, which has the
+    lower energy in most substeps of the free-falling, corner-pinned sheet; where it is kept, Newton starts there
+    and the steps land within 0.1 mm of the plain ones."""
     from drape_amd import learn
     sc = scenes.c1_sheet()
     plain = hip.HipSolver()
@@ -110,14 +112,22 @@ def test_energy_fallback_keeps_the_plain_trajectory(hip):
         for _ in range(6):
             s.step(1 / 60, 1, 5)
         c = s.counters()
-        print(f"accepted {c['warm_start_accepted']} fallbacks {c['warm_start_fallbacks']}", flush=True)
-        assert c["warm_start_accepted" if accepted else "warm_start_fallbacks"] == 6
-        np.testing.assert_array_equal(s.read_positions(), plain.read_positions())
+        diff = float(np.max(np.abs(s.read_positions().astype(np.float64) - plain.read_positions())))
+        print(f"accepted {c['warm_start_accepted']} fallbacks {c['warm_start_fallbacks']}, "
+              f"max |x - x_plain| {diff * 1e3:.6f} mm", flush=True)
+        assert c["warm_start_accepted"] + c["warm_start_fallbacks"] == 6
+        if accepted:
+            assert c["warm_start_accepted"] >= 3
+            assert diff <= 1e-4
+        else:
+            assert c["warm_start_fallbacks"] == 6
+            np.testing.assert_array_equal(s.read_positions(), plain.read_positions())
 
 
 def test_trained_network_guess_beats_previous_velocity_guess(hip):
-    """Trained on drops of the 32x32 sheet from three seeds, the network's dv on a held-out drop is closer to the
-    converged step's dv than the previous-velocity guess (dv = 0)."""
+    """Trained on drops of the 32x32 sheet from three seeds, the network's velocity residual on a held-out drop is
+    closer to the converged step's than the previous-velocity guess's (u = h v: a residual of -h g on the free
+    particles, 0 on the pinned ones)."""
     from drape_amd import learn
     train = []
     for seed in (21, 22, 23):
@@ -126,10 +136,12 @@ def test_trained_network_guess_beats_previous_velocity_guess(hip):
     net = learn.WarmStartNet(seed=4)
     learn.train(net, train, epochs=12, lr=2e-3)
     err_net = err_plain = 0.0
+    sc = scenes.c1_sheet(seed=29)
+    plain = learn.residual_target(sc, np.zeros((sc.n, 3), np.float32), 1 / 60).astype(np.float64)
     for s in held[2:]:
         pred = net.predict(s.graph, s.frame)
         err_net += float(np.sum((pred - s.target) ** 2))
-        err_plain += float(np.sum(s.target.astype(np.float64) ** 2))
+        err_plain += float(np.sum((plain - s.target) ** 2))
     ratio = np.sqrt(err_net / err_plain)
     print(f"held-out guess error, network / previous velocity: {ratio:.3f}", flush=True)
     assert ratio < 0.5
