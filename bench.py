@@ -263,12 +263,12 @@ def batch_bench(steps: int, warmup: int, sample_every: int, shape: tuple, graph:
     }
 
 
-def tube_bench(steps: int) -> dict:
+def tube_bench(steps: int, shape: tuple, graph: bool = True) -> dict:
     """The seamed tube (two 200 x 300 panels sewn into a tube, 120,000 particles) falling onto the capsule
     mannequin and settling: mean time per implicit step over `steps` steps from the start, and the last 100."""
     from drape_amd import HipSolver, scenes
     sc = scenes.c3_tube()
-    s = HipSolver()
+    s = HipSolver(graph=graph, pcg_shape=shape)
     s.build(sc)
     times = []
     for _ in range(steps):
@@ -307,6 +307,9 @@ def main() -> int:
     ap.add_argument("--no-warm-start-ab", action="store_true",
                     help="skip the learned-vs-plain warm start comparison")
     ap.add_argument("--warm-start-weights", default=WEIGHTS)
+    ap.add_argument("--learned", action="store_true",
+                    help="seed Newton with the learned warm start in the timed run (default: the plain guess)")
+    ap.add_argument("--newton", type=int, default=5, help="Newton iteration cap of the timed run")
     ap.add_argument("--pcg-shape", default="0x0",
                     help="launch shape of the one-kernel PCG iteration, wavefronts x slots: 1x8, 1x4, 2x8, 4x8, "
                          "or 1x8h / 1x4h (column window staged in LDS); 0x0 = tune during warm-up")
@@ -317,12 +320,12 @@ def main() -> int:
     ap.add_argument("--batch", action="store_true",
                     help="instead of the sheet benchmark: the 64-sheet batch (one solver, one launch per kernel)")
     args = ap.parse_args()
+    spec = args.pcg_shape.lower()
+    shape = tuple(int(t) for t in spec.rstrip("h").split("x")) + (spec.endswith("h"),)
     if args.tube:
-        print(json.dumps(tube_bench(args.steps)), flush=True)
+        print(json.dumps(tube_bench(args.steps, shape, not args.no_graph)), flush=True)
         return 0
     if args.batch:
-        spec = args.pcg_shape.lower()
-        shape = tuple(int(t) for t in spec.rstrip("h").split("x")) + (spec.endswith("h"),)
         print(json.dumps(batch_bench(args.steps, args.warmup, args.sample_every, shape, not args.no_graph)),
               flush=True)
         return 0
@@ -340,16 +343,19 @@ def main() -> int:
         return scenes.c2_sheet(n=args.n)
 
     sc = make_scene()
-    spec = args.pcg_shape.lower()
-    halo = spec.endswith("h")
-    shape = tuple(int(t) for t in spec.rstrip("h").split("x")) + (halo,)
     s = HipSolver(graph=not args.no_graph, pcg_variant=args.pcg_variant, pcg_shape=shape)
     s.build(sc)
+    net = None
+    if args.learned:
+        from drape_amd import learn
+        net = learn.WarmStartNet()
+        net.load(args.warm_start_weights)
+        s.set_warm_start(net)
     if args.no_refinement:
         s.set_refinement(0)
     s.set_spmv_sampling(args.sample_every)
     for _ in range(args.warmup):
-        s.step(1 / 60, 1, 5)
+        s.step(1 / 60, 1, args.newton)
     window_start = s.snapshot()
     s.reset_counters()
     s.set_spmv_sampling(args.sample_every)
@@ -357,7 +363,7 @@ def main() -> int:
     checkpoints, elapsed = [], 0.0
     for i in range(args.steps):
         t0 = time.perf_counter()
-        s.step(1 / 60, 1, 5)  # returns after the device finished the step
+        s.step(1 / 60, 1, args.newton)  # returns after the device finished the step
         elapsed += time.perf_counter() - t0
         k = args.warmup + i + 1  # steps from rest
         if fixture is not None and k in [int(x) for x in fixture["steps"]]:
@@ -392,8 +398,9 @@ def main() -> int:
             "workload": f"sheet {args.n}x{args.n} ({sc.n} particles) dropped from y=0.4 m on sphere r=0.25 m + "
                         f"ground y=-0.3 m; steps {args.warmup + 1}..{args.warmup + args.steps} of the drop",
             "particles": sc.n,
-            "dt": 1 / 60, "substeps": 1, "newton_max": 5, "pcg_tolerance": 1e-6, "pcg_cap": 2000,
-            "solver": "plain warm start (u = h v)",
+            "dt": 1 / 60, "substeps": 1, "newton_max": args.newton, "pcg_tolerance": 1e-6, "pcg_cap": 2000,
+            "solver": ("learned warm start (u = y + h d, network " + os.path.relpath(args.warm_start_weights, REPO)
+                       + ")") if args.learned else "plain warm start (u = h v)",
             "pcg_form": "one-kernel" if args.pcg_variant == 1 else "two-kernel",
             "pcg_launch_shape": "%dx%d%s" % (s.pcg_shape()[0], s.pcg_shape()[1], "h" if s.pcg_shape()[2] else "")
                                 if args.pcg_variant == 1 else None,
