@@ -55,3 +55,38 @@ def test_batch_counters_sum_segments(hip):
         s.step(1 / 60, 1, 5)
         singles += s.counters()["pcg_iterations"]
     assert c["pcg_iterations"] == singles
+
+
+def test_full_batch_of_64_segments_matches_single_scenes_and_the_oracle(hip, oracle):
+    """The full batch (64 sheets of 2k-40k particles, 891,781 particles in one solver) for 12 steps, through first
+    contact: the smallest, the median and the largest segment equal their single-scene runs bit for bit, and the
+    four smallest segments lie within 1 mm of the FP64 oracle (the reference's short-run parity gate,
+    test_gpu_parity.cpp:54-60)."""
+    batch = scenes.c4_batch()
+    g = hip.HipSolver()
+    g.build_batch(batch)
+    assert g.segment_count() == 64
+    steps = 12
+    for _ in range(steps):
+        g.step(1 / 60, 1, 5)
+    xb = g.read_positions()
+    stats = g.segment_stats()
+    assert not any(x["has_non_finite"] for x in stats)
+    order = sorted(range(len(batch)), key=lambda k: batch[k].n)
+    for k in (order[0], order[len(order) // 2], order[-1]):
+        off, cnt = g.segment_range(k)
+        single = hip.HipSolver()
+        single.build(batch[k])
+        for _ in range(steps):
+            single.step(1 / 60, 1, 5)
+        print(f"segment {k} ({batch[k].name}, {cnt} particles): batched = single-scene bit for bit", flush=True)
+        np.testing.assert_array_equal(xb[off:off + cnt], single.read_positions())
+    for k in order[:4]:
+        off, cnt = g.segment_range(k)
+        o = oracle.OracleSolver()
+        o.build(batch[k])
+        for _ in range(steps):
+            o.step(1 / 60, 1, 5)
+        d = max_distance(xb[off:off + cnt], o.read_positions())
+        print(f"segment {k} ({batch[k].name}, {cnt} particles): oracle distance {d * 1e3:.4f} mm", flush=True)
+        assert d <= 1e-3
