@@ -125,6 +125,7 @@ struct EdgeTable {
 
 Model Model::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   if (!scenes && nScenes) throw std::invalid_argument("null scene list");
+  if (nScenes == 0) throw std::invalid_argument("scene has no particles");
   Model m;
   uint32_t rowCursor = 0;
   for (uint32_t si = 0; si < nScenes; ++si) {
@@ -295,6 +296,7 @@ Model Model::build(const drape_scene_desc* scenes, uint32_t nScenes) {
       m.garments.push_back(g);
     }
     seg.particleCount = static_cast<uint32_t>(m.x.size()) - seg.particleBegin;
+    if (seg.particleCount == 0) throw std::invalid_argument("scene has no particles");
     seg.garmentCount = static_cast<uint32_t>(m.garments.size()) - seg.garmentBegin;
     seg.rowBegin = rowCursor;
     seg.rowCount = std::max<uint32_t>(dk::kTile, (seg.particleCount + dk::kTile - 1) / dk::kTile * dk::kTile);

@@ -69,8 +69,9 @@ class HipSolver {
   void debugSpmv(const float* x, float* y);
   void debugSystem(float* blocks, float* grad, float* precond, uint32_t* contactFlags, float* contactNormals);
 
-  // Learned warm start: the network predicts each step's dv; null restores the previous-velocity guess. The
-  // solver does not own the network.
+  // Learned warm start: the network predicts each step's velocity residual beyond free flight, which seeds Newton
+  // at the inertial target plus h times it; null restores the previous-velocity guess. The solver does not own
+  // the network.
   void setWarmStart(ws::Net* net);
   // Network inputs of the current state in particle order: x, v, the last substep's realised dv (3 each) and the
   // SDF distance and normal (4: normal, distance). Any output may be null.

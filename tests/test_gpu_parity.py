@@ -393,6 +393,13 @@ def test_solver_errors_follow_the_reference_and_leave_it_usable(hip):
     bad.garments[0].mesh.triangle_piece[5] = 1
     with pytest.raises(IndexError, match="triangle piece out of range"):
         hip.HipSolver().build(bad)
+    # A scene without particles is refused before anything reaches the device, alone or in a batch.
+    empty = copy.deepcopy(sc)
+    empty.garments = []
+    with pytest.raises(ValueError, match="no particles"):
+        hip.HipSolver().build(empty)
+    with pytest.raises(ValueError, match="no particles"):
+        hip.HipSolver().build_batch([sc, empty])
     fresh = hip.HipSolver()
     fresh.build(sc)
     for s in (g, fresh):
