@@ -250,7 +250,7 @@ def batch_bench(steps: int, warmup: int, sample_every: int, shape: tuple, graph:
         "dtype": "f32 (FP64 reductions)", "data": "synthetic", "vs_baseline": None,
         "config": {"workload": f"64 sheets, {n} particles (2k-40k each), dropped on spheres/capsules + ground",
                    "particles": n, "particle_steps_per_s": n * steps / elapsed,
-                   "pcg_launch_shape": f"{w}x{k}{'h' if h else ''}",
+                   "pcg_launch_shape": f"{w}x{k}{('', 'h', 'p')[h]}",
                    "newton_per_step": c["newton_iterations"] / steps, "pcg_per_step": c["pcg_iterations"] / steps,
                    "non_finite_segments": sum(bool(x["has_non_finite"]) for x in stats),
                    "max_penetration_mm": 1e3 * max(x["max_body_penetration"] for x in stats)},
@@ -286,7 +286,7 @@ def tube_bench(steps: int, shape: tuple, graph: bool = True) -> dict:
         "config": {"workload": f"tube of two 200x300 panels ({sc.n} particles), seams k = 1e3 N/m, on the capsule "
                                "mannequin, PCG to 1e-6 (cap 2,000), Newton <= 5",
                    "last_100_steps_ms": 1000.0 * sum(tail) / len(tail),
-                   "pcg_launch_shape": f"{w}x{k}{'h' if h else ''}",
+                   "pcg_launch_shape": f"{w}x{k}{('', 'h', 'p')[h]}",
                    "newton_per_step": c["newton_iterations"] / steps, "pcg_per_step": c["pcg_iterations"] / steps,
                    "final_stats": st},
     }
@@ -312,7 +312,8 @@ def main() -> int:
     ap.add_argument("--newton", type=int, default=5, help="Newton iteration cap of the timed run")
     ap.add_argument("--pcg-shape", default="0x0",
                     help="launch shape of the one-kernel PCG iteration, wavefronts x slots: 1x8, 1x4, 2x8, 4x8, "
-                         "or 1x8h / 1x4h (column window staged in LDS); 0x0 = tune during warm-up")
+                         "or 1x8h / 1x4h (column window staged in LDS), or 1x8p (the persistent solve: one cooperative "
+                         "launch per PCG solve); 0x0 = tune during warm-up")
     ap.add_argument("--tee", choices=("draft", "standard", "fine"),
                     help="instead of the sheet benchmark: drape the reference's tee to rest at this quality")
     ap.add_argument("--tube", action="store_true",
@@ -321,7 +322,7 @@ def main() -> int:
                     help="instead of the sheet benchmark: the 64-sheet batch (one solver, one launch per kernel)")
     args = ap.parse_args()
     spec = args.pcg_shape.lower()
-    shape = tuple(int(t) for t in spec.rstrip("h").split("x")) + (spec.endswith("h"),)
+    shape = tuple(int(t) for t in spec.rstrip("hp").split("x")) + (2 if spec.endswith("p") else int(spec.endswith("h")),)
     if args.tube:
         print(json.dumps(tube_bench(args.steps, shape, not args.no_graph)), flush=True)
         return 0
@@ -402,7 +403,7 @@ def main() -> int:
             "solver": ("learned warm start (u = y + h d, network " + os.path.relpath(args.warm_start_weights, REPO)
                        + ")") if args.learned else "plain warm start (u = h v)",
             "pcg_form": "one-kernel" if args.pcg_variant == 1 else "two-kernel",
-            "pcg_launch_shape": "%dx%d%s" % (s.pcg_shape()[0], s.pcg_shape()[1], "h" if s.pcg_shape()[2] else "")
+            "pcg_launch_shape": "%dx%d%s" % (s.pcg_shape()[0], s.pcg_shape()[1], ("", "h", "p")[s.pcg_shape()[2]])
                                 if args.pcg_variant == 1 else None,
             "fp64_true_residual_refinement": not args.no_refinement,
             "graph": not args.no_graph,

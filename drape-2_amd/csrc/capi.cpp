@@ -358,7 +358,7 @@ int drape_solver_set_pcg_variant(drape_solver* s, int32_t variant) {
 int drape_solver_set_pcg_shape(drape_solver* s, int32_t waves, int32_t slots, int32_t halo) {
   return guard([&] {
     need(s, "solver");
-    s->impl.setPcgShape(waves, slots, halo != 0);
+    s->impl.setPcgShape(waves, slots, halo);
   });
 }
 
@@ -368,12 +368,26 @@ int drape_solver_pcg_shape(const drape_solver* s, int32_t* waves, int32_t* slots
     need(waves, "output");
     need(slots, "output");
     need(halo, "output");
-    int w = 0, k = 0;
-    bool h = false;
+    int w = 0, k = 0, h = 0;
     s->impl.pcgShape(w, k, h);
     *waves = w;
     *slots = k;
-    *halo = h ? 1 : 0;
+    *halo = h;
+  });
+}
+
+int drape_solver_set_solve_spin_cap(drape_solver* s, uint32_t polls) {
+  return guard([&] {
+    need(s, "solver");
+    s->impl.setSolveSpinCap(polls);
+  });
+}
+
+int drape_solver_persistent_available(const drape_solver* s, int32_t* available) {
+  return guard([&] {
+    need(s, "solver");
+    need(available, "output");
+    *available = s->impl.persistentAvailable() ? 1 : 0;
   });
 }
 

@@ -1217,6 +1217,284 @@ kPcgStep(View v, uint32_t q) {
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// Persistent solve (kPcgSolve): one cooperative launch runs the one-reduction PCG from the state kPcgStepInit (or a
+// refinement's kTrueResidual) left in parity slot 1 until every segment's solve has ended. Iteration k computes
+// exactly what the k-th kPcgStep<1, 8, true> launch computes (same expressions, same summation orders), so the two
+// forms give bit-identical results.
+//
+// One workgroup owns one 256-row tile for the whole solve, one wavefront per 64-row slice. The slice's matrix (upper
+// block values and columns, the lower slots' transposed blocks and columns; every slice has at most 8 + 8 slots)
+// and its rows' private PCG data (r, ap, step, the preconditioner, the row and contact flags) are loaded once and
+// stay in registers. An iteration moves only the (zr, t, p) window the tile gathers from, the tile's own new
+// (zr, t, p) and 7 FP64 partial sums: about 70 B per row against kPcgStep's 500.
+//
+// Exchange between workgroups. The L2 of each XCD is not coherent with the others, and an agent-scope release
+// fence writes back a whole L2 (buffer_wbl2), which costs microseconds when every workgroup of an XCD issues one.
+// So every value another workgroup reads during the launch is stored with device-scope stores (sc1: written
+// through to the device-coherent level) and loaded with device-scope loads (sc1), and a workgroup announces its
+// stores only after every wavefront has waited for their acknowledgement (s_waitcnt vmcnt(0)). The announcement is
+// a device-scope atomic on the segment's counters, and the waits poll a device-scope generation counter. Nothing
+// that changes during the launch is read through the scalar cache.
+//
+// Segment barrier: a two-level tree over the segment's tiles (groups of kBarrierGroup): the last arrival of a
+// group increments the segment's top counter, the last arrival at the top publishes the generation. Only
+// workgroups of one segment wait for each other; a segment whose solve has ended leaves the launch at once. Every
+// wait is bounded: after spinCap polls, or once the abort flag is set (by the host or by another workgroup that
+// gave up), the workgroup sets the abort flag and leaves, writing nothing; the host then re-runs the solve with
+// kPcgStep launches.
+namespace {
+constexpr uint32_t kBarrierGroup = 16u;
+constexpr int kScDevice = 16;  // cache-policy operand of the buffer builtins: sc1 (device scope)
+typedef uint32_t U3 __attribute__((ext_vector_type(3)));
+
+__device__ inline __amdgpu_buffer_rsrc_t bufferOf(const void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, 0x7fffffff, 0x00020000);
+}
+__device__ inline F3 loadDevice(__amdgpu_buffer_rsrc_t r, uint32_t row) {
+  const U3 u = __builtin_amdgcn_raw_buffer_load_b96(r, row * 12u, 0, kScDevice);
+  return f3(__uint_as_float(u.x), __uint_as_float(u.y), __uint_as_float(u.z));
+}
+__device__ inline void storeDevice(__amdgpu_buffer_rsrc_t r, uint32_t row, F3 x) {
+  U3 u;
+  u.x = __float_as_uint(x.x);
+  u.y = __float_as_uint(x.y);
+  u.z = __float_as_uint(x.z);
+  __builtin_amdgcn_raw_buffer_store_b96(u, r, row * 12u, 0, kScDevice);
+}
+__device__ inline double loadDevice(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void storeDevice(double* p, double x) {
+  __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint32_t loadDevice(uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint32_t addDevice(uint32_t* p) {
+  return __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Arrival of a tile at the segment barrier of iteration `it` (0-based) and the wait for the segment's other tiles.
+// Called by every thread of the workgroup; returns false if the workgroup gave up (abort flag set).
+__device__ inline bool segmentBarrier(const SolveCtl& c, uint32_t s, uint32_t localTile, uint32_t nTiles, uint32_t it,
+                                      uint32_t* quit) {
+  __builtin_amdgcn_s_waitcnt(0);  // this wavefront's device-scope stores are acknowledged
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t* base = c.ctr + 64u * c.segCtr[s];
+    const uint32_t g = localTile / kBarrierGroup, nGroups = (nTiles + kBarrierGroup - 1u) / kBarrierGroup;
+    const uint32_t gsz = min(kBarrierGroup, nTiles - g * kBarrierGroup);
+    const uint32_t want = it + 1u;
+    if (addDevice(base + 128u + 64u * g) == want * gsz - 1u && addDevice(base) == want * nGroups - 1u)
+      __hip_atomic_store(base + 64u, want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t spins = 0;
+    uint32_t gaveUp = 0;
+    while (loadDevice(base + 64u) < want) {
+      if (++spins > c.spinCap || ((spins & 63u) == 0u && loadDevice(c.abortFlag))) {
+        gaveUp = 1u;
+        __hip_atomic_store(c.abortFlag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    *quit = gaveUp;
+  }
+  __syncthreads();
+  return *quit == 0u;
+}
+}  // namespace
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) kPcgSolve(View v, SolveCtl c) {
+  __shared__ double red[kPcgParts * 4], red2[kPcgParts * 4];
+  __shared__ F3 hz[kHaloRows + 1], ht[kHaloRows + 1], hp[kHaloRows + 1];  // entry kHaloRows: the dummy column
+  __shared__ uint32_t quit;
+  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  if (tile >= v.nTiles) return;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t slice = tile * kSlicesPerTile + wave;
+  const uint32_t row = slice * kSlice + lane;
+  if (v.tileDone[0][tile]) {  // the segment's solve has ended (or its Newton loop): pass the flag on
+    if (threadIdx.x == 0) v.tileDone[1][tile] = 1u;
+    return;
+  }
+  const uint32_t s = v.tileSeg[tile];
+  const uint32_t tileBegin = v.tileRange[2u * tile], tileEnd = v.tileRange[2u * tile + 1u];
+  SegState* S = v.state + s;
+  const SegParams* P = v.seg + s;
+  const uint32_t maxCg = P->maxCg;
+  const double cgTolerance = P->cgTolerance;
+  const uint32_t k0 = S->kIter[0];
+  const uint32_t restarted = S->restartActive;
+  const uint32_t kBase = S->kBase;
+  double r0 = S->r0;
+  const uint32_t hb = v.tileHalo[2u * tile], hc = v.tileHalo[2u * tile + 1u];
+
+  // The slice's matrix, as kPcgStep<1, 8, true> loads it in its first round trip; columns become window offsets
+  // (the dummy column: the zero entry at kHaloRows).
+  const uint32_t K = v.sellK[slice], base = v.sellBase[slice], KL = v.lowK[slice], lbase = v.lowBase[slice];
+  uint32_t lc[8], uc[8];
+  float bt[8][9], bv[8][9];
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j) {
+    const uint32_t cl = j < KL ? v.lowCol[lbase + j * kSlice + lane] : v.nRows;
+    const uint32_t bl = j < KL ? v.lowBlk[lbase + j * kSlice + lane] : v.zeroBlock;
+    const uint32_t cu = j < K ? v.sellCol[base / 9u + j * kSlice + lane] : v.nRows;
+    lc[j] = cl == v.nRows ? kHaloRows : cl - hb;
+    uc[j] = cu == v.nRows ? kHaloRows : cu - hb;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      bt[j][e] = v.sellVal[size_t(bl) * 9u + e];
+      bv[j][e] = v.sellVal[base + j * 9u * kSlice + 9u * lane + e];
+    }
+  }
+  // The rows' PCG state of iteration -1 and their private data.
+  F3 oR = v.cgR[1][row], oAp = v.cgAp[1][row], oZ = v.cgZr[1][row], oT = v.cgT[1][row], oP = v.cgP[1][row];
+  F3 oStep = v.step[row];
+  float m[kPrecondFloats];
+#pragma unroll
+  for (uint32_t e = 0; e < kPrecondFloats; ++e) m[e] = v.precond[kPrecondFloats * row + e];
+  const uint32_t oFlags = v.rowFlags[row], oCf = v.cflags[row];
+  const float4 oCn = v.cnormal[row];
+  if (threadIdx.x == 0) hz[kHaloRows] = ht[kHaloRows] = hp[kHaloRows] = f3(0, 0, 0);
+
+  const __amdgpu_buffer_rsrc_t zrBuf[2] = {bufferOf(v.cgZr[0]), bufferOf(v.cgZr[1])};
+  const __amdgpu_buffer_rsrc_t tBuf[2] = {bufferOf(v.cgT[0]), bufferOf(v.cgT[1])};
+  const __amdgpu_buffer_rsrc_t pBuf[2] = {bufferOf(v.cgP[0]), bufferOf(v.cgP[1])};
+  constexpr uint32_t HR = kHaloRows / 256u;
+  uint32_t updates = 0, conv = 0, k = k0, lastQ = 0;
+  double alphaLast = 0.0, rrLast = 0.0;
+  bool wroteR0 = false;
+  for (uint32_t it = 0;; ++it) {
+    const uint32_t q = it & 1u, o = q ^ 1u;
+    k = k0 + it;
+    lastQ = q;
+    // The window (zr, t, p) and the segment's partial sums of the previous iteration, issued together.
+    F3 wz[HR], wt[HR], wp[HR];
+#pragma unroll
+    for (uint32_t r = 0; r < HR; ++r) {
+      const uint32_t i = hb + min(r * 256u + threadIdx.x, hc - 1u);
+      wz[r] = loadDevice(o ? zrBuf[1] : zrBuf[0], i);
+      wt[r] = loadDevice(o ? tBuf[1] : tBuf[0], i);
+      wp[r] = loadDevice(o ? pBuf[1] : pBuf[0], i);
+    }
+    const double* prev = v.cgPart[o];
+    double sum[kPcgParts];
+    uint32_t ti = tileBegin + threadIdx.x;
+#pragma unroll
+    for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < tileEnd ? loadDevice(prev + kPcgParts * ti + n) : 0.0;
+    for (ti += 256u; ti < tileEnd; ti += 256u)
+#pragma unroll
+      for (int n = 0; n < kPcgParts; ++n) sum[n] += loadDevice(prev + kPcgParts * ti + n);
+    blockSumN<kPcgParts, 4, true>(sum, red);
+#pragma unroll
+    for (uint32_t r = 0; r < HR; ++r) {
+      const uint32_t i = r * 256u + threadIdx.x;
+      if (i < hc) {
+        hz[i] = wz[r];
+        ht[i] = wt[r];
+        hp[i] = wp[r];
+      }
+    }
+    __syncthreads();
+
+    // The decision of kPcgStep, from the same sums.
+    const uint32_t kDone = k + kBase;
+    bool update = false;
+    double alpha = 0.0, beta = 0.0, rrNext = sum[kPrR];
+    if (k == 0) {
+      r0 = restarted ? r0 : sqrt(sum[kPrR]);
+      wroteR0 = true;
+      if (!(sum[kPrR] > 0.0))
+        conv = restarted ? kPcgExitTolerance : kPcgExitZero;
+      else if (kDone >= maxCg)
+        conv = kPcgExitCap;
+    } else if (!(sum[kPpAp] > 0.0)) {
+      conv = kPcgExitCurvature;
+    } else {
+      update = true;
+      alpha = sum[kPrZ] / sum[kPpAp];
+      rrNext = sum[kPrR] - 2.0 * alpha * sum[kPrAp] + alpha * alpha * sum[kPapAp];
+      const double rzNext = sum[kPrZ] - 2.0 * alpha * sum[kPrT] + alpha * alpha * sum[kPapT];
+      if (sqrt(fmax(rrNext, 0.0)) <= cgTolerance * r0)
+        conv = kPcgExitTolerance;
+      else if (kDone >= maxCg)
+        conv = kPcgExitCap;
+      else
+        beta = rzNext / sum[kPrZ];
+    }
+    if (update) {
+      ++updates;
+      alphaLast = alpha;
+      rrLast = rrNext;
+    }
+    const float a = float(alpha), b = float(beta);
+    if (update) oStep = oStep + a * oP;
+    if (conv) break;
+
+    // Row sums of A p_k in kPcgStep's order: four groups (slot mod 4), each its lower then its upper slots.
+    float acc[4][3];
+#pragma unroll
+    for (uint32_t g = 0; g < 4; ++g) acc[g][0] = acc[g][1] = acc[g][2] = 0.0f;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j)
+      blockTMulAdd(bt[j], pcgDirection(hz[lc[j]], ht[lc[j]], hp[lc[j]], a, b), acc[j % 4][0], acc[j % 4][1],
+                   acc[j % 4][2]);
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j)
+      blockMulAdd(bv[j], pcgDirection(hz[uc[j]], ht[uc[j]], hp[uc[j]], a, b), acc[j % 4][0], acc[j % 4][1],
+                  acc[j % 4][2]);
+    const float a0 = ((acc[0][0] + acc[1][0]) + acc[2][0]) + acc[3][0];
+    const float a1 = ((acc[0][1] + acc[1][1]) + acc[2][1]) + acc[3][1];
+    const float a2 = ((acc[0][2] + acc[1][2]) + acc[2][2]) + acc[3][2];
+
+    const F3 rk = oR - a * oAp;
+    const F3 pk = pcgDirection(oZ, oT, oP, a, b);
+    const F3 apk = (oFlags & kRowPinned) ? f3(0, 0, 0) : onPlane(oCf, oCn, f3(a0, a1, a2));
+    const F3 zr = applyPrecond(m, oCf, oCn, rk);
+    const F3 tk = applyPrecond(m, oCf, oCn, apk);
+    storeDevice(q ? zrBuf[1] : zrBuf[0], row, zr);
+    storeDevice(q ? tBuf[1] : tBuf[0], row, tk);
+    storeDevice(q ? pBuf[1] : pBuf[0], row, pk);
+    double d[kPcgParts] = {dot64(pk, apk), dot64(rk, zr), dot64(rk, tk), dot64(apk, tk),
+                           dot64(rk, apk), dot64(apk, apk), dot64(rk, rk)};
+    blockSumN<kPcgParts, 4, true>(d, red2);
+    if (threadIdx.x < uint32_t(kPcgParts)) {
+      double x = d[0];
+#pragma unroll
+      for (int n = 1; n < kPcgParts; ++n) x = threadIdx.x == uint32_t(n) ? d[n] : x;
+      storeDevice(v.cgPart[q] + kPcgParts * tile + threadIdx.x, x);
+    }
+    oR = rk;
+    oAp = apk;
+    oZ = zr;
+    oT = tk;
+    oP = pk;
+    if (!segmentBarrier(c, s, tile - tileBegin, tileEnd - tileBegin, it, &quit)) return;
+  }
+
+  // The solve has ended in every workgroup of the segment at the same iteration: the step, the per-tile flags,
+  // and (first tile) the segment's scalars as the sequence of kPcgStep launches leaves them.
+  v.step[row] = oStep;
+  if (threadIdx.x == 0) {
+    v.tileDone[0][tile] = 1u;
+    v.tileDone[1][tile] = 1u;
+  }
+  if (tile == tileBegin && threadIdx.x == 0) {
+    if (wroteR0) S->r0 = r0;
+    S->kIter[lastQ ^ 1u] = k + 1u;
+    if (k > k0) S->kIter[lastQ] = k;
+    if (updates) {
+      S->alpha = alphaLast;
+      S->rr = rrLast;
+      S->pcgIter += updates;
+      S->cntPcg += updates;
+    }
+    S->pcgConverged = 1u;
+    S->pcgExit = conv;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // FP64 true-residual refinement of the FP32 PCG (mixed-precision iterative refinement). The FP32 recursion for r
 // drifts from the true residual b - A step (b = S(-grad)) as the iteration proceeds, so a solve the recursion calls
 // converged can be short of |b - A step| <= tol |r0| (the reference's FP64 solve meets its test on a residual that
@@ -2085,6 +2363,24 @@ void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t s
     hipExtLaunchKernelGGL((kPcgStep<1, 4, true>), grid, dim3(256), 0, st, start, stop, 0, v, q);
   else
     throw std::invalid_argument("PCG launch shape must be 1x8, 1x4, 2x8 or 4x8, or 1x8 / 1x4 with the halo");
+}
+uint32_t pcgSolveCapacity() {
+  int device = 0, perCu = 0;
+  hipDeviceProp_t prop{};
+  if (hipGetDevice(&device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) return 0u;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, reinterpret_cast<const void*>(kPcgSolve), 256, 0) !=
+      hipSuccess)
+    return 0u;
+  // One workgroup per CU: a tile's four wavefronts take one SIMD each (the kernel is built for one wavefront per SIMD).
+  return perCu > 0 ? uint32_t(prop.multiProcessorCount) : 0u;
+}
+hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, hipStream_t st) {
+  launchFill(c.ctr, 0u, uint64_t(counterLines) * 64u + 1u, st);
+  View vv = v;
+  SolveCtl cc = c;
+  void* args[] = {&vv, &cc};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kPcgSolve), dim3(xcdGrid(v.nTiles)), dim3(256), args,
+                                    0, st);
 }
 void launchPcgRefine(const View& v, uint32_t maxRestarts, hipStream_t st) {
   kTrueResidual<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);

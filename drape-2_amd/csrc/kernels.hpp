@@ -139,17 +139,41 @@ constexpr uint32_t kStatsPerSegment = 6;
 void launchStats(const View& v, const double* massF64, double* tilePart, double* segOut, hipStream_t st);
 // Launch shape of the one-kernel PCG iteration: `waves` wavefronts share each 64-row slice, each batch covers
 // `slots` lower and `slots` upper slots, and `halo` stages the tile's column window in LDS. Supported: 1x8, 1x4,
-// 2x8, 4x8, and 1x8 / 1x4 with the halo. Every shape gives bit-identical results.
+// 2x8, 4x8, and 1x8 / 1x4 with the halo; the last entry is the persistent solve (launchPcgSolve), which iterates
+// as 1x8 with the halo. Every shape gives bit-identical results.
 struct PcgShape {
   uint32_t waves = 1, slots = 8;
-  bool halo = false;  // gathers from an LDS window of the tile's columns (every tile's window <= kHaloRows rows)
-  bool operator==(const PcgShape& o) const { return waves == o.waves && slots == o.slots && halo == o.halo; }
+  bool halo = false;        // gathers from an LDS window of the tile's columns (every tile's window <= kHaloRows rows)
+  bool persistent = false;  // one cooperative launch per solve (kPcgSolve) instead of one launch per iteration
+  bool operator==(const PcgShape& o) const {
+    return waves == o.waves && slots == o.slots && halo == o.halo && persistent == o.persistent;
+  }
 };
-constexpr PcgShape kPcgShapes[6] = {{1, 8, false}, {1, 4, false}, {2, 8, false},
-                                    {4, 8, false}, {1, 8, true},  {1, 4, true}};
+constexpr int kPcgShapeCount = 7;
+constexpr int kPcgPersistentShape = 6;
+constexpr PcgShape kPcgShapes[kPcgShapeCount] = {{1, 8, false, false}, {1, 4, false, false}, {2, 8, false, false},
+                                                 {4, 8, false, false}, {1, 8, true, false},  {1, 4, true, false},
+                                                 {1, 8, true, true}};
 // One PCG iteration of the one-reduction form.
 void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t st, hipEvent_t start = nullptr,
                    hipEvent_t stop = nullptr);
+// Persistent solve (kPcgSolve): one cooperative launch runs the one-kernel PCG until every segment's solve has
+// ended, with the matrix and the rows' private data held in registers and a barrier per segment; bit-identical to
+// launchPcgStep with the 1x8 halo shape. It needs the halo window (View::tileHalo), at most 8 + 8 slots per slice and
+// one workgroup per tile resident at once (pcgSolveCapacity).
+struct SolveCtl {
+  uint32_t* ctr;            // per segment 2 + ceil(tiles / 16) counters, 64 words apart: top, generation, groups
+  const uint32_t* segCtr;   // per segment: its first counter, in units of 64 words
+  uint32_t* abortFlag;      // set by a workgroup that gave up waiting (or by the host): every workgroup leaves
+  uint32_t spinCap;         // polls of a barrier's generation counter before a workgroup gives up
+};
+// Counter lines a segment of `tiles` tiles needs.
+inline uint32_t solveCounterLines(uint32_t tiles) { return 2u + (tiles + 15u) / 16u; }
+// Workgroups of kPcgSolve the device holds at once (0 if the kernel cannot be resident).
+uint32_t pcgSolveCapacity();
+// Clears the counters and the abort flag (counterLines * 64 + 1 words at c.ctr, the flag last) and launches the
+// solve cooperatively. Returns the launch's error code (a grid beyond the capacity fails without running).
+hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, hipStream_t st);
 // FP64 true residual of the finished solve and, per segment, a continuation of the solve from it when it is above
 // tol r0 (at most maxRestarts per solve; see kTrueResidual).
 void launchPcgRefine(const View& v, uint32_t maxRestarts, hipStream_t st);

@@ -90,6 +90,7 @@ HipSolver::~HipSolver() {
     (void)hipEventDestroy(pr.second);
   }
   if (hostStates_) (void)hipHostFree(hostStates_);
+  if (solveAbortHost_) (void)hipHostFree(solveAbortHost_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -458,6 +459,25 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   v.lowCol = lowCol_.as<uint32_t>();
   v.lowBlk = lowBlk_.as<uint32_t>();
   v.fieldValues = fieldValues_.as<float>();
+  // Persistent solve: a banded matrix (the halo window), at most 8 + 8 slots per slice, and every tile's workgroup
+  // resident at once; counters per segment for its barrier.
+  {
+    std::vector<uint32_t> segCtr(std::max<size_t>(M.segments.size(), 1), 0u);
+    uint32_t lines = 0;
+    for (size_t sg = 0; sg < M.segments.size(); ++sg) {
+      segCtr[sg] = lines;
+      lines += dk::solveCounterLines(params[sg].tileEnd - params[sg].tileBegin);
+    }
+    solveCtrLines_ = lines;
+    solveCtr_.allocate(sizeof(uint32_t) * (size_t(lines) * 64u + 1u));
+    upload(solveSegCtr_, segCtr);
+    if (!solveAbortHost_)
+      check(hipHostMalloc(reinterpret_cast<void**>(&solveAbortHost_), sizeof(uint32_t), hipHostMallocDefault),
+            "hipHostMalloc");
+    *solveAbortHost_ = 0u;
+    const uint32_t grid = 8u * ((nTiles_ + 7u) / 8u);
+    persistentOk_ = haloFits && maxK_ <= 8u && maxLowK_ <= 8u && grid <= dk::pcgSolveCapacity();
+  }
   built_ = true;
   syncStateToDevice();
   check(hipStreamSynchronize(stream_), "build");
@@ -675,26 +695,29 @@ void HipSolver::setPcgVariant(int variant) {
   pcgVariant_ = variant;
 }
 
-void HipSolver::setPcgShape(int waves, int slots, bool halo) {
-  if (waves == 0 && slots == 0 && !halo) {
+void HipSolver::setPcgShape(int waves, int slots, int halo) {
+  if (waves == 0 && slots == 0 && halo == 0) {
     pcgShape_ = -1;
     return;
   }
-  for (int i = 0; i < kShapeCount; ++i)
-    if (int(dk::kPcgShapes[i].waves) == waves && int(dk::kPcgShapes[i].slots) == slots &&
-        dk::kPcgShapes[i].halo == halo) {
+  for (int i = 0; i < kShapeCount; ++i) {
+    const dk::PcgShape& sh = dk::kPcgShapes[i];
+    const int code = sh.persistent ? 2 : sh.halo ? 1 : 0;
+    if (int(sh.waves) == waves && int(sh.slots) == slots && code == halo) {
       pcgShape_ = i;
       return;
     }
+  }
   throw std::invalid_argument(
-      "PCG launch shape must be 0x0 (tune), 1x8, 1x4, 2x8 or 4x8 (wavefronts x slots), or 1x8 / 1x4 with the halo");
+      "PCG launch shape must be 0x0 (tune), 1x8, 1x4, 2x8 or 4x8 (wavefronts x slots), 1x8 / 1x4 with the halo, or the "
+      "persistent 1x8 solve (halo 2)");
 }
 
-void HipSolver::pcgShape(int& waves, int& slots, bool& halo) const {
+void HipSolver::pcgShape(int& waves, int& slots, int& halo) const {
   const int i = shapeIndex();
   waves = i < 0 ? 0 : int(dk::kPcgShapes[i].waves);
   slots = i < 0 ? 0 : int(dk::kPcgShapes[i].slots);
-  halo = i >= 0 && dk::kPcgShapes[i].halo;
+  halo = i < 0 ? 0 : dk::kPcgShapes[i].persistent ? 2 : dk::kPcgShapes[i].halo ? 1 : 0;
 }
 
 void HipSolver::runPcg() {
@@ -771,6 +794,36 @@ void HipSolver::runPcgTuning(uint32_t maxCg) {
       if (shapeAvailable(i) && tuneLaunches_[i] < tuneLaunches_[o]) o = i;
     const int si = tunedShape_ >= 0 ? tunedShape_ : o;
     const dk::PcgShape shape = dk::kPcgShapes[si];
+    if (shape.persistent) {
+      // The persistent candidate runs the rest of the solve in one launch and counts as a sample of that many
+      // iterations when it ran at least a chunk's worth.
+      pollStates();
+      uint32_t before = 0;
+      for (size_t g = 0; g < model_.segments.size(); ++g) before = std::max(before, hostStates_[g].pcgIter);
+      const auto pr = takeEvents();
+      check(hipEventRecord(pr.first, stream_), "event");
+      const bool ok = runPcgPersistent(false, false);
+      check(hipEventRecord(pr.second, stream_), "event");
+      if (!ok) {
+        eventPool_.push_back(pr);
+        redoSolveWithLaunches(maxCg);
+        return;
+      }
+      pollStates();
+      uint32_t after = 0;
+      for (size_t g = 0; g < model_.segments.size(); ++g) after = std::max(after, hostStates_[g].pcgIter);
+      if (tunedShape_ < 0 && after - before >= kTuneChunk) {
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, pr.first, pr.second), "event time");
+        tuneMs_[si] += ms;
+        tuneLaunches_[si] += after - before;
+        bool sampled = true;
+        for (int i = 0; i < kShapeCount; ++i) sampled = sampled && (!shapeAvailable(i) || tuneLaunches_[i] >= 4u * kTuneChunk);
+        if (sampled) tunedShape_ = fastestSampledShape();
+      }
+      eventPool_.push_back(pr);
+      break;
+    }
     const auto pr = takeEvents();
     check(hipEventRecord(pr.first, stream_), "event");
     // Each shape's chunk is a captured graph, as the solves after tuning replay them: the events then time the
@@ -798,7 +851,12 @@ void HipSolver::runPcgTuning(uint32_t maxCg) {
     if (done) break;
   }
   if (tunedShape_ < 0 && ++tuneSolves_ >= kTuneSolves) tunedShape_ = fastestSampledShape();
-  const dk::PcgShape shape = dk::kPcgShapes[tunedShape_ >= 0 ? tunedShape_ : 0];
+  const int refineIdx = tunedShape_ >= 0 ? tunedShape_ : 0;
+  if (dk::kPcgShapes[refineIdx].persistent && shapeAvailable(refineIdx)) {
+    if (!runPcgPersistent(true, false)) redoSolveWithLaunches(maxCg);
+    return;
+  }
+  const dk::PcgShape shape = dk::kPcgShapes[dk::kPcgShapes[refineIdx].persistent ? 0 : refineIdx];
   for (uint32_t r = 0; r < refineRestarts_; ++r) {
     dk::launchPcgRefine(view_, refineRestarts_, stream_);
     pollStates();
@@ -824,18 +882,103 @@ int HipSolver::fastestSampledShape() const {
   return best < 0 ? 0 : best;
 }
 
+// The rest of the current solve with the persistent kernel: one cooperative launch runs every segment's solve to
+// its end, then (when `refine`) each FP64 true-residual refinement round is a decision and, for the segments that
+// continue, one more launch. With `timeIt` each launch is bracketed by events and counted in the iteration timing
+// by the iterations it ran. A launch that fails, or whose barrier gave up (abort flag), switches the persistent form
+// off for this solver and returns false: the caller then re-runs the solve.
+bool HipSolver::runPcgPersistent(bool refine, bool timeIt) {
+  dk::SolveCtl c{};
+  c.ctr = solveCtr_.as<uint32_t>();
+  c.segCtr = solveSegCtr_.as<uint32_t>();
+  c.abortFlag = c.ctr + size_t(solveCtrLines_) * 64u;
+  c.spinCap = solveSpinCap_;
+  const auto fail = [&] {
+    persistentOk_ = false;
+    ++persistentAborts_;
+    if (pcgShape_ >= 0 && dk::kPcgShapes[pcgShape_].persistent) pcgShape_ = -1;
+    if (tunedShape_ >= 0 && dk::kPcgShapes[tunedShape_].persistent) tunedShape_ = fastestSampledShape();
+    return false;
+  };
+  // Iterations done before each launch (the first launch follows kPcgStepInit when timed: none).
+  uint32_t before = 0;
+  const auto launch = [&]() -> bool {
+    std::pair<hipEvent_t, hipEvent_t> pr{};
+    if (timeIt) {
+      pr = takeEvents();
+      check(hipEventRecord(pr.first, stream_), "event");
+    }
+    const hipError_t e = dk::launchPcgSolve(view_, c, solveCtrLines_, stream_);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      if (timeIt) eventPool_.push_back(pr);
+      return false;
+    }
+    if (timeIt) check(hipEventRecord(pr.second, stream_), "event");
+    check(hipMemcpyAsync(solveAbortHost_, c.abortFlag, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "copy");
+    pollStates();
+    if (*solveAbortHost_) {
+      if (timeIt) eventPool_.push_back(pr);
+      return false;
+    }
+    if (timeIt) {
+      uint32_t after = 0;
+      for (size_t g = 0; g < model_.segments.size(); ++g) after = std::max(after, hostStates_[g].pcgIter);
+      float ms = 0.0f;
+      check(hipEventElapsedTime(&ms, pr.first, pr.second), "event");
+      if (after > before) {
+        spmvTotalMs_ += ms;
+        spmvTimed_ += after - before;
+        spmvTotalBytes_ += double(after - before) * activeLaunchBytes(false);
+      }
+      before = after;
+      eventPool_.push_back(pr);
+    }
+    return true;
+  };
+  if (!launch()) return fail();
+  if (refine)
+    for (uint32_t r = 0; r < refineRestarts_; ++r) {
+      dk::launchPcgRefine(view_, refineRestarts_, stream_);
+      if (!launch()) return fail();
+      bool continued = false;
+      for (size_t g = 0; g < model_.segments.size(); ++g) continued = continued || hostStates_[g].restartActive != 0u;
+      if (!continued) break;
+    }
+  uint32_t total = 0;
+  for (size_t g = 0; g < model_.segments.size(); ++g) total = std::max(total, hostStates_[g].pcgIter);
+  lastPcgIterations_ = total;
+  return true;
+}
+
+// After the persistent form gave up: the whole solve again from its start, one launch per iteration (the results
+// are those the persistent form would have given).
+void HipSolver::redoSolveWithLaunches(uint32_t maxCg) {
+  (void)maxCg;
+  runPcgSingleKernel();
+}
+
 // Chunks of kGraphChunk one-kernel iterations, replayed from a captured hipGraph (or launched eagerly). The
 // iteration index lives on the device, so one graph serves every chunk; iterations after convergence return at
 // once. The host reads the converged flags after the first batch (sized from the previous solve) and then after
 // every chunk.
 void HipSolver::runPcgSingleKernel() {
   const int shapeIdx = shapeIndex();
-  if (shapeIdx >= 0 && !shapeAvailable(shapeIdx))
+  if (shapeIdx >= 0 && !shapeAvailable(shapeIdx)) {
+    if (dk::kPcgShapes[shapeIdx].persistent)
+      throw std::invalid_argument(
+          "PCG persistent solve: needs a banded matrix (every 256-row tile within 1536 rows), at most 8 + 8 slots per "
+          "64-row slice and one resident workgroup per tile, and must not have given up before");
     throw std::invalid_argument("PCG halo shape: a 256-row tile of this matrix reaches more than 1536 rows");
+  }
   dk::launchPcgStepInit(view_, stream_);
   const uint32_t maxCg = maxCgOf(model_);
   if (shapeIdx < 0) {
     runPcgTuning(maxCg);
+    return;
+  }
+  if (dk::kPcgShapes[shapeIdx].persistent) {
+    if (!runPcgPersistent(true, spmvEvery_ > 0)) redoSolveWithLaunches(maxCg);
     return;
   }
   const dk::PcgShape shape = dk::kPcgShapes[shapeIdx];
@@ -1275,6 +1418,7 @@ drape_pcg_report HipSolver::pcgReport() const {
     sum += st.trueRelSum;
   }
   r.true_rel_mean = r.solves ? sum / double(r.solves) : 0.0;
+  r.persistent_aborts = persistentAborts_;
   return r;
 }
 

@@ -51,9 +51,17 @@ class HipSolver {
   int refinement() const { return static_cast<int>(refineRestarts_); }
   // Wavefronts per 64-row slice in the one-kernel PCG iteration: 1, 2 or 4, or 0 to time all three during the
   // first solves and keep the fastest (results are bit-identical for every choice).
-  // Launch shape of the one-kernel PCG iteration (wavefronts per slice x slots per batch; 0x0 = tune).
-  void setPcgShape(int waves, int slots, bool halo);
-  void pcgShape(int& waves, int& slots, bool& halo) const;  // 0x0 while still tuning
+  // Launch shape of the one-kernel PCG iteration (wavefronts per slice x slots per batch; 0x0 = tune); halo 2 = the
+  // persistent solve (one cooperative launch per solve, 1x8 with the halo).
+  void setPcgShape(int waves, int slots, int halo);
+  void pcgShape(int& waves, int& slots, int& halo) const;  // 0x0 while still tuning
+  // Polls of a persistent solve's barrier before a workgroup gives up (the solve is then re-run with one launch per
+  // iteration, and the persistent form is not used again by this solver).
+  void setSolveSpinCap(uint32_t cap) { solveSpinCap_ = cap; }
+  // The persistent solve fits this build (banded matrix, at most 8 + 8 slots per slice, one workgroup per tile
+  // resident) and has not given up.
+  bool persistentAvailable() const { return persistentOk_; }
+  uint64_t persistentAborts() const { return persistentAborts_; }
   // Every `every`-th PCG SpMV launch of the step is bracketed by HIP events (0 = off).
   void setSpmvSampling(int every);
   void spmvTiming(uint64_t& launches, double& meanUs, double& bytesPerLaunch) const;
@@ -104,6 +112,10 @@ class HipSolver {
   int shapeIndex() const { return pcgShape_ >= 0 ? pcgShape_ : tunedShape_; }
   std::pair<hipEvent_t, hipEvent_t> takeEvents();
   void runPcgTuning(uint32_t maxCg);
+  // Runs the rest of the current solve (and its refinement continuations when `refine`) with launchPcgSolve; false
+  // if a barrier gave up or the launch failed (the persistent form is then switched off).
+  bool runPcgPersistent(bool refine, bool timeIt);
+  void redoSolveWithLaunches(uint32_t maxCg);
   void globalStructure(std::vector<uint32_t>& rowStart, std::vector<uint32_t>& colIndex,
                        std::vector<uint32_t>& diagSlot, std::vector<uint32_t>& internalToGlobalSlot) const;
 
@@ -140,12 +152,22 @@ class HipSolver {
   DeviceBuffer cgBuf_[10], cgPart0_, cgPart1_, tileDone_;
   int pcgVariant_ = 1;
   uint32_t refineRestarts_ = 4;
-  static constexpr int kShapeCount = 6;  // dk::kPcgShapes
+  static constexpr int kShapeCount = dk::kPcgShapeCount;
   int pcgShape_ = -1;     // requested index into dk::kPcgShapes (-1 = tune)
   int tunedShape_ = -1;   // chosen by the tuner (-1 until it has measured every shape)
   double tuneMs_[kShapeCount] = {};
   uint32_t tuneLaunches_[kShapeCount] = {};
-  bool shapeAvailable(int i) const { return !dk::kPcgShapes[i].halo || view_.tileHalo != nullptr; }
+  bool shapeAvailable(int i) const {
+    return dk::kPcgShapes[i].persistent ? persistentOk_ : (!dk::kPcgShapes[i].halo || view_.tileHalo != nullptr);
+  }
+  // Persistent solve: counters (per segment dk::solveCounterLines lines of 64 words, then the abort flag), each
+  // segment's first line, the abort flag's pinned host mirror.
+  DeviceBuffer solveCtr_, solveSegCtr_;
+  uint32_t solveCtrLines_ = 0;
+  uint32_t* solveAbortHost_ = nullptr;
+  bool persistentOk_ = false;
+  uint64_t persistentAborts_ = 0;
+  uint32_t solveSpinCap_ = 1u << 20;
   // PCG iterations per captured graph, i.e. between two looks at the convergence flags after a solve's first
   // batch (even: the parity pattern repeats). Measured on the 65k sheet over 8 ... 64: a look at the flags costs
   // more than the launches that find the solve already ended.
@@ -171,7 +193,7 @@ class HipSolver {
   // The first launches of a continuation from the true residual, replayed behind the refinement decision.
   static constexpr uint32_t kRefineChunk = 16;
   Captured refineExec_;
-  Captured tuneExec_[6];  // the tuner's chunk of each launch shape (kShapeCount)
+  Captured tuneExec_[kShapeCount];  // the tuner's chunk of each launch shape
   int refineShape_ = -1;
   static constexpr int kGraphTrials = 2;
   template <class F>

@@ -146,7 +146,8 @@ class Counters(C.Structure):
 class PcgReport(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "solves", "exit_tolerance", "exit_cap", "exit_curvature", "exit_zero", "restarts")] + [
-        (n, C.c_double) for n in ("true_rel_max", "true_rel_mean", "true_rel_last", "true_rel_max_converged")]
+        (n, C.c_double) for n in ("true_rel_max", "true_rel_mean", "true_rel_last", "true_rel_max_converged")] + [
+        ("persistent_aborts", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {n: (int(v) if isinstance(v, int) else float(v)) for n, _ in self._fields_ for v in [getattr(self, n)]}
@@ -321,6 +322,8 @@ def _declare(lib):
         "drape_solver_set_spmv_sampling": (C.c_int, [S, C.c_int32]),
         "drape_solver_set_pcg_variant": (C.c_int, [S, C.c_int32]),
         "drape_solver_set_pcg_shape": (C.c_int, [S, C.c_int32, C.c_int32, C.c_int32]),
+        "drape_solver_set_solve_spin_cap": (C.c_int, [S, C.c_uint32]),
+        "drape_solver_persistent_available": (C.c_int, [S, C.POINTER(C.c_int32)]),
         "drape_solver_pcg_shape": (C.c_int, [S, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                              C.POINTER(C.c_int32)]),
         "drape_solver_spmv_timing": (C.c_int, [S, C.POINTER(C.c_uint64), f64p, f64p]),

@@ -41,6 +41,9 @@ def default_settings() -> dict:
     return out
 
 
+PERSISTENT = (1, 8, 2)  # the persistent solve's shape code
+
+
 class HipSolver:
     """One MI355X solver instance (one segment per scene of a batched build)."""
 
@@ -166,17 +169,29 @@ class HipSolver:
         """1: one kernel per PCG iteration (default); 0: the two-kernel standard form."""
         _check(_lib().drape_solver_set_pcg_variant(self._h, int(variant)))
 
-    def set_pcg_shape(self, waves: int, slots: int, halo: bool = False) -> None:
+    def set_pcg_shape(self, waves: int, slots: int, halo: int = 0) -> None:
         """Launch shape of the one-kernel PCG iteration: (wavefronts per 64-row slice, slots per batch) in
-        (1, 8), (1, 4), (2, 8), (4, 8), or (0, 0) to tune (default); halo=True (with (1, 8) or (1, 4)) gathers from
-        an LDS copy of each tile's column window, for matrices whose every tile spans at most 1,536 rows."""
-        _check(_lib().drape_solver_set_pcg_shape(self._h, int(waves), int(slots), int(bool(halo))))
+        (1, 8), (1, 4), (2, 8), (4, 8), or (0, 0) to tune (default); halo=1 (with (1, 8) or (1, 4)) gathers from
+        an LDS copy of each tile's column window, for matrices whose every tile spans at most 1,536 rows; halo=2
+        (with (1, 8)) is the persistent solve: one cooperative launch per PCG solve (PERSISTENT)."""
+        _check(_lib().drape_solver_set_pcg_shape(self._h, int(waves), int(slots), int(halo)))
 
     def pcg_shape(self) -> tuple:
-        """(waves, slots, halo) of the shape in use; (0, 0, False) while still tuning."""
+        """(waves, slots, halo) of the shape in use, halo 0, 1 or 2 (the persistent solve); (0, 0, 0) while still
+        tuning."""
         w, k, h = C.c_int32(), C.c_int32(), C.c_int32()
         _check(_lib().drape_solver_pcg_shape(self._h, C.byref(w), C.byref(k), C.byref(h)))
-        return (w.value, k.value, bool(h.value))
+        return (w.value, k.value, h.value)
+
+    def set_solve_spin_cap(self, polls: int) -> None:
+        """Polls of a persistent solve's barrier before a workgroup gives up (the solve is then re-run with one
+        launch per iteration and the persistent form switched off)."""
+        _check(_lib().drape_solver_set_solve_spin_cap(self._h, int(polls)))
+
+    def persistent_available(self) -> bool:
+        a = C.c_int32()
+        _check(_lib().drape_solver_persistent_available(self._h, C.byref(a)))
+        return bool(a.value)
 
     def set_spmv_sampling(self, every: int) -> None:
         _check(_lib().drape_solver_set_spmv_sampling(self._h, int(every)))

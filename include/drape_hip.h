@@ -213,6 +213,7 @@ typedef struct drape_pcg_report {
   double true_rel_mean;
   double true_rel_last; /* of the last solve (largest over segments) */
   double true_rel_max_converged; /* largest over the solves that ended on the tolerance test */
+  uint64_t persistent_aborts;    /* persistent solves whose barrier wait gave up and were re-run (halo = 2 below) */
 } drape_pcg_report;
 int drape_solver_pcg_report(const drape_solver* s, drape_pcg_report* out);
 int drape_solver_reset_counters(drape_solver* s);
@@ -235,11 +236,21 @@ int drape_solver_set_refinement(drape_solver* s, int32_t restarts);
  * tiles than CUs), 2x8 and 4x8, or 0x0 (default) to time every shape during the first solves and keep the fastest.
  * halo = 1 (with 1x8 or 1x4) stages each 256-row tile's column window in LDS and gathers from there; it runs only
  * on matrices whose every tile reaches at most 1,536 consecutive rows (a step with it fails otherwise), and the
- * tuner tries it only there. Results are bit-identical for every shape. drape_solver_pcg_shape reports the shape
- * in use (0x0 while still tuning). No reference counterpart: the reference's GPU path has one thread per row
- * (kernels.h:272-289). */
+ * tuner tries it only there. halo = 2 (with 1x8) is the persistent solve: one cooperative launch runs a whole PCG
+ * solve, each 256-row tile's workgroup keeping its slices' matrix and private vectors in registers, the tiles of a
+ * scene meeting at a barrier of their own each iteration (scenes of a batch never wait for each other); it needs
+ * what halo = 1 needs, at most 8 + 8 matrix slots per 64-row slice and one resident workgroup per tile (at most
+ * one tile per CU: 65,536 particle rows on MI355X). If a barrier wait gives up (drape_solver_pcg_report counts
+ * it), the solve is re-run with one launch per iteration and the solver stops using the persistent form. Results
+ * are bit-identical for every shape. drape_solver_pcg_shape reports the shape in use (0x0 while still tuning). No
+ * reference counterpart: the reference's GPU path has one thread per row (kernels.h:272-289). */
 int drape_solver_set_pcg_shape(drape_solver* s, int32_t waves, int32_t slots, int32_t halo);
 int drape_solver_pcg_shape(const drape_solver* s, int32_t* waves, int32_t* slots, int32_t* halo);
+/* Persistent solve: polls of a barrier's generation counter before a workgroup gives up waiting (default 2^20, about
+ * a second); every wait of the kernel is bounded by it. *available (optional) reports whether the persistent solve
+ * fits this build and has not given up. No reference counterpart. */
+int drape_solver_set_solve_spin_cap(drape_solver* s, uint32_t polls);
+int drape_solver_persistent_available(const drape_solver* s, int32_t* available);
 
 /* Kernel timing inside the step: every `every`-th PCG SpMV launch (0 = off) is bracketed by HIP events on the
  * solver's stream. drape_solver_spmv_timing reports how many launches were timed since sampling was set, their

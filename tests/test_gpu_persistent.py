@@ -1,0 +1,108 @@
+"""The persistent PCG solve: one cooperative launch per solve, each tile's workgroup keeping its slices' matrix and
+private vectors in registers, the tiles of each scene meeting at a barrier of their own every iteration.
+
+It computes every iteration as the launch-per-iteration form with the 1x8 halo shape does, so the two must agree bit
+for bit: on the 32x32 sheet, through the 65,536-particle drop's first contact, and segment by segment on a batch of
+uneven sheets (whose scenes never wait for each other). Every wait is bounded: a barrier that gives up sets the
+abort flag, every workgroup leaves, and the solver re-runs that solve with one launch per iteration.
+"""
+import numpy as np
+import pytest
+
+from drape_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+PERSISTENT = (1, 8, 2)
+HALO_LAUNCHES = (1, 8, 1)
+
+
+def _run(hip, sc, shape, steps, spin_cap=None, batch=False):
+    g = hip.HipSolver(pcg_shape=shape)
+    if spin_cap is not None:
+        g.set_solve_spin_cap(spin_cap)
+    if batch:
+        g.build_batch(sc)
+    else:
+        g.build(sc)
+    for _ in range(steps):
+        g.step(1 / 60, 1, 5)
+    return g
+
+
+def _same(a, b):
+    np.testing.assert_array_equal(a.read_positions(), b.read_positions())
+    np.testing.assert_array_equal(a.read_velocities(), b.read_velocities())
+    assert a.counters() == b.counters()
+    ra, rb = a.pcg_report(), b.pcg_report()
+    for key in ("solves", "exit_tolerance", "exit_cap", "exit_curvature", "restarts"):
+        assert ra[key] == rb[key], key
+    assert ra["true_rel_max"] == rb["true_rel_max"]
+
+
+def test_persistent_solve_equals_launches_on_the_small_sheet(hip):
+    sc = scenes.c1_sheet()
+    p = _run(hip, sc, PERSISTENT, 12)
+    assert p.persistent_available() and p.pcg_shape() == PERSISTENT
+    assert p.pcg_report()["persistent_aborts"] == 0
+    launches = _run(hip, sc, HALO_LAUNCHES, 12)
+    print(f"32x32 sheet, 12 steps: counters {p.counters()}", flush=True)
+    _same(p, launches)
+
+
+def test_persistent_solve_equals_launches_through_first_contact_at_65k(hip):
+    """256 tiles, one per CU: the largest single scene the persistent form takes on MI355X. 26 steps reach the
+    sphere (first contact near step 23), where the solves are longest and the refinement continues some of them."""
+    sc = scenes.c2_sheet()
+    p = _run(hip, sc, PERSISTENT, 26)
+    assert p.pcg_report()["persistent_aborts"] == 0
+    launches = _run(hip, sc, HALO_LAUNCHES, 26)
+    print(f"65,536-particle drop, 26 steps: counters {p.counters()}, report {p.pcg_report()}", flush=True)
+    _same(p, launches)
+
+
+def test_persistent_batch_segments_equal_their_launch_runs(hip):
+    """Eight uneven sheets (3,312 to 10,856 particles, 197 tiles) in one persistent launch per solve: each
+    segment's barrier counts only its own tiles, and the batch equals the launch form segment for segment; the
+    smallest and largest segments also equal their single-scene persistent runs."""
+    batch = scenes.c4_batch(count=8, lo=2000, hi=12000, tolerance=1e-6, max_cg=2000)
+    steps = 6
+    p = _run(hip, batch, PERSISTENT, steps, batch=True)
+    assert p.persistent_available() and p.pcg_report()["persistent_aborts"] == 0
+    launches = _run(hip, batch, HALO_LAUNCHES, steps, batch=True)
+    print(f"8-sheet batch, {steps} steps: counters {p.counters()}", flush=True)
+    _same(p, launches)
+    xb = p.read_positions()
+    sizes = [sc.n for sc in batch]
+    for k in (int(np.argmin(sizes)), int(np.argmax(sizes))):
+        off, cnt = p.segment_range(k)
+        single = _run(hip, batch[k], PERSISTENT, steps)
+        np.testing.assert_array_equal(xb[off:off + cnt], single.read_positions())
+
+
+def test_persistent_solve_gives_up_and_the_solve_is_redone_with_launches(hip):
+    """A spin cap of zero makes every barrier wait of a multi-tile scene give up at once (forced from the host; the
+    kernel still exits normally): the solver re-runs the solve with one launch per iteration, switches the
+    persistent form off, and its results are the launch form's."""
+    sc = scenes.c2_sheet(n=96)  # 36 tiles: three barrier groups
+    p = _run(hip, sc, PERSISTENT, 4, spin_cap=0)
+    rep = p.pcg_report()
+    print(f"spin cap 0: report {rep}, shape now {p.pcg_shape()}", flush=True)
+    assert rep["persistent_aborts"] == 1
+    assert not p.persistent_available()
+    assert p.pcg_shape() != PERSISTENT
+    launches = _run(hip, sc, HALO_LAUNCHES, 4)
+    np.testing.assert_array_equal(p.read_positions(), launches.read_positions())
+    assert p.counters()["newton_iterations"] == launches.counters()["newton_iterations"]
+
+
+def test_persistent_solve_is_refused_where_it_does_not_fit(hip):
+    """The seamed tube's tiles reach across the panels (no halo window), and a 288 x 288 sheet has 324 tiles, more
+    than the CUs that hold one each: the persistent form is refused before anything is launched, and a tuned solver
+    never picks it there."""
+    for sc in (scenes.c3_tube(nx=40, ny=60), scenes.c2_sheet(n=288)):
+        g = hip.HipSolver(pcg_shape=PERSISTENT)
+        g.build(sc)
+        assert not g.persistent_available()
+        with pytest.raises(ValueError, match="persistent"):
+            g.step(1 / 60, 1, 5)
