@@ -1233,12 +1233,13 @@ kPcgStep(View v, uint32_t q) {
 // So every value another workgroup reads during the launch is stored with device-scope stores (sc1: written
 // through to the device-coherent level) and loaded with device-scope loads (sc1), and a workgroup announces its
 // stores only after every wavefront has waited for their acknowledgement (s_waitcnt vmcnt(0)). The announcement is
-// a device-scope atomic on the segment's counters, and the waits poll a device-scope generation counter. Nothing
+// a device-scope atomic on the segment's counters, and the waits poll a generation line with device-scope loads. Nothing
 // that changes during the launch is read through the scalar cache.
 //
 // Segment barrier: a two-level tree over the segment's tiles (groups of kBarrierGroup): the last arrival of a
-// group increments the segment's top counter, the last arrival at the top publishes the generation. Only
-// workgroups of one segment wait for each other; a segment whose solve has ended leaves the launch at once. Every
+// group increments the segment's top counter, and the last arrival at the top publishes the generation on one line
+// per group, which that group's tiles poll. Only workgroups of one segment wait for each other; a segment whose
+// solve has ended leaves the launch at once. Every
 // wait is bounded: after spinCap polls, or once the abort flag is set (by the host or by another workgroup that
 // gave up), the workgroup sets the abort flag and leaves, writing nothing; the host then re-runs the solve with
 // kPcgStep launches.
@@ -1261,6 +1262,26 @@ __device__ inline void storeDevice(__amdgpu_buffer_rsrc_t r, uint32_t row, F3 x)
   u.z = __float_as_uint(x.z);
   __builtin_amdgcn_raw_buffer_store_b96(u, r, row * 12u, 0, kScDevice);
 }
+// A tile's kPcgParts FP64 partial sums (56 consecutive bytes) with three 16-byte and one 8-byte device-scope loads.
+typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+typedef uint32_t U2 __attribute__((ext_vector_type(2)));
+__device__ inline double asDouble(uint32_t lo, uint32_t hi) {
+  return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+}
+__device__ inline void loadPartsDevice(__amdgpu_buffer_rsrc_t r, uint32_t tile, double (&x)[7]) {
+  const uint32_t off = tile * 56u;
+  const U4 a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, kScDevice);
+  const U4 b = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u, 0, kScDevice);
+  const U4 c = __builtin_amdgcn_raw_buffer_load_b128(r, off + 32u, 0, kScDevice);
+  const U2 d = __builtin_amdgcn_raw_buffer_load_b64(r, off + 48u, 0, kScDevice);
+  x[0] = asDouble(a.x, a.y);
+  x[1] = asDouble(a.z, a.w);
+  x[2] = asDouble(b.x, b.y);
+  x[3] = asDouble(b.z, b.w);
+  x[4] = asDouble(c.x, c.y);
+  x[5] = asDouble(c.z, c.w);
+  x[6] = asDouble(d.x, d.y);
+}
 __device__ inline double loadDevice(const double* p) {
   return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -1275,7 +1296,10 @@ __device__ inline uint32_t addDevice(uint32_t* p) {
 }
 
 // Arrival of a tile at the segment barrier of iteration `it` (0-based) and the wait for the segment's other tiles.
-// Called by every thread of the workgroup; returns false if the workgroup gave up (abort flag set).
+// Counter lines of a segment with G groups: [0] top, [1 + g] group g's arrivals, [1 + G + g] group g's generation.
+// The last arrival of a group adds one to the top counter; the last arrival at the top publishes the generation on
+// every group's line, and each tile polls its own group's line (at most kBarrierGroup pollers per line). Called by
+// every thread of the workgroup; returns false if the workgroup gave up (abort flag set).
 __device__ inline bool segmentBarrier(const SolveCtl& c, uint32_t s, uint32_t localTile, uint32_t nTiles, uint32_t it,
                                       uint32_t* quit) {
   __builtin_amdgcn_s_waitcnt(0);  // this wavefront's device-scope stores are acknowledged
@@ -1285,11 +1309,12 @@ __device__ inline bool segmentBarrier(const SolveCtl& c, uint32_t s, uint32_t lo
     const uint32_t g = localTile / kBarrierGroup, nGroups = (nTiles + kBarrierGroup - 1u) / kBarrierGroup;
     const uint32_t gsz = min(kBarrierGroup, nTiles - g * kBarrierGroup);
     const uint32_t want = it + 1u;
-    if (addDevice(base + 128u + 64u * g) == want * gsz - 1u && addDevice(base) == want * nGroups - 1u)
-      __hip_atomic_store(base + 64u, want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t spins = 0;
-    uint32_t gaveUp = 0;
-    while (loadDevice(base + 64u) < want) {
+    if (addDevice(base + 64u * (1u + g)) == want * gsz - 1u && addDevice(base) == want * nGroups - 1u)
+      for (uint32_t k = 0; k < nGroups; ++k)
+        __hip_atomic_store(base + 64u * (1u + nGroups + k), want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t* gen = base + 64u * (1u + nGroups + g);
+    uint32_t spins = 0, gaveUp = 0;
+    while (loadDevice(gen) < want) {
       if (++spins > c.spinCap || ((spins & 63u) == 0u && loadDevice(c.abortFlag))) {
         gaveUp = 1u;
         __hip_atomic_store(c.abortFlag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1306,7 +1331,7 @@ __device__ inline bool segmentBarrier(const SolveCtl& c, uint32_t s, uint32_t lo
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) kPcgSolve(View v, SolveCtl c) {
   __shared__ double red[kPcgParts * 4], red2[kPcgParts * 4];
-  __shared__ F3 hz[kHaloRows + 1], ht[kHaloRows + 1], hp[kHaloRows + 1];  // entry kHaloRows: the dummy column
+  __shared__ F3 hpk[kHaloRows + 1];  // p_k of the window's rows; entry kHaloRows: the dummy column
   __shared__ uint32_t quit;
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
@@ -1355,12 +1380,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   for (uint32_t e = 0; e < kPrecondFloats; ++e) m[e] = v.precond[kPrecondFloats * row + e];
   const uint32_t oFlags = v.rowFlags[row], oCf = v.cflags[row];
   const float4 oCn = v.cnormal[row];
-  if (threadIdx.x == 0) hz[kHaloRows] = ht[kHaloRows] = hp[kHaloRows] = f3(0, 0, 0);
+  if (threadIdx.x == 0) hpk[kHaloRows] = f3(0, 0, 0);
 
   const __amdgpu_buffer_rsrc_t zrBuf[2] = {bufferOf(v.cgZr[0]), bufferOf(v.cgZr[1])};
   const __amdgpu_buffer_rsrc_t tBuf[2] = {bufferOf(v.cgT[0]), bufferOf(v.cgT[1])};
   const __amdgpu_buffer_rsrc_t pBuf[2] = {bufferOf(v.cgP[0]), bufferOf(v.cgP[1])};
+  const __amdgpu_buffer_rsrc_t partBuf[2] = {bufferOf(v.cgPart[0]), bufferOf(v.cgPart[1])};
   constexpr uint32_t HR = kHaloRows / 256u;
+  const uint32_t hr = (hc + 255u) / 256u;  // window rows per thread
   uint32_t updates = 0, conv = 0, k = k0, lastQ = 0;
   double alphaLast = 0.0, rrLast = 0.0;
   bool wroteR0 = false;
@@ -1372,30 +1399,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     F3 wz[HR], wt[HR], wp[HR];
 #pragma unroll
     for (uint32_t r = 0; r < HR; ++r) {
-      const uint32_t i = hb + min(r * 256u + threadIdx.x, hc - 1u);
-      wz[r] = loadDevice(o ? zrBuf[1] : zrBuf[0], i);
-      wt[r] = loadDevice(o ? tBuf[1] : tBuf[0], i);
-      wp[r] = loadDevice(o ? pBuf[1] : pBuf[0], i);
-    }
-    const double* prev = v.cgPart[o];
-    double sum[kPcgParts];
-    uint32_t ti = tileBegin + threadIdx.x;
-#pragma unroll
-    for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < tileEnd ? loadDevice(prev + kPcgParts * ti + n) : 0.0;
-    for (ti += 256u; ti < tileEnd; ti += 256u)
-#pragma unroll
-      for (int n = 0; n < kPcgParts; ++n) sum[n] += loadDevice(prev + kPcgParts * ti + n);
-    blockSumN<kPcgParts, 4, true>(sum, red);
-#pragma unroll
-    for (uint32_t r = 0; r < HR; ++r) {
-      const uint32_t i = r * 256u + threadIdx.x;
-      if (i < hc) {
-        hz[i] = wz[r];
-        ht[i] = wt[r];
-        hp[i] = wp[r];
+      if (r < hr) {  // rows past the window's end are not loaded (hr is uniform over the workgroup)
+        const uint32_t i = hb + min(r * 256u + threadIdx.x, hc - 1u);
+        wz[r] = loadDevice(o ? zrBuf[1] : zrBuf[0], i);
+        wt[r] = loadDevice(o ? tBuf[1] : tBuf[0], i);
+        wp[r] = loadDevice(o ? pBuf[1] : pBuf[0], i);
       }
     }
-    __syncthreads();
+    double sum[kPcgParts] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    uint32_t ti = tileBegin + threadIdx.x;
+    if (ti < tileEnd) loadPartsDevice(o ? partBuf[1] : partBuf[0], ti, sum);
+    for (ti += 256u; ti < tileEnd; ti += 256u) {
+      double more[kPcgParts];
+      loadPartsDevice(o ? partBuf[1] : partBuf[0], ti, more);
+#pragma unroll
+      for (int n = 0; n < kPcgParts; ++n) sum[n] += more[n];
+    }
+    blockSumN<kPcgParts, 4, true>(sum, red);
 
     // The decision of kPcgStep, from the same sums.
     const uint32_t kDone = k + kBase;
@@ -1431,18 +1451,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     if (update) oStep = oStep + a * oP;
     if (conv) break;
 
+    // Each window row's search direction once (kPcgStep rebuilds it per gathered column from the same values with
+    // the same expression), to LDS.
+#pragma unroll
+    for (uint32_t r = 0; r < HR; ++r) {
+      const uint32_t i = r * 256u + threadIdx.x;
+      if (r < hr && i < hc) hpk[i] = pcgDirection(wz[r], wt[r], wp[r], a, b);
+    }
+    __syncthreads();
+
     // Row sums of A p_k in kPcgStep's order: four groups (slot mod 4), each its lower then its upper slots.
     float acc[4][3];
 #pragma unroll
     for (uint32_t g = 0; g < 4; ++g) acc[g][0] = acc[g][1] = acc[g][2] = 0.0f;
 #pragma unroll
-    for (uint32_t j = 0; j < 8; ++j)
-      blockTMulAdd(bt[j], pcgDirection(hz[lc[j]], ht[lc[j]], hp[lc[j]], a, b), acc[j % 4][0], acc[j % 4][1],
-                   acc[j % 4][2]);
+    for (uint32_t j = 0; j < 8; ++j) blockTMulAdd(bt[j], hpk[lc[j]], acc[j % 4][0], acc[j % 4][1], acc[j % 4][2]);
 #pragma unroll
-    for (uint32_t j = 0; j < 8; ++j)
-      blockMulAdd(bv[j], pcgDirection(hz[uc[j]], ht[uc[j]], hp[uc[j]], a, b), acc[j % 4][0], acc[j % 4][1],
-                  acc[j % 4][2]);
+    for (uint32_t j = 0; j < 8; ++j) blockMulAdd(bv[j], hpk[uc[j]], acc[j % 4][0], acc[j % 4][1], acc[j % 4][2]);
     const float a0 = ((acc[0][0] + acc[1][0]) + acc[2][0]) + acc[3][0];
     const float a1 = ((acc[0][1] + acc[1][1]) + acc[2][1]) + acc[3][1];
     const float a2 = ((acc[0][2] + acc[1][2]) + acc[2][2]) + acc[3][2];

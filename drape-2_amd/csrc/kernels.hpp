@@ -162,13 +162,13 @@ void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t s
 // launchPcgStep with the 1x8 halo shape. It needs the halo window (View::tileHalo), at most 8 + 8 slots per slice and
 // one workgroup per tile resident at once (pcgSolveCapacity).
 struct SolveCtl {
-  uint32_t* ctr;            // per segment 2 + ceil(tiles / 16) counters, 64 words apart: top, generation, groups
+  uint32_t* ctr;            // per segment 1 + 2 ceil(tiles / 16) lines of 64 words: top, group arrivals, generations
   const uint32_t* segCtr;   // per segment: its first counter, in units of 64 words
   uint32_t* abortFlag;      // set by a workgroup that gave up waiting (or by the host): every workgroup leaves
   uint32_t spinCap;         // polls of a barrier's generation counter before a workgroup gives up
 };
 // Counter lines a segment of `tiles` tiles needs.
-inline uint32_t solveCounterLines(uint32_t tiles) { return 2u + (tiles + 15u) / 16u; }
+inline uint32_t solveCounterLines(uint32_t tiles) { return 1u + 2u * ((tiles + 15u) / 16u); }
 // Workgroups of kPcgSolve the device holds at once (0 if the kernel cannot be resident).
 uint32_t pcgSolveCapacity();
 // Clears the counters and the abort flag (counterLines * 64 + 1 words at c.ctr, the flag last) and launches the
