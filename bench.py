@@ -26,12 +26,21 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "drape-2_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-# HBM bytes per kPcgStep launch on the 256x256 sheet (one wavefront per slice), from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE scaled by a plain copy's readings (x2.0, x1.0): profiles/kPcgStep_sheet65k_kernel_trace.md.
-PCG_STEP_TRAFFIC_BYTES = 35.10e6
-# Mean duration of the kPcgStep launches that ran an iteration on the 256x256 sheet in the rocprofv3 kernel trace of
-# the same command: profiles/kPcgStep_sheet65k_kernel_trace.md.
-KERNEL_TRACE_US = 9.62
+# Per PCG iteration on the 256x256 sheet, from rocprofv3 runs of the same workload: the kernel's mean time in the
+# kernel trace, and its HBM traffic (FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings, x2.0 and x1.0).
+# kPcgStep (one launch per iteration): profiles/kPcgStep_sheet65k_kernel_trace.md (launches that ran an iteration).
+# kPcgSolve (the persistent solve, one launch per solve): profiles/kPcgSolve_sheet65k_kernel_trace.md (kernel time and
+# counter sums over the run divided by its PCG iterations).
+KERNEL_FIGURES = {
+    (1, 8, 0): {"kernel": "kPcgStep (one-kernel PCG iteration, one launch per iteration: update, convergence test, "
+                          "SELL-64 block SpMV with the search direction rebuilt per gathered column, contact filter, "
+                          "FP64 partials)",
+                "trace_us": 9.62, "traffic": 35.10e6},
+    (1, 8, 2): {"kernel": "kPcgSolve (the persistent solve: one launch runs a whole PCG solve of the same one-kernel "
+                          "iteration, the matrix and each row's private vectors held in registers, a barrier per "
+                          "scene per iteration)",
+                "trace_us": 6.377, "traffic": 5.18e6},
+}
 
 
 def cpu_model() -> str:
@@ -151,7 +160,7 @@ def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 4
 def algorithmic_step_bytes(s, counters: dict, steps: int, pcg_launch_bytes: float) -> float:
     """Algorithmic bytes of one implicit step of the timed run: per Newton iteration the assembly (read x start,
     u, each term's constants and the contact state; write the stored blocks and the gradient) and the
-    preconditioner; per PCG iteration one kPcgStep launch; per line-search trial an energy-only evaluation."""
+    preconditioner; per PCG iteration its algorithmic bytes; per line-search trial an energy-only evaluation."""
     z = s.structure_sizes()
     n, S, H, K = z["n_particles"], z["n_stretch"], z["n_hinges"], z["n_seams"]
     upper = 0.5 * (z["n_blocks"] + n)
@@ -381,6 +390,9 @@ def main() -> int:
     pcg_bytes = survey_pcg_bytes(z)
     step_bytes = algorithmic_step_bytes(s, c, args.steps, pcg_bytes)
     per_iter_us = timing["mean_us"]
+    shape_in_use = tuple(s.pcg_shape())
+    persistent = shape_in_use[2] == 2
+    fig = KERNEL_FIGURES.get(shape_in_use) if args.pcg_variant == 1 and args.n == 256 else None
     achieved = pcg_bytes / (per_iter_us * 1e-6) / 1e9 if per_iter_us > 0 else None
     line = {
         "metric": "implicit steps/sec @65k particles; learned-warm-start vs plain at 0.1 mm error",
@@ -417,30 +429,36 @@ def main() -> int:
                 "true_rel_residual_mean": rep["true_rel_mean"], "true_rel_residual_max": rep["true_rel_max"],
                 "true_rel_residual_max_tolerance_exits": rep["true_rel_max_converged"]},
         "roofline": {
-            "kernel": ("kPcgStep (one-kernel PCG iteration: update, convergence test, SELL-64 block SpMV with the "
-                       "search direction rebuilt per gathered column, contact filter, FP64 partials)")
-                      if args.pcg_variant == 1 else
+            "kernel": (fig or KERNEL_FIGURES[(1, 8, 0)])["kernel"] if args.pcg_variant == 1 else
                       "kPcgSpmv (fused p = z + beta p, SELL-64 block SpMV, contact filter, FP64 p.Ap partials)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": (PCG_STEP_TRAFFIC_BYTES if args.pcg_variant == 1 and args.n == 256 and s.pcg_shape() == (1, 8, False)
-                        else None),
+            "traffic": fig["traffic"] if fig else None,
             "bytes_per_launch": pcg_bytes,
             "bytes_definition": "scope-table algorithmic bytes of one PCG iteration: full block SpMV nnzb (36 + 4) "
                                 "+ 4 (n + 1) + 24 n, plus 14 vector passes of 12 n",
             "layout_bytes_per_launch": timing["bytes_per_launch"],
             "per_iteration_us": per_iter_us,
-            "timing": "HIP events around whole replayed 64-launch chunks of the timed steps, counted when the solve "
-                      "was still running after the chunk (every launch did an iteration); the gaps between launches "
-                      "are included",
-            "timed_launches": timing["launches"],
-            "kernel_trace_us": KERNEL_TRACE_US if args.n == 256 else None,
-            "kernel_trace_frac": (pcg_bytes / (KERNEL_TRACE_US * 1e-6) / 1e9 / HBM_PEAK_GBS) if args.n == 256 else None,
+            "timing": ("HIP events around each persistent solve launch of the timed steps, divided by the PCG "
+                       "iterations it ran" if persistent else
+                       "HIP events around whole replayed 64-launch chunks of the timed steps, counted when the solve "
+                       "was still running after the chunk (every launch did an iteration); the gaps between launches "
+                       "are included"),
+            "timed_iterations": timing["launches"],
+            "kernel_trace_us": fig["trace_us"] if fig else None,
+            "kernel_trace_frac": (pcg_bytes / (fig["trace_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS) if fig else None,
         },
     }
+    if persistent:
+        line["roofline"]["on_chip"] = (
+            "the persistent solve keeps the matrix and each row's private vectors in registers for the whole solve, "
+            "so an iteration moves only the gathered window, the rows' new (zr, t, p) and the partial sums: `traffic` "
+            "(HBM bytes per iteration, PMC) is far below the algorithmic bytes, and the iteration's time is set by its "
+            "per-scene barrier and exchange round trips, not by HBM bandwidth; `frac` compares that time with the "
+            "algorithmic bytes of a matrix-streaming iteration at the HBM peak")
     line["step_roofline"] = {
         "bytes_per_step": step_bytes,
         "achieved": step_bytes * value / 1e9,
@@ -448,7 +466,8 @@ def main() -> int:
         "unit": "GB/s",
         "frac": step_bytes * value / 1e9 / HBM_PEAK_GBS,
         "terms": "assembly 24n + 36 S + 24 H + 16 K + 44n + 36 U per Newton iteration, preconditioner 48n per Newton "
-                 "iteration, one kPcgStep launch per PCG iteration, 24n + 36 S + 24 H + 16 K + 32n per line-search "
+                 "iteration, one PCG iteration's scope-table bytes per PCG iteration, 24n + 36 S + 24 H + 16 K + 32n "
+                 "per line-search "
                  "trial (n particles, S stretch, H hinge, K seam terms, U stored blocks)",
     }
     accuracy = {"reference": "FP64 oracle (this project's restatement of CpuSolver), same scene and step settings",

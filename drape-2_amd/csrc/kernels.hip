@@ -747,13 +747,41 @@ __device__ inline double waveTotal(double v) {
   return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
 }
 
+// One level of waveTotal applied to N independent values, each move issued before any add: the N dependency chains
+// interleave instead of running one after another.
+template <int Ctrl, int RowMask, int N>
+__device__ inline void dppAddN(double (&x)[N]) {
+  double t[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) t[i] = dppMove<Ctrl, RowMask>(x[i]);
+#pragma unroll
+  for (int i = 0; i < N; ++i) x[i] += t[i];
+}
+
+// waveTotal of N values at once (the same moves and adds per value, so the same sums).
+template <int N>
+__device__ inline void waveTotalN(double (&x)[N]) {
+  dppAddN<0xB1, 0xf>(x);
+  dppAddN<0x4E, 0xf>(x);
+  dppAddN<0x141, 0xf>(x);
+  dppAddN<0x140, 0xf>(x);
+  dppAddN<0x142, 0xA>(x);
+  dppAddN<0x143, 0xC>(x);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint64_t u = uint64_t(__double_as_longlong(x[i]));
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u)), 63));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u >> 32)), 63));
+    x[i] = __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+  }
+}
+
 // N sums over a workgroup of NW wavefronts (fixed order), returned to every thread. lds >= N * NW doubles.
 // Fresh: `lds` has not been read earlier in this kernel, so the barrier that protects its previous readers is
 // left out.
 template <int N, int NW, bool Fresh = false>
 __device__ inline void blockSumN(double (&x)[N], double* lds) {
-#pragma unroll
-  for (int i = 0; i < N; ++i) x[i] = waveTotal(x[i]);
+  waveTotalN(x);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   if constexpr (!Fresh) __syncthreads();
   if (l == 0)
@@ -2400,12 +2428,12 @@ uint32_t pcgSolveCapacity() {
   return perCu > 0 ? uint32_t(prop.multiProcessorCount) : 0u;
 }
 hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, hipStream_t st) {
+  // An ordinary dispatch (capturable in a graph, and traceable by the profiler, which does not survive cooperative
+  // dispatches): the host has checked that the grid fits the device at once (pcgSolveCapacity), the solver's stream
+  // runs nothing beside it, and a workgroup that found no room would only make its segment's waits give up.
   launchFill(c.ctr, 0u, uint64_t(counterLines) * 64u + 1u, st);
-  View vv = v;
-  SolveCtl cc = c;
-  void* args[] = {&vv, &cc};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kPcgSolve), dim3(xcdGrid(v.nTiles)), dim3(256), args,
-                                    0, st);
+  hipExtLaunchKernelGGL(kPcgSolve, dim3(xcdGrid(v.nTiles)), dim3(256), 0, st, nullptr, nullptr, 0, v, c);
+  return hipGetLastError();
 }
 void launchPcgRefine(const View& v, uint32_t maxRestarts, hipStream_t st) {
   kTrueResidual<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);

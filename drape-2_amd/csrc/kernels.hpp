@@ -172,7 +172,8 @@ inline uint32_t solveCounterLines(uint32_t tiles) { return 1u + 2u * ((tiles + 1
 // Workgroups of kPcgSolve the device holds at once (0 if the kernel cannot be resident).
 uint32_t pcgSolveCapacity();
 // Clears the counters and the abort flag (counterLines * 64 + 1 words at c.ctr, the flag last) and launches the
-// solve cooperatively. Returns the launch's error code (a grid beyond the capacity fails without running).
+// solve (an ordinary dispatch of xcdGrid(nTiles) workgroups; the caller checks the grid against pcgSolveCapacity).
+// Returns the launch's error code.
 hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, hipStream_t st);
 // FP64 true residual of the finished solve and, per segment, a continuation of the solve from it when it is above
 // tol r0 (at most maxRestarts per solve; see kTrueResidual).

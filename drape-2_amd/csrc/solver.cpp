@@ -376,6 +376,7 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   tuneSolves_ = 0;
   std::fill(tuneMs_, tuneMs_ + kShapeCount, 0.0);
   std::fill(tuneLaunches_, tuneLaunches_ + kShapeCount, 0u);
+  std::fill(tuneWarm_, tuneWarm_ + kShapeCount, false);
   sellVal_.allocate(sizeof(float) * (sellFloats_ + dk::kSellSlack));  // zero slack: see kSellSlack
 
   dk::View& v = view_;
@@ -783,9 +784,11 @@ void HipSolver::timedPcgStep(uint32_t parity, dk::PcgShape shape, bool solveStar
 // Solves run before a launch shape is chosen: chunks of kTuneChunk launches rotate over the shapes of
 // dk::kPcgShapes that the matrix admits (the halo shapes need every tile's window to fit; the shape with the fewest
 // samples goes next), each bracketed by HIP events and followed by a poll. A chunk counts only if the solve was
-// still running at its end, so every timed launch did a full iteration. Once every shape has four such chunks the
-// fastest mean is kept; after kTuneSolves solves without that (solves too short to fill a chunk), the fastest shape
-// sampled so far is kept, else 1x8. The shape never changes the results.
+// still running at its end, so every timed launch did a full iteration; each shape's first chunk is not counted (it
+// loads the kernel's code and captures the chunk's graph). The persistent solve takes the rest of a solve as its
+// sample and counts its iterations. Once every shape has four chunks' worth of counted iterations the fastest mean
+// is kept; after kTuneSolves solves without that (solves too short to fill a chunk), the fastest shape sampled so far
+// is kept, else 1x8. The shape never changes the results.
 void HipSolver::runPcgTuning(uint32_t maxCg) {
   uint32_t launched = 0;
   while (launched < maxCg + 1) {
@@ -812,7 +815,9 @@ void HipSolver::runPcgTuning(uint32_t maxCg) {
       pollStates();
       uint32_t after = 0;
       for (size_t g = 0; g < model_.segments.size(); ++g) after = std::max(after, hostStates_[g].pcgIter);
-      if (tunedShape_ < 0 && after - before >= kTuneChunk) {
+      if (tunedShape_ < 0 && after - before >= kTuneChunk && !tuneWarm_[si]) {
+        tuneWarm_[si] = true;  // the first persistent solve loads the kernel's code: not counted
+      } else if (tunedShape_ < 0 && after - before >= kTuneChunk) {
         float ms = 0.0f;
         check(hipEventElapsedTime(&ms, pr.first, pr.second), "event time");
         tuneMs_[si] += ms;
@@ -838,7 +843,9 @@ void HipSolver::runPcgTuning(uint32_t maxCg) {
     launched += kTuneChunk;
     pollStates();
     const bool done = allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; });
-    if (!done && tunedShape_ < 0) {
+    if (!done && tunedShape_ < 0 && !tuneWarm_[o]) {
+      tuneWarm_[o] = true;  // the shape's first chunk loads its kernel and captures its graph: not counted
+    } else if (!done && tunedShape_ < 0) {
       float ms = 0.0f;
       check(hipEventElapsedTime(&ms, pr.first, pr.second), "event time");
       tuneMs_[o] += ms;

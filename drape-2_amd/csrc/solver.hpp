@@ -156,6 +156,7 @@ class HipSolver {
   int pcgShape_ = -1;     // requested index into dk::kPcgShapes (-1 = tune)
   int tunedShape_ = -1;   // chosen by the tuner (-1 until it has measured every shape)
   double tuneMs_[kShapeCount] = {};
+  bool tuneWarm_[kShapeCount] = {};  // the shape's first timed sample was taken (and not counted: it loads code)
   uint32_t tuneLaunches_[kShapeCount] = {};
   bool shapeAvailable(int i) const {
     return dk::kPcgShapes[i].persistent ? persistentOk_ : (!dk::kPcgShapes[i].halo || view_.tileHalo != nullptr);
