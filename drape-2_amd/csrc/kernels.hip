@@ -1245,32 +1245,39 @@ kPcgStep(View v, uint32_t q) {
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Persistent solve (kPcgSolve): one cooperative launch runs the one-reduction PCG from the state kPcgStepInit (or a
-// refinement's kTrueResidual) left in parity slot 1 until every segment's solve has ended. Iteration k computes
-// exactly what the k-th kPcgStep<1, 8, true> launch computes (same expressions, same summation orders), so the two
-// forms give bit-identical results.
+// Persistent solve (kPcgSolve): one launch runs the one-reduction PCG from the state kPcgStepInit (or a refinement's
+// kTrueResidual) left in parity slot 1 until every segment's solve has ended. Iteration k computes exactly what the
+// k-th kPcgStep<1, 8, true> launch computes (same expressions, same summation orders), so the two forms give
+// bit-identical results.
 //
-// One workgroup owns one 256-row tile for the whole solve, one wavefront per 64-row slice. The slice's matrix (upper
-// block values and columns, the lower slots' transposed blocks and columns; every slice has at most 8 + 8 slots)
-// and its rows' private PCG data (r, ap, step, the preconditioner, the row and contact flags) are loaded once and
-// stay in registers. An iteration moves only the (zr, t, p) window the tile gathers from, the tile's own new
-// (zr, t, p) and 7 FP64 partial sums: about 70 B per row against kPcgStep's 500.
+// One workgroup owns one 256-row tile for the whole solve, one wavefront per 64-row slice, one wavefront per SIMD
+// (the kernel's registers allow no second one). The slice's matrix (upper block values and columns, the lower
+// slots' transposed blocks and columns; every slice has at most 8 + 8 slots) and its rows' private PCG data (r, ap,
+// step, the preconditioner, the row and contact flags) are loaded once and stay in registers. An iteration moves only
+// the (zr, t, p) window the tile gathers from, the tile's own new (zr, t, p) and 7 FP64 partial sums: about 80 B per
+// row against kPcgStep's 500.
+//
+// Residency. A segment's tiles wait for each other every iteration, so all of them must be resident together. The
+// host launches either every tile at once (at most one per CU), or, when the tiles outnumber the CUs, a queue:
+// workgroup i takes the tile order[i], each segment's tiles consecutive. Workgroups are dispatched in order, round-
+// robin over the eight XCDs, so a segment of at most 256 tiles has at most 32 on any XCD, and the earliest segment
+// not yet finished has all its tiles resident (everything dispatched before it has left): it finishes and frees its
+// CUs for the segments behind it. No wait relies on this for termination: every wait is bounded (below).
 //
 // Exchange between workgroups. The L2 of each XCD is not coherent with the others, and an agent-scope release
 // fence writes back a whole L2 (buffer_wbl2), which costs microseconds when every workgroup of an XCD issues one.
 // So every value another workgroup reads during the launch is stored with device-scope stores (sc1: written
 // through to the device-coherent level) and loaded with device-scope loads (sc1), and a workgroup announces its
 // stores only after every wavefront has waited for their acknowledgement (s_waitcnt vmcnt(0)). The announcement is
-// a device-scope atomic on the segment's counters, and the waits poll a generation line with device-scope loads. Nothing
-// that changes during the launch is read through the scalar cache.
+// a device-scope atomic on the segment's counters, and the waits poll a generation line with device-scope loads.
+// Nothing that changes during the launch is read through the scalar cache.
 //
 // Segment barrier: a two-level tree over the segment's tiles (groups of kBarrierGroup): the last arrival of a
 // group increments the segment's top counter, and the last arrival at the top publishes the generation on one line
 // per group, which that group's tiles poll. Only workgroups of one segment wait for each other; a segment whose
-// solve has ended leaves the launch at once. Every
-// wait is bounded: after spinCap polls, or once the abort flag is set (by the host or by another workgroup that
-// gave up), the workgroup sets the abort flag and leaves, writing nothing; the host then re-runs the solve with
-// kPcgStep launches.
+// solve has ended leaves the launch at once. Every wait is bounded: after spinCap polls, or once the abort flag is
+// set (by the host or by another workgroup that gave up), the workgroup sets the abort flag and leaves, writing
+// nothing; the host then re-runs the solve with kPcgStep launches.
 namespace {
 constexpr uint32_t kBarrierGroup = 16u;
 constexpr int kScDevice = 16;  // cache-policy operand of the buffer builtins: sc1 (device scope)
@@ -1361,7 +1368,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   __shared__ double red[kPcgParts * 4], red2[kPcgParts * 4];
   __shared__ F3 hpk[kHaloRows + 1];  // p_k of the window's rows; entry kHaloRows: the dummy column
   __shared__ uint32_t quit;
-  const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
+  // Queued launch (more tiles than the device holds at once): workgroup i takes tile order[i]; each segment's tiles
+  // are consecutive in that order, so the XCDs (which receive workgroups round-robin, each in order) hold every
+  // segment's tiles spread over them, and the earliest segment not yet finished always has all its tiles resident.
+  const uint32_t tile = c.order ? (blockIdx.x < v.nTiles ? c.order[blockIdx.x] : v.nTiles) : xcdMap(blockIdx.x, v.nTiles);
   if (tile >= v.nTiles) return;
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   const uint32_t slice = tile * kSlicesPerTile + wave;
@@ -2432,7 +2442,8 @@ hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLine
   // dispatches): the host has checked that the grid fits the device at once (pcgSolveCapacity), the solver's stream
   // runs nothing beside it, and a workgroup that found no room would only make its segment's waits give up.
   launchFill(c.ctr, 0u, uint64_t(counterLines) * 64u + 1u, st);
-  hipExtLaunchKernelGGL(kPcgSolve, dim3(xcdGrid(v.nTiles)), dim3(256), 0, st, nullptr, nullptr, 0, v, c);
+  hipExtLaunchKernelGGL(kPcgSolve, dim3(c.order ? v.nTiles : xcdGrid(v.nTiles)), dim3(256), 0, st, nullptr, nullptr, 0,
+                        v, c);
   return hipGetLastError();
 }
 void launchPcgRefine(const View& v, uint32_t maxRestarts, hipStream_t st) {

@@ -157,15 +157,19 @@ constexpr PcgShape kPcgShapes[kPcgShapeCount] = {{1, 8, false, false}, {1, 4, fa
 // One PCG iteration of the one-reduction form.
 void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t st, hipEvent_t start = nullptr,
                    hipEvent_t stop = nullptr);
-// Persistent solve (kPcgSolve): one cooperative launch runs the one-kernel PCG until every segment's solve has
-// ended, with the matrix and the rows' private data held in registers and a barrier per segment; bit-identical to
-// launchPcgStep with the 1x8 halo shape. It needs the halo window (View::tileHalo), at most 8 + 8 slots per slice and
-// one workgroup per tile resident at once (pcgSolveCapacity).
+// Persistent solve (kPcgSolve): one launch runs the one-kernel PCG until every segment's solve has ended, with the
+// matrix and the rows' private data held in registers and a barrier per segment; bit-identical to launchPcgStep
+// with the 1x8 halo shape. It needs the halo window (View::tileHalo), at most 8 + 8 slots per slice, and every
+// segment's tiles resident at once (pcgSolveCapacity): all tiles of the launch, or, with SolveCtl::order, each
+// segment's (the segments then pass through the resident workgroups in that order).
 struct SolveCtl {
   uint32_t* ctr;            // per segment 1 + 2 ceil(tiles / 16) lines of 64 words: top, group arrivals, generations
   const uint32_t* segCtr;   // per segment: its first counter, in units of 64 words
   uint32_t* abortFlag;      // set by a workgroup that gave up waiting (or by the host): every workgroup leaves
   uint32_t spinCap;         // polls of a barrier's generation counter before a workgroup gives up
+  // Queued launch (null: one launch of resident tiles): the tile of each workgroup, every segment's tiles
+  // consecutive, so that segments run through the device's resident workgroups in this order.
+  const uint32_t* order;
 };
 // Counter lines a segment of `tiles` tiles needs.
 inline uint32_t solveCounterLines(uint32_t tiles) { return 1u + 2u * ((tiles + 15u) / 16u); }

@@ -476,8 +476,20 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
       check(hipHostMalloc(reinterpret_cast<void**>(&solveAbortHost_), sizeof(uint32_t), hipHostMallocDefault),
             "hipHostMalloc");
     *solveAbortHost_ = 0u;
-    const uint32_t grid = 8u * ((nTiles_ + 7u) / 8u);
-    persistentOk_ = haloFits && maxK_ <= 8u && maxLowK_ <= 8u && grid <= dk::pcgSolveCapacity();
+    // Every segment's tiles must be resident together: all tiles in one launch when they fit the device, else a
+    // queue of segments (solveQueued_) in which each segment fits it.
+    const uint32_t grid = 8u * ((nTiles_ + 7u) / 8u), capacity = dk::pcgSolveCapacity();
+    segTiles_.assign(M.segments.size(), {0u, 0u});
+    uint32_t largest = 0;
+    for (size_t sg = 0; sg < M.segments.size(); ++sg) {
+      segTiles_[sg] = {params[sg].tileBegin, params[sg].tileEnd};
+      largest = std::max(largest, params[sg].tileEnd - params[sg].tileBegin);
+    }
+    solveQueued_ = grid > capacity;
+    persistentOk_ = haloFits && maxK_ <= 8u && maxLowK_ <= 8u && capacity > 0 && largest <= capacity;
+    solveOrderHost_.clear();
+    segLastIters_.assign(M.segments.size(), 0u);
+    if (solveQueued_) solveOrder_.allocate(sizeof(uint32_t) * nTiles_);
   }
   built_ = true;
   syncStateToDevice();
@@ -889,8 +901,7 @@ int HipSolver::fastestSampledShape() const {
   return best < 0 ? 0 : best;
 }
 
-// The rest of the current solve with the persistent kernel: one cooperative launch runs every segment's solve to
-// its end, then (when `refine`) each FP64 true-residual refinement round is a decision and, for the segments that
+// The rest of the current solve with the persistent kernel: one launch runs every segment's solve to its end, then (when `refine`) each FP64 true-residual refinement round is a decision and, for the segments that
 // continue, one more launch. With `timeIt` each launch is bracketed by events and counted in the iteration timing
 // by the iterations it ran. A launch that fails, or whose barrier gave up (abort flag), switches the persistent form
 // off for this solver and returns false: the caller then re-runs the solve.
@@ -907,9 +918,14 @@ bool HipSolver::runPcgPersistent(bool refine, bool timeIt) {
     if (tunedShape_ >= 0 && dk::kPcgShapes[tunedShape_].persistent) tunedShape_ = fastestSampledShape();
     return false;
   };
-  // Iterations done before each launch (the first launch follows kPcgStepInit when timed: none).
-  uint32_t before = 0;
+  // Iterations done before each launch, per segment (the first launch follows kPcgStepInit when timed: none).
+  // A segment whose Newton loop had already finished keeps its last solve's count (kPcgStepInit leaves it).
+  std::vector<uint32_t> before(model_.segments.size(), 0u);
+  for (size_t g = 0; g < model_.segments.size(); ++g)
+    if (mirrorFresh_ && hostStates_[g].newtonDone) before[g] = hostStates_[g].pcgIter;
+  c.order = solveQueued_ ? solveOrder_.as<uint32_t>() : nullptr;
   const auto launch = [&]() -> bool {
+    if (solveQueued_) queueSegments();
     std::pair<hipEvent_t, hipEvent_t> pr{};
     if (timeIt) {
       pr = takeEvents();
@@ -929,16 +945,23 @@ bool HipSolver::runPcgPersistent(bool refine, bool timeIt) {
       return false;
     }
     if (timeIt) {
-      uint32_t after = 0;
-      for (size_t g = 0; g < model_.segments.size(); ++g) after = std::max(after, hostStates_[g].pcgIter);
+      // Counted as many iterations as its longest segment ran, carrying the bytes every segment's own iterations
+      // move: bytes over time is then the launch's achieved rate.
+      uint32_t most = 0;
+      double bytes = 0.0;
+      for (size_t g = 0; g < model_.segments.size(); ++g) {
+        const uint32_t ran = hostStates_[g].pcgIter - std::min(before[g], hostStates_[g].pcgIter);
+        most = std::max(most, ran);
+        bytes += double(ran) * launchBytes(segBlocks_[g], segParticles_[g]);
+        before[g] = hostStates_[g].pcgIter;
+      }
       float ms = 0.0f;
       check(hipEventElapsedTime(&ms, pr.first, pr.second), "event");
-      if (after > before) {
+      if (most > 0) {
         spmvTotalMs_ += ms;
-        spmvTimed_ += after - before;
-        spmvTotalBytes_ += double(after - before) * activeLaunchBytes(false);
+        spmvTimed_ += most;
+        spmvTotalBytes_ += bytes;
       }
-      before = after;
       eventPool_.push_back(pr);
     }
     return true;
@@ -953,9 +976,35 @@ bool HipSolver::runPcgPersistent(bool refine, bool timeIt) {
       if (!continued) break;
     }
   uint32_t total = 0;
-  for (size_t g = 0; g < model_.segments.size(); ++g) total = std::max(total, hostStates_[g].pcgIter);
+  for (size_t g = 0; g < model_.segments.size(); ++g) {
+    total = std::max(total, hostStates_[g].pcgIter);
+    segLastIters_[g] = hostStates_[g].pcgIter;
+  }
   lastPcgIterations_ = total;
   return true;
+}
+
+// The queue of a persistent launch with more tiles than the device holds at once: segments by decreasing expected
+// work (tiles times the iterations of the segment's last solve), each segment's tiles consecutive, so that the
+// longest solves start first and the short ones fill the CUs they leave; segments whose Newton loop has finished
+// (their tiles leave at once) go last. Uploaded only when it changes.
+void HipSolver::queueSegments() {
+  std::vector<std::pair<double, uint32_t>> work;
+  for (uint32_t sg = 0; sg < segTiles_.size(); ++sg) {
+    const double tiles = double(segTiles_[sg].second - segTiles_[sg].first);
+    const bool idle = mirrorFresh_ && hostStates_[sg].newtonDone != 0u;
+    work.push_back({idle ? -1.0 : tiles * double(std::max<uint32_t>(1u, segLastIters_[sg])), sg});
+  }
+  std::stable_sort(work.begin(), work.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  std::vector<uint32_t> order;
+  order.reserve(nTiles_);
+  for (const auto& w : work)
+    for (uint32_t t = segTiles_[w.second].first; t < segTiles_[w.second].second; ++t) order.push_back(t);
+  if (order == solveOrderHost_) return;
+  solveOrderHost_ = order;
+  check(hipMemcpyAsync(solveOrder_.as<void>(), solveOrderHost_.data(), sizeof(uint32_t) * order.size(),
+                       hipMemcpyHostToDevice, stream_),
+        "copy");
 }
 
 // After the persistent form gave up: the whole solve again from its start, one launch per iteration (the results

@@ -58,8 +58,8 @@ class HipSolver {
   // Polls of a persistent solve's barrier before a workgroup gives up (the solve is then re-run with one launch per
   // iteration, and the persistent form is not used again by this solver).
   void setSolveSpinCap(uint32_t cap) { solveSpinCap_ = cap; }
-  // The persistent solve fits this build (banded matrix, at most 8 + 8 slots per slice, one workgroup per tile
-  // resident) and has not given up.
+  // The persistent solve fits this build (banded matrix, at most 8 + 8 slots per slice, each segment's tiles
+  // resident together) and has not given up.
   bool persistentAvailable() const { return persistentOk_; }
   uint64_t persistentAborts() const { return persistentAborts_; }
   // Every `every`-th PCG SpMV launch of the step is bracketed by HIP events (0 = off).
@@ -167,6 +167,12 @@ class HipSolver {
   uint32_t solveCtrLines_ = 0;
   uint32_t* solveAbortHost_ = nullptr;
   bool persistentOk_ = false;
+  bool solveQueued_ = false;                       // more tiles than resident workgroups: segments queue
+  DeviceBuffer solveOrder_;                        // the queue's tile order (device)
+  std::vector<uint32_t> solveOrderHost_;           // the order last uploaded
+  std::vector<std::pair<uint32_t, uint32_t>> segTiles_;  // per segment: first tile, end tile
+  std::vector<uint32_t> segLastIters_;             // per segment: PCG iterations of its last persistent solve
+  void queueSegments();
   uint64_t persistentAborts_ = 0;
   uint32_t solveSpinCap_ = 1u << 20;
   // PCG iterations per captured graph, i.e. between two looks at the convergence flags after a solve's first

@@ -80,6 +80,20 @@ def test_persistent_batch_segments_equal_their_launch_runs(hip):
         np.testing.assert_array_equal(xb[off:off + cnt], single.read_positions())
 
 
+def test_persistent_queue_on_the_full_64_sheet_batch(hip):
+    """The 64 uneven sheets (891,781 particles, 3,517 tiles) outnumber the CUs, so the persistent solve queues them:
+    each sheet's tiles (at most 153) resident together, the sheets passing through the CUs largest expected work
+    first. Three steps through the drop equal the launch form bit for bit, with no barrier giving up."""
+    batch = scenes.c4_batch()
+    steps = 3
+    p = _run(hip, batch, PERSISTENT, steps, batch=True)
+    rep = p.pcg_report()
+    print(f"64-sheet batch, {steps} steps: counters {p.counters()}, aborts {rep['persistent_aborts']}", flush=True)
+    assert p.persistent_available() and rep["persistent_aborts"] == 0
+    launches = _run(hip, batch, HALO_LAUNCHES, steps, batch=True)
+    _same(p, launches)
+
+
 def test_persistent_solve_gives_up_and_the_solve_is_redone_with_launches(hip):
     """A spin cap of zero makes every barrier wait of a multi-tile scene give up at once (forced from the host; the
     kernel still exits normally): the solver re-runs the solve with one launch per iteration, switches the
@@ -97,9 +111,9 @@ def test_persistent_solve_gives_up_and_the_solve_is_redone_with_launches(hip):
 
 
 def test_persistent_solve_is_refused_where_it_does_not_fit(hip):
-    """The seamed tube's tiles reach across the panels (no halo window), and a 288 x 288 sheet has 324 tiles, more
-    than the CUs that hold one each: the persistent form is refused before anything is launched, and a tuned solver
-    never picks it there."""
+    """The seamed tube's tiles reach across the panels (no halo window), and a 288 x 288 sheet is one scene of 324
+    tiles, more than the CUs that hold one each (a scene's tiles wait for each other, so they must all be resident):
+    the persistent form is refused before anything is launched, and a tuned solver never picks it there."""
     for sc in (scenes.c3_tube(nx=40, ny=60), scenes.c2_sheet(n=288)):
         g = hip.HipSolver(pcg_shape=PERSISTENT)
         g.build(sc)
