@@ -95,13 +95,15 @@ WEIGHTS = os.path.join(REPO, "drape-2_amd", "drape_amd", "weights", "warmstart.n
 
 
 def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 4, 5), ref_iterations: int = 40,
-                  sample_every: int = 2) -> dict:
+                  sample_every: int = 2, fixture=None) -> dict:
     """Learned warm start against the previous-velocity guess at matched error. Each method steps the drop with
     Newton capped at each of `caps`; every `sample_every`-th timed step the converged plain solve (Newton <=
     ref_iterations, same PCG) from that method's own state before the step judges the step's error (max and RMS
     particle distance, mm). Reports steps/s over the timed steps (the reference solves excluded), iteration counts,
     how often the energy check kept the learned guess, and for every (learned cap, plain cap) pair the speed ratio
-    and the ratios of the three error measures (below 1: the learned run is the more accurate)."""
+    and the ratios of the three error measures (below 1: the learned run is the more accurate). With the FP64-oracle
+    fixture of the drop (positions at given steps from rest, plain guess, Newton <= 5), each run's own trajectory
+    from rest is also compared with the oracle at those steps, beside the oracle's self-spread there."""
     from drape_amd import HipSolver, learn
     net = learn.WarmStartNet()
     net.load(net_path)
@@ -116,6 +118,7 @@ def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 4
             if learned:
                 s.set_warm_start(net)
             errs, rms, t = [], [], 0.0
+            oracle = {}
             for i in range(warmup + steps):
                 check = i >= warmup and (i - warmup) % sample_every == 0
                 if check:
@@ -126,6 +129,12 @@ def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 4
                 s.step(1 / 60, 1, k)
                 if i >= warmup:
                     t += time.perf_counter() - t0
+                if fixture is not None and i + 1 in [int(x) for x in fixture["steps"]]:
+                    d = np.linalg.norm(s.read_positions().astype(np.float64) - fixture[f"x{i + 1}"].astype(np.float64),
+                                       axis=1)
+                    oracle[str(i + 1)] = {"max_mm": float(d.max()) * 1e3,
+                                          "rms_mm": float(np.sqrt(np.mean(d * d))) * 1e3,
+                                          "oracle_self_spread_mm": float(fixture[f"spread{i + 1}"][0]) * 1e3}
                 if check:
                     ref.restore(snap)
                     ref.step(1 / 60, 1, ref_iterations)
@@ -140,7 +149,8 @@ def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 4
                          "max_step_error_mm": max(errs), "median_step_error_mm": float(np.median(errs)),
                          "steps_within_0p1mm": int(sum(e <= 0.1 for e in errs)), "checked_steps": len(errs),
                          "learned_guess_kept": c["warm_start_accepted"],
-                         "learned_guess_rejected": c["warm_start_fallbacks"]})
+                         "learned_guess_rejected": c["warm_start_fallbacks"],
+                         "fp64_oracle_from_rest": oracle})
     plain = [r for r in runs if r["guess"] != "learned"]
     pairs = []
     for r in runs:
@@ -153,7 +163,8 @@ def warm_start_ab(make_scene, warmup: int, steps: int, net_path: str, caps=(3, 4
                           "mean_max_error_ratio": r["mean_max_error_mm"] / p["mean_max_error_mm"],
                           "worst_step_error_ratio": r["max_step_error_mm"] / p["max_step_error_mm"]})
     return {"error_reference": f"converged plain solve (Newton <= {ref_iterations}) from the method's own state "
-                               f"before the step, every {sample_every} timed steps",
+                               f"before the step, every {sample_every} timed steps; fp64_oracle_from_rest: the run's "
+                               f"own trajectory from rest against the FP64 oracle's (plain guess, Newton <= 5)",
             "weights": os.path.relpath(net_path, REPO), "runs": runs, "learned_vs_plain": pairs}
 
 
@@ -494,7 +505,8 @@ def main() -> int:
                                                   "rms_mm": float(np.sqrt(np.mean(d * d))) * 1e3}
     line["accuracy"] = accuracy
     if not args.no_warm_start_ab and os.path.exists(args.warm_start_weights):
-        line["warm_start_ab"] = warm_start_ab(make_scene, args.warmup, args.steps, args.warm_start_weights)
+        line["warm_start_ab"] = warm_start_ab(make_scene, args.warmup, args.steps, args.warm_start_weights,
+                                              fixture=fixture)
     print(json.dumps(line), flush=True)
     return 0
 
