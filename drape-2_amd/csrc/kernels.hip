@@ -1364,8 +1364,30 @@ __device__ inline bool segmentBarrier(const SolveCtl& c, uint32_t s, uint32_t lo
 }
 }  // namespace
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) kPcgSolve(View v, SolveCtl c) {
+// The window of each exchanged vector: with one workgroup per CU it is loaded into registers (3 rows a thread on a
+// sheet) and only the rows' search directions go to LDS; with two it goes to LDS as a byte copy of memory, loaded
+// straight into LDS (buffer_load ... lds: 16 bytes a lane, device scope) in 16-byte chunks from the chunk that holds
+// the window's first row, which takes no registers (two workgroups per CU leave 256 registers a lane): row i of the
+// window then starts `delta` floats into the copy, plus 3 i.
+constexpr uint32_t kHaloChunks = (kHaloRows * 12u + 12u + 15u) / 16u;
+
+__device__ inline __amdgpu_buffer_rsrc_t bufferOfBytes(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, int(bytes), 0x00020000);
+}
+__device__ inline F3 ldsF3(const float* p) { return f3(p[0], p[1], p[2]); }
+// 16 bytes a lane from a device-scope buffer at byte `offset` straight into LDS at lds + 16 * lane (lds uniform over
+// the wavefront).
+__device__ inline void loadChunkToLds(__amdgpu_buffer_rsrc_t r, float* lds, uint32_t offset) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, lds, 16, offset, 0, 0, kScDevice);
+}
+
+// PerCu: workgroups per CU (1: each of a tile's wavefronts has a SIMD to itself; 2: two tiles share a CU, for
+// launches with more tiles than CUs). The arithmetic is the same.
+template <int PerCu>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu, PerCu))) kPcgSolve(View v, SolveCtl c) {
   __shared__ double red[kPcgParts * 4], red2[kPcgParts * 4];
+  constexpr bool Dma = PerCu == 2;
+  __shared__ __attribute__((aligned(16))) float hraw[Dma ? 3 : 1][Dma ? kHaloChunks * 4 : 4];  // zr, t, p copies
   __shared__ F3 hpk[kHaloRows + 1];  // p_k of the window's rows; entry kHaloRows: the dummy column
   __shared__ uint32_t quit;
   // Queued launch (more tiles than the device holds at once): workgroup i takes tile order[i]; each segment's tiles
@@ -1391,6 +1413,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   const uint32_t kBase = S->kBase;
   double r0 = S->r0;
   const uint32_t hb = v.tileHalo[2u * tile], hc = v.tileHalo[2u * tile + 1u];
+  const uint32_t chunk0 = (12u * hb) & ~15u, delta = (12u * hb - chunk0) / 4u;
+  const uint32_t nChunks = (12u * hc + 4u * delta + 15u) / 16u;
+  const uint32_t own = delta + 3u * (row - hb);  // the row's (zr, t, p) in the window copies
 
   // The slice's matrix, as kPcgStep<1, 8, true> loads it in its first round trip; columns become window offsets
   // (the dummy column: the zero entry at kHaloRows).
@@ -1410,8 +1435,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
       bv[j][e] = v.sellVal[base + j * 9u * kSlice + 9u * lane + e];
     }
   }
-  // The rows' PCG state of iteration -1 and their private data.
-  F3 oR = v.cgR[1][row], oAp = v.cgAp[1][row], oZ = v.cgZr[1][row], oT = v.cgT[1][row], oP = v.cgP[1][row];
+  // The rows' private PCG state of iteration -1 (with the LDS window their zr, t and p come with it).
+  F3 oR = v.cgR[1][row], oAp = v.cgAp[1][row];
+  F3 oZ = f3(0, 0, 0), oT = f3(0, 0, 0), oP = f3(0, 0, 0);
+  if constexpr (!Dma) {
+    oZ = v.cgZr[1][row];
+    oT = v.cgT[1][row];
+    oP = v.cgP[1][row];
+  }
   F3 oStep = v.step[row];
   float m[kPrecondFloats];
 #pragma unroll
@@ -1420,12 +1451,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   const float4 oCn = v.cnormal[row];
   if (threadIdx.x == 0) hpk[kHaloRows] = f3(0, 0, 0);
 
-  const __amdgpu_buffer_rsrc_t zrBuf[2] = {bufferOf(v.cgZr[0]), bufferOf(v.cgZr[1])};
-  const __amdgpu_buffer_rsrc_t tBuf[2] = {bufferOf(v.cgT[0]), bufferOf(v.cgT[1])};
-  const __amdgpu_buffer_rsrc_t pBuf[2] = {bufferOf(v.cgP[0]), bufferOf(v.cgP[1])};
+  // The vectors' sizes: nRows + 1 entries and 16 zero bytes past them, which cover the last window chunk.
+  const uint32_t vecBytes = 12u * (v.nRows + 1u) + 16u;
+  const __amdgpu_buffer_rsrc_t zrBuf[2] = {bufferOfBytes(v.cgZr[0], vecBytes), bufferOfBytes(v.cgZr[1], vecBytes)};
+  const __amdgpu_buffer_rsrc_t tBuf[2] = {bufferOfBytes(v.cgT[0], vecBytes), bufferOfBytes(v.cgT[1], vecBytes)};
+  const __amdgpu_buffer_rsrc_t pBuf[2] = {bufferOfBytes(v.cgP[0], vecBytes), bufferOfBytes(v.cgP[1], vecBytes)};
   const __amdgpu_buffer_rsrc_t partBuf[2] = {bufferOf(v.cgPart[0]), bufferOf(v.cgPart[1])};
-  constexpr uint32_t HR = kHaloRows / 256u;
-  const uint32_t hr = (hc + 255u) / 256u;  // window rows per thread
   uint32_t updates = 0, conv = 0, k = k0, lastQ = 0;
   double alphaLast = 0.0, rrLast = 0.0;
   bool wroteR0 = false;
@@ -1433,15 +1464,28 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     const uint32_t q = it & 1u, o = q ^ 1u;
     k = k0 + it;
     lastQ = q;
-    // The window (zr, t, p) and the segment's partial sums of the previous iteration, issued together.
+    // The window's (zr, t, p) and the segment's partial sums of the previous iteration, issued together.
+    constexpr uint32_t HR = Dma ? 1u : kHaloRows / 256u;
     F3 wz[HR], wt[HR], wp[HR];
+    if constexpr (Dma) {
+      for (uint32_t j0 = wave * 64u; j0 < nChunks; j0 += 256u) {  // uniform over the wavefront
+        const uint32_t j = j0 + lane;
+        if (j < nChunks) {
+          loadChunkToLds(o ? zrBuf[1] : zrBuf[0], &hraw[0][4u * j0], chunk0 + 16u * j);
+          loadChunkToLds(o ? tBuf[1] : tBuf[0], &hraw[1][4u * j0], chunk0 + 16u * j);
+          loadChunkToLds(o ? pBuf[1] : pBuf[0], &hraw[2][4u * j0], chunk0 + 16u * j);
+        }
+      }
+    } else {
+      const uint32_t hr = (hc + 255u) / 256u;  // window rows per thread (uniform over the workgroup)
 #pragma unroll
-    for (uint32_t r = 0; r < HR; ++r) {
-      if (r < hr) {  // rows past the window's end are not loaded (hr is uniform over the workgroup)
-        const uint32_t i = hb + min(r * 256u + threadIdx.x, hc - 1u);
-        wz[r] = loadDevice(o ? zrBuf[1] : zrBuf[0], i);
-        wt[r] = loadDevice(o ? tBuf[1] : tBuf[0], i);
-        wp[r] = loadDevice(o ? pBuf[1] : pBuf[0], i);
+      for (uint32_t r = 0; r < HR; ++r) {
+        if (r < hr) {
+          const uint32_t i = hb + min(r * 256u + threadIdx.x, hc - 1u);
+          wz[r] = loadDevice(o ? zrBuf[1] : zrBuf[0], i);
+          wt[r] = loadDevice(o ? tBuf[1] : tBuf[0], i);
+          wp[r] = loadDevice(o ? pBuf[1] : pBuf[0], i);
+        }
       }
     }
     double sum[kPcgParts] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1453,6 +1497,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
 #pragma unroll
       for (int n = 0; n < kPcgParts; ++n) sum[n] += more[n];
     }
+    if constexpr (Dma) __builtin_amdgcn_s_waitcnt(0);  // this wavefront's chunks are in LDS; the barrier below shows them
     blockSumN<kPcgParts, 4, true>(sum, red);
 
     // The decision of kPcgStep, from the same sums.
@@ -1486,15 +1531,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
       rrLast = rrNext;
     }
     const float a = float(alpha), b = float(beta);
+    if constexpr (Dma) oP = ldsF3(&hraw[2][own]);
     if (update) oStep = oStep + a * oP;
     if (conv) break;
 
     // Each window row's search direction once (kPcgStep rebuilds it per gathered column from the same values with
     // the same expression), to LDS.
+    if constexpr (Dma) {
+      for (uint32_t i = threadIdx.x; i < hc; i += 256u) {
+        const uint32_t w = delta + 3u * i;
+        hpk[i] = pcgDirection(ldsF3(&hraw[0][w]), ldsF3(&hraw[1][w]), ldsF3(&hraw[2][w]), a, b);
+      }
+    } else {
 #pragma unroll
-    for (uint32_t r = 0; r < HR; ++r) {
-      const uint32_t i = r * 256u + threadIdx.x;
-      if (r < hr && i < hc) hpk[i] = pcgDirection(wz[r], wt[r], wp[r], a, b);
+      for (uint32_t r = 0; r < HR; ++r) {
+        const uint32_t i = r * 256u + threadIdx.x;
+        if (i < hc) hpk[i] = pcgDirection(wz[r], wt[r], wp[r], a, b);
+      }
     }
     __syncthreads();
 
@@ -1511,7 +1564,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     const float a2 = ((acc[0][2] + acc[1][2]) + acc[2][2]) + acc[3][2];
 
     const F3 rk = oR - a * oAp;
-    const F3 pk = pcgDirection(oZ, oT, oP, a, b);
+    const F3 pk = Dma ? hpk[row - hb] : pcgDirection(oZ, oT, oP, a, b);  // (the same expression either way)
     const F3 apk = (oFlags & kRowPinned) ? f3(0, 0, 0) : onPlane(oCf, oCn, f3(a0, a1, a2));
     const F3 zr = applyPrecond(m, oCf, oCn, rk);
     const F3 tk = applyPrecond(m, oCf, oCn, apk);
@@ -1529,9 +1582,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     }
     oR = rk;
     oAp = apk;
-    oZ = zr;
-    oT = tk;
-    oP = pk;
+    if constexpr (!Dma) {
+      oZ = zr;
+      oT = tk;
+      oP = pk;
+    }
     if (!segmentBarrier(c, s, tile - tileBegin, tileEnd - tileBegin, it, &quit)) return;
   }
 
@@ -2427,23 +2482,25 @@ void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t s
   else
     throw std::invalid_argument("PCG launch shape must be 1x8, 1x4, 2x8 or 4x8, or 1x8 / 1x4 with the halo");
 }
-uint32_t pcgSolveCapacity() {
-  int device = 0, perCu = 0;
+uint32_t pcgSolveCapacity(int perCu) {
+  int device = 0, fits = 0;
   hipDeviceProp_t prop{};
   if (hipGetDevice(&device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) return 0u;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, reinterpret_cast<const void*>(kPcgSolve), 256, 0) !=
-      hipSuccess)
-    return 0u;
-  // One workgroup per CU: a tile's four wavefronts take one SIMD each (the kernel is built for one wavefront per SIMD).
-  return perCu > 0 ? uint32_t(prop.multiProcessorCount) : 0u;
+  const void* fn = perCu == 2 ? reinterpret_cast<const void*>(kPcgSolve<2>) : reinterpret_cast<const void*>(kPcgSolve<1>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fits, fn, 256, 0) != hipSuccess) return 0u;
+  // perCu workgroups per CU (the kernel's registers and LDS are sized for exactly that many).
+  return fits >= perCu ? uint32_t(prop.multiProcessorCount) * uint32_t(perCu) : 0u;
 }
-hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, hipStream_t st) {
+hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, int perCu, hipStream_t st) {
   // An ordinary dispatch (capturable in a graph, and traceable by the profiler, which does not survive cooperative
   // dispatches): the host has checked that the grid fits the device at once (pcgSolveCapacity), the solver's stream
   // runs nothing beside it, and a workgroup that found no room would only make its segment's waits give up.
   launchFill(c.ctr, 0u, uint64_t(counterLines) * 64u + 1u, st);
-  hipExtLaunchKernelGGL(kPcgSolve, dim3(c.order ? v.nTiles : xcdGrid(v.nTiles)), dim3(256), 0, st, nullptr, nullptr, 0,
-                        v, c);
+  const dim3 grid(c.order ? v.nTiles : xcdGrid(v.nTiles));
+  if (perCu == 2)
+    hipExtLaunchKernelGGL(kPcgSolve<2>, grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
+  else  // (its registers allow one workgroup per CU)
+    hipExtLaunchKernelGGL(kPcgSolve<1>, grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
   return hipGetLastError();
 }
 void launchPcgRefine(const View& v, uint32_t maxRestarts, hipStream_t st) {

@@ -173,12 +173,13 @@ struct SolveCtl {
 };
 // Counter lines a segment of `tiles` tiles needs.
 inline uint32_t solveCounterLines(uint32_t tiles) { return 1u + 2u * ((tiles + 15u) / 16u); }
-// Workgroups of kPcgSolve the device holds at once (0 if the kernel cannot be resident).
-uint32_t pcgSolveCapacity();
+// Workgroups of kPcgSolve the device holds at once with perCu (1 or 2) workgroups per CU (0 if that many cannot be
+// resident).
+uint32_t pcgSolveCapacity(int perCu);
 // Clears the counters and the abort flag (counterLines * 64 + 1 words at c.ctr, the flag last) and launches the
 // solve (an ordinary dispatch of xcdGrid(nTiles) workgroups; the caller checks the grid against pcgSolveCapacity).
 // Returns the launch's error code.
-hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, hipStream_t st);
+hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, int perCu, hipStream_t st);
 // FP64 true residual of the finished solve and, per segment, a continuation of the solve from it when it is above
 // tol r0 (at most maxRestarts per solve; see kTrueResidual).
 void launchPcgRefine(const View& v, uint32_t maxRestarts, hipStream_t st);

@@ -184,8 +184,9 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
     }
     meshGraph_ = std::make_unique<ws::Graph>(nRows_, rs.data(), nb.data(), rest.data(), mask.data());
   }
-  // Row arrays a gathered column may index hold the zero dummy entry nRows as well.
-  for (DeviceBuffer& b : cgBuf_) b.allocate(vecBytes);
+  // Row arrays a gathered column may index hold the zero dummy entry nRows as well; the PCG vectors carry 16 more
+  // zero bytes, the most the persistent solve's 16-byte window chunks read past the dummy entry.
+  for (DeviceBuffer& b : cgBuf_) b.allocate(vecBytes + 16u);
   precond_.allocate(sizeof(float) * dk::kPrecondFloats * (nRows_ + 1));
   cnormal_.allocate(sizeof(float4) * (nRows_ + 1));
   cflags_.allocate(sizeof(uint32_t) * (nRows_ + 1));
@@ -478,7 +479,11 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
     *solveAbortHost_ = 0u;
     // Every segment's tiles must be resident together: all tiles in one launch when they fit the device, else a
     // queue of segments (solveQueued_) in which each segment fits it.
-    const uint32_t grid = 8u * ((nTiles_ + 7u) / 8u), capacity = dk::pcgSolveCapacity();
+    // One workgroup per CU when every tile fits that way (each wavefront then has a SIMD to itself), else two.
+    const uint32_t grid = 8u * ((nTiles_ + 7u) / 8u);
+    const uint32_t capacity1 = dk::pcgSolveCapacity(1), capacity2 = dk::pcgSolveCapacity(2);
+    solvePerCu_ = grid <= capacity1 || capacity2 == 0 ? 1 : 2;
+    const uint32_t capacity = solvePerCu_ == 1 ? capacity1 : capacity2;
     segTiles_.assign(M.segments.size(), {0u, 0u});
     uint32_t largest = 0;
     for (size_t sg = 0; sg < M.segments.size(); ++sg) {
@@ -931,7 +936,7 @@ bool HipSolver::runPcgPersistent(bool refine, bool timeIt) {
       pr = takeEvents();
       check(hipEventRecord(pr.first, stream_), "event");
     }
-    const hipError_t e = dk::launchPcgSolve(view_, c, solveCtrLines_, stream_);
+    const hipError_t e = dk::launchPcgSolve(view_, c, solveCtrLines_, solvePerCu_, stream_);
     if (e != hipSuccess) {
       (void)hipGetLastError();
       if (timeIt) eventPool_.push_back(pr);

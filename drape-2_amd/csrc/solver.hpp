@@ -168,6 +168,7 @@ class HipSolver {
   uint32_t* solveAbortHost_ = nullptr;
   bool persistentOk_ = false;
   bool solveQueued_ = false;                       // more tiles than resident workgroups: segments queue
+  int solvePerCu_ = 1;                             // workgroups per CU of the persistent launch (1 or 2)
   DeviceBuffer solveOrder_;                        // the queue's tile order (device)
   std::vector<uint32_t> solveOrderHost_;           // the order last uploaded
   std::vector<std::pair<uint32_t, uint32_t>> segTiles_;  // per segment: first tile, end tile

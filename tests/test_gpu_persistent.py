@@ -1,5 +1,5 @@
-"""The persistent PCG solve: one cooperative launch per solve, each tile's workgroup keeping its slices' matrix and
-private vectors in registers, the tiles of each scene meeting at a barrier of their own every iteration.
+"""The persistent PCG solve: one launch per solve, each tile's workgroup keeping its slices' matrix and private vectors
+in registers, the tiles of each scene meeting at a barrier of their own every iteration.
 
 It computes every iteration as the launch-per-iteration form with the 1x8 halo shape does, so the two must agree bit
 for bit: on the 32x32 sheet, through the 65,536-particle drop's first contact, and segment by segment on a batch of
@@ -110,11 +110,22 @@ def test_persistent_solve_gives_up_and_the_solve_is_redone_with_launches(hip):
     assert p.counters()["newton_iterations"] == launches.counters()["newton_iterations"]
 
 
+def test_persistent_solve_two_tiles_per_cu_on_one_large_scene(hip):
+    """A 288 x 288 sheet is one scene of 324 tiles: more than the CUs, so its tiles run two to a CU, all resident
+    together in one launch. Three steps equal the launch form bit for bit."""
+    sc = scenes.c2_sheet(n=288)
+    p = _run(hip, sc, PERSISTENT, 3)
+    assert p.pcg_report()["persistent_aborts"] == 0
+    launches = _run(hip, sc, HALO_LAUNCHES, 3)
+    _same(p, launches)
+
+
 def test_persistent_solve_is_refused_where_it_does_not_fit(hip):
-    """The seamed tube's tiles reach across the panels (no halo window), and a 288 x 288 sheet is one scene of 324
-    tiles, more than the CUs that hold one each (a scene's tiles wait for each other, so they must all be resident):
-    the persistent form is refused before anything is launched, and a tuned solver never picks it there."""
-    for sc in (scenes.c3_tube(nx=40, ny=60), scenes.c2_sheet(n=288)):
+    """The seamed tube's tiles reach across the panels (no halo window), and a 384 x 384 sheet is one scene of 576
+    tiles, more than the device holds at two a CU (a scene's tiles wait for each other, so they must all be
+    resident): the persistent form is refused before anything is launched, and a tuned solver never picks it
+    there."""
+    for sc in (scenes.c3_tube(nx=40, ny=60), scenes.c2_sheet(n=384)):
         g = hip.HipSolver(pcg_shape=PERSISTENT)
         g.build(sc)
         assert not g.persistent_available()
