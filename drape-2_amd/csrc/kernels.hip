@@ -1317,9 +1317,6 @@ __device__ inline void loadPartsDevice(__amdgpu_buffer_rsrc_t r, uint32_t tile, 
   x[5] = asDouble(c.z, c.w);
   x[6] = asDouble(d.x, d.y);
 }
-__device__ inline double loadDevice(const double* p) {
-  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ inline void storeDevice(double* p, double x) {
   __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -1369,7 +1366,7 @@ __device__ inline bool segmentBarrier(const SolveCtl& c, uint32_t s, uint32_t lo
 // straight into LDS (buffer_load ... lds: 16 bytes a lane, device scope) in 16-byte chunks from the chunk that holds
 // the window's first row, which takes no registers (two workgroups per CU leave 256 registers a lane): row i of the
 // window then starts `delta` floats into the copy, plus 3 i.
-constexpr uint32_t kHaloChunks = (kHaloRows * 12u + 12u + 15u) / 16u;
+constexpr uint32_t kHaloChunks = (kHaloRows * 12u + kSolveWindows * 15u + 15u) / 16u;
 
 __device__ inline __amdgpu_buffer_rsrc_t bufferOfBytes(const void* p, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, int(bytes), 0x00020000);
@@ -1382,8 +1379,9 @@ __device__ inline void loadChunkToLds(__amdgpu_buffer_rsrc_t r, float* lds, uint
 }
 
 // PerCu: workgroups per CU (1: each of a tile's wavefronts has a SIMD to itself; 2: two tiles share a CU, for
-// launches with more tiles than CUs). The arithmetic is the same.
-template <int PerCu>
+// launches with more tiles than CUs). Extra: some slice has more than 8 upper or lower slots (seam rows); the slots
+// past the eighth come from memory every iteration. The arithmetic is the same in every instance.
+template <int PerCu, bool Extra>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu, PerCu))) kPcgSolve(View v, SolveCtl c) {
   __shared__ double red[kPcgParts * 4], red2[kPcgParts * 4];
   constexpr bool Dma = PerCu == 2;
@@ -1412,23 +1410,57 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
   const uint32_t restarted = S->restartActive;
   const uint32_t kBase = S->kBase;
   double r0 = S->r0;
-  const uint32_t hb = v.tileHalo[2u * tile], hc = v.tileHalo[2u * tile + 1u];
-  const uint32_t chunk0 = (12u * hb) & ~15u, delta = (12u * hb - chunk0) / 4u;
-  const uint32_t nChunks = (12u * hc + 4u * delta + 15u) / 16u;
-  const uint32_t own = delta + 3u * (row - hb);  // the row's (zr, t, p) in the window copies
+  // The tile's window: up to kSolveWindows row ranges, concatenated in LDS (range w from entry off[w]); with the
+  // LDS copies, range w's bytes start in chunk ch[w], its first row `delta[w]` floats in. The table lives in LDS
+  // (read at uniform addresses): in registers it would crowd out the matrix.
+  __shared__ uint32_t wf[kSolveWindows], wc[kSolveWindows], off[kSolveWindows], ch[kSolveWindows],
+      delta[kSolveWindows], totals[2];
+  if (threadIdx.x == 0) {
+    uint32_t entries = 0, chunks = 0;
+    for (uint32_t w = 0; w < kSolveWindows; ++w) {
+      wf[w] = v.tileWindows[2u * (kSolveWindows * tile + w)];
+      wc[w] = v.tileWindows[2u * (kSolveWindows * tile + w) + 1u];
+      off[w] = entries;
+      ch[w] = chunks;
+      delta[w] = (12u * wf[w] - ((12u * wf[w]) & ~15u)) / 4u;
+      entries += wc[w];
+      chunks += wc[w] ? (12u * wc[w] + 4u * delta[w] + 15u) / 16u : 0u;
+    }
+    totals[0] = entries;
+    totals[1] = chunks;
+  }
+  __syncthreads();
+  const uint32_t hc = totals[0];
+  const bool oneRange = wc[1] == 0u;  // (most tiles: every column in one range)
+  const uint32_t first0 = wf[0];
+  // Window entry of a column (the dummy column: the zero entry at kHaloRows), and the range of a window entry.
+  const auto entryOf = [&](uint32_t col) {
+    uint32_t e = kHaloRows;
+    for (uint32_t w = 0; w < kSolveWindows; ++w)
+      if (col - wf[w] < wc[w]) e = off[w] + (col - wf[w]);
+    return col == v.nRows ? kHaloRows : e;
+  };
+  const auto rangeOf = [&](uint32_t e) {
+    uint32_t r = 0;
+    for (uint32_t w = 1; w < kSolveWindows; ++w)
+      if (wc[w] && e >= off[w]) r = w;
+    return r;
+  };
+  const uint32_t ownEntry = entryOf(row);
+  const uint32_t ownRange = rangeOf(ownEntry);
+  const uint32_t own = 4u * ch[ownRange] + delta[ownRange] + 3u * (ownEntry - off[ownRange]);  // in the copies
 
   // The slice's matrix, as kPcgStep<1, 8, true> loads it in its first round trip; columns become window offsets
   // (the dummy column: the zero entry at kHaloRows).
   const uint32_t K = v.sellK[slice], base = v.sellBase[slice], KL = v.lowK[slice], lbase = v.lowBase[slice];
-  uint32_t lc[8], uc[8];
+  uint32_t ent[8];  // window entries of slot j's lower (low 16 bits) and upper (high 16 bits) column
   float bt[8][9], bv[8][9];
 #pragma unroll
   for (uint32_t j = 0; j < 8; ++j) {
     const uint32_t cl = j < KL ? v.lowCol[lbase + j * kSlice + lane] : v.nRows;
     const uint32_t bl = j < KL ? v.lowBlk[lbase + j * kSlice + lane] : v.zeroBlock;
     const uint32_t cu = j < K ? v.sellCol[base / 9u + j * kSlice + lane] : v.nRows;
-    lc[j] = cl == v.nRows ? kHaloRows : cl - hb;
-    uc[j] = cu == v.nRows ? kHaloRows : cu - hb;
+    ent[j] = entryOf(cl) | (entryOf(cu) << 16);
 #pragma unroll
     for (int e = 0; e < 9; ++e) {
       bt[j][e] = v.sellVal[size_t(bl) * 9u + e];
@@ -1468,12 +1500,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
     constexpr uint32_t HR = Dma ? 1u : kHaloRows / 256u;
     F3 wz[HR], wt[HR], wp[HR];
     if constexpr (Dma) {
-      for (uint32_t j0 = wave * 64u; j0 < nChunks; j0 += 256u) {  // uniform over the wavefront
-        const uint32_t j = j0 + lane;
-        if (j < nChunks) {
-          loadChunkToLds(o ? zrBuf[1] : zrBuf[0], &hraw[0][4u * j0], chunk0 + 16u * j);
-          loadChunkToLds(o ? tBuf[1] : tBuf[0], &hraw[1][4u * j0], chunk0 + 16u * j);
-          loadChunkToLds(o ? pBuf[1] : pBuf[0], &hraw[2][4u * j0], chunk0 + 16u * j);
+      for (uint32_t w = 0; w < kSolveWindows; ++w) {
+        const uint32_t first = (12u * wf[w]) & ~15u;
+        const uint32_t n = wc[w] ? (12u * wc[w] + 4u * delta[w] + 15u) / 16u : 0u;
+        for (uint32_t j0 = wave * 64u; j0 < n; j0 += 256u) {  // uniform over the wavefront
+          const uint32_t j = j0 + lane;
+          if (j < n) {
+            loadChunkToLds(o ? zrBuf[1] : zrBuf[0], &hraw[0][4u * (ch[w] + j0)], first + 16u * j);
+            loadChunkToLds(o ? tBuf[1] : tBuf[0], &hraw[1][4u * (ch[w] + j0)], first + 16u * j);
+            loadChunkToLds(o ? pBuf[1] : pBuf[0], &hraw[2][4u * (ch[w] + j0)], first + 16u * j);
+          }
         }
       }
     } else {
@@ -1481,7 +1517,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
 #pragma unroll
       for (uint32_t r = 0; r < HR; ++r) {
         if (r < hr) {
-          const uint32_t i = hb + min(r * 256u + threadIdx.x, hc - 1u);
+          const uint32_t e = min(r * 256u + threadIdx.x, hc - 1u);
+          uint32_t i = first0 + e;
+          if (!oneRange) {
+            const uint32_t w = rangeOf(e);
+            i = wf[w] + (e - off[w]);
+          }
           wz[r] = loadDevice(o ? zrBuf[1] : zrBuf[0], i);
           wt[r] = loadDevice(o ? tBuf[1] : tBuf[0], i);
           wp[r] = loadDevice(o ? pBuf[1] : pBuf[0], i);
@@ -1538,9 +1579,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
     // Each window row's search direction once (kPcgStep rebuilds it per gathered column from the same values with
     // the same expression), to LDS.
     if constexpr (Dma) {
+      const uint32_t delta0 = delta[0];
       for (uint32_t i = threadIdx.x; i < hc; i += 256u) {
-        const uint32_t w = delta + 3u * i;
-        hpk[i] = pcgDirection(ldsF3(&hraw[0][w]), ldsF3(&hraw[1][w]), ldsF3(&hraw[2][w]), a, b);
+        uint32_t at = delta0 + 3u * i;
+        if (!oneRange) {
+          const uint32_t w = rangeOf(i);
+          at = 4u * ch[w] + delta[w] + 3u * (i - off[w]);
+        }
+        hpk[i] = pcgDirection(ldsF3(&hraw[0][at]), ldsF3(&hraw[1][at]), ldsF3(&hraw[2][at]), a, b);
       }
     } else {
 #pragma unroll
@@ -1556,15 +1602,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
 #pragma unroll
     for (uint32_t g = 0; g < 4; ++g) acc[g][0] = acc[g][1] = acc[g][2] = 0.0f;
 #pragma unroll
-    for (uint32_t j = 0; j < 8; ++j) blockTMulAdd(bt[j], hpk[lc[j]], acc[j % 4][0], acc[j % 4][1], acc[j % 4][2]);
+    for (uint32_t j = 0; j < 8; ++j)
+      blockTMulAdd(bt[j], hpk[ent[j] & 0xffffu], acc[j % 4][0], acc[j % 4][1], acc[j % 4][2]);
+    for (uint32_t t = 8; Extra && t < KL; ++t) {  // rows with more than 8 lower slots: the rest from memory
+      const uint32_t cl = v.lowCol[lbase + t * kSlice + lane], bl = v.lowBlk[lbase + t * kSlice + lane];
+      blockTMulAdd(v.sellVal + size_t(bl) * 9u, hpk[entryOf(cl)], acc[(t - 8u) % 4][0], acc[(t - 8u) % 4][1],
+                   acc[(t - 8u) % 4][2]);
+    }
 #pragma unroll
-    for (uint32_t j = 0; j < 8; ++j) blockMulAdd(bv[j], hpk[uc[j]], acc[j % 4][0], acc[j % 4][1], acc[j % 4][2]);
+    for (uint32_t j = 0; j < 8; ++j) blockMulAdd(bv[j], hpk[ent[j] >> 16], acc[j % 4][0], acc[j % 4][1], acc[j % 4][2]);
+    for (uint32_t t = 8; Extra && t < K; ++t) {  // rows with more than 8 upper slots
+      const uint32_t cu = v.sellCol[base / 9u + t * kSlice + lane];
+      blockMulAdd(v.sellVal + base + t * 9u * kSlice + 9u * lane, hpk[entryOf(cu)], acc[(t - 8u) % 4][0],
+                  acc[(t - 8u) % 4][1], acc[(t - 8u) % 4][2]);
+    }
     const float a0 = ((acc[0][0] + acc[1][0]) + acc[2][0]) + acc[3][0];
     const float a1 = ((acc[0][1] + acc[1][1]) + acc[2][1]) + acc[3][1];
     const float a2 = ((acc[0][2] + acc[1][2]) + acc[2][2]) + acc[3][2];
 
     const F3 rk = oR - a * oAp;
-    const F3 pk = Dma ? hpk[row - hb] : pcgDirection(oZ, oT, oP, a, b);  // (the same expression either way)
+    const F3 pk = Dma ? hpk[ownEntry] : pcgDirection(oZ, oT, oP, a, b);  // (the same expression either way)
     const F3 apk = (oFlags & kRowPinned) ? f3(0, 0, 0) : onPlane(oCf, oCn, f3(a0, a1, a2));
     const F3 zr = applyPrecond(m, oCf, oCn, rk);
     const F3 tk = applyPrecond(m, oCf, oCn, apk);
@@ -2482,25 +2539,33 @@ void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t s
   else
     throw std::invalid_argument("PCG launch shape must be 1x8, 1x4, 2x8 or 4x8, or 1x8 / 1x4 with the halo");
 }
-uint32_t pcgSolveCapacity(int perCu) {
+uint32_t pcgSolveCapacity(int perCu, bool extra) {
   int device = 0, fits = 0;
   hipDeviceProp_t prop{};
   if (hipGetDevice(&device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) return 0u;
-  const void* fn = perCu == 2 ? reinterpret_cast<const void*>(kPcgSolve<2>) : reinterpret_cast<const void*>(kPcgSolve<1>);
+  const void* fn = perCu == 2 ? (extra ? reinterpret_cast<const void*>(kPcgSolve<2, true>)
+                                       : reinterpret_cast<const void*>(kPcgSolve<2, false>))
+                              : (extra ? reinterpret_cast<const void*>(kPcgSolve<1, true>)
+                                       : reinterpret_cast<const void*>(kPcgSolve<1, false>));
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fits, fn, 256, 0) != hipSuccess) return 0u;
   // perCu workgroups per CU (the kernel's registers and LDS are sized for exactly that many).
   return fits >= perCu ? uint32_t(prop.multiProcessorCount) * uint32_t(perCu) : 0u;
 }
-hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, int perCu, hipStream_t st) {
+hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, int perCu, bool extra,
+                          hipStream_t st) {
   // An ordinary dispatch (capturable in a graph, and traceable by the profiler, which does not survive cooperative
   // dispatches): the host has checked that the grid fits the device at once (pcgSolveCapacity), the solver's stream
   // runs nothing beside it, and a workgroup that found no room would only make its segment's waits give up.
   launchFill(c.ctr, 0u, uint64_t(counterLines) * 64u + 1u, st);
   const dim3 grid(c.order ? v.nTiles : xcdGrid(v.nTiles));
-  if (perCu == 2)
-    hipExtLaunchKernelGGL(kPcgSolve<2>, grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
-  else  // (its registers allow one workgroup per CU)
-    hipExtLaunchKernelGGL(kPcgSolve<1>, grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
+  if (perCu == 2 && extra)
+    hipExtLaunchKernelGGL((kPcgSolve<2, true>), grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
+  else if (perCu == 2)
+    hipExtLaunchKernelGGL((kPcgSolve<2, false>), grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
+  else if (extra)  // (the one-per-CU instances' registers allow one workgroup per CU)
+    hipExtLaunchKernelGGL((kPcgSolve<1, true>), grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
+  else
+    hipExtLaunchKernelGGL((kPcgSolve<1, false>), grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
   return hipGetLastError();
 }
 void launchPcgRefine(const View& v, uint32_t maxRestarts, hipStream_t st) {

@@ -57,6 +57,8 @@ struct View {
   const uint32_t* tileSeg;
   const uint32_t* tileRange;  // per tile: its segment's first tile and end tile
   const uint32_t* tileHalo;   // per tile: first row and row count of the window holding all its columns
+  // per tile: kSolveWindows (first row, row count) ranges holding all its columns (count 0: unused), for kPcgSolve
+  const uint32_t* tileWindows;
   const SegParams* seg;
   SegState* state;
   double* partA;  // per tile
@@ -159,9 +161,11 @@ void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t s
                    hipEvent_t stop = nullptr);
 // Persistent solve (kPcgSolve): one launch runs the one-kernel PCG until every segment's solve has ended, with the
 // matrix and the rows' private data held in registers and a barrier per segment; bit-identical to launchPcgStep
-// with the 1x8 halo shape. It needs the halo window (View::tileHalo), at most 8 + 8 slots per slice, and every
-// segment's tiles resident at once (pcgSolveCapacity): all tiles of the launch, or, with SolveCtl::order, each
-// segment's (the segments then pass through the resident workgroups in that order).
+// with the 1x8 halo shape. It needs every tile's columns within kSolveWindows row ranges of kHaloRows rows in all
+// (View::tileWindows), at most 16 + 16 slots per slice (8 + 8 held in registers), and every segment's tiles resident
+// at once (pcgSolveCapacity): all tiles of the launch, or, with SolveCtl::order, each segment's (the segments then
+// pass through the resident workgroups in that order).
+constexpr uint32_t kSolveWindows = 4;  // row ranges of a tile's gathered window in the persistent solve
 struct SolveCtl {
   uint32_t* ctr;            // per segment 1 + 2 ceil(tiles / 16) lines of 64 words: top, group arrivals, generations
   const uint32_t* segCtr;   // per segment: its first counter, in units of 64 words
@@ -174,12 +178,13 @@ struct SolveCtl {
 // Counter lines a segment of `tiles` tiles needs.
 inline uint32_t solveCounterLines(uint32_t tiles) { return 1u + 2u * ((tiles + 15u) / 16u); }
 // Workgroups of kPcgSolve the device holds at once with perCu (1 or 2) workgroups per CU (0 if that many cannot be
-// resident).
-uint32_t pcgSolveCapacity(int perCu);
+// resident); extra: the instance for slices with more than 8 upper or lower slots.
+uint32_t pcgSolveCapacity(int perCu, bool extra);
 // Clears the counters and the abort flag (counterLines * 64 + 1 words at c.ctr, the flag last) and launches the
 // solve (an ordinary dispatch of xcdGrid(nTiles) workgroups; the caller checks the grid against pcgSolveCapacity).
 // Returns the launch's error code.
-hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, int perCu, hipStream_t st);
+hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, int perCu, bool extra,
+                          hipStream_t st);
 // FP64 true residual of the finished solve and, per segment, a continuation of the solve from it when it is above
 // tol r0 (at most maxRestarts per solve; see kTrueResidual).
 void launchPcgRefine(const View& v, uint32_t maxRestarts, hipStream_t st);

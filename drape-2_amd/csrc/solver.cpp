@@ -344,7 +344,7 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
     throw std::invalid_argument("hip: a 64-row slice has more than 28 blocks on or above the diagonal in one row");
   // Per tile, the window of rows [first, first + count) that holds every column of its rows and the rows
   // themselves; the halo launch shapes run only when every window fits kHaloRows (a banded matrix).
-  bool haloFits = true;
+  bool haloFits = true, windowsFit = true;
   {
     std::vector<uint32_t> rs(nRows_ + 1), ci(std::max<uint32_t>(nBlocks_, 1));
     fromDevice(rs.data(), rowStart_.as<void>(), sizeof(uint32_t) * (nRows_ + 1));
@@ -372,6 +372,43 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
       haloFits = haloFits && hi - lo + 1 <= dk::kHaloRows;
     }
     upload(tileHalo_, halo);
+    // The persistent solve's windows: per tile at most dk::kSolveWindows row ranges holding every column of its rows
+    // (and the rows themselves), kHaloRows rows in all. The tile's distinct columns, split where consecutive ones lie
+    // more than 64 rows apart, the closest ranges merged until at most kSolveWindows remain.
+    std::vector<uint32_t> win(2 * size_t(dk::kSolveWindows) * nTiles_, 0u);
+    windowsFit = true;
+    std::vector<uint32_t> cols;
+    for (uint32_t t = 0; t < nTiles_ && windowsFit; ++t) {
+      cols.clear();
+      for (uint32_t r = t * dk::kTile; r < (t + 1) * dk::kTile; ++r) {
+        cols.push_back(r);
+        for (uint32_t k = rs[r]; k < rs[r + 1]; ++k) cols.push_back(ci[k]);
+      }
+      std::sort(cols.begin(), cols.end());
+      cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+      std::vector<std::pair<uint32_t, uint32_t>> runs{{cols[0], cols[0]}};
+      for (size_t k = 1; k < cols.size(); ++k) {
+        if (cols[k] - runs.back().second <= 64u)
+          runs.back().second = cols[k];
+        else
+          runs.push_back({cols[k], cols[k]});
+      }
+      while (runs.size() > dk::kSolveWindows) {
+        size_t best = 0;
+        for (size_t k = 1; k + 1 < runs.size(); ++k)
+          if (runs[k + 1].first - runs[k].second < runs[best + 1].first - runs[best].second) best = k;
+        runs[best].second = runs[best + 1].second;
+        runs.erase(runs.begin() + long(best) + 1);
+      }
+      uint32_t total = 0;
+      for (size_t k = 0; k < runs.size(); ++k) {
+        win[2 * (dk::kSolveWindows * size_t(t) + k)] = runs[k].first;
+        win[2 * (dk::kSolveWindows * size_t(t) + k) + 1] = runs[k].second - runs[k].first + 1;
+        total += runs[k].second - runs[k].first + 1;
+      }
+      windowsFit = total <= dk::kHaloRows;
+    }
+    upload(tileWindows_, win);
   }
   tunedShape_ = -1;
   tuneSolves_ = 0;
@@ -432,6 +469,7 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   v.tileSeg = tileSeg_.as<uint32_t>();
   v.tileRange = tileRange_.as<uint32_t>();
   v.tileHalo = haloFits ? tileHalo_.as<uint32_t>() : nullptr;
+  v.tileWindows = windowsFit ? tileWindows_.as<uint32_t>() : nullptr;
   v.seg = segParams_.as<dk::SegParams>();
   v.state = segState_.as<dk::SegState>();
   v.partA = partA_.as<double>();
@@ -481,7 +519,8 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
     // queue of segments (solveQueued_) in which each segment fits it.
     // One workgroup per CU when every tile fits that way (each wavefront then has a SIMD to itself), else two.
     const uint32_t grid = 8u * ((nTiles_ + 7u) / 8u);
-    const uint32_t capacity1 = dk::pcgSolveCapacity(1), capacity2 = dk::pcgSolveCapacity(2);
+    solveExtra_ = maxK_ > 8u || maxLowK_ > 8u;
+    const uint32_t capacity1 = dk::pcgSolveCapacity(1, solveExtra_), capacity2 = dk::pcgSolveCapacity(2, solveExtra_);
     solvePerCu_ = grid <= capacity1 || capacity2 == 0 ? 1 : 2;
     const uint32_t capacity = solvePerCu_ == 1 ? capacity1 : capacity2;
     segTiles_.assign(M.segments.size(), {0u, 0u});
@@ -491,7 +530,7 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
       largest = std::max(largest, params[sg].tileEnd - params[sg].tileBegin);
     }
     solveQueued_ = grid > capacity;
-    persistentOk_ = haloFits && maxK_ <= 8u && maxLowK_ <= 8u && capacity > 0 && largest <= capacity;
+    persistentOk_ = windowsFit && maxK_ <= 16u && maxLowK_ <= 16u && capacity > 0 && largest <= capacity;
     solveOrderHost_.clear();
     segLastIters_.assign(M.segments.size(), 0u);
     if (solveQueued_) solveOrder_.allocate(sizeof(uint32_t) * nTiles_);
@@ -936,7 +975,7 @@ bool HipSolver::runPcgPersistent(bool refine, bool timeIt) {
       pr = takeEvents();
       check(hipEventRecord(pr.first, stream_), "event");
     }
-    const hipError_t e = dk::launchPcgSolve(view_, c, solveCtrLines_, solvePerCu_, stream_);
+    const hipError_t e = dk::launchPcgSolve(view_, c, solveCtrLines_, solvePerCu_, solveExtra_, stream_);
     if (e != hipSuccess) {
       (void)hipGetLastError();
       if (timeIt) eventPool_.push_back(pr);

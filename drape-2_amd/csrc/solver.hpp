@@ -58,8 +58,8 @@ class HipSolver {
   // Polls of a persistent solve's barrier before a workgroup gives up (the solve is then re-run with one launch per
   // iteration, and the persistent form is not used again by this solver).
   void setSolveSpinCap(uint32_t cap) { solveSpinCap_ = cap; }
-  // The persistent solve fits this build (banded matrix, at most 8 + 8 slots per slice, each segment's tiles
-  // resident together) and has not given up.
+  // The persistent solve fits this build (each tile's columns within kSolveWindows row ranges of kHaloRows rows in
+  // all, at most 16 + 16 slots per slice, each segment's tiles resident together) and has not given up.
   bool persistentAvailable() const { return persistentOk_; }
   uint64_t persistentAborts() const { return persistentAborts_; }
   // Every `every`-th PCG SpMV launch of the step is bracketed by HIP events (0 = off).
@@ -142,7 +142,7 @@ class HipSolver {
   mutable std::unique_ptr<nrm::Mesh> normalMesh_;  // built on the first normals request after a build
   mutable DeviceBuffer normalsOut_;
   ws::Net* net_ = nullptr;
-  DeviceBuffer tileRange_, tileHalo_;
+  DeviceBuffer tileRange_, tileHalo_, tileWindows_;
   DeviceBuffer tileSeg_, segParams_, segState_, partA_, partB_, partC_, partE_, partMax_;
   DeviceBuffer stretch_, hinges_, seams_, fabrics_, garments_;
   DeviceBuffer incCount_, incStart_, inc_, rowLen_, rowStart_, colIndex_, diagSlot_;
@@ -169,6 +169,7 @@ class HipSolver {
   bool persistentOk_ = false;
   bool solveQueued_ = false;                       // more tiles than resident workgroups: segments queue
   int solvePerCu_ = 1;                             // workgroups per CU of the persistent launch (1 or 2)
+  bool solveExtra_ = false;                        // some slice has more than 8 upper or lower slots
   DeviceBuffer solveOrder_;                        // the queue's tile order (device)
   std::vector<uint32_t> solveOrderHost_;           // the order last uploaded
   std::vector<std::pair<uint32_t, uint32_t>> segTiles_;  // per segment: first tile, end tile

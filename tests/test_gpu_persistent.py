@@ -120,12 +120,42 @@ def test_persistent_solve_two_tiles_per_cu_on_one_large_scene(hip):
     _same(p, launches)
 
 
+def test_persistent_solve_on_the_seamed_tube(hip):
+    """The 120,000-particle seamed tube: 469 tiles (two to a CU), each tile's columns in up to four row ranges (its
+    own panel's neighbourhood and the seam partners' on the other panel), seam rows with 9 upper and 9 lower slots
+    (the slots past the eighth read from memory each iteration). Three steps equal one launch per iteration bit for
+    bit."""
+    sc = scenes.c3_tube()
+    p = _run(hip, sc, PERSISTENT, 3)
+    print(f"tube, 3 steps: counters {p.counters()}, aborts {p.pcg_report()['persistent_aborts']}", flush=True)
+    assert p.persistent_available() and p.pcg_report()["persistent_aborts"] == 0
+    launches = _run(hip, sc, (1, 8, 0), 3)
+    _same(p, launches)
+
+
+def _scattered_sheet(n=64, seed=5):
+    """A sheet whose vertices are numbered in random order: every tile's columns scatter over the whole matrix."""
+    sc = scenes.c2_sheet(n=n, spacing=0.01)
+    g = sc.garments[0]
+    m = g.mesh
+    perm = np.random.default_rng(seed).permutation(m.n)  # new index of vertex i is inv[i]
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(m.n)
+    m.rest = m.rest[perm]
+    m.vertex_area = m.vertex_area[perm]
+    m.boundary = m.boundary[perm]
+    m.vertex_piece = m.vertex_piece[perm]
+    m.triangles = inv[m.triangles].astype(np.uint32)
+    g.initial_positions = g.initial_positions[perm]
+    return sc
+
+
 def test_persistent_solve_is_refused_where_it_does_not_fit(hip):
-    """The seamed tube's tiles reach across the panels (no halo window), and a 384 x 384 sheet is one scene of 576
-    tiles, more than the device holds at two a CU (a scene's tiles wait for each other, so they must all be
-    resident): the persistent form is refused before anything is launched, and a tuned solver never picks it
-    there."""
-    for sc in (scenes.c3_tube(nx=40, ny=60), scenes.c2_sheet(n=384)):
+    """A sheet numbered in random order has tiles whose columns need more than four row ranges or 1,536 rows, and a
+    384 x 384 sheet is one scene of 576 tiles, more than the device holds at two a CU (a scene's tiles wait for each
+    other, so they must all be resident): the persistent form is refused before anything is launched, and a tuned
+    solver never picks it there."""
+    for sc in (_scattered_sheet(), scenes.c2_sheet(n=384)):
         g = hip.HipSolver(pcg_shape=PERSISTENT)
         g.build(sc)
         assert not g.persistent_available()
