@@ -447,7 +447,7 @@ def test_pcg_launch_shapes_bit_identical_beyond_256_tiles(hip):
 def test_pcg_halo_shape_needs_a_banded_matrix(hip):
     """The tube's seams tie each row of the front panel to a row of the back panel 2,400 rows away, so its tiles'
     column windows exceed the LDS window: the halo shape is refused before anything is launched, and a tuned
-    solver never tries it there."""
+    solver never picks it there (it may pick the persistent solve, whose window takes up to four row ranges)."""
     sc = scenes.c3_tube(nx=40, ny=60)
     g = hip.HipSolver(pcg_shape=(1, 8, True))
     g.build(sc)
@@ -458,7 +458,7 @@ def test_pcg_halo_shape_needs_a_banded_matrix(hip):
     for _ in range(20):
         t.step(1 / 60, 1, 5)
     print(f"tuned shape on the tube: {t.pcg_shape()}", flush=True)
-    assert not t.pcg_shape()[2]
+    assert t.pcg_shape()[2] != 1
 
 
 def test_sheet_drop_contact_phase_stays_bounded(hip):
