@@ -324,6 +324,53 @@ def tube_bench(steps: int, shape: tuple, graph: bool = True) -> dict:
     }
 
 
+def newton_curve(net_path: str, steps: int = 60, every: int = 3, caps=(1, 2, 3, 5, 8, 12, 20, 30),
+                 ref_iterations: int = 40) -> dict:
+    """How far a step is from the converged solve after k Newton iterations. The sheet drop runs with the plain
+    guess (Newton <= 5); every `every`-th step from step 3, its state is restored into a plain-guess and a learned-
+    guess solver, each steps once with Newton capped at every k of `caps`, and the max particle distance (mm) to the
+    Newton <= `ref_iterations` plain solve from the same state is recorded. Restoring a snapshot clears the realised
+    dv the network reads, so the learned guess here sees dv_prev = 0."""
+    from drape_amd import HipSolver, learn, scenes
+    net = learn.WarmStartNet()
+    net.load(net_path)
+    sc = scenes.c2_sheet()
+    run, ref, plain, learned = (HipSolver() for _ in range(4))
+    for s in (run, ref, plain, learned):
+        s.build(sc)
+    learned.set_warm_start(net)
+    rows = []
+    for i in range(steps + 2):
+        if i >= 2 and (i - 2) % every == 0:
+            snap = run.snapshot()
+            ref.restore(snap)
+            ref.reset_counters()
+            ref.step(1 / 60, 1, ref_iterations)
+            x_ref = ref.read_positions().astype(np.float64)
+            row = {"step": i + 1, "ref_newton": ref.counters()["newton_iterations"]}
+            for name, s in (("plain", plain), ("learned", learned)):
+                errs = []
+                for k in caps:
+                    s.restore(snap)
+                    s.reset_counters()
+                    s.step(1 / 60, 1, k)
+                    errs.append(round(float(np.linalg.norm(s.read_positions() - x_ref, axis=1).max()) * 1e3, 4))
+                row[name] = errs
+                if name == "learned":
+                    row["learned_guess_kept"] = bool(s.counters()["warm_start_accepted"])
+            rows.append(row)
+            print(json.dumps(row), file=sys.stderr, flush=True)
+        run.step(1 / 60, 1, 5)
+    summary = {}
+    for name in ("plain", "learned"):
+        summary[name] = {str(k): {"within_0p1mm": sum(r[name][j] <= 0.1 for r in rows),
+                                  "median_max_mm": float(np.median([r[name][j] for r in rows]))}
+                         for j, k in enumerate(caps)}
+    return {"metric": "max distance to the converged solve after k Newton iterations, sheet drop",
+            "unit": "mm", "caps": list(caps), "reference_newton_cap": ref_iterations, "checked_steps": len(rows),
+            "summary": summary, "steps": rows}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -352,6 +399,8 @@ def main() -> int:
                     help="instead of the sheet benchmark: the seamed tube settling on the mannequin (--steps steps)")
     ap.add_argument("--batch", action="store_true",
                     help="instead of the sheet benchmark: the 64-sheet batch (one solver, one launch per kernel)")
+    ap.add_argument("--newton-curve", action="store_true",
+                    help="instead of the sheet benchmark: distance to the converged solve after k Newton iterations")
     args = ap.parse_args()
     spec = args.pcg_shape.lower()
     shape = tuple(int(t) for t in spec.rstrip("hp").split("x")) + (2 if spec.endswith("p") else int(spec.endswith("h")),)
@@ -364,6 +413,9 @@ def main() -> int:
         return 0
     if args.tee:
         print(json.dumps(tee_bench(args.tee, args.no_cpu_baseline)), flush=True)
+        return 0
+    if args.newton_curve:
+        print(json.dumps(newton_curve(args.warm_start_weights)), flush=True)
         return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
@@ -450,7 +502,8 @@ def main() -> int:
         "pcg": {"solves": rep["solves"], "exit_tolerance": rep["exit_tolerance"], "exit_cap": rep["exit_cap"],
                 "exit_curvature": rep["exit_curvature"], "refinement_restarts": rep["restarts"],
                 "true_rel_residual_mean": rep["true_rel_mean"], "true_rel_residual_max": rep["true_rel_max"],
-                "true_rel_residual_max_tolerance_exits": rep["true_rel_max_converged"]},
+                "true_rel_residual_max_tolerance_exits": rep["true_rel_max_converged"],
+                "true_rel_over_tol_or_fp32_floor_max": rep["true_rel_excess_max"]},
         "roofline": {
             "kernel": (fig or KERNEL_FIGURES[(1, 8, 0)])["kernel"] if args.pcg_variant == 1 else
                       "kPcgSpmv (fused p = z + beta p, SELL-64 block SpMV, contact filter, FP64 p.Ap partials)",

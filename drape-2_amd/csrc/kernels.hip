@@ -183,9 +183,15 @@ __device__ inline F3 sellRowProduct(const View& v, uint32_t slice, uint32_t lane
 struct D3 {
   double x, y, z;
 };
-template <class Gather>
-__device__ inline D3 sellRowProduct64(const View& v, uint32_t slice, uint32_t lane, Gather gather) {
+// With Sq, *sq also accumulates the square of every product term (b x)^2 of the row's three sums.
+template <bool Sq = false, class Gather>
+__device__ inline D3 sellRowProduct64(const View& v, uint32_t slice, uint32_t lane, Gather gather,
+                                      double* sq = nullptr) {
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  auto squares = [&](const float* b, F3 x, int i0, int i1, int i2) {
+    const double p0 = double(b[i0]) * x.x, p1 = double(b[i1]) * x.y, p2 = double(b[i2]) * x.z;
+    return p0 * p0 + p1 * p1 + p2 * p2;
+  };
   if (v.lowK) {
     const uint32_t KL = v.lowK[slice];
     const uint32_t lb = v.lowBase[slice] + lane;
@@ -196,6 +202,7 @@ __device__ inline D3 sellRowProduct64(const View& v, uint32_t slice, uint32_t la
       a0 += double(b[0]) * x.x + double(b[3]) * x.y + double(b[6]) * x.z;
       a1 += double(b[1]) * x.x + double(b[4]) * x.y + double(b[7]) * x.z;
       a2 += double(b[2]) * x.x + double(b[5]) * x.y + double(b[8]) * x.z;
+      if constexpr (Sq) *sq += squares(b, x, 0, 3, 6) + squares(b, x, 1, 4, 7) + squares(b, x, 2, 5, 8);
     }
   }
   const uint32_t K = v.sellK[slice];
@@ -208,6 +215,7 @@ __device__ inline D3 sellRowProduct64(const View& v, uint32_t slice, uint32_t la
     a0 += double(b[0]) * x.x + double(b[1]) * x.y + double(b[2]) * x.z;
     a1 += double(b[3]) * x.x + double(b[4]) * x.y + double(b[5]) * x.z;
     a2 += double(b[6]) * x.x + double(b[7]) * x.y + double(b[8]) * x.z;
+    if constexpr (Sq) *sq += squares(b, x, 0, 1, 2) + squares(b, x, 3, 4, 5) + squares(b, x, 6, 7, 8);
   }
   return D3{a0, a1, a2};
 }
@@ -1250,17 +1258,20 @@ kPcgStep(View v, uint32_t q) {
 // k-th kPcgStep<1, 8, true> launch computes (same expressions, same summation orders), so the two forms give
 // bit-identical results.
 //
-// One workgroup owns one 256-row tile for the whole solve, one wavefront per 64-row slice, one wavefront per SIMD
-// (the kernel's registers allow no second one). The slice's matrix (upper block values and columns, the lower
-// slots' transposed blocks and columns; every slice has at most 8 + 8 slots) and its rows' private PCG data (r, ap,
+// One workgroup owns one 256-row tile for the whole solve, one wavefront per 64-row slice. PerCu = 1 gives each
+// wavefront a SIMD to itself and holds the gathered window in registers; PerCu = 2 puts two workgroups on a CU
+// within 256 registers and stages the window in LDS with buffer loads to LDS. The slice's matrix (upper block values
+// and columns, the lower slots' transposed blocks and columns: the first 8 + 8 slots; with Extra, a seam row's slots
+// past the eighth, at most 16 + 16, are read from memory every iteration) and its rows' private PCG data (r, ap,
 // step, the preconditioner, the row and contact flags) are loaded once and stay in registers. An iteration moves only
-// the (zr, t, p) window the tile gathers from, the tile's own new (zr, t, p) and 7 FP64 partial sums: about 80 B per
-// row against kPcgStep's 500.
+// the (zr, t, p) window the tile gathers from (up to kSolveWindows row ranges), the tile's own new (zr, t, p) and 7
+// FP64 partial sums: about 80 B per row against kPcgStep's 500.
 //
 // Residency. A segment's tiles wait for each other every iteration, so all of them must be resident together. The
-// host launches either every tile at once (at most one per CU), or, when the tiles outnumber the CUs, a queue:
-// workgroup i takes the tile order[i], each segment's tiles consecutive. Workgroups are dispatched in order, round-
-// robin over the eight XCDs, so a segment of at most 256 tiles has at most 32 on any XCD, and the earliest segment
+// host launches either every tile at once (one or two per CU), or, when the tiles outnumber the resident
+// workgroups, a queue: workgroup i takes the tile order[i], each segment's tiles consecutive. Workgroups are
+// dispatched in order, round-robin over the eight XCDs, so a segment of at most 512 tiles has at most 64 on any XCD
+// (32 CUs, two each), and the earliest segment
 // not yet finished has all its tiles resident (everything dispatched before it has left): it finishes and frees its
 // CUs for the segments behind it. No wait relies on this for termination: every wait is bounded (below).
 //
@@ -1675,7 +1686,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
 // converged can be short of |b - A step| <= tol |r0| (the reference's FP64 solve meets its test on a residual that
 // stays close to the true one, cpu_solver.cpp:188-204). Per row: r = S(-grad - A step) from an FP64 product, and the
 // continuation's first state written as the output of a kernel "-1" (r, zr = S M r, p = ap = t = 0; step kept);
-// per tile: |r|^2 and the partial sums the next kPcgStep reads.
+// per tile: |r|^2 and the partial sums the next kPcgStep reads, and in partC the rows' sum of (A_ij step_j)^2 over
+// every product term, from which kRefineDecide estimates the part of |r| that rounding step to FP32 leaves.
 __global__ void __launch_bounds__(256) kTrueResidual(View v) {
   __shared__ double lds[kPcgParts * 4];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
@@ -1687,12 +1699,14 @@ __global__ void __launch_bounds__(256) kTrueResidual(View v) {
   const uint32_t slice = tile * kSlicesPerTile + wave;
   const uint32_t row = slice * kSlice + lane;
   const F3* __restrict__ stp = v.step;
-  const D3 as = sellRowProduct64(v, slice, lane, [&](uint32_t c) { return stp[c]; });
+  double sq = 0.0;
+  const D3 as = sellRowProduct64<true>(v, slice, lane, [&](uint32_t c) { return stp[c]; }, &sq);
   const F3 g = v.grad[row];
   const F3 resid = f3(float(-double(g.x) - as.x), float(-double(g.y) - as.y), float(-double(g.z) - as.z));
   const uint32_t cf = v.cflags[row];
   const float4 plane = v.cnormal[row];
-  const F3 r = (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : onPlane(cf, plane, resid);
+  const bool pinned = v.rowFlags[row] & kRowPinned;
+  const F3 r = pinned ? f3(0, 0, 0) : onPlane(cf, plane, resid);
   const F3 zr = applyPrecond(v.precond + kPrecondFloats * row, cf, plane, r);
   const F3 zero = f3(0, 0, 0);
   v.cgR[1][row] = r;
@@ -1701,21 +1715,30 @@ __global__ void __launch_bounds__(256) kTrueResidual(View v) {
   v.cgZr[1][row] = zr;
   v.cgT[1][row] = zero;
   double d[kPcgParts] = {0.0, dot64(r, zr), 0.0, 0.0, 0.0, 0.0, dot64(r, r)};
+  double f[1] = {pinned ? 0.0 : sq};
   blockSumN<kPcgParts, 4>(d, lds);
+  blockSumN<1, 4>(f, lds);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < kPcgParts; ++i) v.cgPart[1][kPcgParts * tile + i] = d[i];
     v.partB[tile] = d[kPrR];
+    v.partC[tile] = f[0];
   }
 }
 
 // A continuation pays when the recursion has drifted well away from the true residual, not for a true residual a
 // few percent above the tolerance (FP32 recursions end there routinely): it starts above kRefineFactor tol r0.
+// Nor does it pay once the true residual is down to what rounding step to FP32 leaves: step is stored in FP32, and
+// each component's rounding error, uniform within half a unit in the last place (2^-24 |step_j| relative), puts
+// about 2^-24 sqrt(sum (A_ij step_j)^2 / 3) into |b - A step| however exact the solve (the sum over every product
+// term of every row). Continuations go on while the true residual is above kRefineFactor times that floor as well;
+// below it, a continuation moves step by a fraction of its FP32 rounding.
 constexpr double kRefineFactor = 4.0;
 
 // Per segment: a solve that stopped on the recursion's tolerance test continues from its true residual when that
-// is above kRefineFactor tol r0 and iterations remain (at most `maxRestarts` continuations per solve). Every tile of the segment
-// gets its solve-ended flag for the next kPcgStep; the host looks at pcgConverged afterwards.
+// is above kRefineFactor max(tol r0, the FP32 rounding floor) and iterations remain (at most `maxRestarts`
+// continuations per solve). Every tile of the segment gets its solve-ended flag for the next kPcgStep; the host
+// looks at pcgConverged afterwards.
 __global__ void __launch_bounds__(256) kRefineDecide(View v, uint32_t maxRestarts) {
   __shared__ double lds[8];
   const uint32_t s = blockIdx.x;
@@ -1723,8 +1746,10 @@ __global__ void __launch_bounds__(256) kRefineDecide(View v, uint32_t maxRestart
   const SegParams& P = v.seg[s];
   const bool candidate = !S->newtonDone && S->pcgExit == kPcgExitTolerance;
   const double trueSq = candidate ? rangeSum256(v.partB, P.tileBegin, P.tileEnd, lds) : 0.0;
-  const bool restart = candidate && sqrt(trueSq) > kRefineFactor * P.cgTolerance * S->r0 && S->pcgIter < P.maxCg &&
-                       S->restartsThisSolve < maxRestarts;
+  const double termSq = candidate ? rangeSum256(v.partC, P.tileBegin, P.tileEnd, lds) : 0.0;
+  const double floor32 = 0x1p-24 * sqrt(termSq / 3.0);
+  const bool restart = candidate && sqrt(trueSq) > kRefineFactor * fmax(P.cgTolerance * S->r0, floor32) &&
+                       S->pcgIter < P.maxCg && S->restartsThisSolve < maxRestarts;
   for (uint32_t t = P.tileBegin + threadIdx.x; t < P.tileEnd; t += 256u) v.tileDone[0][t] = restart ? 0u : 1u;
   if (threadIdx.x == 0) {
     S->restartActive = restart ? 1u : 0u;
@@ -1740,8 +1765,8 @@ __global__ void __launch_bounds__(256) kRefineDecide(View v, uint32_t maxRestart
 
 // After the solve, from one product A step over every row: each held particle's normal force from the unfiltered
 // residual -grad - A step (cpu_solver.cpp:210-212: the reference keeps that residual as a running vector; here it is
-// formed explicitly), the FP64 norm partials of the filtered true residual S(-grad - A step) (pinned rows excluded)
-// for the solve report, and the Newton slope grad . step partials.
+// formed explicitly), the FP64 norm partials of the filtered true residual S(-grad - A step) and of its FP32 rounding
+// floor's terms (pinned rows excluded) for the solve report, and the Newton slope grad . step partials.
 __global__ void __launch_bounds__(256) kContactForceSlope(View v) {
   __shared__ double lds[8];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
@@ -1755,16 +1780,20 @@ __global__ void __launch_bounds__(256) kContactForceSlope(View v) {
   const F3 gi = v.grad[row];
   const F3 si = v.step[row];
   const F3* __restrict__ stp = v.step;
-  const D3 as = sellRowProduct64(v, slice, lane, [&](uint32_t c) { return stp[c]; });
+  double sq = 0.0;
+  const D3 as = sellRowProduct64<true>(v, slice, lane, [&](uint32_t c) { return stp[c]; }, &sq);
   const F3 residual = f3(float(-double(gi.x) - as.x), float(-double(gi.y) - as.y), float(-double(gi.z) - as.z));
   const float4 plane = v.cnormal[row];
   v.cforce[row] = (cf & kContactActive) ? fmaxf(0.0f, -dot(residual, f3(plane.x, plane.y, plane.z))) : 0.0f;
-  const F3 filtered = (v.rowFlags[row] & kRowPinned) ? f3(0, 0, 0) : onPlane(cf, plane, residual);
+  const bool pinned = v.rowFlags[row] & kRowPinned;
+  const F3 filtered = pinned ? f3(0, 0, 0) : onPlane(cf, plane, residual);
   const double slope = blockSum256(dot64(gi, si), lds);
   const double trueSq = blockSum256(dot64(filtered, filtered), lds);
+  const double termSq = blockSum256(pinned ? 0.0 : sq, lds);
   if (threadIdx.x == 0) {
     v.partA[tile] = slope;
     v.partB[tile] = trueSq;
+    v.partC[tile] = termSq;
   }
 }
 
@@ -1778,9 +1807,11 @@ __global__ void __launch_bounds__(256) kNewtonSlope(View v) {
   const SegParams& P = v.seg[s];
   const double slope = rangeSum256(v.partA, P.tileBegin, P.tileEnd, lds);
   const double trueSq = rangeSum256(v.partB, P.tileBegin, P.tileEnd, lds);
+  const double termSq = rangeSum256(v.partC, P.tileBegin, P.tileEnd, lds);
   if (threadIdx.x == 0) {
     S->slope = slope;
     const double rel = S->r0 > 0.0 ? sqrt(trueSq) / S->r0 : 0.0;
+    const double floorRel = S->r0 > 0.0 ? 0x1p-24 * sqrt(termSq / 3.0) / S->r0 : 0.0;
     S->cntSolves += 1u;
     S->cntExitTolerance += S->pcgExit == kPcgExitTolerance ? 1u : 0u;
     S->cntExitCap += S->pcgExit == kPcgExitCap ? 1u : 0u;
@@ -1789,7 +1820,10 @@ __global__ void __launch_bounds__(256) kNewtonSlope(View v) {
     S->trueRelLast = rel;
     S->trueRelSum += rel;
     S->trueRelMax = fmax(S->trueRelMax, rel);
-    if (S->pcgExit == kPcgExitTolerance) S->trueRelMaxConverged = fmax(S->trueRelMaxConverged, rel);
+    if (S->pcgExit == kPcgExitTolerance) {
+      S->trueRelMaxConverged = fmax(S->trueRelMaxConverged, rel);
+      S->trueExcessMax = fmax(S->trueExcessMax, rel / fmax(P.cgTolerance, floorRel));
+    }
     if (!(slope < 0.0)) {
       S->newtonDone = 1u;
     } else {

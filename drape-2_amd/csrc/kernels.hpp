@@ -63,7 +63,7 @@ struct View {
   SegState* state;
   double* partA;  // per tile
   double* partB;  // per tile
-  double* partC;  // per tile
+  double* partC;  // per tile (also the true residual's squared product terms, kTrueResidual -> kRefineDecide)
   double* partE;  // per slice
   float* partMax; // per tile
   // terms

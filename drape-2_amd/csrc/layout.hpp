@@ -160,6 +160,9 @@ struct SegState {
   uint32_t restartActive, kBase, restartsThisSolve, pad3;
   uint64_t cntSolves, cntExitTolerance, cntExitCap, cntExitCurvature, cntExitZero, cntRestarts;
   double trueRelMax, trueRelSum, trueRelLast, trueRelMaxConverged;  // the last: over solves ended on the tolerance test
+  // Over solves ended on the tolerance test: the largest true residual over max(tol |r0|, the part of it that
+  // rounding step to FP32 leaves, estimated as in kRefineDecide).
+  double trueExcessMax;
 };
 
 // Why a PCG solve ended (cpu_solver.cpp:188-204 stopping rules).

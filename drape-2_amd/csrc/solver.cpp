@@ -499,8 +499,8 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   v.lowCol = lowCol_.as<uint32_t>();
   v.lowBlk = lowBlk_.as<uint32_t>();
   v.fieldValues = fieldValues_.as<float>();
-  // Persistent solve: a banded matrix (the halo window), at most 8 + 8 slots per slice, and every tile's workgroup
-  // resident at once; counters per segment for its barrier.
+  // Persistent solve: every tile's columns in at most four row ranges (kHaloRows rows in all), at most 16 + 16 slots
+  // per slice, and each segment's tiles resident together; counters per segment for its barrier.
   {
     std::vector<uint32_t> segCtr(std::max<size_t>(M.segments.size(), 1), 0u);
     uint32_t lines = 0;
@@ -1067,8 +1067,9 @@ void HipSolver::runPcgSingleKernel() {
   if (shapeIdx >= 0 && !shapeAvailable(shapeIdx)) {
     if (dk::kPcgShapes[shapeIdx].persistent)
       throw std::invalid_argument(
-          "PCG persistent solve: needs a banded matrix (every 256-row tile within 1536 rows), at most 8 + 8 slots per "
-          "64-row slice and one resident workgroup per tile, and must not have given up before");
+          "PCG persistent solve: needs every 256-row tile's columns within four row ranges of 1536 rows in all, at "
+          "most 16 + 16 slots per 64-row slice, each scene's tiles resident together (512 per scene), and must not "
+          "have given up before");
     throw std::invalid_argument("PCG halo shape: a 256-row tile of this matrix reaches more than 1536 rows");
   }
   dk::launchPcgStepInit(view_, stream_);
@@ -1515,6 +1516,7 @@ drape_pcg_report HipSolver::pcgReport() const {
     r.true_rel_max = std::max(r.true_rel_max, st.trueRelMax);
     r.true_rel_last = std::max(r.true_rel_last, st.trueRelLast);
     r.true_rel_max_converged = std::max(r.true_rel_max_converged, st.trueRelMaxConverged);
+    r.true_rel_excess_max = std::max(r.true_rel_excess_max, st.trueExcessMax);
     sum += st.trueRelSum;
   }
   r.true_rel_mean = r.solves ? sum / double(r.solves) : 0.0;
@@ -1529,7 +1531,7 @@ void HipSolver::resetCounters() {
     dk::SegState& st = hostStates_[s];
     st.cntSubsteps = st.cntNewton = st.cntPcg = st.cntLs = st.cntWarmRejected = st.cntWarmAccepted = 0;
     st.cntSolves = st.cntExitTolerance = st.cntExitCap = st.cntExitCurvature = st.cntExitZero = st.cntRestarts = 0;
-    st.trueRelMax = st.trueRelSum = st.trueRelLast = st.trueRelMaxConverged = 0.0;
+    st.trueRelMax = st.trueRelSum = st.trueRelLast = st.trueRelMaxConverged = st.trueExcessMax = 0.0;
   }
   check(hipMemcpyAsync(segState_.as<void>(), hostStates_, sizeof(dk::SegState) * model_.segments.size(),
                        hipMemcpyHostToDevice, stream_),

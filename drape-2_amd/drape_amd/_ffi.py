@@ -147,7 +147,7 @@ class PcgReport(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "solves", "exit_tolerance", "exit_cap", "exit_curvature", "exit_zero", "restarts")] + [
         (n, C.c_double) for n in ("true_rel_max", "true_rel_mean", "true_rel_last", "true_rel_max_converged")] + [
-        ("persistent_aborts", C.c_uint64)]
+        ("persistent_aborts", C.c_uint64), ("true_rel_excess_max", C.c_double)]
 
     def as_dict(self) -> dict:
         return {n: (int(v) if isinstance(v, int) else float(v)) for n, _ in self._fields_ for v in [getattr(self, n)]}

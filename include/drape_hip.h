@@ -214,6 +214,11 @@ typedef struct drape_pcg_report {
   double true_rel_last; /* of the last solve (largest over segments) */
   double true_rel_max_converged; /* largest over the solves that ended on the tolerance test */
   uint64_t persistent_aborts;    /* persistent solves whose barrier wait gave up and were re-run (halo = 2 below) */
+  /* Over the solves that ended on the tolerance test: the largest true relative residual divided by max(tol, its
+   * FP32 rounding floor). The floor is what storing step in FP32 leaves in |S(-grad - A step)| however exact the
+   * solve: each component's rounding error, uniform within half a unit in the last place, gives an expected
+   * 2^-24 sqrt(sum (A_ij step_j)^2 / 3) over every product term, relative to |r0|. */
+  double true_rel_excess_max;
 } drape_pcg_report;
 int drape_solver_pcg_report(const drape_solver* s, drape_pcg_report* out);
 int drape_solver_reset_counters(drape_solver* s);
@@ -228,8 +233,9 @@ int drape_solver_set_graph(drape_solver* s, int32_t enabled);
 int drape_solver_set_pcg_variant(drape_solver* s, int32_t variant);
 /* FP64 true-residual refinement of the one-kernel PCG (no counterpart in the reference, whose CG runs in FP64):
  * after a solve stops on the FP32 recursion's test |r| <= tol |r0|, its true residual S(-grad - A step) is formed
- * with FP64 accumulation, and a solve whose true residual is still above tol |r0| continues from it (same r0, same
- * iteration cap), at most `restarts` times per solve. 0 turns it off; default 4. */
+ * with FP64 accumulation, and a solve whose true residual is still above 4 max(tol |r0|, its FP32 rounding floor;
+ * drape_pcg_report.true_rel_excess_max) continues from it (same r0, same iteration cap), at most `restarts` times
+ * per solve. 0 turns it off; default 4. */
 int drape_solver_set_refinement(drape_solver* s, int32_t restarts);
 /* Launch shape of the one-kernel PCG iteration: `waves` wavefronts per 64-row slice and `slots` lower and upper
  * matrix slots per batch; 1x8, 1x4 (smaller register footprint: two workgroups per CU, for launches with many more
@@ -239,8 +245,10 @@ int drape_solver_set_refinement(drape_solver* s, int32_t restarts);
  * tuner tries it only there. halo = 2 (with 1x8) is the persistent solve: one cooperative launch runs a whole PCG
  * solve, each 256-row tile's workgroup keeping its slices' matrix and private vectors in registers, the tiles of a
  * scene meeting at a barrier of their own each iteration (scenes of a batch never wait for each other); it needs
- * what halo = 1 needs, at most 8 + 8 matrix slots per 64-row slice and one resident workgroup per tile (at most
- * one tile per CU: 65,536 particle rows on MI355X). If a barrier wait gives up (drape_solver_pcg_report counts
+ * every 256-row tile's columns within at most four row ranges of 1,536 rows in all, at most 16 + 16 matrix slots
+ * per 64-row slice, and every scene's tiles resident together (two workgroups per CU: at most 512 tiles, 131,072
+ * particle rows, per scene on MI355X; a batch's scenes pass through the CUs one after another when their tiles
+ * outnumber that). If a barrier wait gives up (drape_solver_pcg_report counts
  * it), the solve is re-run with one launch per iteration and the solver stops using the persistent form. Results
  * are bit-identical for every shape. drape_solver_pcg_shape reports the shape in use (0x0 while still tuning). No
  * reference counterpart: the reference's GPU path has one thread per row (kernels.h:272-289). */

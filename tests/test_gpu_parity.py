@@ -691,9 +691,10 @@ def test_reference_default_settings_parity_60_frames(hip, oracle, name):
 
 def test_true_residual_refinement_meets_the_tolerance_on_the_true_residual(hip):
     """The FP32 recursion stops PCG on |r| <= tol |r0| while the true residual S(-grad - A step) may have drifted
-    far above it (the 65k drop's contact phase, steps 11-22: up to 9e-4 against tol 1e-6). Continuing such solves
-    from their FP64 true residual brings the worst and the mean true residual down by more than tenfold (measured:
-    9.1e-4 -> 4.0e-5, 1.1e-4 -> 5.6e-6 with at most 8 continuations per solve)."""
+    far above it (the 65k drop, steps 1-22: up to 9e-4 against tol 1e-6). Continuing such solves from their FP64
+    true residual ends every solve within 4 times the larger of tol |r0| and the residual's FP32 rounding floor
+    (what storing step in FP32 leaves however exact the solve; in free fall it is 40-60 tol, so the refinement stops
+    there). Without refinement solves end far above both."""
     def run(restarts):
         g = hip.HipSolver()
         g.build(scenes.c2_sheet())
@@ -705,9 +706,9 @@ def test_true_residual_refinement_meets_the_tolerance_on_the_true_residual(hip):
     on, x_on = run(8)
     print(f"refinement off: {off}\nrefinement on: {on}", flush=True)
     assert off["restarts"] == 0 and on["restarts"] > 0
-    assert off["true_rel_max_converged"] > 4e-6
-    assert on["true_rel_max_converged"] < off["true_rel_max_converged"] / 10
-    assert on["true_rel_mean"] < off["true_rel_mean"] / 10
+    assert off["true_rel_max_converged"] > 4e-6 and off["true_rel_excess_max"] > 8
+    assert on["true_rel_excess_max"] <= 4
+    assert on["true_rel_max_converged"] < off["true_rel_max_converged"] / 5
     assert np.all(np.isfinite(x_on))
 
 
