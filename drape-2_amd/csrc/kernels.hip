@@ -725,7 +725,8 @@ __global__ void __launch_bounds__(256) kPcgUpdate(View v, uint32_t k, const F3* 
 // issued before the segment reduction.
 
 namespace {
-constexpr int kPcgParts = 7;  // per tile: p.ap, r.zr, r.t, ap.t, r.ap, ap.ap, r.r
+constexpr int kPcgParts = 7;  // per tile: p.ap, r.zr, r.t, ap.t, r.ap, ap.ap, r.r (kPartStride doubles apart)
+static_assert(kPcgParts <= int(kPartStride), "a tile's partial sums overlap the next tile's");
 enum : int { kPpAp = 0, kPrZ, kPrT, kPapT, kPrAp, kPapAp, kPrR };
 }  // namespace
 
@@ -849,7 +850,7 @@ __global__ void __launch_bounds__(256) kPcgStepInit(View v) {
   blockSumN<kPcgParts, 4>(d, lds);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int i = 0; i < kPcgParts; ++i) v.cgPart[1][kPcgParts * tile + i] = d[i];
+    for (int i = 0; i < kPcgParts; ++i) v.cgPart[1][kPartStride * tile + i] = d[i];
 }
 
 // A vector store the kernel never reads back (the next launch reads it from beyond the L2): marked nontemporal.
@@ -1004,7 +1005,7 @@ kPcgStep(View v, uint32_t q) {
     // zeros: the sums round the same way in every shape.
     ti = threadIdx.x < 256u ? tileBegin + threadIdx.x : tileEnd;
 #pragma unroll
-    for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < tileEnd ? prev[kPcgParts * ti + n] : 0.0;
+    for (int n = 0; n < kPcgParts; ++n) sum[n] = ti < tileEnd ? prev[kPartStride * ti + n] : 0.0;
     __builtin_amdgcn_sched_barrier(0);
   };
   if constexpr (U) {
@@ -1098,7 +1099,7 @@ kPcgStep(View v, uint32_t q) {
   // Scalars of this kernel (identical in every workgroup of the segment), while the gathers are in flight.
   for (ti += 256u; ti < tileEnd; ti += 256u)
 #pragma unroll
-    for (int n = 0; n < kPcgParts; ++n) sum[n] += prev[kPcgParts * ti + n];
+    for (int n = 0; n < kPcgParts; ++n) sum[n] += prev[kPartStride * ti + n];
   blockSumN<kPcgParts, NW, true>(sum, red);
   uint32_t conv = 0u;
   bool update = false;
@@ -1249,7 +1250,7 @@ kPcgStep(View v, uint32_t q) {
   blockSumN<kPcgParts, NW, true>(d, red2);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int i = 0; i < kPcgParts; ++i) par(v.cgPart, q)[kPcgParts * tile + i] = d[i];
+    for (int i = 0; i < kPcgParts; ++i) par(v.cgPart, q)[kPartStride * tile + i] = d[i];
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -1315,7 +1316,7 @@ __device__ inline double asDouble(uint32_t lo, uint32_t hi) {
   return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
 }
 __device__ inline void loadPartsDevice(__amdgpu_buffer_rsrc_t r, uint32_t tile, double (&x)[7]) {
-  const uint32_t off = tile * 56u;
+  const uint32_t off = tile * (8u * kPartStride);
   const U4 a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, kScDevice);
   const U4 b = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u, 0, kScDevice);
   const U4 c = __builtin_amdgcn_raw_buffer_load_b128(r, off + 32u, 0, kScDevice);
@@ -1646,7 +1647,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
       double x = d[0];
 #pragma unroll
       for (int n = 1; n < kPcgParts; ++n) x = threadIdx.x == uint32_t(n) ? d[n] : x;
-      storeDevice(v.cgPart[q] + kPcgParts * tile + threadIdx.x, x);
+      storeDevice(v.cgPart[q] + kPartStride * tile + threadIdx.x, x);
     }
     oR = rk;
     oAp = apk;
@@ -1720,7 +1721,7 @@ __global__ void __launch_bounds__(256) kTrueResidual(View v) {
   blockSumN<1, 4>(f, lds);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < kPcgParts; ++i) v.cgPart[1][kPcgParts * tile + i] = d[i];
+    for (int i = 0; i < kPcgParts; ++i) v.cgPart[1][kPartStride * tile + i] = d[i];
     v.partB[tile] = d[kPrR];
     v.partC[tile] = f[0];
   }

@@ -46,7 +46,7 @@ struct View {
   F3* cgP[2];
   F3* cgZr[2];
   F3* cgT[2];
-  double* cgPart[2];
+  double* cgPart[2];  // per tile: the PCG's FP64 partial sums, kPartStride doubles apart
   uint32_t* tileDone[2];  // per tile, by kernel parity: 1 once the tile's PCG solve has ended (kPcgStep)
   float* precond;  // kPrecondFloats per row: the symmetric block-Jacobi inverse's upper triangle (00 01 02 11 12 22)
   float4* cnormal; // xyz normal, w = offset r - phi(xStart)
@@ -165,6 +165,8 @@ void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t s
 // (View::tileWindows), at most 16 + 16 slots per slice (8 + 8 held in registers), and every segment's tiles resident
 // at once (pcgSolveCapacity): all tiles of the launch, or, with SolveCtl::order, each segment's (the segments then
 // pass through the resident workgroups in that order).
+// Distance in doubles between consecutive tiles' PCG partial sums (View::cgPart).
+constexpr uint32_t kPartStride = 7;
 constexpr uint32_t kSolveWindows = 4;  // row ranges of a tile's gathered window in the persistent solve
 struct SolveCtl {
   uint32_t* ctr;            // per segment 1 + 2 ceil(tiles / 16) lines of 64 words: top, group arrivals, generations

@@ -261,8 +261,8 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   statsTile_.allocate(sizeof(double) * dk::kStatsPerSegment * nTiles_);
   statsSeg_.allocate(sizeof(double) * dk::kStatsPerSegment * std::max<size_t>(M.segments.size(), 1));
   tileDone_.allocate(sizeof(uint32_t) * 2 * nTiles_);
-  cgPart0_.allocate(sizeof(double) * 7 * nTiles_);
-  cgPart1_.allocate(sizeof(double) * 7 * nTiles_);
+  cgPart0_.allocate(sizeof(double) * dk::kPartStride * nTiles_);
+  cgPart1_.allocate(sizeof(double) * dk::kPartStride * nTiles_);
 
   // Terms in internal row indices.
   std::vector<dk::Stretch32> st(M.stretch.size());
