@@ -1479,6 +1479,38 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
       bv[j][e] = v.sellVal[base + j * 9u * kSlice + 9u * lane + e];
     }
   }
+  // Extra: the slots past the eighth (seam rows), read once into an LDS list: per lane its real lower slots 8..,
+  // then its real upper slots 8.. (xb their blocks as stored, xe their window entries), from entry xAt on. A real
+  // slot of a pinned column keeps its block and gathers the zero entry, as kPcgStep does; the slice's padding slots
+  // past a row's own count are added as the zero products they are.
+  // Each lane's first entry and counts stay in LDS too (xinfo: first | lower << 8 | upper << 12), not in registers
+  // the matrix needs.
+  __shared__ float xb[Extra ? kSolveExtraCap * 9u : 1u];
+  __shared__ uint32_t xe[Extra ? kSolveExtraCap : 1u], xinfo[Extra ? 256 : 1], xn;
+  if constexpr (Extra) {
+    if (threadIdx.x == 0) xn = 0u;
+    __syncthreads();
+    const uint32_t kd = v.diagSlot[row], lower = kd - v.rowStart[row], upper = v.rowStart[row + 1u] - kd;
+    const uint32_t xl = lower > 8u ? lower - 8u : 0u, xu = upper > 8u ? upper - 8u : 0u;
+    const uint32_t xAt = xl + xu ? atomicAdd(&xn, xl + xu) : 0u;
+    xinfo[threadIdx.x] = (xAt & 0xffu) | xl << 8 | xu << 12;
+    __syncthreads();
+    if (xn > kSolveExtraCap) {  // the host admits a matrix only when every tile's list fits: give up, as a wait would
+      if (threadIdx.x == 0) __hip_atomic_store(c.abortFlag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+    for (uint32_t i = 0; i < xl; ++i) {
+      const uint32_t at = lbase + (8u + i) * kSlice + lane;
+      xe[xAt + i] = entryOf(v.lowCol[at]);
+      const float* b = v.sellVal + size_t(v.lowBlk[at]) * 9u;
+      for (uint32_t e = 0; e < 9u; ++e) xb[(xAt + i) * 9u + e] = b[e];
+    }
+    for (uint32_t i = 0; i < xu; ++i) {
+      xe[xAt + xl + i] = entryOf(v.sellCol[base / 9u + (8u + i) * kSlice + lane]);
+      const float* b = v.sellVal + base + (8u + i) * 9u * kSlice + 9u * lane;
+      for (uint32_t e = 0; e < 9u; ++e) xb[(xAt + xl + i) * 9u + e] = b[e];
+    }
+  }
   // The rows' private PCG state of iteration -1 (with the LDS window their zr, t and p come with it).
   F3 oR = v.cgR[1][row], oAp = v.cgAp[1][row];
   F3 oZ = f3(0, 0, 0), oT = f3(0, 0, 0), oP = f3(0, 0, 0);
@@ -1616,17 +1648,35 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
 #pragma unroll
     for (uint32_t j = 0; j < 8; ++j)
       blockTMulAdd(bt[j], hpk[ent[j] & 0xffffu], acc[j % 4][0], acc[j % 4][1], acc[j % 4][2]);
-    for (uint32_t t = 8; Extra && t < KL; ++t) {  // rows with more than 8 lower slots: the rest from memory
-      const uint32_t cl = v.lowCol[lbase + t * kSlice + lane], bl = v.lowBlk[lbase + t * kSlice + lane];
-      blockTMulAdd(v.sellVal + size_t(bl) * 9u, hpk[entryOf(cl)], acc[(t - 8u) % 4][0], acc[(t - 8u) % 4][1],
-                   acc[(t - 8u) % 4][2]);
+    if constexpr (Extra) {  // lower slots 8 + i of the slice (a slice-uniform bound): the LDS list, or a zero product
+      const uint32_t info = xinfo[threadIdx.x], xAt = info & 0xffu, xl = (info >> 8) & 0xfu;
+#pragma unroll
+      for (uint32_t i = 0; i < 8u; ++i) {
+        if (8u + i >= KL) break;
+        if (i < xl) {
+          blockTMulAdd(&xb[(xAt + i) * 9u], hpk[xe[xAt + i]], acc[i % 4][0], acc[i % 4][1], acc[i % 4][2]);
+        } else {
+          acc[i % 4][0] += 0.0f;
+          acc[i % 4][1] += 0.0f;
+          acc[i % 4][2] += 0.0f;
+        }
+      }
     }
 #pragma unroll
     for (uint32_t j = 0; j < 8; ++j) blockMulAdd(bv[j], hpk[ent[j] >> 16], acc[j % 4][0], acc[j % 4][1], acc[j % 4][2]);
-    for (uint32_t t = 8; Extra && t < K; ++t) {  // rows with more than 8 upper slots
-      const uint32_t cu = v.sellCol[base / 9u + t * kSlice + lane];
-      blockMulAdd(v.sellVal + base + t * 9u * kSlice + 9u * lane, hpk[entryOf(cu)], acc[(t - 8u) % 4][0],
-                  acc[(t - 8u) % 4][1], acc[(t - 8u) % 4][2]);
+    if constexpr (Extra) {  // upper slots 8 + i
+      const uint32_t info = xinfo[threadIdx.x], xAt = info & 0xffu, xl = (info >> 8) & 0xfu, xu = info >> 12;
+#pragma unroll
+      for (uint32_t i = 0; i < 8u; ++i) {
+        if (8u + i >= K) break;
+        if (i < xu) {
+          blockMulAdd(&xb[(xAt + xl + i) * 9u], hpk[xe[xAt + xl + i]], acc[i % 4][0], acc[i % 4][1], acc[i % 4][2]);
+        } else {
+          acc[i % 4][0] += 0.0f;
+          acc[i % 4][1] += 0.0f;
+          acc[i % 4][2] += 0.0f;
+        }
+      }
     }
     const float a0 = ((acc[0][0] + acc[1][0]) + acc[2][0]) + acc[3][0];
     const float a1 = ((acc[0][1] + acc[1][1]) + acc[2][1]) + acc[3][1];

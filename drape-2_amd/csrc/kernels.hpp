@@ -168,6 +168,8 @@ void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t s
 // Distance in doubles between consecutive tiles' PCG partial sums (View::cgPart).
 constexpr uint32_t kPartStride = 7;
 constexpr uint32_t kSolveWindows = 4;  // row ranges of a tile's gathered window in the persistent solve
+// Per tile, the persistent solve's LDS list of matrix slots past the eighth (a row's upper or lower slots 9..16).
+constexpr uint32_t kSolveExtraCap = 64;
 struct SolveCtl {
   uint32_t* ctr;            // per segment 1 + 2 ceil(tiles / 16) lines of 64 words: top, group arrivals, generations
   const uint32_t* segCtr;   // per segment: its first counter, in units of 64 words

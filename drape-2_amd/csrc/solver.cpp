@@ -409,6 +409,19 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
       windowsFit = total <= dk::kHaloRows;
     }
     upload(tileWindows_, win);
+    // Per tile, the slots past the eighth of its rows (upper: the diagonal and the columns after it; lower: those
+    // before), which the persistent solve keeps in an LDS list of kSolveExtraCap entries.
+    tileExtrasMax_ = 0;
+    for (uint32_t t = 0; t < nTiles_; ++t) {
+      uint32_t extras = 0;
+      for (uint32_t r = t * dk::kTile; r < (t + 1) * dk::kTile; ++r) {
+        const uint32_t lower = uint32_t(std::lower_bound(ci.begin() + rs[r], ci.begin() + rs[r + 1], r) -
+                                        (ci.begin() + rs[r]));
+        const uint32_t upper = rs[r + 1] - rs[r] - lower;
+        extras += (lower > 8u ? lower - 8u : 0u) + (upper > 8u ? upper - 8u : 0u);
+      }
+      tileExtrasMax_ = std::max(tileExtrasMax_, extras);
+    }
   }
   tunedShape_ = -1;
   tuneSolves_ = 0;
@@ -530,7 +543,8 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
       largest = std::max(largest, params[sg].tileEnd - params[sg].tileBegin);
     }
     solveQueued_ = grid > capacity;
-    persistentOk_ = windowsFit && maxK_ <= 16u && maxLowK_ <= 16u && capacity > 0 && largest <= capacity;
+    persistentOk_ = windowsFit && maxK_ <= 16u && maxLowK_ <= 16u && tileExtrasMax_ <= dk::kSolveExtraCap &&
+                    capacity > 0 && largest <= capacity;
     solveOrderHost_.clear();
     segLastIters_.assign(M.segments.size(), 0u);
     if (solveQueued_) solveOrder_.allocate(sizeof(uint32_t) * nTiles_);

@@ -59,7 +59,8 @@ class HipSolver {
   // iteration, and the persistent form is not used again by this solver).
   void setSolveSpinCap(uint32_t cap) { solveSpinCap_ = cap; }
   // The persistent solve fits this build (each tile's columns within kSolveWindows row ranges of kHaloRows rows in
-  // all, at most 16 + 16 slots per slice, each segment's tiles resident together) and has not given up.
+  // all, at most 16 + 16 slots per slice and kSolveExtraCap slots past the eighth per tile, each segment's tiles
+  // resident together) and has not given up.
   bool persistentAvailable() const { return persistentOk_; }
   uint64_t persistentAborts() const { return persistentAborts_; }
   // Every `every`-th PCG SpMV launch of the step is bracketed by HIP events (0 = off).
@@ -170,6 +171,7 @@ class HipSolver {
   bool solveQueued_ = false;                       // more tiles than resident workgroups: segments queue
   int solvePerCu_ = 1;                             // workgroups per CU of the persistent launch (1 or 2)
   bool solveExtra_ = false;                        // some slice has more than 8 upper or lower slots
+  uint32_t tileExtrasMax_ = 0;                     // most slots past the eighth in one tile's rows
   DeviceBuffer solveOrder_;                        // the queue's tile order (device)
   std::vector<uint32_t> solveOrderHost_;           // the order last uploaded
   std::vector<std::pair<uint32_t, uint32_t>> segTiles_;  // per segment: first tile, end tile
