@@ -133,6 +133,37 @@ def test_persistent_solve_on_the_seamed_tube(hip):
     _same(p, launches)
 
 
+def _sheet_with_long_rows():
+    """The 32 x 32 sheet (pinned at vertices 0 and 31) with extra seam springs that give a few rows 9 to 12 lower or
+    upper slots: row 500 is sewn to the pinned corner 0, to 40-44 and to 900-904, row 600 to 0 and 30-33 (every pair
+    a seam of its own, so no seam hinges)."""
+    sc = scenes.c1_sheet()
+    m = sc.garments[0].mesh
+    pairs = [(500, b) for b in (0, 40, 41, 42, 43, 44, 900, 901, 902, 903, 904)] + [(600, b) for b in (0, 30, 31, 32, 33)]
+    m.seam_pairs = np.array([(a, b, k, 0) for k, (a, b) in enumerate(pairs)], dtype=np.uint32)
+    return sc
+
+
+def test_persistent_solve_with_rows_past_sixteen_slots_in_all(hip):
+    """Rows with up to four lower and four upper slots past the eighth, among them a pinned column (a real slot whose
+    gathered value is zero), and lanes of the same slices with none: the LDS list of the extra slots and the padding
+    slots' zero products keep the persistent solve bit-identical to one launch per iteration."""
+    sc = _sheet_with_long_rows()
+    probe = hip.HipSolver()
+    probe.build(sc)
+    st = probe.structure()
+    rs, ci = st["row_start"].astype(np.int64), st["col_index"].astype(np.int64)
+    lower = np.array([np.searchsorted(ci[rs[r]:rs[r + 1]], r) for r in range(len(rs) - 1)])
+    upper = np.diff(rs) - lower
+    print(f"largest lower / upper slot counts: {lower.max()} / {upper.max()}", flush=True)
+    assert lower.max() > 9 and upper.max() > 9 and lower.max() <= 16 and upper.max() <= 16
+    p = _run(hip, sc, PERSISTENT, 4)
+    assert p.persistent_available() and p.pcg_report()["persistent_aborts"] == 0
+    launches = _run(hip, sc, (1, 8, 0), 4)
+    assert np.all(np.isfinite(p.read_positions()))
+    _same(p, launches)
+
+
 def _scattered_sheet(n=64, seed=5):
     """A sheet whose vertices are numbered in random order: every tile's columns scatter over the whole matrix."""
     sc = scenes.c2_sheet(n=n, spacing=0.01)
