@@ -1378,7 +1378,9 @@ __device__ inline bool segmentBarrier(const SolveCtl& c, uint32_t s, uint32_t lo
 // straight into LDS (buffer_load ... lds: 16 bytes a lane, device scope) in 16-byte chunks from the chunk that holds
 // the window's first row, which takes no registers (two workgroups per CU leave 256 registers a lane): row i of the
 // window then starts `delta` floats into the copy, plus 3 i.
-constexpr uint32_t kHaloChunks = (kHaloRows * 12u + kSolveWindows * 15u + 15u) / 16u;
+// Chunks of the copies: a range of c rows starting d floats into its first chunk (d < 4) spans
+// ceil((12 c + 4 d) / 16) <= (12 c + 24) / 16 chunks (12 c + 4 d is a multiple of 4).
+constexpr uint32_t kHaloChunks = (kHaloRows * 12u + kSolveWindows * 24u) / 16u;
 
 __device__ inline __amdgpu_buffer_rsrc_t bufferOfBytes(const void* p, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, int(bytes), 0x00020000);
@@ -1717,6 +1719,324 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
     v.tileDone[1][tile] = 1u;
   }
   if (tile == tileBegin && threadIdx.x == 0) {
+    if (wroteR0) S->r0 = r0;
+    S->kIter[lastQ ^ 1u] = k + 1u;
+    if (k > k0) S->kIter[lastQ] = k;
+    if (updates) {
+      S->alpha = alphaLast;
+      S->rr = rrLast;
+      S->pcgIter += updates;
+      S->cntPcg += updates;
+    }
+    S->pcgConverged = 1u;
+    S->pcgExit = conv;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Persistent solve by tile pairs (kPcgSolvePair): where a scene's tiles outnumber the CUs, one 512-thread workgroup
+// per CU takes two consecutive tiles of a segment (waves 0-3 the first, 4-7 the second; the last workgroup of a
+// segment with an odd tile count has one), instead of two 256-thread workgroups per CU taking one each. The pair
+// stages one window, the union of its tiles' columns (up to kPairWindows row ranges, kPairRows rows), forms p once per
+// row of it, reads the segment's partial sums once and meets the segment's barrier once. Per row and per tile it
+// computes what kPcgSolve<2, Extra> computes, in the same order: the tiles' partial sums stay per tile, each summed
+// over its own four wavefronts, and the segment totals are the sums of the same per-tile partials in the same order.
+
+// N sums over one half of a 512-thread workgroup (wavefronts 4 half .. 4 half + 3), in blockSumN's order, returned
+// to every thread of the workgroup; every wavefront calls it. lds >= 2 * N * 4 doubles. Fresh: as in blockSumN.
+template <int N, bool Fresh = false>
+__device__ inline void halfSumN(double (&x)[N], double* lds, uint32_t half) {
+  waveTotalN(x);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if constexpr (!Fresh) __syncthreads();
+  if (l == 0)
+#pragma unroll
+    for (int i = 0; i < N; ++i) lds[(w >> 2) * N * 4 + i * 4 + (w & 3)] = x[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) t += lds[half * N * 4 + i * 4 + j];
+    x[i] = t;
+  }
+}
+
+constexpr uint32_t kPairChunks = (kPairRows * 12u + kPairWindows * 24u) / 16u;  // as kHaloChunks
+
+template <bool Extra>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) kPcgSolvePair(View v, SolveCtl c) {
+  __shared__ double red[2 * kPcgParts * 4], red2[2 * kPcgParts * 4];
+  __shared__ __attribute__((aligned(16))) float hraw[3][kPairChunks * 4];  // zr, t, p copies of the pair's window
+  __shared__ F3 hpk[kPairRows + 1];  // p_k of the window's rows; entry kPairRows: the dummy column
+  __shared__ uint32_t quit;
+  // The workgroup's pair: its first tile and tile count (queued launch: workgroup i takes pair order[i]).
+  const uint32_t wg = c.order ? (blockIdx.x < v.nPairs ? c.order[blockIdx.x] : v.nPairs) : xcdMap(blockIdx.x, v.nPairs);
+  if (wg >= v.nPairs) return;
+  const uint32_t pt = v.pairTiles[wg], first = pt & 0x7fffffffu, count = 1u + (pt >> 31);
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, half = wave >> 2;
+  const bool has = half < count;  // (uniform over each wavefront) this half has a tile; else it mirrors the first
+  const uint32_t tile = first + (has ? half : 0u);
+  const uint32_t slice = tile * kSlicesPerTile + (wave & 3u);
+  const uint32_t row = slice * kSlice + lane;
+  if (v.tileDone[0][first]) {  // the segment's solve has ended (or its Newton loop): pass the flags on
+    if (threadIdx.x < count) v.tileDone[1][first + threadIdx.x] = 1u;
+    return;
+  }
+  const uint32_t s = v.tileSeg[first];
+  const uint32_t tileBegin = v.tileRange[2u * first], tileEnd = v.tileRange[2u * first + 1u];
+  const uint32_t localWg = (first - tileBegin) / 2u, nWg = (tileEnd - tileBegin + 1u) / 2u;
+  SegState* S = v.state + s;
+  const SegParams* P = v.seg + s;
+  const uint32_t maxCg = P->maxCg;
+  const double cgTolerance = P->cgTolerance;
+  const uint32_t k0 = S->kIter[0];
+  const uint32_t restarted = S->restartActive;
+  const uint32_t kBase = S->kBase;
+  double r0 = S->r0;
+  // The pair's window: up to kPairWindows row ranges, concatenated (range w from entry off[w], its bytes from chunk
+  // ch[w] of the copies, its first row delta[w] floats into that chunk).
+  __shared__ uint32_t wf[kPairWindows], wc[kPairWindows], off[kPairWindows], ch[kPairWindows], delta[kPairWindows],
+      totals[2];
+  if (threadIdx.x == 0) {
+    uint32_t entries = 0, chunks = 0;
+    for (uint32_t w = 0; w < kPairWindows; ++w) {
+      wf[w] = v.pairWindows[2u * (kPairWindows * wg + w)];
+      wc[w] = v.pairWindows[2u * (kPairWindows * wg + w) + 1u];
+      off[w] = entries;
+      ch[w] = chunks;
+      delta[w] = (12u * wf[w] - ((12u * wf[w]) & ~15u)) / 4u;
+      entries += wc[w];
+      chunks += wc[w] ? (12u * wc[w] + 4u * delta[w] + 15u) / 16u : 0u;
+    }
+    totals[0] = entries;
+    totals[1] = chunks;
+  }
+  __syncthreads();
+  const uint32_t hc = totals[0];
+  const bool oneRange = wc[1] == 0u;
+  const auto entryOf = [&](uint32_t col) {
+    uint32_t e = kPairRows;
+    for (uint32_t w = 0; w < kPairWindows; ++w)
+      if (col - wf[w] < wc[w]) e = off[w] + (col - wf[w]);
+    return col == v.nRows ? kPairRows : e;
+  };
+  const auto rangeOf = [&](uint32_t e) {
+    uint32_t r = 0;
+    for (uint32_t w = 1; w < kPairWindows; ++w)
+      if (wc[w] && e >= off[w]) r = w;
+    return r;
+  };
+  const uint32_t ownEntry = entryOf(row);
+  const uint32_t ownRange = rangeOf(ownEntry);
+  const uint32_t own = 4u * ch[ownRange] + delta[ownRange] + 3u * (ownEntry - off[ownRange]);
+
+  const uint32_t K = v.sellK[slice], base = v.sellBase[slice], KL = v.lowK[slice], lbase = v.lowBase[slice];
+  uint32_t ent[8];
+  float bt[8][9], bv[8][9];
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j) {
+    const uint32_t cl = j < KL ? v.lowCol[lbase + j * kSlice + lane] : v.nRows;
+    const uint32_t bl = j < KL ? v.lowBlk[lbase + j * kSlice + lane] : v.zeroBlock;
+    const uint32_t cu = j < K ? v.sellCol[base / 9u + j * kSlice + lane] : v.nRows;
+    ent[j] = entryOf(cl) | (entryOf(cu) << 16);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      bt[j][e] = v.sellVal[size_t(bl) * 9u + e];
+      bv[j][e] = v.sellVal[base + j * 9u * kSlice + 9u * lane + e];
+    }
+  }
+  // Extra: the slots past the eighth in the pair's LDS list, as in kPcgSolve (2 kSolveExtraCap entries a pair).
+  constexpr uint32_t kCap = 2u * kSolveExtraCap;
+  __shared__ float xb[Extra ? kCap * 9u : 1u];
+  __shared__ uint32_t xe[Extra ? kCap : 1u], xinfo[Extra ? 512 : 1], xn;
+  if constexpr (Extra) {
+    if (threadIdx.x == 0) xn = 0u;
+    __syncthreads();
+    const uint32_t kd = v.diagSlot[row], lower = kd - v.rowStart[row], upper = v.rowStart[row + 1u] - kd;
+    const uint32_t xl = has && lower > 8u ? lower - 8u : 0u, xu = has && upper > 8u ? upper - 8u : 0u;
+    const uint32_t xAt = xl + xu ? atomicAdd(&xn, xl + xu) : 0u;
+    xinfo[threadIdx.x] = (xAt & 0xffu) | xl << 8 | xu << 12;
+    __syncthreads();
+    if (xn > kCap) {  // the host admits a matrix only when every pair's list fits: give up, as a wait would
+      if (threadIdx.x == 0) __hip_atomic_store(c.abortFlag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+    for (uint32_t i = 0; i < xl; ++i) {
+      const uint32_t at = lbase + (8u + i) * kSlice + lane;
+      xe[xAt + i] = entryOf(v.lowCol[at]);
+      const float* b = v.sellVal + size_t(v.lowBlk[at]) * 9u;
+      for (uint32_t e = 0; e < 9u; ++e) xb[(xAt + i) * 9u + e] = b[e];
+    }
+    for (uint32_t i = 0; i < xu; ++i) {
+      xe[xAt + xl + i] = entryOf(v.sellCol[base / 9u + (8u + i) * kSlice + lane]);
+      const float* b = v.sellVal + base + (8u + i) * 9u * kSlice + 9u * lane;
+      for (uint32_t e = 0; e < 9u; ++e) xb[(xAt + xl + i) * 9u + e] = b[e];
+    }
+  }
+  F3 oR = v.cgR[1][row], oAp = v.cgAp[1][row], oP = f3(0, 0, 0);
+  F3 oStep = v.step[row];
+  float m[kPrecondFloats];
+#pragma unroll
+  for (uint32_t e = 0; e < kPrecondFloats; ++e) m[e] = v.precond[kPrecondFloats * row + e];
+  const uint32_t oFlags = v.rowFlags[row], oCf = v.cflags[row];
+  const float4 oCn = v.cnormal[row];
+  if (threadIdx.x == 0) hpk[kPairRows] = f3(0, 0, 0);
+
+  const uint32_t vecBytes = 12u * (v.nRows + 1u) + 16u;
+  const __amdgpu_buffer_rsrc_t zrBuf[2] = {bufferOfBytes(v.cgZr[0], vecBytes), bufferOfBytes(v.cgZr[1], vecBytes)};
+  const __amdgpu_buffer_rsrc_t tBuf[2] = {bufferOfBytes(v.cgT[0], vecBytes), bufferOfBytes(v.cgT[1], vecBytes)};
+  const __amdgpu_buffer_rsrc_t pBuf[2] = {bufferOfBytes(v.cgP[0], vecBytes), bufferOfBytes(v.cgP[1], vecBytes)};
+  const __amdgpu_buffer_rsrc_t partBuf[2] = {bufferOf(v.cgPart[0]), bufferOf(v.cgPart[1])};
+  uint32_t updates = 0, conv = 0, k = k0, lastQ = 0;
+  double alphaLast = 0.0, rrLast = 0.0;
+  bool wroteR0 = false;
+  for (uint32_t it = 0;; ++it) {
+    const uint32_t q = it & 1u, o = q ^ 1u;
+    k = k0 + it;
+    lastQ = q;
+    // The window's (zr, t, p) straight into LDS, and (first 256 threads) the segment's partial sums, issued together.
+    for (uint32_t w = 0; w < kPairWindows; ++w) {
+      const uint32_t firstByte = (12u * wf[w]) & ~15u;
+      const uint32_t n = wc[w] ? (12u * wc[w] + 4u * delta[w] + 15u) / 16u : 0u;
+      for (uint32_t j0 = wave * 64u; j0 < n; j0 += 512u) {  // uniform over the wavefront
+        const uint32_t j = j0 + lane;
+        if (j < n) {
+          loadChunkToLds(o ? zrBuf[1] : zrBuf[0], &hraw[0][4u * (ch[w] + j0)], firstByte + 16u * j);
+          loadChunkToLds(o ? tBuf[1] : tBuf[0], &hraw[1][4u * (ch[w] + j0)], firstByte + 16u * j);
+          loadChunkToLds(o ? pBuf[1] : pBuf[0], &hraw[2][4u * (ch[w] + j0)], firstByte + 16u * j);
+        }
+      }
+    }
+    double sum[kPcgParts] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (threadIdx.x < 256u) {
+      uint32_t ti = tileBegin + threadIdx.x;
+      if (ti < tileEnd) loadPartsDevice(o ? partBuf[1] : partBuf[0], ti, sum);
+      for (ti += 256u; ti < tileEnd; ti += 256u) {
+        double more[kPcgParts];
+        loadPartsDevice(o ? partBuf[1] : partBuf[0], ti, more);
+#pragma unroll
+        for (int n = 0; n < kPcgParts; ++n) sum[n] += more[n];
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // this wavefront's chunks are in LDS; the barrier below shows them
+    halfSumN<kPcgParts, true>(sum, red, 0u);  // the first half's threads hold the 256-thread order's sums
+
+    const uint32_t kDone = k + kBase;
+    bool update = false;
+    double alpha = 0.0, beta = 0.0, rrNext = sum[kPrR];
+    if (k == 0) {
+      r0 = restarted ? r0 : sqrt(sum[kPrR]);
+      wroteR0 = true;
+      if (!(sum[kPrR] > 0.0))
+        conv = restarted ? kPcgExitTolerance : kPcgExitZero;
+      else if (kDone >= maxCg)
+        conv = kPcgExitCap;
+    } else if (!(sum[kPpAp] > 0.0)) {
+      conv = kPcgExitCurvature;
+    } else {
+      update = true;
+      alpha = sum[kPrZ] / sum[kPpAp];
+      rrNext = sum[kPrR] - 2.0 * alpha * sum[kPrAp] + alpha * alpha * sum[kPapAp];
+      const double rzNext = sum[kPrZ] - 2.0 * alpha * sum[kPrT] + alpha * alpha * sum[kPapT];
+      if (sqrt(fmax(rrNext, 0.0)) <= cgTolerance * r0)
+        conv = kPcgExitTolerance;
+      else if (kDone >= maxCg)
+        conv = kPcgExitCap;
+      else
+        beta = rzNext / sum[kPrZ];
+    }
+    if (update) {
+      ++updates;
+      alphaLast = alpha;
+      rrLast = rrNext;
+    }
+    const float a = float(alpha), b = float(beta);
+    oP = ldsF3(&hraw[2][own]);
+    if (update) oStep = oStep + a * oP;
+    if (conv) break;
+
+    for (uint32_t i = threadIdx.x; i < hc; i += 512u) {
+      uint32_t at = delta[0] + 3u * i;
+      if (!oneRange) {
+        const uint32_t w = rangeOf(i);
+        at = 4u * ch[w] + delta[w] + 3u * (i - off[w]);
+      }
+      hpk[i] = pcgDirection(ldsF3(&hraw[0][at]), ldsF3(&hraw[1][at]), ldsF3(&hraw[2][at]), a, b);
+    }
+    __syncthreads();
+
+    float acc[4][3];
+#pragma unroll
+    for (uint32_t g = 0; g < 4; ++g) acc[g][0] = acc[g][1] = acc[g][2] = 0.0f;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j)
+      blockTMulAdd(bt[j], hpk[ent[j] & 0xffffu], acc[j % 4][0], acc[j % 4][1], acc[j % 4][2]);
+    if constexpr (Extra) {
+      const uint32_t info = xinfo[threadIdx.x], xAt = info & 0xffu, xl = (info >> 8) & 0xfu;
+#pragma unroll
+      for (uint32_t i = 0; i < 8u; ++i) {
+        if (8u + i >= KL) break;
+        if (i < xl) {
+          blockTMulAdd(&xb[(xAt + i) * 9u], hpk[xe[xAt + i]], acc[i % 4][0], acc[i % 4][1], acc[i % 4][2]);
+        } else {
+          acc[i % 4][0] += 0.0f;
+          acc[i % 4][1] += 0.0f;
+          acc[i % 4][2] += 0.0f;
+        }
+      }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) blockMulAdd(bv[j], hpk[ent[j] >> 16], acc[j % 4][0], acc[j % 4][1], acc[j % 4][2]);
+    if constexpr (Extra) {
+      const uint32_t info = xinfo[threadIdx.x], xAt = info & 0xffu, xl = (info >> 8) & 0xfu, xu = info >> 12;
+#pragma unroll
+      for (uint32_t i = 0; i < 8u; ++i) {
+        if (8u + i >= K) break;
+        if (i < xu) {
+          blockMulAdd(&xb[(xAt + xl + i) * 9u], hpk[xe[xAt + xl + i]], acc[i % 4][0], acc[i % 4][1], acc[i % 4][2]);
+        } else {
+          acc[i % 4][0] += 0.0f;
+          acc[i % 4][1] += 0.0f;
+          acc[i % 4][2] += 0.0f;
+        }
+      }
+    }
+    const float a0 = ((acc[0][0] + acc[1][0]) + acc[2][0]) + acc[3][0];
+    const float a1 = ((acc[0][1] + acc[1][1]) + acc[2][1]) + acc[3][1];
+    const float a2 = ((acc[0][2] + acc[1][2]) + acc[2][2]) + acc[3][2];
+
+    const F3 rk = oR - a * oAp;
+    const F3 pk = hpk[ownEntry];
+    const F3 apk = (oFlags & kRowPinned) ? f3(0, 0, 0) : onPlane(oCf, oCn, f3(a0, a1, a2));
+    const F3 zr = applyPrecond(m, oCf, oCn, rk);
+    const F3 tk = applyPrecond(m, oCf, oCn, apk);
+    if (has) {
+      storeDevice(q ? zrBuf[1] : zrBuf[0], row, zr);
+      storeDevice(q ? tBuf[1] : tBuf[0], row, tk);
+      storeDevice(q ? pBuf[1] : pBuf[0], row, pk);
+    }
+    double d[kPcgParts] = {dot64(pk, apk), dot64(rk, zr), dot64(rk, tk), dot64(apk, tk),
+                           dot64(rk, apk), dot64(apk, apk), dot64(rk, rk)};
+    halfSumN<kPcgParts, true>(d, red2, half);
+    const uint32_t t256 = threadIdx.x & 255u;
+    if (has && t256 < uint32_t(kPcgParts)) {
+      double x = d[0];
+#pragma unroll
+      for (int n = 1; n < kPcgParts; ++n) x = t256 == uint32_t(n) ? d[n] : x;
+      storeDevice(v.cgPart[q] + kPartStride * tile + t256, x);
+    }
+    oR = rk;
+    oAp = apk;
+    if (!segmentBarrier(c, s, localWg, nWg, it, &quit)) return;
+  }
+
+  if (has) v.step[row] = oStep;
+  if (threadIdx.x < count) {
+    v.tileDone[0][first + threadIdx.x] = 1u;
+    v.tileDone[1][first + threadIdx.x] = 1u;
+  }
+  if (first == tileBegin && threadIdx.x == 0) {
     if (wroteR0) S->r0 = r0;
     S->kIter[lastQ ^ 1u] = k + 1u;
     if (k > k0) S->kIter[lastQ] = k;
@@ -2636,6 +2956,26 @@ uint32_t pcgSolveCapacity(int perCu, bool extra) {
   // perCu workgroups per CU (the kernel's registers and LDS are sized for exactly that many).
   return fits >= perCu ? uint32_t(prop.multiProcessorCount) * uint32_t(perCu) : 0u;
 }
+uint32_t pcgSolvePairCapacity(bool extra) {
+  int device = 0, fits = 0;
+  hipDeviceProp_t prop{};
+  if (hipGetDevice(&device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) return 0u;
+  const void* fn = extra ? reinterpret_cast<const void*>(kPcgSolvePair<true>)
+                         : reinterpret_cast<const void*>(kPcgSolvePair<false>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fits, fn, 512, 0) != hipSuccess) return 0u;
+  return fits >= 1 ? uint32_t(prop.multiProcessorCount) : 0u;
+}
+
+hipError_t launchPcgSolvePairs(const View& v, const SolveCtl& c, uint32_t counterLines, bool extra, hipStream_t st) {
+  launchFill(c.ctr, 0u, uint64_t(counterLines) * 64u + 1u, st);
+  const dim3 grid(c.order ? v.nPairs : xcdGrid(v.nPairs));
+  if (extra)
+    hipExtLaunchKernelGGL((kPcgSolvePair<true>), grid, dim3(512), 0, st, nullptr, nullptr, 0, v, c);
+  else
+    hipExtLaunchKernelGGL((kPcgSolvePair<false>), grid, dim3(512), 0, st, nullptr, nullptr, 0, v, c);
+  return hipGetLastError();
+}
+
 hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, int perCu, bool extra,
                           hipStream_t st) {
   // An ordinary dispatch (capturable in a graph, and traceable by the profiler, which does not survive cooperative

@@ -13,6 +13,7 @@ namespace dk {
 // "dummy column" that padding and pinned-column SELL entries point at.
 struct View {
   uint32_t nRows, nTiles, nSlices, nSegments;
+  uint32_t nPairs;  // workgroups of the persistent solve by tile pairs (kPcgSolvePair)
   uint32_t nStretch, nHinge, nSeam;
   uint32_t zeroBlock;  // index (in blocks) of an all-zero block after the last slice's values
   uint32_t uniformK, uniformKL;  // > 0: every slice has this many upper / lower slots (0: see sellK, lowK)
@@ -59,6 +60,10 @@ struct View {
   const uint32_t* tileHalo;   // per tile: first row and row count of the window holding all its columns
   // per tile: kSolveWindows (first row, row count) ranges holding all its columns (count 0: unused), for kPcgSolve
   const uint32_t* tileWindows;
+  // per pair workgroup of kPcgSolvePair: its first tile | (tiles - 1) << 31, and kPairWindows (first row, row count)
+  // ranges holding every column of its tiles' rows (count 0: unused)
+  const uint32_t* pairTiles;
+  const uint32_t* pairWindows;
   const SegParams* seg;
   SegState* state;
   double* partA;  // per tile
@@ -170,6 +175,9 @@ constexpr uint32_t kPartStride = 7;
 constexpr uint32_t kSolveWindows = 4;  // row ranges of a tile's gathered window in the persistent solve
 // Per tile, the persistent solve's LDS list of matrix slots past the eighth (a row's upper or lower slots 9..16).
 constexpr uint32_t kSolveExtraCap = 64;
+// The persistent solve by tile pairs: row ranges and rows of a pair's window.
+constexpr uint32_t kPairWindows = 8;
+constexpr uint32_t kPairRows = 3072;
 struct SolveCtl {
   uint32_t* ctr;            // per segment 1 + 2 ceil(tiles / 16) lines of 64 words: top, group arrivals, generations
   const uint32_t* segCtr;   // per segment: its first counter, in units of 64 words
@@ -189,6 +197,10 @@ uint32_t pcgSolveCapacity(int perCu, bool extra);
 // Returns the launch's error code.
 hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, int perCu, bool extra,
                           hipStream_t st);
+// Pair workgroups of kPcgSolvePair the device holds at once (one per CU; 0 if one cannot be resident).
+uint32_t pcgSolvePairCapacity(bool extra);
+// As launchPcgSolve, by tile pairs: xcdGrid(nPairs) workgroups of 512 threads (with c.order: nPairs, order[i] a pair).
+hipError_t launchPcgSolvePairs(const View& v, const SolveCtl& c, uint32_t counterLines, bool extra, hipStream_t st);
 // FP64 true residual of the finished solve and, per segment, a continuation of the solve from it when it is above
 // tol r0 (at most maxRestarts per solve; see kTrueResidual).
 void launchPcgRefine(const View& v, uint32_t maxRestarts, hipStream_t st);

@@ -422,6 +422,56 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
       }
       tileExtrasMax_ = std::max(tileExtrasMax_, extras);
     }
+    // The persistent solve by tile pairs: per segment, consecutive tiles two to a workgroup (the last alone when the
+    // count is odd); each pair's window the union of its tiles' columns, in at most kPairWindows ranges of kPairRows
+    // rows in all, and its LDS list of slots past the eighth at most 2 kSolveExtraCap long.
+    std::vector<uint32_t> pairTiles, pairWin;
+    segPairs_.assign(M.segments.size(), {0u, 0u});
+    pairsFit_ = true;
+    for (size_t sg = 0; sg < M.segments.size(); ++sg) {
+      segPairs_[sg].first = uint32_t(pairTiles.size());
+      for (uint32_t t = params[sg].tileBegin; t < params[sg].tileEnd; t += 2u) {
+        const uint32_t n = std::min(2u, params[sg].tileEnd - t);
+        pairTiles.push_back(t | (n - 1u) << 31);
+        cols.clear();
+        uint32_t extras = 0;
+        for (uint32_t r = t * dk::kTile; r < (t + n) * dk::kTile; ++r) {
+          cols.push_back(r);
+          for (uint32_t k = rs[r]; k < rs[r + 1]; ++k) cols.push_back(ci[k]);
+          const uint32_t lower = uint32_t(std::lower_bound(ci.begin() + rs[r], ci.begin() + rs[r + 1], r) -
+                                          (ci.begin() + rs[r]));
+          const uint32_t upper = rs[r + 1] - rs[r] - lower;
+          extras += (lower > 8u ? lower - 8u : 0u) + (upper > 8u ? upper - 8u : 0u);
+        }
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+        std::vector<std::pair<uint32_t, uint32_t>> runs{{cols[0], cols[0]}};
+        for (size_t k = 1; k < cols.size(); ++k) {
+          if (cols[k] - runs.back().second <= 64u)
+            runs.back().second = cols[k];
+          else
+            runs.push_back({cols[k], cols[k]});
+        }
+        while (runs.size() > dk::kPairWindows) {
+          size_t best = 0;
+          for (size_t k = 1; k + 1 < runs.size(); ++k)
+            if (runs[k + 1].first - runs[k].second < runs[best + 1].first - runs[best].second) best = k;
+          runs[best].second = runs[best + 1].second;
+          runs.erase(runs.begin() + long(best) + 1);
+        }
+        uint32_t total = 0;
+        for (uint32_t k = 0; k < dk::kPairWindows; ++k) {
+          pairWin.push_back(k < runs.size() ? runs[k].first : 0u);
+          pairWin.push_back(k < runs.size() ? runs[k].second - runs[k].first + 1 : 0u);
+          if (k < runs.size()) total += runs[k].second - runs[k].first + 1;
+        }
+        pairsFit_ = pairsFit_ && total <= dk::kPairRows && extras <= 2u * dk::kSolveExtraCap;
+      }
+      segPairs_[sg].second = uint32_t(pairTiles.size());
+    }
+    nPairs_ = uint32_t(pairTiles.size());
+    upload(pairTiles_, pairTiles);
+    upload(pairWindows_, pairWin);
   }
   tunedShape_ = -1;
   tuneSolves_ = 0;
@@ -483,6 +533,9 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
   v.tileRange = tileRange_.as<uint32_t>();
   v.tileHalo = haloFits ? tileHalo_.as<uint32_t>() : nullptr;
   v.tileWindows = windowsFit ? tileWindows_.as<uint32_t>() : nullptr;
+  v.nPairs = nPairs_;
+  v.pairTiles = pairTiles_.as<uint32_t>();
+  v.pairWindows = pairWindows_.as<uint32_t>();
   v.seg = segParams_.as<dk::SegParams>();
   v.state = segState_.as<dk::SegState>();
   v.partA = partA_.as<double>();
@@ -537,14 +590,24 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
     solvePerCu_ = grid <= capacity1 || capacity2 == 0 ? 1 : 2;
     const uint32_t capacity = solvePerCu_ == 1 ? capacity1 : capacity2;
     segTiles_.assign(M.segments.size(), {0u, 0u});
-    uint32_t largest = 0;
+    uint32_t largest = 0, largestPairs = 0;
     for (size_t sg = 0; sg < M.segments.size(); ++sg) {
       segTiles_[sg] = {params[sg].tileBegin, params[sg].tileEnd};
       largest = std::max(largest, params[sg].tileEnd - params[sg].tileBegin);
+      largestPairs = std::max(largestPairs, segPairs_[sg].second - segPairs_[sg].first);
     }
-    solveQueued_ = grid > capacity;
-    persistentOk_ = windowsFit && maxK_ <= 16u && maxLowK_ <= 16u && tileExtrasMax_ <= dk::kSolveExtraCap &&
-                    capacity > 0 && largest <= capacity;
+    // Where the tiles outnumber the CUs, tile pairs (one 512-thread workgroup a CU) when their windows fit.
+    const uint32_t pairCapacity = solvePerCu_ == 2 ? dk::pcgSolvePairCapacity(solveExtra_) : 0u;
+    solvePairs_ = solvePerCu_ == 2 && pairsFit_ && maxK_ <= 16u && maxLowK_ <= 16u && pairCapacity > 0 &&
+                  largestPairs <= pairCapacity;
+    if (solvePairs_) {
+      solveQueued_ = 8u * ((nPairs_ + 7u) / 8u) > pairCapacity;
+      persistentOk_ = true;
+    } else {
+      solveQueued_ = grid > capacity;
+      persistentOk_ = windowsFit && maxK_ <= 16u && maxLowK_ <= 16u && tileExtrasMax_ <= dk::kSolveExtraCap &&
+                      capacity > 0 && largest <= capacity;
+    }
     solveOrderHost_.clear();
     segLastIters_.assign(M.segments.size(), 0u);
     if (solveQueued_) solveOrder_.allocate(sizeof(uint32_t) * nTiles_);
@@ -989,7 +1052,8 @@ bool HipSolver::runPcgPersistent(bool refine, bool timeIt) {
       pr = takeEvents();
       check(hipEventRecord(pr.first, stream_), "event");
     }
-    const hipError_t e = dk::launchPcgSolve(view_, c, solveCtrLines_, solvePerCu_, solveExtra_, stream_);
+    const hipError_t e = solvePairs_ ? dk::launchPcgSolvePairs(view_, c, solveCtrLines_, solveExtra_, stream_)
+                                     : dk::launchPcgSolve(view_, c, solveCtrLines_, solvePerCu_, solveExtra_, stream_);
     if (e != hipSuccess) {
       (void)hipGetLastError();
       if (timeIt) eventPool_.push_back(pr);
@@ -1056,8 +1120,10 @@ void HipSolver::queueSegments() {
   std::stable_sort(work.begin(), work.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
   std::vector<uint32_t> order;
   order.reserve(nTiles_);
-  for (const auto& w : work)
-    for (uint32_t t = segTiles_[w.second].first; t < segTiles_[w.second].second; ++t) order.push_back(t);
+  for (const auto& w : work) {
+    const auto& span = solvePairs_ ? segPairs_[w.second] : segTiles_[w.second];  // workgroups: pairs or tiles
+    for (uint32_t t = span.first; t < span.second; ++t) order.push_back(t);
+  }
   if (order == solveOrderHost_) return;
   solveOrderHost_ = order;
   check(hipMemcpyAsync(solveOrder_.as<void>(), solveOrderHost_.data(), sizeof(uint32_t) * order.size(),

@@ -172,6 +172,11 @@ class HipSolver {
   int solvePerCu_ = 1;                             // workgroups per CU of the persistent launch (1 or 2)
   bool solveExtra_ = false;                        // some slice has more than 8 upper or lower slots
   uint32_t tileExtrasMax_ = 0;                     // most slots past the eighth in one tile's rows
+  bool solvePairs_ = false;                        // the persistent solve runs by tile pairs (kPcgSolvePair)
+  bool pairsFit_ = false;                          // every pair's window and extra slots fit kPcgSolvePair
+  uint32_t nPairs_ = 0;
+  DeviceBuffer pairTiles_, pairWindows_;
+  std::vector<std::pair<uint32_t, uint32_t>> segPairs_;  // per segment: its pair workgroups [first, end)
   DeviceBuffer solveOrder_;                        // the queue's tile order (device)
   std::vector<uint32_t> solveOrderHost_;           // the order last uploaded
   std::vector<std::pair<uint32_t, uint32_t>> segTiles_;  // per segment: first tile, end tile
