@@ -1261,7 +1261,8 @@ kPcgStep(View v, uint32_t q) {
 //
 // One workgroup owns one 256-row tile for the whole solve, one wavefront per 64-row slice. PerCu = 1 gives each
 // wavefront a SIMD to itself and holds the gathered window in registers; PerCu = 2 puts two workgroups on a CU
-// within 256 registers and stages the window in LDS with buffer loads to LDS. The slice's matrix (upper block values
+// within 256 registers and stages the window in LDS with buffer loads to LDS (kPcgSolvePair, below, does the same
+// with one workgroup per pair of tiles, which the host prefers where the pairs' windows fit). The slice's matrix (upper block values
 // and columns, the lower slots' transposed blocks and columns: the first 8 + 8 slots; with Extra, a seam row's slots
 // past the eighth, at most 16 + 16, are read from memory every iteration) and its rows' private PCG data (r, ap,
 // step, the preconditioner, the row and contact flags) are loaded once and stay in registers. An iteration moves only

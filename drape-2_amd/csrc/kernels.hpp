@@ -197,6 +197,9 @@ uint32_t pcgSolveCapacity(int perCu, bool extra);
 // Returns the launch's error code.
 hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, int perCu, bool extra,
                           hipStream_t st);
+// Persistent solve by tile pairs (kPcgSolvePair): for launches whose tiles outnumber the CUs, one 512-thread
+// workgroup per CU takes two consecutive tiles of a segment, with one window (the union of the tiles' columns, at
+// most kPairWindows ranges of kPairRows rows in all; View::pairWindows); bit-identical to kPcgSolve.
 // Pair workgroups of kPcgSolvePair the device holds at once (one per CU; 0 if one cannot be resident).
 uint32_t pcgSolvePairCapacity(bool extra);
 // As launchPcgSolve, by tile pairs: xcdGrid(nPairs) workgroups of 512 threads (with c.order: nPairs, order[i] a pair).

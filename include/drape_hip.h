@@ -246,9 +246,9 @@ int drape_solver_set_refinement(drape_solver* s, int32_t restarts);
  * solve, each 256-row tile's workgroup keeping its slices' matrix and private vectors in registers, the tiles of a
  * scene meeting at a barrier of their own each iteration (scenes of a batch never wait for each other); it needs
  * every 256-row tile's columns within at most four row ranges of 1,536 rows in all, at most 16 + 16 matrix slots
- * per 64-row slice, and every scene's tiles resident together (two workgroups per CU: at most 512 tiles, 131,072
- * particle rows, per scene on MI355X; a batch's scenes pass through the CUs one after another when their tiles
- * outnumber that). If a barrier wait gives up (drape_solver_pcg_report counts
+ * per 64-row slice, and every scene's tiles resident together (two tiles per CU — a 512-thread workgroup per pair of
+ * tiles where their windows fit, else two workgroups — at most 512 tiles, 131,072 particle rows, per scene on
+ * MI355X; a batch's scenes pass through the CUs one after another when their tiles outnumber that). If a barrier wait gives up (drape_solver_pcg_report counts
  * it), the solve is re-run with one launch per iteration and the solver stops using the persistent form. Results
  * are bit-identical for every shape. drape_solver_pcg_shape reports the shape in use (0x0 while still tuning). No
  * reference counterpart: the reference's GPU path has one thread per row (kernels.h:272-289). */
