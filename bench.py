@@ -391,7 +391,7 @@ def main() -> int:
     ap.add_argument("--newton", type=int, default=5, help="Newton iteration cap of the timed run")
     ap.add_argument("--pcg-shape", default="0x0",
                     help="launch shape of the one-kernel PCG iteration, wavefronts x slots: 1x8, 1x4, 2x8, 4x8, "
-                         "or 1x8h / 1x4h (column window staged in LDS), or 1x8p (the persistent solve: one cooperative "
+                         "or 1x8h / 1x4h (column window staged in LDS), or 1x8p (the persistent solve: one kernel "
                          "launch per PCG solve); 0x0 = tune during warm-up")
     ap.add_argument("--tee", choices=("draft", "standard", "fine"),
                     help="instead of the sheet benchmark: drape the reference's tee to rest at this quality")

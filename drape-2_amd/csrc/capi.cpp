@@ -484,6 +484,8 @@ int drape_solver_debug_system(drape_solver* s, float* blocks, float* grad, float
   });
 }
 
+}  // extern "C"
+
 namespace {
 std::vector<bake::Component> bakeComponents(const drape_body_mesh* components, uint32_t n) {
   need(components, "components");
@@ -505,6 +507,8 @@ bake::Options bakeOptions(const drape_bake_options* o) {
   return b;
 }
 }  // namespace
+
+extern "C" {
 
 int drape_bake_field_grid(const drape_body_mesh* components, uint32_t n, const drape_bake_options* options,
                           drape_field_desc* out) {

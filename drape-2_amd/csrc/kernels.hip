@@ -1259,10 +1259,9 @@ kPcgStep(View v, uint32_t q) {
 // k-th kPcgStep<1, 8, true> launch computes (same expressions, same summation orders), so the two forms give
 // bit-identical results.
 //
-// One workgroup owns one 256-row tile for the whole solve, one wavefront per 64-row slice. PerCu = 1 gives each
-// wavefront a SIMD to itself and holds the gathered window in registers; PerCu = 2 puts two workgroups on a CU
-// within 256 registers and stages the window in LDS with buffer loads to LDS (kPcgSolvePair, below, does the same
-// with one workgroup per pair of tiles, which the host prefers where the pairs' windows fit). The slice's matrix (upper block values
+// One workgroup owns one 256-row tile for the whole solve, one wavefront per 64-row slice, each wavefront with a
+// SIMD to itself, the gathered window in registers. Where the tiles outnumber the CUs, kPcgSolvePair (below) runs
+// two tiles per 512-thread workgroup, one workgroup per CU, with the window staged in LDS. The slice's matrix (upper block values
 // and columns, the lower slots' transposed blocks and columns: the first 8 + 8 slots; with Extra, a seam row's slots
 // past the eighth, at most 16 + 16, are read from memory every iteration) and its rows' private PCG data (r, ap,
 // step, the preconditioner, the row and contact flags) are loaded once and stay in registers. An iteration moves only
@@ -1374,14 +1373,11 @@ __device__ inline bool segmentBarrier(const SolveCtl& c, uint32_t s, uint32_t lo
 }
 }  // namespace
 
-// The window of each exchanged vector: with one workgroup per CU it is loaded into registers (3 rows a thread on a
-// sheet) and only the rows' search directions go to LDS; with two it goes to LDS as a byte copy of memory, loaded
-// straight into LDS (buffer_load ... lds: 16 bytes a lane, device scope) in 16-byte chunks from the chunk that holds
-// the window's first row, which takes no registers (two workgroups per CU leave 256 registers a lane): row i of the
-// window then starts `delta` floats into the copy, plus 3 i.
-// Chunks of the copies: a range of c rows starting d floats into its first chunk (d < 4) spans
-// ceil((12 c + 4 d) / 16) <= (12 c + 24) / 16 chunks (12 c + 4 d is a multiple of 4).
-constexpr uint32_t kHaloChunks = (kHaloRows * 12u + kSolveWindows * 24u) / 16u;
+// The window of each exchanged vector: kPcgSolve loads it into registers (3 rows a thread on a sheet) and only the
+// rows' search directions go to LDS. kPcgSolvePair, whose two wavefronts per SIMD leave 256 registers a lane, copies
+// it to LDS byte for byte, loaded straight into LDS (buffer_load ... lds: 16 bytes a lane, device scope) in 16-byte
+// chunks from the chunk that holds the window's first row: row i of the window then starts `delta` floats into the
+// copy, plus 3 i.
 
 __device__ inline __amdgpu_buffer_rsrc_t bufferOfBytes(const void* p, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, int(bytes), 0x00020000);
@@ -1393,14 +1389,11 @@ __device__ inline void loadChunkToLds(__amdgpu_buffer_rsrc_t r, float* lds, uint
   __builtin_amdgcn_raw_ptr_buffer_load_lds(r, lds, 16, offset, 0, 0, kScDevice);
 }
 
-// PerCu: workgroups per CU (1: each of a tile's wavefronts has a SIMD to itself; 2: two tiles share a CU, for
-// launches with more tiles than CUs). Extra: some slice has more than 8 upper or lower slots (seam rows); the slots
-// past the eighth come from memory every iteration. The arithmetic is the same in every instance.
-template <int PerCu, bool Extra>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu, PerCu))) kPcgSolve(View v, SolveCtl c) {
+// Extra: some slice has more than 8 upper or lower slots (seam rows); the slots past the eighth go to an LDS list.
+// The arithmetic is the same in every instance.
+template <bool Extra>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) kPcgSolve(View v, SolveCtl c) {
   __shared__ double red[kPcgParts * 4], red2[kPcgParts * 4];
-  constexpr bool Dma = PerCu == 2;
-  __shared__ __attribute__((aligned(16))) float hraw[Dma ? 3 : 1][Dma ? kHaloChunks * 4 : 4];  // zr, t, p copies
   __shared__ F3 hpk[kHaloRows + 1];  // p_k of the window's rows; entry kHaloRows: the dummy column
   __shared__ uint32_t quit;
   // Queued launch (more tiles than the device holds at once): workgroup i takes tile order[i]; each segment's tiles
@@ -1425,24 +1418,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
   const uint32_t restarted = S->restartActive;
   const uint32_t kBase = S->kBase;
   double r0 = S->r0;
-  // The tile's window: up to kSolveWindows row ranges, concatenated in LDS (range w from entry off[w]); with the
-  // LDS copies, range w's bytes start in chunk ch[w], its first row `delta[w]` floats in. The table lives in LDS
-  // (read at uniform addresses): in registers it would crowd out the matrix.
-  __shared__ uint32_t wf[kSolveWindows], wc[kSolveWindows], off[kSolveWindows], ch[kSolveWindows],
-      delta[kSolveWindows], totals[2];
+  // The tile's window: up to kSolveWindows row ranges, concatenated in LDS (range w from entry off[w]). The table
+  // lives in LDS (read at uniform addresses): in registers it would crowd out the matrix.
+  __shared__ uint32_t wf[kSolveWindows], wc[kSolveWindows], off[kSolveWindows], totals[1];
   if (threadIdx.x == 0) {
-    uint32_t entries = 0, chunks = 0;
+    uint32_t entries = 0;
     for (uint32_t w = 0; w < kSolveWindows; ++w) {
       wf[w] = v.tileWindows[2u * (kSolveWindows * tile + w)];
       wc[w] = v.tileWindows[2u * (kSolveWindows * tile + w) + 1u];
       off[w] = entries;
-      ch[w] = chunks;
-      delta[w] = (12u * wf[w] - ((12u * wf[w]) & ~15u)) / 4u;
       entries += wc[w];
-      chunks += wc[w] ? (12u * wc[w] + 4u * delta[w] + 15u) / 16u : 0u;
     }
     totals[0] = entries;
-    totals[1] = chunks;
   }
   __syncthreads();
   const uint32_t hc = totals[0];
@@ -1461,9 +1448,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
       if (wc[w] && e >= off[w]) r = w;
     return r;
   };
-  const uint32_t ownEntry = entryOf(row);
-  const uint32_t ownRange = rangeOf(ownEntry);
-  const uint32_t own = 4u * ch[ownRange] + delta[ownRange] + 3u * (ownEntry - off[ownRange]);  // in the copies
 
   // The slice's matrix, as kPcgStep<1, 8, true> loads it in its first round trip; columns become window offsets
   // (the dummy column: the zero entry at kHaloRows).
@@ -1514,14 +1498,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
       for (uint32_t e = 0; e < 9u; ++e) xb[(xAt + xl + i) * 9u + e] = b[e];
     }
   }
-  // The rows' private PCG state of iteration -1 (with the LDS window their zr, t and p come with it).
+  // The rows' private PCG state of iteration -1.
   F3 oR = v.cgR[1][row], oAp = v.cgAp[1][row];
-  F3 oZ = f3(0, 0, 0), oT = f3(0, 0, 0), oP = f3(0, 0, 0);
-  if constexpr (!Dma) {
-    oZ = v.cgZr[1][row];
-    oT = v.cgT[1][row];
-    oP = v.cgP[1][row];
-  }
+  F3 oZ = v.cgZr[1][row], oT = v.cgT[1][row], oP = v.cgP[1][row];
   F3 oStep = v.step[row];
   float m[kPrecondFloats];
 #pragma unroll
@@ -1544,36 +1523,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
     k = k0 + it;
     lastQ = q;
     // The window's (zr, t, p) and the segment's partial sums of the previous iteration, issued together.
-    constexpr uint32_t HR = Dma ? 1u : kHaloRows / 256u;
+    constexpr uint32_t HR = kHaloRows / 256u;
     F3 wz[HR], wt[HR], wp[HR];
-    if constexpr (Dma) {
-      for (uint32_t w = 0; w < kSolveWindows; ++w) {
-        const uint32_t first = (12u * wf[w]) & ~15u;
-        const uint32_t n = wc[w] ? (12u * wc[w] + 4u * delta[w] + 15u) / 16u : 0u;
-        for (uint32_t j0 = wave * 64u; j0 < n; j0 += 256u) {  // uniform over the wavefront
-          const uint32_t j = j0 + lane;
-          if (j < n) {
-            loadChunkToLds(o ? zrBuf[1] : zrBuf[0], &hraw[0][4u * (ch[w] + j0)], first + 16u * j);
-            loadChunkToLds(o ? tBuf[1] : tBuf[0], &hraw[1][4u * (ch[w] + j0)], first + 16u * j);
-            loadChunkToLds(o ? pBuf[1] : pBuf[0], &hraw[2][4u * (ch[w] + j0)], first + 16u * j);
-          }
-        }
-      }
-    } else {
-      const uint32_t hr = (hc + 255u) / 256u;  // window rows per thread (uniform over the workgroup)
+    const uint32_t hr = (hc + 255u) / 256u;  // window rows per thread (uniform over the workgroup)
 #pragma unroll
-      for (uint32_t r = 0; r < HR; ++r) {
-        if (r < hr) {
-          const uint32_t e = min(r * 256u + threadIdx.x, hc - 1u);
-          uint32_t i = first0 + e;
-          if (!oneRange) {
-            const uint32_t w = rangeOf(e);
-            i = wf[w] + (e - off[w]);
-          }
-          wz[r] = loadDevice(o ? zrBuf[1] : zrBuf[0], i);
-          wt[r] = loadDevice(o ? tBuf[1] : tBuf[0], i);
-          wp[r] = loadDevice(o ? pBuf[1] : pBuf[0], i);
+    for (uint32_t r = 0; r < HR; ++r) {
+      if (r < hr) {
+        const uint32_t e = min(r * 256u + threadIdx.x, hc - 1u);
+        uint32_t i = first0 + e;
+        if (!oneRange) {
+          const uint32_t w = rangeOf(e);
+          i = wf[w] + (e - off[w]);
         }
+        wz[r] = loadDevice(o ? zrBuf[1] : zrBuf[0], i);
+        wt[r] = loadDevice(o ? tBuf[1] : tBuf[0], i);
+        wp[r] = loadDevice(o ? pBuf[1] : pBuf[0], i);
       }
     }
     double sum[kPcgParts] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1585,7 +1549,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
 #pragma unroll
       for (int n = 0; n < kPcgParts; ++n) sum[n] += more[n];
     }
-    if constexpr (Dma) __builtin_amdgcn_s_waitcnt(0);  // this wavefront's chunks are in LDS; the barrier below shows them
     blockSumN<kPcgParts, 4, true>(sum, red);
 
     // The decision of kPcgStep, from the same sums.
@@ -1619,28 +1582,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
       rrLast = rrNext;
     }
     const float a = float(alpha), b = float(beta);
-    if constexpr (Dma) oP = ldsF3(&hraw[2][own]);
     if (update) oStep = oStep + a * oP;
     if (conv) break;
 
     // Each window row's search direction once (kPcgStep rebuilds it per gathered column from the same values with
     // the same expression), to LDS.
-    if constexpr (Dma) {
-      const uint32_t delta0 = delta[0];
-      for (uint32_t i = threadIdx.x; i < hc; i += 256u) {
-        uint32_t at = delta0 + 3u * i;
-        if (!oneRange) {
-          const uint32_t w = rangeOf(i);
-          at = 4u * ch[w] + delta[w] + 3u * (i - off[w]);
-        }
-        hpk[i] = pcgDirection(ldsF3(&hraw[0][at]), ldsF3(&hraw[1][at]), ldsF3(&hraw[2][at]), a, b);
-      }
-    } else {
 #pragma unroll
-      for (uint32_t r = 0; r < HR; ++r) {
-        const uint32_t i = r * 256u + threadIdx.x;
-        if (i < hc) hpk[i] = pcgDirection(wz[r], wt[r], wp[r], a, b);
-      }
+    for (uint32_t r = 0; r < HR; ++r) {
+      const uint32_t i = r * 256u + threadIdx.x;
+      if (i < hc) hpk[i] = pcgDirection(wz[r], wt[r], wp[r], a, b);
     }
     __syncthreads();
 
@@ -1686,7 +1636,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
     const float a2 = ((acc[0][2] + acc[1][2]) + acc[2][2]) + acc[3][2];
 
     const F3 rk = oR - a * oAp;
-    const F3 pk = Dma ? hpk[ownEntry] : pcgDirection(oZ, oT, oP, a, b);  // (the same expression either way)
+    const F3 pk = pcgDirection(oZ, oT, oP, a, b);
     const F3 apk = (oFlags & kRowPinned) ? f3(0, 0, 0) : onPlane(oCf, oCn, f3(a0, a1, a2));
     const F3 zr = applyPrecond(m, oCf, oCn, rk);
     const F3 tk = applyPrecond(m, oCf, oCn, apk);
@@ -1704,11 +1654,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
     }
     oR = rk;
     oAp = apk;
-    if constexpr (!Dma) {
-      oZ = zr;
-      oT = tk;
-      oP = pk;
-    }
+    oZ = zr;
+    oT = tk;
+    oP = pk;
     if (!segmentBarrier(c, s, tile - tileBegin, tileEnd - tileBegin, it, &quit)) return;
   }
 
@@ -1740,7 +1688,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PerCu,
 // segment with an odd tile count has one), instead of two 256-thread workgroups per CU taking one each. The pair
 // stages one window, the union of its tiles' columns (up to kPairWindows row ranges, kPairRows rows), forms p once per
 // row of it, reads the segment's partial sums once and meets the segment's barrier once. Per row and per tile it
-// computes what kPcgSolve<2, Extra> computes, in the same order: the tiles' partial sums stay per tile, each summed
+// computes what kPcgSolve<Extra> computes for a tile, in the same order: the tiles' partial sums stay per tile, each summed
 // over its own four wavefronts, and the segment totals are the sums of the same per-tile partials in the same order.
 
 // N sums over one half of a 512-thread workgroup (wavefronts 4 half .. 4 half + 3), in blockSumN's order, returned
@@ -2945,18 +2893,16 @@ void launchPcgStep(const View& v, uint32_t parity, PcgShape shape, hipStream_t s
   else
     throw std::invalid_argument("PCG launch shape must be 1x8, 1x4, 2x8 or 4x8, or 1x8 / 1x4 with the halo");
 }
-uint32_t pcgSolveCapacity(int perCu, bool extra) {
+uint32_t pcgSolveCapacity(bool extra) {
   int device = 0, fits = 0;
   hipDeviceProp_t prop{};
   if (hipGetDevice(&device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) return 0u;
-  const void* fn = perCu == 2 ? (extra ? reinterpret_cast<const void*>(kPcgSolve<2, true>)
-                                       : reinterpret_cast<const void*>(kPcgSolve<2, false>))
-                              : (extra ? reinterpret_cast<const void*>(kPcgSolve<1, true>)
-                                       : reinterpret_cast<const void*>(kPcgSolve<1, false>));
+  const void* fn = extra ? reinterpret_cast<const void*>(kPcgSolve<true>) : reinterpret_cast<const void*>(kPcgSolve<false>);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fits, fn, 256, 0) != hipSuccess) return 0u;
-  // perCu workgroups per CU (the kernel's registers and LDS are sized for exactly that many).
-  return fits >= perCu ? uint32_t(prop.multiProcessorCount) * uint32_t(perCu) : 0u;
+  // one workgroup per CU (the kernel's registers are sized for exactly that)
+  return fits >= 1 ? uint32_t(prop.multiProcessorCount) : 0u;
 }
+
 uint32_t pcgSolvePairCapacity(bool extra) {
   int device = 0, fits = 0;
   hipDeviceProp_t prop{};
@@ -2977,21 +2923,16 @@ hipError_t launchPcgSolvePairs(const View& v, const SolveCtl& c, uint32_t counte
   return hipGetLastError();
 }
 
-hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, int perCu, bool extra,
-                          hipStream_t st) {
+hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, bool extra, hipStream_t st) {
   // An ordinary dispatch (capturable in a graph, and traceable by the profiler, which does not survive cooperative
   // dispatches): the host has checked that the grid fits the device at once (pcgSolveCapacity), the solver's stream
   // runs nothing beside it, and a workgroup that found no room would only make its segment's waits give up.
   launchFill(c.ctr, 0u, uint64_t(counterLines) * 64u + 1u, st);
   const dim3 grid(c.order ? v.nTiles : xcdGrid(v.nTiles));
-  if (perCu == 2 && extra)
-    hipExtLaunchKernelGGL((kPcgSolve<2, true>), grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
-  else if (perCu == 2)
-    hipExtLaunchKernelGGL((kPcgSolve<2, false>), grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
-  else if (extra)  // (the one-per-CU instances' registers allow one workgroup per CU)
-    hipExtLaunchKernelGGL((kPcgSolve<1, true>), grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
+  if (extra)
+    hipExtLaunchKernelGGL((kPcgSolve<true>), grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
   else
-    hipExtLaunchKernelGGL((kPcgSolve<1, false>), grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
+    hipExtLaunchKernelGGL((kPcgSolve<false>), grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
   return hipGetLastError();
 }
 void launchPcgRefine(const View& v, uint32_t maxRestarts, hipStream_t st) {

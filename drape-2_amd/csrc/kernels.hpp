@@ -151,7 +151,7 @@ void launchStats(const View& v, const double* massF64, double* tilePart, double*
 struct PcgShape {
   uint32_t waves = 1, slots = 8;
   bool halo = false;        // gathers from an LDS window of the tile's columns (every tile's window <= kHaloRows rows)
-  bool persistent = false;  // one cooperative launch per solve (kPcgSolve) instead of one launch per iteration
+  bool persistent = false;  // one launch per solve (kPcgSolve, kPcgSolvePair) instead of one launch per iteration
   bool operator==(const PcgShape& o) const {
     return waves == o.waves && slots == o.slots && halo == o.halo && persistent == o.persistent;
   }
@@ -189,14 +189,13 @@ struct SolveCtl {
 };
 // Counter lines a segment of `tiles` tiles needs.
 inline uint32_t solveCounterLines(uint32_t tiles) { return 1u + 2u * ((tiles + 15u) / 16u); }
-// Workgroups of kPcgSolve the device holds at once with perCu (1 or 2) workgroups per CU (0 if that many cannot be
-// resident); extra: the instance for slices with more than 8 upper or lower slots.
-uint32_t pcgSolveCapacity(int perCu, bool extra);
+// Workgroups of kPcgSolve the device holds at once, one per CU (0 if one cannot be resident); extra: the instance
+// for slices with more than 8 upper or lower slots.
+uint32_t pcgSolveCapacity(bool extra);
 // Clears the counters and the abort flag (counterLines * 64 + 1 words at c.ctr, the flag last) and launches the
 // solve (an ordinary dispatch of xcdGrid(nTiles) workgroups; the caller checks the grid against pcgSolveCapacity).
 // Returns the launch's error code.
-hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, int perCu, bool extra,
-                          hipStream_t st);
+hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, bool extra, hipStream_t st);
 // Persistent solve by tile pairs (kPcgSolvePair): for launches whose tiles outnumber the CUs, one 512-thread
 // workgroup per CU takes two consecutive tiles of a segment, with one window (the union of the tiles' columns, at
 // most kPairWindows ranges of kPairRows rows in all; View::pairWindows); bit-identical to kPcgSolve.

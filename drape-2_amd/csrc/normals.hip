@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(256) kNormalize(uint32_t n, const dk::F3* __re
 }
 
 Mesh::Mesh(uint32_t n, const uint32_t* triangles, uint32_t nTriangles, const uint32_t* welds, uint32_t nWelds)
-    : n_(n), nTri_(nTriangles) {
+    : n_(n) {
   for (size_t i = 0; i < 3 * size_t(nTriangles); ++i)
     if (triangles[i] >= n) throw std::out_of_range("normals: triangle vertex out of range");
   for (size_t i = 0; i < 2 * size_t(nWelds); ++i)

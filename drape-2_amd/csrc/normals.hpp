@@ -41,7 +41,7 @@ class Mesh {
   void compute(const dk::F3* pos, dk::F3* out, hipStream_t st) const;
 
  private:
-  uint32_t n_ = 0, nTri_ = 0, nLayers_ = 0;
+  uint32_t n_ = 0, nLayers_ = 0;
   std::vector<uint32_t> layerStart_;  // host copy: nLayers + 1
   dh::DeviceBuffer tri_, vtxStart_, vtxTri_, weldA_, weldB_, layerStartDev_, acc_;
 };

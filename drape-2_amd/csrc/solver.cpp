@@ -582,31 +582,26 @@ void HipSolver::build(const drape_scene_desc* scenes, uint32_t nScenes) {
             "hipHostMalloc");
     *solveAbortHost_ = 0u;
     // Every segment's tiles must be resident together: all tiles in one launch when they fit the device, else a
-    // queue of segments (solveQueued_) in which each segment fits it.
-    // One workgroup per CU when every tile fits that way (each wavefront then has a SIMD to itself), else two.
+    // queue of segments (solveQueued_) in which each segment fits it. One tile per workgroup and per CU when every
+    // tile fits that way (each wavefront then has a SIMD to itself), else tile pairs, one pair per CU.
     const uint32_t grid = 8u * ((nTiles_ + 7u) / 8u);
     solveExtra_ = maxK_ > 8u || maxLowK_ > 8u;
-    const uint32_t capacity1 = dk::pcgSolveCapacity(1, solveExtra_), capacity2 = dk::pcgSolveCapacity(2, solveExtra_);
-    solvePerCu_ = grid <= capacity1 || capacity2 == 0 ? 1 : 2;
-    const uint32_t capacity = solvePerCu_ == 1 ? capacity1 : capacity2;
+    const uint32_t capacity = dk::pcgSolveCapacity(solveExtra_);
     segTiles_.assign(M.segments.size(), {0u, 0u});
-    uint32_t largest = 0, largestPairs = 0;
+    uint32_t largestPairs = 0;
     for (size_t sg = 0; sg < M.segments.size(); ++sg) {
       segTiles_[sg] = {params[sg].tileBegin, params[sg].tileEnd};
-      largest = std::max(largest, params[sg].tileEnd - params[sg].tileBegin);
       largestPairs = std::max(largestPairs, segPairs_[sg].second - segPairs_[sg].first);
     }
-    // Where the tiles outnumber the CUs, tile pairs (one 512-thread workgroup a CU) when their windows fit.
-    const uint32_t pairCapacity = solvePerCu_ == 2 ? dk::pcgSolvePairCapacity(solveExtra_) : 0u;
-    solvePairs_ = solvePerCu_ == 2 && pairsFit_ && maxK_ <= 16u && maxLowK_ <= 16u && pairCapacity > 0 &&
-                  largestPairs <= pairCapacity;
+    solvePairs_ = grid > capacity;
     if (solvePairs_) {
+      const uint32_t pairCapacity = dk::pcgSolvePairCapacity(solveExtra_);
       solveQueued_ = 8u * ((nPairs_ + 7u) / 8u) > pairCapacity;
-      persistentOk_ = true;
+      persistentOk_ = pairsFit_ && maxK_ <= 16u && maxLowK_ <= 16u && pairCapacity > 0 && largestPairs <= pairCapacity;
     } else {
-      solveQueued_ = grid > capacity;
+      solveQueued_ = false;
       persistentOk_ = windowsFit && maxK_ <= 16u && maxLowK_ <= 16u && tileExtrasMax_ <= dk::kSolveExtraCap &&
-                      capacity > 0 && largest <= capacity;
+                      capacity > 0;
     }
     solveOrderHost_.clear();
     segLastIters_.assign(M.segments.size(), 0u);
@@ -1053,7 +1048,7 @@ bool HipSolver::runPcgPersistent(bool refine, bool timeIt) {
       check(hipEventRecord(pr.first, stream_), "event");
     }
     const hipError_t e = solvePairs_ ? dk::launchPcgSolvePairs(view_, c, solveCtrLines_, solveExtra_, stream_)
-                                     : dk::launchPcgSolve(view_, c, solveCtrLines_, solvePerCu_, solveExtra_, stream_);
+                                     : dk::launchPcgSolve(view_, c, solveCtrLines_, solveExtra_, stream_);
     if (e != hipSuccess) {
       (void)hipGetLastError();
       if (timeIt) eventPool_.push_back(pr);
@@ -1147,9 +1142,9 @@ void HipSolver::runPcgSingleKernel() {
   if (shapeIdx >= 0 && !shapeAvailable(shapeIdx)) {
     if (dk::kPcgShapes[shapeIdx].persistent)
       throw std::invalid_argument(
-          "PCG persistent solve: needs every 256-row tile's columns within four row ranges of 1536 rows in all, at "
-          "most 16 + 16 slots per 64-row slice, each scene's tiles resident together (512 per scene), and must not "
-          "have given up before");
+          "PCG persistent solve: needs every 256-row tile's columns within four row ranges of 1536 rows in all (or, "
+          "with more tiles than CUs, every tile pair's within eight ranges of 3072 rows), at most 16 + 16 slots per "
+          "64-row slice, each scene's tiles resident together (512 per scene), and must not have given up before");
     throw std::invalid_argument("PCG halo shape: a 256-row tile of this matrix reaches more than 1536 rows");
   }
   dk::launchPcgStepInit(view_, stream_);

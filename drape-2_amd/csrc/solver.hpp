@@ -52,15 +52,16 @@ class HipSolver {
   // Wavefronts per 64-row slice in the one-kernel PCG iteration: 1, 2 or 4, or 0 to time all three during the
   // first solves and keep the fastest (results are bit-identical for every choice).
   // Launch shape of the one-kernel PCG iteration (wavefronts per slice x slots per batch; 0x0 = tune); halo 2 = the
-  // persistent solve (one cooperative launch per solve, 1x8 with the halo).
+  // persistent solve (one launch per solve, 1x8 with the halo).
   void setPcgShape(int waves, int slots, int halo);
   void pcgShape(int& waves, int& slots, int& halo) const;  // 0x0 while still tuning
   // Polls of a persistent solve's barrier before a workgroup gives up (the solve is then re-run with one launch per
   // iteration, and the persistent form is not used again by this solver).
   void setSolveSpinCap(uint32_t cap) { solveSpinCap_ = cap; }
-  // The persistent solve fits this build (each tile's columns within kSolveWindows row ranges of kHaloRows rows in
-  // all, at most 16 + 16 slots per slice and kSolveExtraCap slots past the eighth per tile, each segment's tiles
-  // resident together) and has not given up.
+  // The persistent solve fits this build (at most 16 + 16 slots per slice; one tile per CU: each tile's columns
+  // within kSolveWindows row ranges of kHaloRows rows in all and at most kSolveExtraCap slots past the eighth; tile
+  // pairs where the tiles outnumber the CUs: each pair's within kPairWindows ranges of kPairRows rows and twice the
+  // slots; each segment's tiles resident together) and has not given up.
   bool persistentAvailable() const { return persistentOk_; }
   uint64_t persistentAborts() const { return persistentAborts_; }
   // Every `every`-th PCG SpMV launch of the step is bracketed by HIP events (0 = off).
@@ -169,7 +170,6 @@ class HipSolver {
   uint32_t* solveAbortHost_ = nullptr;
   bool persistentOk_ = false;
   bool solveQueued_ = false;                       // more tiles than resident workgroups: segments queue
-  int solvePerCu_ = 1;                             // workgroups per CU of the persistent launch (1 or 2)
   bool solveExtra_ = false;                        // some slice has more than 8 upper or lower slots
   uint32_t tileExtrasMax_ = 0;                     // most slots past the eighth in one tile's rows
   bool solvePairs_ = false;                        // the persistent solve runs by tile pairs (kPcgSolvePair)

@@ -242,13 +242,14 @@ int drape_solver_set_refinement(drape_solver* s, int32_t restarts);
  * tiles than CUs), 2x8 and 4x8, or 0x0 (default) to time every shape during the first solves and keep the fastest.
  * halo = 1 (with 1x8 or 1x4) stages each 256-row tile's column window in LDS and gathers from there; it runs only
  * on matrices whose every tile reaches at most 1,536 consecutive rows (a step with it fails otherwise), and the
- * tuner tries it only there. halo = 2 (with 1x8) is the persistent solve: one cooperative launch runs a whole PCG
+ * tuner tries it only there. halo = 2 (with 1x8) is the persistent solve: one kernel launch runs a whole PCG
  * solve, each 256-row tile's workgroup keeping its slices' matrix and private vectors in registers, the tiles of a
  * scene meeting at a barrier of their own each iteration (scenes of a batch never wait for each other); it needs
- * every 256-row tile's columns within at most four row ranges of 1,536 rows in all, at most 16 + 16 matrix slots
- * per 64-row slice, and every scene's tiles resident together (two tiles per CU — a 512-thread workgroup per pair of
- * tiles where their windows fit, else two workgroups — at most 512 tiles, 131,072 particle rows, per scene on
- * MI355X; a batch's scenes pass through the CUs one after another when their tiles outnumber that). If a barrier wait gives up (drape_solver_pcg_report counts
+ * at most 16 + 16 matrix slots per 64-row slice and every scene's tiles resident together: one tile per CU, each
+ * 256-row tile's columns within at most four row ranges of 1,536 rows in all, or, where the tiles outnumber the CUs,
+ * a 512-thread workgroup per pair of consecutive tiles, each pair's columns within eight row ranges of 3,072 rows
+ * (at most 512 tiles, 131,072 particle rows, per scene on MI355X; a batch's scenes pass through the CUs one after
+ * another when their tiles outnumber that). If a barrier wait gives up (drape_solver_pcg_report counts
  * it), the solve is re-run with one launch per iteration and the solver stops using the persistent form. Results
  * are bit-identical for every shape. drape_solver_pcg_shape reports the shape in use (0x0 while still tuning). No
  * reference counterpart: the reference's GPU path has one thread per row (kernels.h:272-289). */
