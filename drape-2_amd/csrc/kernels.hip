@@ -1522,33 +1522,33 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     const uint32_t q = it & 1u, o = q ^ 1u;
     k = k0 + it;
     lastQ = q;
-    // The window's (zr, t, p) and the segment's partial sums of the previous iteration, issued together.
-    constexpr uint32_t HR = kHaloRows / 256u;
+    // The window's (zr, t, p) and the segment's partial sums of the previous iteration (at most 256 tiles: one a
+    // thread). Loads complete in the order they are issued, and the sums are needed first: the window rows past the
+    // third of each thread (wide windows only) go before them, the first three rows (every thread, the index clamped
+    // to the window) after them, so that the sums wait for no window row.
+    constexpr uint32_t HR = kHaloRows / 256u, HR0 = 3u;
     F3 wz[HR], wt[HR], wp[HR];
     const uint32_t hr = (hc + 255u) / 256u;  // window rows per thread (uniform over the workgroup)
-#pragma unroll
-    for (uint32_t r = 0; r < HR; ++r) {
-      if (r < hr) {
-        const uint32_t e = min(r * 256u + threadIdx.x, hc - 1u);
-        uint32_t i = first0 + e;
-        if (!oneRange) {
-          const uint32_t w = rangeOf(e);
-          i = wf[w] + (e - off[w]);
-        }
-        wz[r] = loadDevice(o ? zrBuf[1] : zrBuf[0], i);
-        wt[r] = loadDevice(o ? tBuf[1] : tBuf[0], i);
-        wp[r] = loadDevice(o ? pBuf[1] : pBuf[0], i);
+    const auto loadRow = [&](uint32_t r) {
+      const uint32_t e = min(r * 256u + threadIdx.x, hc - 1u);
+      uint32_t i = first0 + e;
+      if (!oneRange) {
+        const uint32_t w = rangeOf(e);
+        i = wf[w] + (e - off[w]);
       }
-    }
-    double sum[kPcgParts] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    uint32_t ti = tileBegin + threadIdx.x;
-    if (ti < tileEnd) loadPartsDevice(o ? partBuf[1] : partBuf[0], ti, sum);
-    for (ti += 256u; ti < tileEnd; ti += 256u) {
-      double more[kPcgParts];
-      loadPartsDevice(o ? partBuf[1] : partBuf[0], ti, more);
+      wz[r] = loadDevice(o ? zrBuf[1] : zrBuf[0], i);
+      wt[r] = loadDevice(o ? tBuf[1] : tBuf[0], i);
+      wp[r] = loadDevice(o ? pBuf[1] : pBuf[0], i);
+    };
 #pragma unroll
-      for (int n = 0; n < kPcgParts; ++n) sum[n] += more[n];
-    }
+    for (uint32_t r = HR0; r < HR; ++r)
+      if (r < hr) loadRow(r);
+    double sum[kPcgParts] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const uint32_t ti = tileBegin + threadIdx.x;
+    if (ti < tileEnd) loadPartsDevice(o ? partBuf[1] : partBuf[0], ti, sum);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (uint32_t r = 0; r < HR0; ++r) loadRow(r);
     blockSumN<kPcgParts, 4, true>(sum, red);
 
     // The decision of kPcgStep, from the same sums.
