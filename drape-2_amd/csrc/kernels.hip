@@ -1844,21 +1844,37 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
     const uint32_t q = it & 1u, o = q ^ 1u;
     k = k0 + it;
     lastQ = q;
-    // The window's (zr, t, p) straight into LDS, and (first 256 threads) the segment's partial sums, issued together.
-    for (uint32_t w = 0; w < kPairWindows; ++w) {
-      const uint32_t firstByte = (12u * wf[w]) & ~15u;
-      const uint32_t n = wc[w] ? (12u * wc[w] + 4u * delta[w] + 15u) / 16u : 0u;
-      for (uint32_t j0 = wave * 64u; j0 < n; j0 += 512u) {  // uniform over the wavefront
-        const uint32_t j = j0 + lane;
-        if (j < n) {
-          loadChunkToLds(o ? zrBuf[1] : zrBuf[0], &hraw[0][4u * (ch[w] + j0)], firstByte + 16u * j);
-          loadChunkToLds(o ? tBuf[1] : tBuf[0], &hraw[1][4u * (ch[w] + j0)], firstByte + 16u * j);
-          loadChunkToLds(o ? pBuf[1] : pBuf[0], &hraw[2][4u * (ch[w] + j0)], firstByte + 16u * j);
+    // The first half's wavefronts load the segment's partial sums (the 256-thread order) and reduce them; meanwhile
+    // the second half's copy the window's (zr, t, p) straight into LDS, needed only after the decision. (With the
+    // extra-slot list, Extra, the split costs spilled registers and measured slower: there every wavefront copies its
+    // share of the window, the first half's also load the sums, and the window is waited for before the reduction.)
+    double sum[kPcgParts] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if constexpr (Extra) {
+      for (uint32_t w = 0; w < kPairWindows; ++w) {
+        const uint32_t firstByte = (12u * wf[w]) & ~15u;
+        const uint32_t n = wc[w] ? (12u * wc[w] + 4u * delta[w] + 15u) / 16u : 0u;
+        for (uint32_t j0 = wave * 64u; j0 < n; j0 += 512u) {  // uniform over the wavefront
+          const uint32_t j = j0 + lane;
+          if (j < n) {
+            loadChunkToLds(o ? zrBuf[1] : zrBuf[0], &hraw[0][4u * (ch[w] + j0)], firstByte + 16u * j);
+            loadChunkToLds(o ? tBuf[1] : tBuf[0], &hraw[1][4u * (ch[w] + j0)], firstByte + 16u * j);
+            loadChunkToLds(o ? pBuf[1] : pBuf[0], &hraw[2][4u * (ch[w] + j0)], firstByte + 16u * j);
+          }
         }
       }
-    }
-    double sum[kPcgParts] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (threadIdx.x < 256u) {
+      if (half == 0u) {
+        uint32_t ti = tileBegin + threadIdx.x;
+        if (ti < tileEnd) loadPartsDevice(o ? partBuf[1] : partBuf[0], ti, sum);
+        for (ti += 256u; ti < tileEnd; ti += 256u) {
+          double more[kPcgParts];
+          loadPartsDevice(o ? partBuf[1] : partBuf[0], ti, more);
+#pragma unroll
+          for (int n = 0; n < kPcgParts; ++n) sum[n] += more[n];
+        }
+      }
+      __builtin_amdgcn_s_waitcnt(0);  // this wavefront's chunks are in LDS; the barrier below shows them
+      halfSumN<kPcgParts, true>(sum, red, 0u);  // the first half's threads hold the 256-thread order's sums
+    } else if (half == 0u) {
       uint32_t ti = tileBegin + threadIdx.x;
       if (ti < tileEnd) loadPartsDevice(o ? partBuf[1] : partBuf[0], ti, sum);
       for (ti += 256u; ti < tileEnd; ti += 256u) {
@@ -1867,9 +1883,34 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
         for (int n = 0; n < kPcgParts; ++n) sum[n] += more[n];
       }
+      waveTotalN(sum);
+      if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < kPcgParts; ++i) red[i * 4 + wave] = sum[i];
+    } else {
+      for (uint32_t w = 0; w < kPairWindows; ++w) {
+        const uint32_t firstByte = (12u * wf[w]) & ~15u;
+        const uint32_t n = wc[w] ? (12u * wc[w] + 4u * delta[w] + 15u) / 16u : 0u;
+        for (uint32_t j0 = (wave - 4u) * 64u; j0 < n; j0 += 256u) {  // uniform over the wavefront
+          const uint32_t j = j0 + lane;
+          if (j < n) {
+            loadChunkToLds(o ? zrBuf[1] : zrBuf[0], &hraw[0][4u * (ch[w] + j0)], firstByte + 16u * j);
+            loadChunkToLds(o ? tBuf[1] : tBuf[0], &hraw[1][4u * (ch[w] + j0)], firstByte + 16u * j);
+            loadChunkToLds(o ? pBuf[1] : pBuf[0], &hraw[2][4u * (ch[w] + j0)], firstByte + 16u * j);
+          }
+        }
+      }
     }
-    __builtin_amdgcn_s_waitcnt(0);  // this wavefront's chunks are in LDS; the barrier below shows them
-    halfSumN<kPcgParts, true>(sum, red, 0u);  // the first half's threads hold the 256-thread order's sums
+    if constexpr (!Extra) {
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < kPcgParts; ++i) {  // blockSumN's order over the first half's four wavefronts
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t += red[i * 4 + j];
+        sum[i] = t;
+      }
+    }
 
     const uint32_t kDone = k + kBase;
     bool update = false;
@@ -1901,6 +1942,10 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
       rrLast = rrNext;
     }
     const float a = float(alpha), b = float(beta);
+    if constexpr (!Extra) {
+      __builtin_amdgcn_s_waitcnt(0);  // (second half) this wavefront's chunks are in LDS; the barrier shows them
+      __syncthreads();
+    }
     oP = ldsF3(&hraw[2][own]);
     if (update) oStep = oStep + a * oP;
     if (conv) break;
