@@ -39,7 +39,7 @@ KERNEL_FIGURES = {
     (1, 8, 2): {"kernel": "kPcgSolve (the persistent solve: one launch runs a whole PCG solve of the same one-kernel "
                           "iteration, the matrix and each row's private vectors held in registers, a barrier per "
                           "scene per iteration)",
-                "trace_us": 6.326, "traffic": 5.18e6},
+                "trace_us": 6.301, "traffic": 5.18e6},
 }
 
 
