@@ -173,7 +173,8 @@ class HipSolver:
         """Launch shape of the one-kernel PCG iteration: (wavefronts per 64-row slice, slots per batch) in
         (1, 8), (1, 4), (2, 8), (4, 8), or (0, 0) to tune (default); halo=1 (with (1, 8) or (1, 4)) gathers from
         an LDS copy of each tile's column window, for matrices whose every tile spans at most 1,536 rows; halo=2
-        (with (1, 8)) is the persistent solve: one cooperative launch per PCG solve (PERSISTENT)."""
+        (with (1, 8)) is the persistent solve: one launch per PCG solve, each scene's tiles resident together
+        (PERSISTENT)."""
         _check(_lib().drape_solver_set_pcg_shape(self._h, int(waves), int(slots), int(halo)))
 
     def pcg_shape(self) -> tuple:
