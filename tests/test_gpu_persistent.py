@@ -121,15 +121,29 @@ def test_persistent_solve_two_tiles_per_cu_on_one_large_scene(hip):
 
 
 def test_persistent_solve_on_the_seamed_tube(hip):
-    """The 120,000-particle seamed tube: 469 tiles (two to a CU), each tile's columns in up to four row ranges (its
-    own panel's neighbourhood and the seam partners' on the other panel), seam rows with 9 upper and 9 lower slots
-    (the slots past the eighth read from memory each iteration). Three steps equal one launch per iteration bit for
-    bit."""
+    """The 120,000-particle seamed tube: 469 tiles, run by pairs (235 workgroups, one per CU; the last pair has one
+    tile), each pair's columns in up to eight row ranges (its panels' neighbourhoods and the seam partners' on the
+    other panel), seam rows with 9 upper and 9 lower slots (the slots past the eighth held in the pair's LDS list).
+    Three steps equal one launch per iteration bit for bit."""
     sc = scenes.c3_tube()
     p = _run(hip, sc, PERSISTENT, 3)
     print(f"tube, 3 steps: counters {p.counters()}, aborts {p.pcg_report()['persistent_aborts']}", flush=True)
     assert p.persistent_available() and p.pcg_report()["persistent_aborts"] == 0
     launches = _run(hip, sc, (1, 8, 0), 3)
+    _same(p, launches)
+
+
+def test_persistent_pairs_queued_on_a_batch_of_seamed_tubes(hip):
+    """Three seamed tubes of 120,000, 78,000 and 48,000 particles batched: 962 tiles, 482 pairs, more than the CUs,
+    so the seam rows' pairs are queued through the CUs tube by tube (each tube's pairs resident together, at most
+    235). Two steps equal one launch per iteration bit for bit, tube by tube, with no barrier giving up."""
+    batch = [scenes.c3_tube(), scenes.c3_tube(nx=150, ny=260, seed=5), scenes.c3_tube(nx=120, ny=200, seed=6)]
+    steps = 2
+    p = _run(hip, batch, PERSISTENT, steps, batch=True)
+    rep = p.pcg_report()
+    print(f"3-tube batch, {steps} steps: counters {p.counters()}, aborts {rep['persistent_aborts']}", flush=True)
+    assert p.persistent_available() and p.pcg_shape() == PERSISTENT and rep["persistent_aborts"] == 0
+    launches = _run(hip, batch, (1, 8, 0), steps, batch=True)
     _same(p, launches)
 
 
