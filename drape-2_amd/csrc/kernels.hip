@@ -1291,7 +1291,6 @@ kPcgStep(View v, uint32_t q) {
 // set (by the host or by another workgroup that gave up), the workgroup sets the abort flag and leaves, writing
 // nothing; the host then re-runs the solve with kPcgStep launches.
 namespace {
-constexpr uint32_t kBarrierGroup = 16u;
 constexpr int kScDevice = 16;  // cache-policy operand of the buffer builtins: sc1 (device scope)
 typedef uint32_t U3 __attribute__((ext_vector_type(3)));
 

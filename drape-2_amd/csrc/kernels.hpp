@@ -187,8 +187,12 @@ struct SolveCtl {
   // consecutive, so that segments run through the device's resident workgroups in this order.
   const uint32_t* order;
 };
-// Counter lines a segment of `tiles` tiles needs.
-inline uint32_t solveCounterLines(uint32_t tiles) { return 1u + 2u * ((tiles + 15u) / 16u); }
+// Tiles (or tile pairs) per group of the persistent solve's two-level segment barrier.
+constexpr uint32_t kBarrierGroup = 16u;
+// Counter lines a segment of `tiles` tiles needs: the top counter, and per group its arrivals and its generation.
+inline uint32_t solveCounterLines(uint32_t tiles) {
+  return 1u + 2u * ((tiles + kBarrierGroup - 1u) / kBarrierGroup);
+}
 // Workgroups of kPcgSolve the device holds at once, one per CU (0 if one cannot be resident); extra: the instance
 // for slices with more than 8 upper or lower slots.
 uint32_t pcgSolveCapacity(bool extra);
