@@ -2052,10 +2052,10 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
 // continuation's first state written as the output of a kernel "-1" (r, zr = S M r, p = ap = t = 0; step kept);
 // per tile: |r|^2 and the partial sums the next kPcgStep reads, and in partC the rows' sum of (A_ij step_j)^2 over
 // every product term, from which kRefineDecide estimates the part of |r| that rounding step to FP32 leaves.
-__global__ void __launch_bounds__(256) kTrueResidual(View v) {
+__global__ void __launch_bounds__(256) kTrueResidual(View v, const uint32_t* abortFlag) {
   __shared__ double lds[kPcgParts * 4];
   const uint32_t tile = xcdMap(blockIdx.x, v.nTiles);
-  if (tile >= v.nTiles) return;
+  if (tile >= v.nTiles || (abortFlag && *abortFlag)) return;
   const uint32_t s = v.tileSeg[tile];
   const SegState* S = v.state + s;
   if (S->newtonDone || S->pcgExit != kPcgExitTolerance) return;
@@ -2103,12 +2103,13 @@ constexpr double kRefineFactor = 4.0;
 // is above kRefineFactor max(tol r0, the FP32 rounding floor) and iterations remain (at most `maxRestarts`
 // continuations per solve). Every tile of the segment gets its solve-ended flag for the next kPcgStep; the host
 // looks at pcgConverged afterwards.
-__global__ void __launch_bounds__(256) kRefineDecide(View v, uint32_t maxRestarts) {
+__global__ void __launch_bounds__(256) kRefineDecide(View v, uint32_t maxRestarts, const uint32_t* abortFlag) {
   __shared__ double lds[8];
   const uint32_t s = blockIdx.x;
   SegState* S = v.state + s;
   const SegParams& P = v.seg[s];
-  const bool candidate = !S->newtonDone && S->pcgExit == kPcgExitTolerance;
+  const bool aborted = abortFlag && *abortFlag;  // the persistent solve before gave up: no continuation
+  const bool candidate = !aborted && !S->newtonDone && S->pcgExit == kPcgExitTolerance;
   const double trueSq = candidate ? rangeSum256(v.partB, P.tileBegin, P.tileEnd, lds) : 0.0;
   const double termSq = candidate ? rangeSum256(v.partC, P.tileBegin, P.tileEnd, lds) : 0.0;
   const double floor32 = 0x1p-24 * sqrt(termSq / 3.0);
@@ -2957,8 +2958,9 @@ uint32_t pcgSolvePairCapacity(bool extra) {
   return fits >= 1 ? uint32_t(prop.multiProcessorCount) : 0u;
 }
 
-hipError_t launchPcgSolvePairs(const View& v, const SolveCtl& c, uint32_t counterLines, bool extra, hipStream_t st) {
-  launchFill(c.ctr, 0u, uint64_t(counterLines) * 64u + 1u, st);
+hipError_t launchPcgSolvePairs(const View& v, const SolveCtl& c, uint32_t counterLines, bool extra, bool clearAbort,
+                               hipStream_t st) {
+  launchFill(c.ctr, 0u, uint64_t(counterLines) * 64u + (clearAbort ? 1u : 0u), st);
   const dim3 grid(c.order ? v.nPairs : xcdGrid(v.nPairs));
   if (extra)
     hipExtLaunchKernelGGL((kPcgSolvePair<true>), grid, dim3(512), 0, st, nullptr, nullptr, 0, v, c);
@@ -2967,11 +2969,12 @@ hipError_t launchPcgSolvePairs(const View& v, const SolveCtl& c, uint32_t counte
   return hipGetLastError();
 }
 
-hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, bool extra, hipStream_t st) {
+hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, bool extra, bool clearAbort,
+                          hipStream_t st) {
   // An ordinary dispatch (capturable in a graph, and traceable by the profiler, which does not survive cooperative
   // dispatches): the host has checked that the grid fits the device at once (pcgSolveCapacity), the solver's stream
   // runs nothing beside it, and a workgroup that found no room would only make its segment's waits give up.
-  launchFill(c.ctr, 0u, uint64_t(counterLines) * 64u + 1u, st);
+  launchFill(c.ctr, 0u, uint64_t(counterLines) * 64u + (clearAbort ? 1u : 0u), st);
   const dim3 grid(c.order ? v.nTiles : xcdGrid(v.nTiles));
   if (extra)
     hipExtLaunchKernelGGL((kPcgSolve<true>), grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
@@ -2979,9 +2982,9 @@ hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLine
     hipExtLaunchKernelGGL((kPcgSolve<false>), grid, dim3(256), 0, st, nullptr, nullptr, 0, v, c);
   return hipGetLastError();
 }
-void launchPcgRefine(const View& v, uint32_t maxRestarts, hipStream_t st) {
-  kTrueResidual<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);
-  kRefineDecide<<<v.nSegments, 256, 0, st>>>(v, maxRestarts);
+void launchPcgRefine(const View& v, uint32_t maxRestarts, const uint32_t* abortFlag, hipStream_t st) {
+  kTrueResidual<<<xcdGrid(v.nTiles), 256, 0, st>>>(v, abortFlag);
+  kRefineDecide<<<v.nSegments, 256, 0, st>>>(v, maxRestarts, abortFlag);
 }
 void launchContactForceSlope(const View& v, hipStream_t st) {
   kContactForceSlope<<<xcdGrid(v.nTiles), 256, 0, st>>>(v);

@@ -196,20 +196,23 @@ inline uint32_t solveCounterLines(uint32_t tiles) {
 // Workgroups of kPcgSolve the device holds at once, one per CU (0 if one cannot be resident); extra: the instance
 // for slices with more than 8 upper or lower slots.
 uint32_t pcgSolveCapacity(bool extra);
-// Clears the counters and the abort flag (counterLines * 64 + 1 words at c.ctr, the flag last) and launches the
-// solve (an ordinary dispatch of xcdGrid(nTiles) workgroups; the caller checks the grid against pcgSolveCapacity).
-// Returns the launch's error code.
-hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, bool extra, hipStream_t st);
+// Clears the counters (counterLines * 64 words at c.ctr) and, with clearAbort, the abort flag after them, and
+// launches the solve (an ordinary dispatch of xcdGrid(nTiles) workgroups; the caller checks the grid against
+// pcgSolveCapacity). A flag left set makes the launch's waits give up at once. Returns the launch's error code.
+hipError_t launchPcgSolve(const View& v, const SolveCtl& c, uint32_t counterLines, bool extra, bool clearAbort,
+                          hipStream_t st);
 // Persistent solve by tile pairs (kPcgSolvePair): for launches whose tiles outnumber the CUs, one 512-thread
 // workgroup per CU takes two consecutive tiles of a segment, with one window (the union of the tiles' columns, at
 // most kPairWindows ranges of kPairRows rows in all; View::pairWindows); bit-identical to kPcgSolve.
 // Pair workgroups of kPcgSolvePair the device holds at once (one per CU; 0 if one cannot be resident).
 uint32_t pcgSolvePairCapacity(bool extra);
 // As launchPcgSolve, by tile pairs: xcdGrid(nPairs) workgroups of 512 threads (with c.order: nPairs, order[i] a pair).
-hipError_t launchPcgSolvePairs(const View& v, const SolveCtl& c, uint32_t counterLines, bool extra, hipStream_t st);
+hipError_t launchPcgSolvePairs(const View& v, const SolveCtl& c, uint32_t counterLines, bool extra, bool clearAbort,
+                               hipStream_t st);
 // FP64 true residual of the finished solve and, per segment, a continuation of the solve from it when it is above
-// tol r0 (at most maxRestarts per solve; see kTrueResidual).
-void launchPcgRefine(const View& v, uint32_t maxRestarts, hipStream_t st);
+// tol r0 (at most maxRestarts per solve; see kTrueResidual). abortFlag (or null): when it is set, the persistent
+// solve before gave up, and no segment continues.
+void launchPcgRefine(const View& v, uint32_t maxRestarts, const uint32_t* abortFlag, hipStream_t st);
 void launchContactForceSlope(const View& v, hipStream_t st);
 void launchNewtonSlope(const View& v, hipStream_t st);
 void launchEnergyTrial(const View& v, hipStream_t st);

@@ -993,7 +993,7 @@ void HipSolver::runPcgTuning(uint32_t maxCg) {
   }
   const dk::PcgShape shape = dk::kPcgShapes[dk::kPcgShapes[refineIdx].persistent ? 0 : refineIdx];
   for (uint32_t r = 0; r < refineRestarts_; ++r) {
-    dk::launchPcgRefine(view_, refineRestarts_, stream_);
+    dk::launchPcgRefine(view_, refineRestarts_, nullptr, stream_);
     pollStates();
     if (allSegments([](const dk::SegState& s) { return s.newtonDone != 0u || s.pcgConverged != 0u; })) break;
     for (uint32_t n = 0; n < maxCg + 1; n += kTuneChunk) {
@@ -1017,10 +1017,14 @@ int HipSolver::fastestSampledShape() const {
   return best < 0 ? 0 : best;
 }
 
-// The rest of the current solve with the persistent kernel: one launch runs every segment's solve to its end, then (when `refine`) each FP64 true-residual refinement round is a decision and, for the segments that
-// continue, one more launch. With `timeIt` each launch is bracketed by events and counted in the iteration timing
-// by the iterations it ran. A launch that fails, or whose barrier gave up (abort flag), switches the persistent form
-// off for this solver and returns false: the caller then re-runs the solve.
+// The rest of the current solve with the persistent kernel: one launch runs every segment's solve to its end, then
+// (when `refine`) each FP64 true-residual refinement round is a decision and, for the segments that continue, one
+// more launch. The first round goes out right behind the solve, and the host looks at the flags once after both:
+// the abort flag is cleared only before the solve, so a solve that gave up makes the round a no-op (no segment
+// continues, and the continuation's waits give up at once) and is still seen. With `timeIt` the launches between
+// two looks are bracketed by events and counted in the iteration timing by the iterations they ran. A launch that
+// fails, or whose barrier gave up (abort flag), switches the persistent form off for this solver and returns false:
+// the caller then re-runs the solve.
 bool HipSolver::runPcgPersistent(bool refine, bool timeIt) {
   dk::SolveCtl c{};
   c.ctr = solveCtr_.as<uint32_t>();
@@ -1034,36 +1038,44 @@ bool HipSolver::runPcgPersistent(bool refine, bool timeIt) {
     if (tunedShape_ >= 0 && dk::kPcgShapes[tunedShape_].persistent) tunedShape_ = fastestSampledShape();
     return false;
   };
-  // Iterations done before each launch, per segment (the first launch follows kPcgStepInit when timed: none).
+  // Iterations done before each look, per segment (the first launch follows kPcgStepInit when timed: none).
   // A segment whose Newton loop had already finished keeps its last solve's count (kPcgStepInit leaves it).
   std::vector<uint32_t> before(model_.segments.size(), 0u);
   for (size_t g = 0; g < model_.segments.size(); ++g)
     if (mirrorFresh_ && hostStates_[g].newtonDone) before[g] = hostStates_[g].pcgIter;
   c.order = solveQueued_ ? solveOrder_.as<uint32_t>() : nullptr;
-  const auto launch = [&]() -> bool {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> timed;  // the launches since the last look
+  const auto launch = [&](bool clearAbort) -> bool {
     if (solveQueued_) queueSegments();
     std::pair<hipEvent_t, hipEvent_t> pr{};
     if (timeIt) {
       pr = takeEvents();
       check(hipEventRecord(pr.first, stream_), "event");
     }
-    const hipError_t e = solvePairs_ ? dk::launchPcgSolvePairs(view_, c, solveCtrLines_, solveExtra_, stream_)
-                                     : dk::launchPcgSolve(view_, c, solveCtrLines_, solveExtra_, stream_);
+    const hipError_t e = solvePairs_ ? dk::launchPcgSolvePairs(view_, c, solveCtrLines_, solveExtra_, clearAbort, stream_)
+                                     : dk::launchPcgSolve(view_, c, solveCtrLines_, solveExtra_, clearAbort, stream_);
     if (e != hipSuccess) {
       (void)hipGetLastError();
       if (timeIt) eventPool_.push_back(pr);
       return false;
     }
-    if (timeIt) check(hipEventRecord(pr.second, stream_), "event");
+    if (timeIt) {
+      check(hipEventRecord(pr.second, stream_), "event");
+      timed.push_back(pr);
+    }
+    return true;
+  };
+  const auto look = [&]() -> bool {
     check(hipMemcpyAsync(solveAbortHost_, c.abortFlag, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "copy");
     pollStates();
     if (*solveAbortHost_) {
-      if (timeIt) eventPool_.push_back(pr);
+      for (const auto& pr : timed) eventPool_.push_back(pr);
+      timed.clear();
       return false;
     }
     if (timeIt) {
       // Counted as many iterations as its longest segment ran, carrying the bytes every segment's own iterations
-      // move: bytes over time is then the launch's achieved rate.
+      // move: bytes over time is then the launches' achieved rate.
       uint32_t most = 0;
       double bytes = 0.0;
       for (size_t g = 0; g < model_.segments.size(); ++g) {
@@ -1073,24 +1085,36 @@ bool HipSolver::runPcgPersistent(bool refine, bool timeIt) {
         before[g] = hostStates_[g].pcgIter;
       }
       float ms = 0.0f;
-      check(hipEventElapsedTime(&ms, pr.first, pr.second), "event");
+      for (const auto& pr : timed) {
+        float part = 0.0f;
+        check(hipEventElapsedTime(&part, pr.first, pr.second), "event");
+        ms += part;
+        eventPool_.push_back(pr);
+      }
+      timed.clear();
       if (most > 0) {
         spmvTotalMs_ += ms;
         spmvTimed_ += most;
         spmvTotalBytes_ += bytes;
       }
-      eventPool_.push_back(pr);
     }
     return true;
   };
-  if (!launch()) return fail();
+  const auto continued = [&] {
+    bool any = false;
+    for (size_t g = 0; g < model_.segments.size(); ++g) any = any || hostStates_[g].restartActive != 0u;
+    return any;
+  };
+  if (!launch(true)) return fail();
+  if (refine && refineRestarts_ > 0) {
+    dk::launchPcgRefine(view_, refineRestarts_, c.abortFlag, stream_);
+    if (!launch(false)) return fail();
+  }
+  if (!look()) return fail();
   if (refine)
-    for (uint32_t r = 0; r < refineRestarts_; ++r) {
-      dk::launchPcgRefine(view_, refineRestarts_, stream_);
-      if (!launch()) return fail();
-      bool continued = false;
-      for (size_t g = 0; g < model_.segments.size(); ++g) continued = continued || hostStates_[g].restartActive != 0u;
-      if (!continued) break;
+    for (uint32_t r = 1; r < refineRestarts_ && continued(); ++r) {
+      dk::launchPcgRefine(view_, refineRestarts_, c.abortFlag, stream_);
+      if (!launch(false) || !look()) return fail();
     }
   uint32_t total = 0;
   for (size_t g = 0; g < model_.segments.size(); ++g) {
@@ -1225,7 +1249,7 @@ void HipSolver::runPcgSingleKernel() {
   // The decision and a first short chunk of the continuation go out together (the chunk's launches are no-ops for
   // a segment that does not continue), so a look at the flags follows both.
   for (uint32_t r = 0; r < refineRestarts_; ++r) {
-    dk::launchPcgRefine(view_, refineRestarts_, stream_);
+    dk::launchPcgRefine(view_, refineRestarts_, nullptr, stream_);
     if (useGraph_) {
       if (!refineExec_.exec || refineShape_ != shapeIdx) {
         refineExec_.release();
