@@ -1056,7 +1056,11 @@ bool HipSolver::runPcgPersistent(bool refine, bool timeIt) {
                                      : dk::launchPcgSolve(view_, c, solveCtrLines_, solveExtra_, clearAbort, stream_);
     if (e != hipSuccess) {
       (void)hipGetLastError();
-      if (timeIt) eventPool_.push_back(pr);
+      if (timeIt) {
+        timed.push_back(pr);
+        for (const auto& t : timed) eventPool_.push_back(t);
+        timed.clear();
+      }
       return false;
     }
     if (timeIt) {
